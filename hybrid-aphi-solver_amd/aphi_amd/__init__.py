@@ -1,0 +1,260 @@
+"""Python host mirror of the A-Phi drop-in path, over the C-ABI of include/aphi_gpu.h.
+
+Names follow the reference's call surface (include/aphi_solver/assembly.hpp,
+factorization.hpp): ``refill``, ``factorize_ldlt``, ``solve``, ``solve_symmetric``,
+``backward_error``. Error behaviour follows the reference's conventions:
+std::invalid_argument -> ValueError, std::logic_error -> LogicError,
+std::runtime_error -> RuntimeError, a pivot at or below the floor -> the call
+returns ``ok = False`` with its stats filled.
+
+Device arrays are torch tensors (PyTorch is only the allocator here); every
+number is computed by libaphi_amd.so. There is no CPU fallback.
+"""
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from ._capi import (APHI_FP32, APHI_FP64, APHI_NATURAL, APHI_ORDER_AMD, APHI_ORDER_NATURAL, APHI_ORDER_RCM,
+                    APHI_ROW_SCALED, APHI_SCALED_PHI)
+
+__all__ = [
+    "LogicError", "DeviceError", "CubeConfig", "Problem", "Context", "Mesh", "frequency_sweep",
+    "APHI_FP32", "APHI_FP64", "APHI_NATURAL", "APHI_ROW_SCALED", "APHI_SCALED_PHI",
+    "APHI_ORDER_AMD", "APHI_ORDER_NATURAL", "APHI_ORDER_RCM", "lib",
+]
+
+
+class LogicError(RuntimeError):
+    """std::logic_error of the reference: the symbolic and numeric passes disagree."""
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure."""
+
+
+def lib():
+    return _capi.load()
+
+
+def _check(status):
+    if status == _capi.APHI_OK:
+        return
+    msg = lib().aphi_last_error().decode()
+    if status == _capi.APHI_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if status == _capi.APHI_LOGIC_ERROR:
+        raise LogicError(msg)
+    if status in (_capi.APHI_DEVICE_ERROR, _capi.APHI_OUT_OF_MEMORY):
+        raise DeviceError(msg)
+    raise RuntimeError(msg)
+
+
+def frequency_sweep(start, stop, points, logarithmic=True):
+    """expand_frequency_sweep (src/input_file.cpp:19-50)."""
+    out = np.zeros(points, dtype=np.float64)
+    _check(lib().aphi_frequency_sweep(start, stop, points, int(logarithmic),
+                                      out.ctypes.data_as(_capi.DP)))
+    return out
+
+
+@dataclass
+class CubeConfig:
+    """A Kuhn cube with a conductor bar spanning z and voltage ports on its ends."""
+    n: int = 6
+    side: float = 0.1
+    bar: tuple = (2, 4, 2, 4)          # hex columns [i0, i1) x [j0, j1)
+    sigma: float = 5.8e7
+    eps_r: float = 1.0
+    mu_r: float = 1.0
+    v_top: complex = 1.0
+    v_bottom: complex = 0.0
+    ordering: int = APHI_ORDER_AMD
+    relax_small: int = 0
+    relax_fill: float = 0.0
+
+    def spec(self):
+        s = _capi.CubeSpec()
+        s.n, s.side = self.n, self.side
+        s.bar_i0, s.bar_i1, s.bar_j0, s.bar_j1 = self.bar
+        s.sigma, s.eps_r, s.mu_r = self.sigma, self.eps_r, self.mu_r
+        s.air_sigma, s.air_eps_r, s.air_mu_r = 0.0, 1.0, 1.0
+        vt, vb = complex(self.v_top), complex(self.v_bottom)
+        s.v_top_re, s.v_top_im, s.v_bottom_re, s.v_bottom_im = vt.real, vt.imag, vb.real, vb.imag
+        s.ordering, s.relax_small, s.relax_fill = self.ordering, self.relax_small, self.relax_fill
+        return s
+
+
+class Problem:
+    """Host symbolic problem: mesh, binding, gauge, DOF map, pattern, scatter map,
+    ordering, elimination tree and supernodal plan (all integer, bit-exact)."""
+
+    def __init__(self, config: CubeConfig):
+        self.config = config
+        h = C.c_void_p()
+        spec = config.spec()
+        _check(lib().aphi_problem_create_cube(C.byref(spec), C.byref(h)))
+        self._h = h
+        c = _capi.ProblemCounts()
+        _check(lib().aphi_problem_counts_get(self._h, C.byref(c)))
+        self.counts = {k: getattr(c, k) for k, _ in _capi.ProblemCounts._fields_}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().aphi_problem_destroy(h)
+            self._h = None
+
+    @property
+    def n(self):
+        return int(self.counts["num_total"])
+
+    def ints(self, name):
+        cnt = C.c_int64()
+        _check(lib().aphi_problem_get_int(self._h, name.encode(), None, 0, C.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=np.int64)
+        _check(lib().aphi_problem_get_int(self._h, name.encode(), out.ctypes.data_as(_capi.I64P), cnt.value,
+                                          C.byref(cnt)))
+        return out
+
+    def doubles(self, name):
+        cnt = C.c_int64()
+        _check(lib().aphi_problem_get_double(self._h, name.encode(), None, 0, C.byref(cnt)))
+        out = np.zeros(cnt.value, dtype=np.float64)
+        _check(lib().aphi_problem_get_double(self._h, name.encode(), out.ctypes.data_as(_capi.DP), cnt.value,
+                                             C.byref(cnt)))
+        return out
+
+    def describe(self):
+        d = _capi.MeshDesc()
+        _check(lib().aphi_problem_describe(self._h, C.byref(d)))
+        return d
+
+
+class Context:
+    """One HIP device and one stream (default priority)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _check(lib().aphi_context_create(device, C.byref(h)))
+        self._h = h
+        self.device = device
+
+    def synchronize(self):
+        _check(lib().aphi_context_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().aphi_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def _dev_ptr(t):
+    import torch
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.complex128):
+        raise ValueError("expected a complex128 device tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+class Mesh:
+    """Device-resident per-mesh plan and numeric state (what the reference host
+    binds once per mesh and reuses at every frequency)."""
+
+    def __init__(self, ctx: Context, problem: Problem):
+        self.ctx = ctx
+        self.problem = problem  # keeps the described host arrays alive during upload
+        desc = problem.describe()
+        h = C.c_void_p()
+        _check(lib().aphi_mesh_create(ctx._h, C.byref(desc), C.byref(h)))
+        self._h = h
+        self.n = problem.n
+        self.nnz = int(problem.counts["nnz"])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().aphi_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def device_bytes(self):
+        return int(lib().aphi_mesh_device_bytes(self._h))
+
+    # refill(SymmetricSystem&, ..., omega, conditioning) -- assembly.hpp:197
+    def refill(self, omega, conditioning=APHI_ROW_SCALED):
+        _check(lib().aphi_refill(self._h, float(omega), int(conditioning)))
+
+    def system(self):
+        vals = np.zeros(self.nnz, dtype=np.complex128)
+        rhs = np.zeros(self.n, dtype=np.complex128)
+        _check(lib().aphi_system_download(self._h, vals.ctypes.data_as(_capi.DP), rhs.ctypes.data_as(_capi.DP)))
+        return vals, rhs
+
+    def upload_system(self, values, rhs):
+        v = np.ascontiguousarray(values, dtype=np.complex128)
+        r = np.ascontiguousarray(rhs, dtype=np.complex128)
+        if v.size != self.nnz or r.size != self.n:
+            raise ValueError("upload_system: wrong sizes")
+        _check(lib().aphi_system_upload(self._h, v.ctypes.data_as(_capi.DP), r.ctypes.data_as(_capi.DP)))
+
+    def rhs_device(self):
+        """The assembled right-hand side as a device tensor (a copy)."""
+        import torch
+        vals, rhs = self.system()
+        return torch.from_numpy(rhs).to(f"cuda:{self.ctx.device}")
+
+    def equilibrate(self, iterations=10):
+        d = np.zeros(self.n, dtype=np.float64)
+        _check(lib().aphi_equilibrate(self._h, int(iterations), d.ctypes.data_as(_capi.DP)))
+        return d
+
+    # factorize_ldlt -- factorization.hpp:82
+    def factorize_ldlt(self, precision=APHI_FP64, pivot_floor=0.0):
+        st = _capi.FactorStats()
+        status = lib().aphi_factorize(self._h, int(precision), float(pivot_floor), C.byref(st))
+        if status != _capi.APHI_PIVOT_FAILED:
+            _check(status)
+        return {k: getattr(st, k) for k, _ in _capi.FactorStats._fields_}
+
+    # solve -- factorization.hpp:110 (device tensors, n x nrhs column-major = (nrhs, n) row-major)
+    def solve(self, b, x=None):
+        import torch
+        b2 = b.reshape(-1, self.n)
+        if x is None:
+            x = torch.empty_like(b2)
+        _check(lib().aphi_solve(self._h, b2.shape[0], _dev_ptr(b2), self.n, _dev_ptr(x), self.n))
+        return x.reshape(b.shape)
+
+    # solve_symmetric -- factorization.hpp:206, with FP64 refinement
+    def solve_symmetric(self, b, x=None, precision=APHI_FP64, equilibration_iterations=10, pivot_floor=0.0,
+                        max_refinements=0, tolerance=1e-12):
+        import torch
+        b2 = b.reshape(-1, self.n)
+        if x is None:
+            x = torch.empty_like(b2)
+        opt = _capi.SolveOptions(int(precision), int(equilibration_iterations), float(pivot_floor),
+                                 int(max_refinements), float(tolerance))
+        rep = _capi.SolveReport()
+        be = np.zeros(b2.shape[0], dtype=np.float64)
+        status = lib().aphi_solve_symmetric(self._h, b2.shape[0], _dev_ptr(b2), self.n, _dev_ptr(x), self.n,
+                                            C.byref(opt), C.byref(rep), be.ctypes.data_as(_capi.DP))
+        if status != _capi.APHI_PIVOT_FAILED:
+            _check(status)
+        report = {k: getattr(rep, k) for k, _ in _capi.SolveReport._fields_}
+        report["backward_errors"] = be
+        return x.reshape(b.shape), report
+
+    # backward_error / relative_residual -- factorization.hpp:140, :130
+    def backward_error(self, x, b):
+        x2 = x.reshape(-1, self.n)
+        b2 = b.reshape(-1, self.n)
+        be = np.zeros(b2.shape[0])
+        rr = np.zeros(b2.shape[0])
+        _check(lib().aphi_backward_error(self._h, b2.shape[0], _dev_ptr(x2), self.n, _dev_ptr(b2), self.n,
+                                         be.ctypes.data_as(_capi.DP), rr.ctypes.data_as(_capi.DP)))
+        return be, rr

@@ -1,0 +1,741 @@
+// C-ABI over the device path: context and stream, per-mesh plan upload, refill,
+// equilibration, level-scheduled supernodal factorization, multi-RHS solves and
+// FP64 iterative refinement.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../device/factor.hpp"
+#include "status.hpp"
+
+using aphi_capi::DeviceError;
+using aphi_capi::PivotFailure;
+using aphi_dev::cd;
+using Complex = std::complex<double>;
+
+namespace {
+
+constexpr double kMu0 = 4.0e-7 * 3.14159265358979323846;
+constexpr double kEps0 = 8.8541878128e-12;
+constexpr int kSmallMax = 96;  // fronts up to this many rows are factored in LDS by one workgroup
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw DeviceError(std::string(what) + ": " + hipGetErrorString(e), e == hipErrorOutOfMemory ? 2 : 1);
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void reserve(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        if (b == 0) b = 256;
+        HIPCHK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+template <typename T>
+void upload(DevBuf& b, const T* src, size_t n) {
+    b.reserve(n * sizeof(T));
+    if (n) HIPCHK(hipMemcpy(b.p, src, n * sizeof(T), hipMemcpyHostToDevice));
+}
+
+size_t csize(int precision) { return precision == APHI_FP64 ? sizeof(cd) : sizeof(aphi_dev::cf); }
+
+struct Level {
+    int small_count = 0, small_max_m = 0, large_count = 0, ea_blocks = 0;
+    int64_t off = 0;       // first node of this level in the node list: small ones, then large ones
+    int64_t ea_off = 0;    // this level's extend-add block starts
+    std::vector<int> m, k; // per node (small then large)
+    int count() const { return small_count + large_count; }
+};
+
+}  // namespace
+
+struct aphi_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct aphi_mesh {
+    aphi_context* ctx = nullptr;
+    int n = 0, num_nodes = 0, num_tets = 0, num_bodies = 0, num_super = 0, num_levels = 0, num_values = 0;
+    int64_t nnz = 0, num_targets = 0, rows_total = 0, factor_entries = 0, cb_entries = 0, predicted_nnz = 0;
+    double flops = 0.0;
+    std::vector<double> sigma, eps_r, mu_r;
+    std::vector<Complex> value_tab;
+    std::vector<Level> levels;
+    int max_level_count = 0;
+    int64_t max_bwd_partials = 0;  // count * row blocks over levels, per NB*nrhs
+    // plan on the device
+    DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
+    DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start;
+    // numeric state
+    DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
+    DevBuf P, CB, mws;
+    DevBuf bt, W, X, bpart, colscale, res, norms;
+    int solve_nrhs_cap = 0;
+    bool assembled = false, equilibrated = false, factored = false;
+    int factor_precision = -1;
+    double amax = 0.0, eq_min = 1.0, eq_max = 1.0;
+    aphi_factor_stats last_stats{};
+
+    aphi_dev::FrontPlan plan() const {
+        aphi_dev::FrontPlan fp;
+        fp.first = first.as<int>();
+        fp.row_ptr = srow_ptr.as<int64_t>();
+        fp.rows = srows.as<int>();
+        fp.rel = rel.as<int>();
+        fp.child_ptr = child_ptr.as<int>();
+        fp.child = child.as<int>();
+        fp.panel_off = panel_off.as<int64_t>();
+        fp.cb_off = cb_off.as<int64_t>();
+        fp.col_ref = col_ref.as<int>();
+        return fp;
+    }
+    hipStream_t st() const { return ctx->stream; }
+};
+
+namespace {
+
+unsigned blocks_for(int64_t n, int per = 256) { return static_cast<unsigned>((n + per - 1) / per); }
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw std::invalid_argument(msg);
+}
+
+// ------------------------------------------------------------------ numerics helpers
+
+void compute_amax(aphi_mesh* m) {
+    const int parts = 256;
+    m->partials.reserve(std::max<size_t>(m->partials.bytes, parts * sizeof(double)));
+    hipLaunchKernelGGL(aphi_dev::k_absmax_partial, dim3(parts), dim3(256), 0, m->st(), m->nnz, m->A.as<cd>(),
+                       m->partials.as<double>());
+    std::vector<double> h(parts);
+    HIPCHK(hipMemcpyAsync(h.data(), m->partials.p, parts * sizeof(double), hipMemcpyDeviceToHost, m->st()));
+    HIPCHK(hipStreamSynchronize(m->st()));
+    m->amax = *std::max_element(h.begin(), h.end());
+}
+
+void equilibrate(aphi_mesh* m, int iterations, double* d_out) {
+    require(m->assembled, "equilibrate: the system has not been assembled (call aphi_refill or aphi_system_upload)");
+    require(iterations >= 0, "equilibrate: iterations must be >= 0");
+    hipStream_t s = m->st();
+    const int n = m->n;
+    m->d.reserve(n * sizeof(double));
+    m->Aeq.reserve(m->nnz * sizeof(cd));
+    hipLaunchKernelGGL(aphi_dev::k_fill_d, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<int64_t>(n), m->d.as<double>(), 1.0);
+    if (iterations > 0) {
+        m->work.reserve(m->nnz * sizeof(cd));
+        m->rowmax.reserve(n * sizeof(double));
+        m->fac.reserve(n * sizeof(double));
+        hipLaunchKernelGGL(aphi_dev::k_copy_cd, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->A.as<cd>(), m->work.as<cd>());
+        hipLaunchKernelGGL(aphi_dev::k_fill_d, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<int64_t>(n), m->rowmax.as<double>(), 0.0);
+        for (int it = 0; it < iterations; ++it) {
+            hipLaunchKernelGGL(aphi_dev::k_eq_rowmax, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->row_of.as<int>(),
+                               m->col.as<int>(), m->work.as<cd>(), m->rowmax.as<double>());
+            hipLaunchKernelGGL(aphi_dev::k_eq_factor, dim3(blocks_for(n)), dim3(256), 0, s, n, m->rowmax.as<double>(),
+                               m->fac.as<double>(), m->d.as<double>());
+            hipLaunchKernelGGL(aphi_dev::k_eq_scale, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->row_of.as<int>(),
+                               m->col.as<int>(), m->fac.as<double>(), m->work.as<cd>());
+        }
+        hipLaunchKernelGGL(aphi_dev::k_eq_apply, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->row_of.as<int>(),
+                           m->col.as<int>(), m->d.as<double>(), m->A.as<cd>(), m->Aeq.as<cd>());
+    } else {
+        hipLaunchKernelGGL(aphi_dev::k_copy_cd, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->A.as<cd>(), m->Aeq.as<cd>());
+    }
+    std::vector<double> h(static_cast<size_t>(n));
+    HIPCHK(hipMemcpyAsync(h.data(), m->d.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    m->eq_min = n ? *std::min_element(h.begin(), h.end()) : 1.0;
+    m->eq_max = n ? *std::max_element(h.begin(), h.end()) : 1.0;
+    if (d_out) std::memcpy(d_out, h.data(), h.size() * sizeof(double));
+    m->equilibrated = true;
+    m->factored = false;
+}
+
+template <typename T>
+void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
+    hipStream_t s = m->st();
+    const aphi_dev::FrontPlan fp = m->plan();
+    const size_t cs = sizeof(typename aphi_dev::cplx<T>::type);
+    m->P.reserve(static_cast<size_t>(m->factor_entries) * cs);
+    m->CB.reserve(static_cast<size_t>(std::max<int64_t>(m->cb_entries, 1)) * cs);
+    m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
+    m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
+    aphi_dev::FactorStatsDev init{std::numeric_limits<double>::infinity(), 0.0, 0.0, INT_MAX, 0};
+    HIPCHK(hipMemcpyAsync(m->stats.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    HIPCHK(hipEventRecord(m->ctx->ev0, s));
+    aphi_dev::launch_zero<T>(s, m->factor_entries, m->P.p);
+    aphi_dev::launch_scatter_panel<T>(s, m->nnz, m->s2p.as<int64_t>(), m->Aeq.as<cd>(), m->P.p);
+    const int* nodes = m->nodes.as<int>();
+    auto* st = m->stats.as<aphi_dev::FactorStatsDev>();
+    for (const Level& L : m->levels) {
+        if (L.small_count)
+            aphi_dev::launch_front_small<T>(s, nodes + L.off, L.small_count, L.small_max_m, fp, m->P.p, m->CB.p, pivot_floor, st);
+        if (!L.large_count) continue;
+        const int* lg = nodes + L.off + L.small_count;
+        aphi_dev::launch_extend_add<T>(s, lg, m->ea_start.as<int>() + L.ea_off, L.large_count, L.ea_blocks, fp, m->P.p, m->CB.p);
+        int maxk = 0;
+        for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
+        for (int b = 0; b < maxk; b += aphi_dev::kNB) {
+            int rb = 0, tiles = 0;
+            for (int i = 0; i < L.large_count; ++i) {
+                const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
+                if (kk <= b) continue;
+                const int nb = std::min(aphi_dev::kNB, kk - b);
+                const int rem = mm - b - nb;
+                if (rem <= 0) continue;
+                rb = std::max(rb, (rem + 63) / 64);
+                const int t = (rem + 63) / 64;
+                tiles = std::max(tiles, t * (t + 1) / 2);
+            }
+            aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, tiles, fp, m->P.p, m->CB.p, m->mws.p, pivot_floor, st);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(m->ctx->ev1, s));
+    aphi_dev::FactorStatsDev h{};
+    HIPCHK(hipMemcpyAsync(&h, m->stats.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, m->ctx->ev0, m->ctx->ev1));
+    aphi_factor_stats r{};
+    r.ok = h.fail_col == INT_MAX ? 1 : 0;
+    r.columns_done = r.ok ? m->n : h.fail_col + 1;
+    r.smallest_pivot = m->n ? h.smallest_pivot : 0.0;
+    r.largest_pivot = h.largest_pivot;
+    r.largest_multiplier = h.largest_multiplier;
+    r.nnz = m->predicted_nnz;
+    r.milliseconds = ms;
+    r.flops = m->flops;
+    *out = r;
+    m->last_stats = r;
+    m->factored = r.ok != 0;
+}
+
+void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
+    if (nrhs <= m->solve_nrhs_cap) return;
+    const size_t cs = sizeof(cd);  // sized for the wider type; FP32 uses the front half
+    m->bt.reserve(static_cast<size_t>(m->n) * nrhs * cs);
+    m->X.reserve(static_cast<size_t>(m->n) * nrhs * cs);
+    m->W.reserve(static_cast<size_t>(m->rows_total) * nrhs * cs);
+    m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kNB * nrhs * cs);
+    m->colscale.reserve(nrhs * sizeof(double));
+    m->res.reserve(static_cast<size_t>(m->n) * nrhs * sizeof(cd));
+    m->norms.reserve(static_cast<size_t>(64) * nrhs * 3 * sizeof(double));
+    m->partials.reserve(std::max<size_t>(m->partials.bytes, static_cast<size_t>(64) * nrhs * sizeof(double)));
+    m->solve_nrhs_cap = nrhs;
+}
+
+// x (+)= D * P^T * solve(P * D * b) with the current factor
+template <typename T>
+void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ldx, int accumulate) {
+    hipStream_t s = m->st();
+    const aphi_dev::FrontPlan fp = m->plan();
+    const int n = m->n;
+    const int mode = std::is_same<T, float>::value ? 1 : 0;
+    aphi_dev::launch_column_scale(s, n, nrhs, b, ldb, m->partials.as<double>(), m->colscale.as<double>(), mode);
+    aphi_dev::launch_perm_in<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), b, ldb, m->colscale.as<double>(), m->bt.p);
+    const int* nodes = m->nodes.as<int>();
+    for (const Level& L : m->levels) {
+        const int cnt = L.count();
+        aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, nrhs, fp, m->bt.p, n, m->W.p);
+        int maxk = 0;
+        for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
+        for (int b0 = 0; b0 < maxk; b0 += aphi_dev::kNB) {
+            int rb = 0;
+            for (int i = 0; i < cnt; ++i) {
+                const int kk = L.k[static_cast<size_t>(i)];
+                if (kk <= b0) continue;
+                const int rem = L.m[static_cast<size_t>(i)] - b0 - std::min(aphi_dev::kNB, kk - b0);
+                if (rem > 0) rb = std::max(rb, (rem + 255) / 256);
+            }
+            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->W.p);
+        }
+    }
+    for (int li = m->num_levels - 1; li >= 0; --li) {
+        const Level& L = m->levels[static_cast<size_t>(li)];
+        const int cnt = L.count();
+        int maxk = 0;
+        for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
+        if (maxk == 0) continue;
+        for (int b0 = ((maxk - 1) / aphi_dev::kNB) * aphi_dev::kNB; b0 >= 0; b0 -= aphi_dev::kNB) {
+            int rb = 0;
+            for (int i = 0; i < cnt; ++i) {
+                const int kk = L.k[static_cast<size_t>(i)];
+                if (kk <= b0) continue;
+                const int rem = L.m[static_cast<size_t>(i)] - b0 - std::min(aphi_dev::kNB, kk - b0);
+                if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
+            }
+            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->W.p, m->X.p, n, m->bpart.p);
+        }
+    }
+    aphi_dev::launch_perm_out<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), m->X.p, m->colscale.as<double>(), x, ldx,
+                                 accumulate);
+    HIPCHK(hipGetLastError());
+}
+
+void solve_any(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ldx, int accumulate) {
+    require(m->factored, "solve: there is no usable factor (call aphi_factorize first)");
+    ensure_solve_buffers(m, nrhs);
+    if (m->factor_precision == APHI_FP64) solve_t<double>(m, nrhs, b, ldb, x, ldx, accumulate);
+    else solve_t<float>(m, nrhs, b, ldb, x, ldx, accumulate);
+}
+
+// backward error and relative residual per RHS; leaves r = b - A x in m->res
+void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int64_t ldb, double* be, double* rr) {
+    hipStream_t s = m->st();
+    const int n = m->n;
+    ensure_solve_buffers(m, nrhs);
+    cd* res = m->res.as<cd>();
+    hipLaunchKernelGGL(aphi_dev::k_symv, dim3(blocks_for(n), nrhs), dim3(256), 0, s, n, nrhs, m->row_ptr.as<int>(),
+                       m->col.as<int>(), m->ttptr.as<int>(), m->tslot.as<int>(), m->trow.as<int>(), m->A.as<cd>(), x, ldx,
+                       res, static_cast<int64_t>(n));
+    hipLaunchKernelGGL(aphi_dev::k_residual, dim3(blocks_for(n), nrhs), dim3(256), 0, s, n, nrhs, b, ldb, res,
+                       static_cast<int64_t>(n));
+    const int parts = 64;
+    double* nrm = m->norms.as<double>();
+    hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, res, static_cast<int64_t>(n), nrm);
+    hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, x, ldx, nrm + parts * nrhs);
+    hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, b, ldb, nrm + 2 * parts * nrhs);
+    std::vector<double> h(static_cast<size_t>(3 * parts * nrhs));
+    HIPCHK(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    for (int r = 0; r < nrhs; ++r) {
+        double sr = 0, sx = 0, sb = 0;
+        for (int p = 0; p < parts; ++p) {
+            sr += h[static_cast<size_t>(r * parts + p)];
+            sx += h[static_cast<size_t>(parts * nrhs + r * parts + p)];
+            sb += h[static_cast<size_t>(2 * parts * nrhs + r * parts + p)];
+        }
+        const double num = std::sqrt(sr), den = m->amax * std::sqrt(sx) + std::sqrt(sb);
+        if (be) be[r] = den > 0.0 ? num / den : num;
+        if (rr) rr[r] = sb > 0.0 ? num / std::sqrt(sb) : num;
+    }
+}
+
+void factorize(aphi_mesh* m, int precision, double pivot_floor, aphi_factor_stats* out) {
+    require(m->assembled, "factorize: the system has not been assembled");
+    require(precision == APHI_FP64 || precision == APHI_FP32, "factorize: unknown precision");
+    if (!m->equilibrated) equilibrate(m, 0, nullptr);
+    if (m->factor_precision != precision) {
+        m->P.release();
+        m->CB.release();
+        m->mws.release();
+    }
+    m->factor_precision = precision;
+    if (precision == APHI_FP64) factorize_t<double>(m, pivot_floor, out);
+    else factorize_t<float>(m, pivot_floor, out);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C-ABI
+
+extern "C" int aphi_context_create(int device, aphi_context** out) {
+    return aphi_capi::guarded([&] {
+        require(out != nullptr, "aphi_context_create: null output");
+        *out = nullptr;
+        int count = 0;
+        HIPCHK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count)
+            throw std::invalid_argument("aphi_context_create: no HIP device " + std::to_string(device));
+        HIPCHK(hipSetDevice(device));
+        auto* c = new aphi_context;
+        c->device = device;
+        HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&c->ev0));
+        HIPCHK(hipEventCreate(&c->ev1));
+        *out = c;
+    });
+}
+
+extern "C" void aphi_context_destroy(aphi_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int aphi_context_synchronize(aphi_context* c) {
+    return aphi_capi::guarded([&] {
+        require(c != nullptr, "aphi_context_synchronize: null context");
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+
+extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi_mesh** out) {
+    return aphi_capi::guarded([&] {
+        require(ctx && d && out, "aphi_mesh_create: null argument");
+        *out = nullptr;
+        require(d->n > 0 && d->nnz > 0 && d->num_tets > 0 && d->num_super > 0, "aphi_mesh_create: empty mesh");
+        require(d->num_values >= 0 && d->num_values <= 8, "aphi_mesh_create: at most 8 prescribed values");
+        HIPCHK(hipSetDevice(ctx->device));
+        auto m = std::make_unique<aphi_mesh>();
+        m->ctx = ctx;
+        m->n = d->n;
+        m->nnz = d->nnz;
+        m->num_nodes = d->num_nodes;
+        m->num_tets = d->num_tets;
+        m->num_bodies = d->num_bodies;
+        m->num_super = d->num_super;
+        m->num_levels = d->num_levels;
+        m->num_values = d->num_values;
+        m->num_targets = d->nnz + d->n;
+        m->cb_entries = d->cb_pool_entries;
+        m->flops = d->flops;
+        m->predicted_nnz = d->predicted_nnz;
+        m->sigma.assign(d->body_sigma, d->body_sigma + d->num_bodies);
+        m->eps_r.assign(d->body_eps_r, d->body_eps_r + d->num_bodies);
+        m->mu_r.assign(d->body_mu_r, d->body_mu_r + d->num_bodies);
+        for (int i = 0; i < d->num_values; ++i) m->value_tab.emplace_back(d->values[2 * i], d->values[2 * i + 1]);
+        const int n = d->n, ns = d->num_super;
+        const int64_t nc = d->target_ptr[m->num_targets];
+        for (int t = 0; t < d->num_tets; ++t)
+            if (d->tet_body[t] < 0 || d->tet_body[t] >= d->num_bodies)
+                throw std::invalid_argument("aphi_mesh_create: a tet's body index is out of range");
+
+        upload(m->coords, d->coords, static_cast<size_t>(d->num_nodes) * 3);
+        upload(m->tets, d->tet_nodes, static_cast<size_t>(d->num_tets) * 4);
+        upload(m->tet_body, d->tet_body, static_cast<size_t>(d->num_tets));
+        upload(m->tptr, d->target_ptr, static_cast<size_t>(m->num_targets) + 1);
+        upload(m->entry, d->entry, static_cast<size_t>(nc));
+        upload(m->kind, d->kind, static_cast<size_t>(nc));
+        for (int64_t c = 0; c < nc; ++c) {
+            const int kk = d->kind[c] & 0x0f;
+            if (kk <= 5 && d->entry[c] >= static_cast<uint32_t>(d->num_tets) * 196u)
+                throw std::invalid_argument("aphi_mesh_create: an assembly contribution names a tet outside the mesh");
+            if ((d->kind[c] >> 5) >= std::max(d->num_values, 1) && (kk == 4 || kk == 5 || kk == 7 || kk == 8))
+                throw std::invalid_argument("aphi_mesh_create: a contribution names an unknown prescribed value");
+        }
+        upload(m->row_ptr, d->row_ptr, static_cast<size_t>(n) + 1);
+        upload(m->col, d->col_index, static_cast<size_t>(d->nnz));
+        {
+            std::vector<int> row_of(static_cast<size_t>(d->nnz));
+            std::vector<int> tp(static_cast<size_t>(n) + 1, 0);
+            for (int r = 0; r < n; ++r)
+                for (int k = d->row_ptr[r]; k < d->row_ptr[r + 1]; ++k) {
+                    row_of[static_cast<size_t>(k)] = r;
+                    const int c = d->col_index[k];
+                    if (c < r || c >= n) throw std::invalid_argument("aphi_mesh_create: the pattern is not upper triangular");
+                    if (c != r) ++tp[static_cast<size_t>(c) + 1];
+                }
+            for (int i = 0; i < n; ++i) tp[static_cast<size_t>(i) + 1] += tp[static_cast<size_t>(i)];
+            std::vector<int> ts(static_cast<size_t>(tp[static_cast<size_t>(n)])), tr(ts.size());
+            std::vector<int> cur(tp.begin(), tp.end() - 1);
+            for (int r = 0; r < n; ++r)
+                for (int k = d->row_ptr[r]; k < d->row_ptr[r + 1]; ++k) {
+                    const int c = d->col_index[k];
+                    if (c == r) continue;
+                    ts[static_cast<size_t>(cur[static_cast<size_t>(c)])] = k;
+                    tr[static_cast<size_t>(cur[static_cast<size_t>(c)]++)] = r;
+                }
+            upload(m->row_of, row_of.data(), row_of.size());
+            upload(m->ttptr, tp.data(), tp.size());
+            upload(m->tslot, ts.data(), ts.size());
+            upload(m->trow, tr.data(), tr.size());
+        }
+        upload(m->perm, d->factor_perm, static_cast<size_t>(n));
+        upload(m->col_ref, d->col_ref, static_cast<size_t>(n));
+        upload(m->first, d->super_first, static_cast<size_t>(ns) + 1);
+        upload(m->srow_ptr, d->super_row_ptr, static_cast<size_t>(ns) + 1);
+        m->rows_total = d->super_row_ptr[ns];
+        upload(m->srows, d->super_rows, static_cast<size_t>(m->rows_total));
+        upload(m->rel, d->super_rel, static_cast<size_t>(m->rows_total));
+        upload(m->panel_off, d->panel_offset, static_cast<size_t>(ns) + 1);
+        m->factor_entries = d->panel_offset[ns];
+        upload(m->cb_off, d->cb_offset, static_cast<size_t>(ns));
+        upload(m->s2p, d->slot_to_panel, static_cast<size_t>(d->nnz));
+        for (int64_t k = 0; k < d->nnz; ++k)
+            if (d->slot_to_panel[k] < 0 || d->slot_to_panel[k] >= m->factor_entries)
+                throw std::invalid_argument("aphi_mesh_create: slot_to_panel points outside the factor");
+
+        // children (ascending), levels (small fronts first, then large ones)
+        std::vector<int> cptr(static_cast<size_t>(ns) + 1, 0);
+        for (int s = 0; s < ns; ++s) {
+            const int p = d->super_parent[s];
+            if (p >= 0) {
+                if (p <= s || p >= ns) throw std::invalid_argument("aphi_mesh_create: supernode parents must follow their children");
+                ++cptr[static_cast<size_t>(p) + 1];
+            }
+        }
+        for (int s = 0; s < ns; ++s) cptr[static_cast<size_t>(s) + 1] += cptr[static_cast<size_t>(s)];
+        std::vector<int> ch(static_cast<size_t>(cptr[static_cast<size_t>(ns)]));
+        {
+            std::vector<int> cur(cptr.begin(), cptr.end() - 1);
+            for (int s = 0; s < ns; ++s)
+                if (d->super_parent[s] >= 0) ch[static_cast<size_t>(cur[static_cast<size_t>(d->super_parent[s])]++)] = s;
+        }
+        upload(m->child_ptr, cptr.data(), cptr.size());
+        upload(m->child, ch.data(), ch.size());
+        m->levels.assign(static_cast<size_t>(d->num_levels), Level{});
+        std::vector<std::vector<int>> small(static_cast<size_t>(d->num_levels)), large(static_cast<size_t>(d->num_levels));
+        for (int s = 0; s < ns; ++s) {
+            const int L = d->super_level[s];
+            if (L < 0 || L >= d->num_levels) throw std::invalid_argument("aphi_mesh_create: supernode level out of range");
+            const int64_t mm = d->super_row_ptr[s + 1] - d->super_row_ptr[s];
+            (mm <= kSmallMax ? small : large)[static_cast<size_t>(L)].push_back(s);
+        }
+        std::vector<int> node_list, ea;
+        for (int L = 0; L < d->num_levels; ++L) {
+            Level& lv = m->levels[static_cast<size_t>(L)];
+            lv.off = static_cast<int64_t>(node_list.size());
+            lv.ea_off = static_cast<int64_t>(ea.size());
+            lv.small_count = static_cast<int>(small[static_cast<size_t>(L)].size());
+            lv.large_count = static_cast<int>(large[static_cast<size_t>(L)].size());
+            int blk = 0, maxrb = 0;
+            for (int pass = 0; pass < 2; ++pass)
+                for (int s : (pass == 0 ? small : large)[static_cast<size_t>(L)]) {
+                    node_list.push_back(s);
+                    const int mm = static_cast<int>(d->super_row_ptr[s + 1] - d->super_row_ptr[s]);
+                    const int kk = d->super_first[s + 1] - d->super_first[s];
+                    lv.m.push_back(mm);
+                    lv.k.push_back(kk);
+                    if (pass == 0) lv.small_max_m = std::max(lv.small_max_m, mm);
+                    else {
+                        ea.push_back(blk);
+                        blk += (mm + 63) / 64;
+                    }
+                    maxrb = std::max(maxrb, (mm - std::min(kk, aphi_dev::kNB) + 63) / 64);
+                }
+            lv.ea_blocks = blk;
+            ea.push_back(blk);
+            m->max_level_count = std::max(m->max_level_count, lv.count());
+            m->max_bwd_partials = std::max<int64_t>(m->max_bwd_partials, static_cast<int64_t>(lv.count()) * std::max(maxrb, 1));
+        }
+        upload(m->nodes, node_list.data(), node_list.size());
+        upload(m->ea_start, ea.data(), ea.size());
+        m->flag.reserve(sizeof(int));
+        *out = m.release();
+    });
+}
+
+extern "C" void aphi_mesh_destroy(aphi_mesh* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    delete m;
+}
+
+extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
+    if (!m) return 0;
+    const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
+                           &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
+                           &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
+                           &m->ea_start, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
+                           &m->bpart, &m->colscale, &m->res, &m->norms};
+    int64_t total = 0;
+    for (const DevBuf* b : all) total += static_cast<int64_t>(b->bytes);
+    return total;
+}
+
+extern "C" int aphi_refill(aphi_mesh* m, double omega, int conditioning) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_refill: null mesh");
+        HIPCHK(hipSetDevice(m->ctx->device));
+        // formulation_scales + the symmetric guard of src/assembly.cpp:12-45, 484-491
+        const Complex jw(0.0, omega);
+        Complex row(1.0, 0.0), column(1.0, 0.0);
+        if (conditioning == APHI_NATURAL) {
+            throw std::invalid_argument(
+                "refill: conditioning = natural does not give a symmetric matrix, so only half of it cannot be "
+                "stored. Use row_scaled or scaled_phi for symmetric storage.");
+        } else if (conditioning == APHI_ROW_SCALED) {
+            if (omega == 0.0)
+                throw std::invalid_argument("formulation_scales: the row-scaled formulation divides the Phi rows by "
+                                            "j*omega, which is undefined at DC.");
+            row = Complex(1.0, 0.0) / jw;
+        } else if (conditioning == APHI_SCALED_PHI) {
+            if (omega == 0.0)
+                throw std::invalid_argument("formulation_scales: the scaled-potential formulation substitutes "
+                                            "Phi = j*omega*Phi', which is undefined at DC.");
+            column = jw;
+        } else {
+            throw std::invalid_argument("formulation_scales: unknown formulation");
+        }
+        aphi_dev::FreqScales sc{};
+        const auto put = [](cd& dst, const Complex& v) { dst = cd{v.real(), v.imag()}; };
+        put(sc.column, column);
+        put(sc.row_jw, row * jw);
+        put(sc.row_column, row * column);
+        for (int i = 0; i < m->num_values; ++i) {
+            const Complex v = m->value_tab[static_cast<size_t>(i)];
+            put(sc.over_column[i], v / column);
+            put(sc.times_row[i], row * v);
+        }
+        // element_coefficients, src/element_matrix.cpp:10-25
+        std::vector<aphi_dev::BodyCoef> coef(static_cast<size_t>(m->num_bodies));
+        for (int b = 0; b < m->num_bodies; ++b) {
+            const double mu = m->mu_r[static_cast<size_t>(b)] * kMu0;
+            const double eps = m->eps_r[static_cast<size_t>(b)] * kEps0;
+            coef[static_cast<size_t>(b)].nu = 1.0 / mu;
+            coef[static_cast<size_t>(b)].alpha = cd{-omega * omega * eps, omega * m->sigma[static_cast<size_t>(b)]};
+            coef[static_cast<size_t>(b)].beta = cd{m->sigma[static_cast<size_t>(b)], omega * eps};
+        }
+        hipStream_t s = m->st();
+        m->coef.reserve(coef.size() * sizeof(aphi_dev::BodyCoef));
+        HIPCHK(hipMemcpyAsync(m->coef.p, coef.data(), coef.size() * sizeof(aphi_dev::BodyCoef), hipMemcpyHostToDevice, s));
+        m->E.reserve(static_cast<size_t>(m->num_tets) * 196 * sizeof(cd));
+        m->A.reserve(static_cast<size_t>(m->nnz) * sizeof(cd));
+        m->rhs.reserve(static_cast<size_t>(m->n) * sizeof(cd));
+        HIPCHK(hipMemsetAsync(m->flag.p, 0, sizeof(int), s));
+        hipLaunchKernelGGL(aphi_dev::k_element, dim3(blocks_for(m->num_tets, 16)), dim3(256), 0, s, m->num_tets,
+                           m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(),
+                           m->coef.as<aphi_dev::BodyCoef>(), m->E.as<cd>(), m->flag.as<int>());
+        hipLaunchKernelGGL(aphi_dev::k_assemble_reduce, dim3(blocks_for(m->num_targets)), dim3(256), 0, s, m->num_targets,
+                           static_cast<int>(m->nnz), m->tptr.as<int64_t>(), m->entry.as<uint32_t>(), m->kind.as<uint8_t>(),
+                           m->E.as<cd>(), sc, m->A.as<cd>(), m->rhs.as<cd>());
+        int flag = 0;
+        HIPCHK(hipMemcpyAsync(&flag, m->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipGetLastError());
+        if (flag) throw std::runtime_error("compute_tet_geometry: degenerate (zero-volume) tet");
+        m->assembled = true;
+        m->equilibrated = false;
+        m->factored = false;
+        compute_amax(m);
+    });
+}
+
+extern "C" int aphi_system_download(aphi_mesh* m, double* values, double* rhs) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_system_download: null mesh");
+        require(m->assembled, "aphi_system_download: nothing assembled");
+        if (values) HIPCHK(hipMemcpyAsync(values, m->A.p, static_cast<size_t>(m->nnz) * sizeof(cd), hipMemcpyDeviceToHost, m->st()));
+        if (rhs) HIPCHK(hipMemcpyAsync(rhs, m->rhs.p, static_cast<size_t>(m->n) * sizeof(cd), hipMemcpyDeviceToHost, m->st()));
+        HIPCHK(hipStreamSynchronize(m->st()));
+    });
+}
+
+extern "C" int aphi_system_upload(aphi_mesh* m, const double* values, const double* rhs) {
+    return aphi_capi::guarded([&] {
+        require(m && values && rhs, "aphi_system_upload: null argument");
+        m->A.reserve(static_cast<size_t>(m->nnz) * sizeof(cd));
+        m->rhs.reserve(static_cast<size_t>(m->n) * sizeof(cd));
+        HIPCHK(hipMemcpyAsync(m->A.p, values, static_cast<size_t>(m->nnz) * sizeof(cd), hipMemcpyHostToDevice, m->st()));
+        HIPCHK(hipMemcpyAsync(m->rhs.p, rhs, static_cast<size_t>(m->n) * sizeof(cd), hipMemcpyHostToDevice, m->st()));
+        HIPCHK(hipStreamSynchronize(m->st()));
+        m->assembled = true;
+        m->equilibrated = false;
+        m->factored = false;
+        compute_amax(m);
+    });
+}
+
+extern "C" const void* aphi_rhs_device(const aphi_mesh* m) { return m ? m->rhs.p : nullptr; }
+
+extern "C" int aphi_equilibrate(aphi_mesh* m, int iterations, double* d_out) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_equilibrate: null mesh");
+        equilibrate(m, iterations, d_out);
+    });
+}
+
+extern "C" int aphi_factorize(aphi_mesh* m, int precision, double pivot_floor, aphi_factor_stats* stats) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_factorize: null mesh");
+        aphi_factor_stats st{};
+        factorize(m, precision, pivot_floor, &st);
+        if (stats) *stats = st;
+        if (!st.ok) throw PivotFailure("factorize_ldlt: a pivot at or below the floor stopped the factorization");
+    });
+}
+
+extern "C" int aphi_solve(aphi_mesh* m, int nrhs, const void* b, int64_t ldb, void* x, int64_t ldx) {
+    return aphi_capi::guarded([&] {
+        require(m && b && x, "aphi_solve: null argument");
+        require(nrhs >= 1 && ldb >= m->n && ldx >= m->n, "aphi_solve: bad nrhs or leading dimension");
+        solve_any(m, nrhs, static_cast<const cd*>(b), ldb, static_cast<cd*>(x), ldx, 0);
+        HIPCHK(hipStreamSynchronize(m->st()));
+    });
+}
+
+extern "C" int aphi_backward_error(aphi_mesh* m, int nrhs, const void* x, int64_t ldx, const void* b, int64_t ldb,
+                                   double* be, double* rr) {
+    return aphi_capi::guarded([&] {
+        require(m && x && b, "aphi_backward_error: null argument");
+        require(m->assembled, "aphi_backward_error: nothing assembled");
+        require(nrhs >= 1 && ldb >= m->n && ldx >= m->n, "aphi_backward_error: bad nrhs or leading dimension");
+        measure(m, nrhs, static_cast<const cd*>(x), ldx, static_cast<const cd*>(b), ldb, be, rr);
+    });
+}
+
+extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int64_t ldb, void* xv, int64_t ldx,
+                                    const aphi_solve_options* opt, aphi_solve_report* rep, double* backward_errors) {
+    return aphi_capi::guarded([&] {
+        require(m && bv && xv && opt && rep, "aphi_solve_symmetric: null argument");
+        require(nrhs >= 1 && ldb >= m->n && ldx >= m->n, "aphi_solve_symmetric: bad nrhs or leading dimension");
+        require(opt->max_refinements >= 0, "aphi_solve_symmetric: max_refinements must be >= 0");
+        const auto t0 = std::chrono::steady_clock::now();
+        const cd* b = static_cast<const cd*>(bv);
+        cd* x = static_cast<cd*>(xv);
+        aphi_solve_report r{};
+        equilibrate(m, opt->equilibration_iterations, nullptr);
+        r.equilibration_min = m->eq_min;
+        r.equilibration_max = m->eq_max;
+        aphi_factor_stats fs{};
+        factorize(m, opt->precision, opt->pivot_floor, &fs);
+        r.smallest_pivot = fs.smallest_pivot;
+        r.largest_pivot = fs.largest_pivot;
+        r.largest_multiplier = fs.largest_multiplier;
+        r.factor_nnz = fs.nnz;
+        r.columns_done = fs.columns_done;
+        r.factorize_ms = fs.milliseconds;
+        if (!fs.ok) {
+            *rep = r;
+            throw PivotFailure("solve_symmetric: the factorization hit a pivot at or below the floor");
+        }
+        const auto s0 = std::chrono::steady_clock::now();
+        solve_any(m, nrhs, b, ldb, x, ldx, 0);
+        std::vector<double> be(static_cast<size_t>(nrhs)), rr(static_cast<size_t>(nrhs));
+        int steps = 0;
+        for (;;) {
+            measure(m, nrhs, x, ldx, b, ldb, be.data(), rr.data());
+            if (opt->max_refinements == 0) break;
+            const double worst = *std::max_element(be.begin(), be.end());
+            if (worst <= opt->tolerance || steps >= opt->max_refinements) break;
+            solve_any(m, nrhs, m->res.as<cd>(), m->n, x, ldx, 1);
+            ++steps;
+        }
+        const auto s1 = std::chrono::steady_clock::now();
+        r.refinement_steps = steps;
+        r.backward_error = *std::max_element(be.begin(), be.end());
+        r.residual = *std::max_element(rr.begin(), rr.end());
+        r.solve_ms = std::chrono::duration<double, std::milli>(s1 - s0).count();
+        r.total_ms = std::chrono::duration<double, std::milli>(s1 - t0).count();
+        r.ok = 1;
+        if (backward_errors) std::memcpy(backward_errors, be.data(), be.size() * sizeof(double));
+        *rep = r;
+    });
+}

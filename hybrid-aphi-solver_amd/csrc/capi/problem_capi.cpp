@@ -1,0 +1,288 @@
+// C-ABI over the host symbolic problem: build a cube configuration end to end
+// (mesh -> binding -> DOF map -> pattern -> scatter map -> assembly plan ->
+// ordering -> elimination tree -> supernodes) and expose its integer arrays.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../host/aphi_host.hpp"
+#include "status.hpp"
+
+using namespace aphi_amd;
+
+struct aphi_problem {
+    CubeSpec spec;
+    Mesh mesh;
+    BoundProblem bound;
+    DofMap dofs;
+    SparsityPattern pattern;
+    ScatterMap map;
+    AssemblyPlan plan;
+    SolverAnalysis analysis;
+    SupernodePlan sp;
+    std::vector<int64_t> s2p;
+    // flat copies handed out by aphi_problem_describe
+    std::vector<double> coords;
+    std::vector<int32_t> tet_nodes, tet_body, col_ref, super_level;
+    std::vector<double> body_sigma, body_eps_r, body_mu_r, values;
+    double analyze_ms = 0.0, build_ms = 0.0;
+};
+
+namespace aphi_capi {
+namespace {
+thread_local std::string g_error;
+}
+void set_error(const std::string& m) { g_error = m; }
+void clear_error() { g_error.clear(); }
+}  // namespace aphi_capi
+
+extern "C" const char* aphi_last_error(void) { return aphi_capi::g_error.c_str(); }
+extern "C" const char* aphi_version(void) { return "aphi-mi355x 0.1"; }
+
+extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem** out) {
+    return aphi_capi::guarded([&] {
+        if (!spec || !out) throw std::invalid_argument("aphi_problem_create_cube: null argument");
+        *out = nullptr;
+        auto p = std::make_unique<aphi_problem>();
+        const auto t0 = std::chrono::steady_clock::now();
+        CubeSpec& s = p->spec;
+        s.n = spec->n;
+        s.side = spec->side;
+        s.bar_i0 = spec->bar_i0;
+        s.bar_i1 = spec->bar_i1;
+        s.bar_j0 = spec->bar_j0;
+        s.bar_j1 = spec->bar_j1;
+        s.sigma = spec->sigma;
+        s.eps_r = spec->eps_r;
+        s.mu_r = spec->mu_r;
+        s.air_sigma = spec->air_sigma;
+        s.air_eps_r = spec->air_eps_r;
+        s.air_mu_r = spec->air_mu_r;
+        s.v_top = Complex(spec->v_top_re, spec->v_top_im);
+        s.v_bottom = Complex(spec->v_bottom_re, spec->v_bottom_im);
+        if (!(s.side > 0.0)) throw std::invalid_argument("aphi_problem_create_cube: side must be positive");
+        if (spec->ordering < 0 || spec->ordering > 2) throw std::invalid_argument("aphi_problem_create_cube: unknown ordering");
+        p->mesh = make_kuhn_cube(s);
+        p->bound = bind_cube(p->mesh, s);
+        p->dofs = build_dof_map(p->bound, p->mesh);
+        p->pattern = build_sparsity(p->dofs, p->bound, p->mesh, true);
+        p->map = build_scatter_map(p->pattern, p->dofs, p->mesh, p->bound);
+        p->plan = build_assembly_plan(p->pattern, p->map, p->dofs, p->mesh, p->bound);
+        const auto t1 = std::chrono::steady_clock::now();
+        p->analysis = analyze(p->pattern, static_cast<Ordering>(spec->ordering));
+        const auto t2 = std::chrono::steady_clock::now();
+        SupernodeOptions opt;
+        if (spec->relax_small > 0) opt.relax_small = spec->relax_small;
+        if (spec->relax_fill > 0.0) opt.relax_fill = spec->relax_fill;
+        p->sp = build_supernodes(p->analysis, opt);
+        p->s2p = slot_to_panel(p->pattern, p->analysis, p->sp);
+        const auto t3 = std::chrono::steady_clock::now();
+        p->analyze_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        p->build_ms = std::chrono::duration<double, std::milli>((t1 - t0) + (t3 - t2)).count();
+
+        const Mesh& m = p->mesh;
+        p->coords.resize(static_cast<size_t>(m.num_nodes()) * 3);
+        for (int i = 0; i < m.num_nodes(); ++i) {
+            p->coords[3 * static_cast<size_t>(i)] = m.nodes[static_cast<size_t>(i)].x;
+            p->coords[3 * static_cast<size_t>(i) + 1] = m.nodes[static_cast<size_t>(i)].y;
+            p->coords[3 * static_cast<size_t>(i) + 2] = m.nodes[static_cast<size_t>(i)].z;
+        }
+        p->tet_nodes.resize(static_cast<size_t>(m.num_tets()) * 4);
+        for (int t = 0; t < m.num_tets(); ++t)
+            for (int c = 0; c < 4; ++c) p->tet_nodes[4 * static_cast<size_t>(t) + c] = m.tets[static_cast<size_t>(t)][static_cast<size_t>(c)];
+        p->tet_body.assign(p->plan.tet_body.begin(), p->plan.tet_body.end());
+        for (const Body& b : p->bound.bodies) {
+            p->body_sigma.push_back(b.sigma);
+            p->body_eps_r.push_back(b.eps_r);
+            p->body_mu_r.push_back(b.mu_r);
+        }
+        for (const Complex& v : p->plan.fixed_value) {
+            p->values.push_back(v.real());
+            p->values.push_back(v.imag());
+        }
+        const int n = p->sp.n;
+        p->col_ref.resize(static_cast<size_t>(n));
+        for (int j = 0; j < n; ++j)
+            p->col_ref[static_cast<size_t>(j)] = p->analysis.permutation.iperm[static_cast<size_t>(p->sp.perm[static_cast<size_t>(j)])];
+        p->super_level.assign(p->sp.level.begin(), p->sp.level.end());
+        *out = p.release();
+    });
+}
+
+extern "C" void aphi_problem_destroy(aphi_problem* p) { delete p; }
+
+extern "C" int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out) {
+    return aphi_capi::guarded([&] {
+        if (!p || !out) throw std::invalid_argument("aphi_problem_counts_get: null argument");
+        aphi_problem_counts c{};
+        c.nodes = p->mesh.num_nodes();
+        c.edges = p->mesh.num_edges();
+        c.faces = p->mesh.num_faces();
+        c.tets = p->mesh.num_tets();
+        c.num_a = p->dofs.num_a;
+        c.num_phi = p->dofs.num_phi;
+        c.num_ports = p->dofs.num_ports;
+        c.num_total = p->dofs.num_total;
+        c.nnz = static_cast<int64_t>(p->pattern.nnz());
+        c.predicted_nnz = static_cast<int64_t>(p->analysis.predicted_nnz);
+        c.num_roots = p->analysis.num_roots;
+        for (EdgeDof e : p->dofs.edge_state) {
+            if (e == EdgeDof::Tree) ++c.tree_edges;
+            else if (e == EdgeDof::Dirichlet) ++c.dirichlet_edges;
+            else ++c.free_edges;
+        }
+        c.num_super = p->sp.num_super;
+        c.num_levels = p->sp.num_levels;
+        c.factor_entries = static_cast<int64_t>(p->sp.factor_entries);
+        c.cb_pool_entries = p->sp.cb_pool_entries;
+        c.contributions = static_cast<int64_t>(p->plan.entry.size());
+        c.flops = p->sp.flops;
+        c.analyze_ms = p->analyze_ms;
+        c.build_ms = p->build_ms;
+        *out = c;
+    });
+}
+
+namespace {
+
+template <typename V>
+void copy_out(const V& v, int64_t* out, int64_t capacity, int64_t* count) {
+    *count = static_cast<int64_t>(v.size());
+    const int64_t n = std::min<int64_t>(capacity, *count);
+    for (int64_t i = 0; i < n; ++i) out[i] = static_cast<int64_t>(v[static_cast<size_t>(i)]);
+}
+
+}  // namespace
+
+extern "C" int aphi_problem_get_int(const aphi_problem* p, const char* name, int64_t* out, int64_t capacity,
+                                    int64_t* count) {
+    return aphi_capi::guarded([&] {
+        if (!p || !name || !count || (capacity > 0 && !out)) throw std::invalid_argument("aphi_problem_get_int: null argument");
+        const std::string k(name);
+        const Mesh& m = p->mesh;
+        std::vector<int64_t> tmp;
+        if (k == "tet_nodes") return copy_out(p->tet_nodes, out, capacity, count);
+        if (k == "edges") {
+            for (const auto& e : m.edges) { tmp.push_back(e.first); tmp.push_back(e.second); }
+        } else if (k == "tet_edges") {
+            for (const auto& te : m.tet_edges) for (int e : te) tmp.push_back(e);
+        } else if (k == "tet_edge_signs") {
+            for (const auto& ts : m.tet_edge_signs) for (int8_t s : ts) tmp.push_back(s);
+        } else if (k == "tet_body") {
+            return copy_out(p->tet_body, out, capacity, count);
+        } else if (k == "dirichlet_edge") {
+            return copy_out(p->bound.dirichlet_edge, out, capacity, count);
+        } else if (k == "tree_edge") {
+            return copy_out(p->bound.gauge.is_tree_edge, out, capacity, count);
+        } else if (k == "edge_index") {
+            return copy_out(p->dofs.edge_index, out, capacity, count);
+        } else if (k == "phi_index") {
+            return copy_out(p->dofs.phi_index, out, capacity, count);
+        } else if (k == "port_index") {
+            return copy_out(p->dofs.port_index, out, capacity, count);
+        } else if (k == "row_ptr") {
+            return copy_out(p->pattern.row_ptr, out, capacity, count);
+        } else if (k == "col_index") {
+            return copy_out(p->pattern.col_index, out, capacity, count);
+        } else if (k == "perm") {
+            return copy_out(p->analysis.permutation.perm, out, capacity, count);
+        } else if (k == "iperm") {
+            return copy_out(p->analysis.permutation.iperm, out, capacity, count);
+        } else if (k == "parent") {
+            return copy_out(p->analysis.parent, out, capacity, count);
+        } else if (k == "row_count") {
+            return copy_out(p->analysis.row_count, out, capacity, count);
+        } else if (k == "col_count") {
+            return copy_out(p->analysis.col_count, out, capacity, count);
+        } else if (k == "scatter_live") {
+            return copy_out(p->map.live, out, capacity, count);
+        } else if (k == "scatter_pos") {
+            return copy_out(p->map.pos, out, capacity, count);
+        } else if (k == "scatter_offset") {
+            return copy_out(p->map.offset, out, capacity, count);
+        } else if (k == "scatter_slot") {
+            return copy_out(p->map.slot, out, capacity, count);
+        } else if (k == "super_first") {
+            return copy_out(p->sp.first, out, capacity, count);
+        } else if (k == "super_parent") {
+            return copy_out(p->sp.sparent, out, capacity, count);
+        } else if (k == "super_row_ptr") {
+            return copy_out(p->sp.row_ptr, out, capacity, count);
+        } else if (k == "super_rows") {
+            return copy_out(p->sp.rows, out, capacity, count);
+        } else if (k == "super_level") {
+            return copy_out(p->sp.level, out, capacity, count);
+        } else if (k == "factor_perm") {
+            return copy_out(p->sp.perm, out, capacity, count);
+        } else if (k == "slot_to_panel") {
+            return copy_out(p->s2p, out, capacity, count);
+        } else {
+            throw std::invalid_argument("aphi_problem_get_int: unknown array '" + k + "'");
+        }
+        copy_out(tmp, out, capacity, count);
+    });
+}
+
+extern "C" int aphi_problem_get_double(const aphi_problem* p, const char* name, double* out, int64_t capacity,
+                                       int64_t* count) {
+    return aphi_capi::guarded([&] {
+        if (!p || !name || !count || (capacity > 0 && !out)) throw std::invalid_argument("aphi_problem_get_double: null argument");
+        const std::string k(name);
+        if (k != "coords") throw std::invalid_argument("aphi_problem_get_double: unknown array '" + k + "'");
+        *count = static_cast<int64_t>(p->coords.size());
+        const int64_t n = std::min<int64_t>(capacity, *count);
+        std::memcpy(out, p->coords.data(), static_cast<size_t>(n) * sizeof(double));
+    });
+}
+
+extern "C" int aphi_frequency_sweep(double start, double stop, int points, int logarithmic, double* out) {
+    return aphi_capi::guarded([&] {
+        if (!out) throw std::invalid_argument("aphi_frequency_sweep: null output");
+        const std::vector<double> f = expand_frequency_sweep(start, stop, points, logarithmic != 0);
+        std::memcpy(out, f.data(), f.size() * sizeof(double));
+    });
+}
+
+extern "C" int aphi_problem_describe(const aphi_problem* p, aphi_mesh_desc* d) {
+    return aphi_capi::guarded([&] {
+        if (!p || !d) throw std::invalid_argument("aphi_problem_describe: null argument");
+        *d = aphi_mesh_desc{};
+        d->num_nodes = p->mesh.num_nodes();
+        d->num_tets = p->mesh.num_tets();
+        d->num_bodies = static_cast<int32_t>(p->bound.bodies.size());
+        d->n = p->dofs.num_total;
+        d->nnz = static_cast<int64_t>(p->pattern.nnz());
+        d->coords = p->coords.data();
+        d->tet_nodes = p->tet_nodes.data();
+        d->tet_body = p->tet_body.data();
+        d->body_sigma = p->body_sigma.data();
+        d->body_eps_r = p->body_eps_r.data();
+        d->body_mu_r = p->body_mu_r.data();
+        d->target_ptr = p->plan.target_ptr.data();
+        d->entry = p->plan.entry.data();
+        d->kind = p->plan.kind.data();
+        d->num_values = static_cast<int32_t>(p->plan.fixed_value.size());
+        d->values = p->values.data();
+        d->row_ptr = p->pattern.row_ptr.data();
+        d->col_index = p->pattern.col_index.data();
+        d->factor_perm = p->sp.perm.data();
+        d->col_ref = p->col_ref.data();
+        d->num_super = p->sp.num_super;
+        d->num_levels = p->sp.num_levels;
+        d->super_first = p->sp.first.data();
+        d->super_parent = p->sp.sparent.data();
+        d->super_row_ptr = p->sp.row_ptr.data();
+        d->super_rows = p->sp.rows.data();
+        d->super_rel = p->sp.rel.data();
+        d->super_level = p->super_level.data();
+        d->panel_offset = p->sp.panel_offset.data();
+        d->cb_offset = p->sp.cb_offset.data();
+        d->cb_pool_entries = p->sp.cb_pool_entries;
+        d->slot_to_panel = p->s2p.data();
+        d->flops = p->sp.flops;
+        d->predicted_nnz = static_cast<int64_t>(p->analysis.predicted_nnz);
+    });
+}

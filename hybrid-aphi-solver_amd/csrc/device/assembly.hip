@@ -1,0 +1,409 @@
+// Device refill (element kernels + deterministic slot reduction), symmetric Ruiz
+// equilibration, and the FP64 operator kernels the solve needs (A*x residual,
+// norms, permutation with scaling).
+//
+// Floating-point contraction is disabled in this file: the element entries and
+// their reduction reproduce the reference's rounding step for step
+// (src/basis_functions.cpp:15-139, src/element_matrix.cpp:27-117,
+// src/assembly.cpp:255-422), so a device refill can be compared bit for bit.
+#pragma clang fp contract(off)
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace aphi_dev {
+
+namespace {
+
+__constant__ int kEV[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {2, 3}, {1, 3}};
+
+constexpr int kTetsPerBlock = 16;
+constexpr int kLanesPerTet = 16;
+
+// Gauss-Jordan with partial pivoting on [1 x y z] rows (src/basis_functions.cpp:15-57),
+// fully unrolled so everything stays in registers. Returns false for a pivot
+// below 1e-14 (a degenerate tet).
+__device__ bool invert4(double m[4][4], double inv[4][4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) inv[i][j] = (i == j) ? 1.0 : 0.0;
+#pragma unroll
+    for (int col = 0; col < 4; ++col) {
+        int prow = col;
+        double pval = fabs(m[col][col]);
+#pragma unroll
+        for (int row = col + 1; row < 4; ++row) {
+            if (fabs(m[row][col]) > pval) {
+                prow = row;
+                pval = fabs(m[row][col]);
+            }
+        }
+        if (pval < 1e-14) return false;
+#pragma unroll
+        for (int row = col + 1; row < 4; ++row) {
+            if (row == prow) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    double t = m[row][k]; m[row][k] = m[col][k]; m[col][k] = t;
+                    t = inv[row][k]; inv[row][k] = inv[col][k]; inv[col][k] = t;
+                }
+            }
+        }
+        const double piv = m[col][col];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            m[col][k] /= piv;
+            inv[col][k] /= piv;
+        }
+#pragma unroll
+        for (int row = 0; row < 4; ++row) {
+            if (row == col) continue;
+            const double f = m[row][col];
+            if (f == 0.0) continue;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                m[row][k] -= f * m[col][k];
+                inv[row][k] -= f * inv[col][k];
+            }
+        }
+    }
+    return true;
+}
+
+struct Geo {
+    double g[4][3];  // grad L_i
+    double vol;
+};
+
+__device__ __forceinline__ void whitney(const Geo& G, int e, const double* L, double out[3]) {
+    const int vi = kEV[e][0], vj = kEV[e][1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = G.g[vj][c] * L[vi] - G.g[vi][c] * L[vj];
+}
+
+__device__ __forceinline__ void p2grad(const Geo& G, int a, const double* L, double out[3]) {
+    if (a < 4) {
+        const double s = 4.0 * L[a] - 1.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c] = G.g[a][c] * s;
+    } else {
+        const int v0 = kEV[a - 4][0], v1 = kEV[a - 4][1];
+        const double s1 = 4.0 * L[v1], s0 = 4.0 * L[v0];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[c] = G.g[v0][c] * s1 + G.g[v1][c] * s0;
+    }
+}
+
+__device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+}  // namespace
+
+// One 16-lane group per tet: lane 0 builds the geometry into LDS, then the 16
+// lanes split the 196 element entries (36 AA, 60 APhi, 100 PhiPhi).
+__global__ void __launch_bounds__(256) k_element(int num_tets, const double* __restrict__ coords,
+                                                 const int* __restrict__ tets, const int* __restrict__ tet_body,
+                                                 const BodyCoef* __restrict__ coef, cd* __restrict__ E,
+                                                 int* __restrict__ error_flag) {
+    __shared__ Geo geo[kTetsPerBlock];
+    __shared__ double qp[4][4];
+    const int local = threadIdx.x / kLanesPerTet;
+    const int lane = threadIdx.x % kLanesPerTet;
+    const int t = blockIdx.x * kTetsPerBlock + local;
+    if (threadIdx.x == 0) {
+        // src/quadrature.cpp:16-35
+        const double s5 = sqrt(5.0);
+        const double a = (5.0 + 3.0 * s5) / 20.0;
+        const double b = (5.0 - s5) / 20.0;
+        for (int k = 0; k < 4; ++k)
+            for (int i = 0; i < 4; ++i) qp[k][i] = (i == k) ? a : b;
+    }
+    if (lane == 0 && t < num_tets) {
+        double m[4][4], inv[4][4];
+        double p[4][3];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int v = tets[4 * t + j];
+            p[j][0] = coords[3 * v + 0];
+            p[j][1] = coords[3 * v + 1];
+            p[j][2] = coords[3 * v + 2];
+            m[j][0] = 1.0;
+            m[j][1] = p[j][0];
+            m[j][2] = p[j][1];
+            m[j][3] = p[j][2];
+        }
+        Geo G;
+        if (!invert4(m, inv)) {
+            atomicOr(error_flag, 1);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) G.g[i][0] = G.g[i][1] = G.g[i][2] = 0.0;
+            G.vol = 0.0;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                G.g[i][0] = inv[1][i];
+                G.g[i][1] = inv[2][i];
+                G.g[i][2] = inv[3][i];
+            }
+            // |signed_tet_volume|, src/mesh.cpp:151-161
+            double a[3], b[3], c[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                a[k] = p[1][k] - p[0][k];
+                b[k] = p[2][k] - p[0][k];
+                c[k] = p[3][k] - p[0][k];
+            }
+            const double x = a[1] * b[2] - a[2] * b[1];
+            const double y = a[2] * b[0] - a[0] * b[2];
+            const double z = a[0] * b[1] - a[1] * b[0];
+            G.vol = fabs((x * c[0] + y * c[1] + z * c[2]) / 6.0);
+        }
+        geo[local] = G;
+    }
+    __syncthreads();
+    if (t >= num_tets) return;
+    const Geo& G = geo[local];
+    const BodyCoef cf = coef[tet_body[t]];
+    cd* out = E + static_cast<int64_t>(t) * 196;
+    const double wvol = 0.25 * G.vol;
+
+    for (int e = lane; e < 196; e += kLanesPerTet) {
+        cd acc;
+        if (e < 36) {
+            // kernel_AA: nu*V*curl.curl, then alpha * (w * W_i.W_j) per point
+            const int i = e / 6, j = e % 6;
+            double ci[3], cj[3];
+            {
+                const int a0 = kEV[i][0], a1 = kEV[i][1], b0 = kEV[j][0], b1 = kEV[j][1];
+                ci[0] = (G.g[a0][1] * G.g[a1][2] - G.g[a0][2] * G.g[a1][1]) * 2.0;
+                ci[1] = (G.g[a0][2] * G.g[a1][0] - G.g[a0][0] * G.g[a1][2]) * 2.0;
+                ci[2] = (G.g[a0][0] * G.g[a1][1] - G.g[a0][1] * G.g[a1][0]) * 2.0;
+                cj[0] = (G.g[b0][1] * G.g[b1][2] - G.g[b0][2] * G.g[b1][1]) * 2.0;
+                cj[1] = (G.g[b0][2] * G.g[b1][0] - G.g[b0][0] * G.g[b1][2]) * 2.0;
+                cj[2] = (G.g[b0][0] * G.g[b1][1] - G.g[b0][1] * G.g[b1][0]) * 2.0;
+            }
+            acc.re = cf.nu * G.vol * dot3(ci, cj);
+            acc.im = 0.0;
+            if (!(cf.alpha.re == 0.0 && cf.alpha.im == 0.0)) {
+                for (int k = 0; k < 4; ++k) {
+                    double wi[3], wj[3];
+                    whitney(G, i, qp[k], wi);
+                    whitney(G, j, qp[k], wj);
+                    const double s = wvol * dot3(wi, wj);
+                    acc = cadd(acc, cscale(cf.alpha, s));
+                }
+            }
+        } else if (e < 96) {
+            // kernel_APhi: beta * (w * W_i . grad S_b)
+            const int i = (e - 36) / 10, b = (e - 36) % 10;
+            acc = {0.0, 0.0};
+            if (!(cf.beta.re == 0.0 && cf.beta.im == 0.0)) {
+                for (int k = 0; k < 4; ++k) {
+                    double wi[3], gb[3];
+                    whitney(G, i, qp[k], wi);
+                    p2grad(G, b, qp[k], gb);
+                    const double s = wvol * dot3(wi, gb);
+                    acc = cadd(acc, cscale(cf.beta, s));
+                }
+            }
+        } else {
+            // kernel_PhiPhi: beta * (w * grad S_a . grad S_b)
+            const int a = (e - 96) / 10, b = (e - 96) % 10;
+            acc = {0.0, 0.0};
+            if (!(cf.beta.re == 0.0 && cf.beta.im == 0.0)) {
+                for (int k = 0; k < 4; ++k) {
+                    double ga[3], gb[3];
+                    p2grad(G, a, qp[k], ga);
+                    p2grad(G, b, qp[k], gb);
+                    const double s = wvol * dot3(ga, gb);
+                    acc = cadd(acc, cscale(cf.beta, s));
+                }
+            }
+        }
+        out[e] = acc;
+    }
+}
+
+// One thread per target (a matrix slot, or an rhs entry at num_slots + i). The
+// contributions are summed in the recorded order, each formed exactly as the
+// reference's scatter forms it.
+__global__ void __launch_bounds__(256) k_assemble_reduce(int64_t num_targets, int num_slots,
+                                                         const int64_t* __restrict__ tptr,
+                                                         const uint32_t* __restrict__ entry,
+                                                         const uint8_t* __restrict__ kind,
+                                                         const cd* __restrict__ E, FreqScales sc,
+                                                         cd* __restrict__ values, cd* __restrict__ rhs) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= num_targets) return;
+    cd acc = {0.0, 0.0};
+    for (int64_t c = tptr[t]; c < tptr[t + 1]; ++c) {
+        const int k = kind[c];
+        const double sgn = (k & 0x10) ? -1.0 : 1.0;
+        const int vid = k >> 5;
+        switch (k & 0x0f) {
+            case 0:  // (cp*cq) * aa
+                acc = cadd(acc, cscale(E[entry[c]], sgn));
+                break;
+            case 1:  // (column * (ce*cn)) * ap
+                acc = cadd(acc, cmul(cscale(sc.column, sgn), E[entry[c]]));
+                break;
+            case 2:  // ((row*jw) * (cn*ce)) * ap
+                acc = cadd(acc, cmul(cscale(sc.row_jw, sgn), E[entry[c]]));
+                break;
+            case 3:  // ((row*column) * (cp*cq)) * pp
+                acc = cadd(acc, cmul(cscale(sc.row_column, sgn), E[entry[c]]));
+                break;
+            case 4:  // rhs -= ((column*ce) * ap) * fixed
+                acc = csub(acc, cmul(cmul(cscale(sc.column, sgn), E[entry[c]]), sc.over_column[vid]));
+                break;
+            case 5:  // rhs -= (((row*column)*cp) * pp) * fixed
+                acc = csub(acc, cmul(cmul(cscale(sc.row_column, sgn), E[entry[c]]), sc.over_column[vid]));
+                break;
+            case 6:  // voltage port diagonal
+                acc = cadd(acc, cd{1.0, 0.0});
+                break;
+            case 7:  // voltage port rhs: V / column
+                acc = cadd(acc, sc.over_column[vid]);
+                break;
+            case 8:  // current port rhs: row * I
+                acc = cadd(acc, sc.times_row[vid]);
+                break;
+            default:
+                break;
+        }
+    }
+    if (t < num_slots) values[t] = acc;
+    else rhs[t - num_slots] = acc;
+}
+
+// ------------------------------------------------------------------ equilibration
+// src/equilibration.cpp:94-139: every stored entry counts toward both its row and
+// its column. Max is exact, so the atomic max is order independent.
+
+__global__ void k_copy_cd(int64_t n, const cd* __restrict__ src, cd* __restrict__ dst) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_fill_d(int64_t n, double* __restrict__ p, double v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_eq_rowmax(int64_t nnz, const int* __restrict__ row_of, const int* __restrict__ col,
+                            const cd* __restrict__ w, double* __restrict__ rowmax) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const double mag = hypot(w[k].re, w[k].im);
+    atomic_max_nonneg(&rowmax[row_of[k]], mag);
+    atomic_max_nonneg(&rowmax[col[k]], mag);
+}
+
+__global__ void k_eq_factor(int n, double* __restrict__ rowmax, double* __restrict__ factor, double* __restrict__ d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double rm = rowmax[i];
+    const double f = rm > 0.0 ? 1.0 / sqrt(rm) : 1.0;
+    factor[i] = f;
+    d[i] *= f;
+    rowmax[i] = 0.0;
+}
+
+__global__ void k_eq_scale(int64_t nnz, const int* __restrict__ row_of, const int* __restrict__ col,
+                           const double* __restrict__ factor, cd* __restrict__ w) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    w[k] = cscale(w[k], factor[row_of[k]] * factor[col[k]]);
+}
+
+// apply_symmetric_equilibration (src/equilibration.cpp:141-162): values * (d_i d_j, 0)
+__global__ void k_eq_apply(int64_t nnz, const int* __restrict__ row_of, const int* __restrict__ col,
+                           const double* __restrict__ d, const cd* __restrict__ a, cd* __restrict__ out) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    out[k] = cmul(a[k], cd{d[row_of[k]] * d[col[k]], 0.0});
+}
+
+// ------------------------------------------------------------------ operator
+
+// y = A x for the symmetric upper-stored A (include/aphi_solver/sparse_symmetric.hpp:116-131),
+// accumulated per row in the reference's order: the mirrored entries (r, i), r < i
+// ascending, then row i's own entries. Multi-RHS, column-major with leading dim ld.
+__global__ void k_symv(int n, int nrhs, const int* __restrict__ row_ptr, const int* __restrict__ col,
+                       const int* __restrict__ tptr, const int* __restrict__ tslot, const int* __restrict__ trow,
+                       const cd* __restrict__ a, const cd* __restrict__ x, int64_t ldx, cd* __restrict__ y, int64_t ldy) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n || r >= nrhs) return;
+    const cd* xr = x + r * ldx;
+    cd acc = {0.0, 0.0};
+    for (int t = tptr[i]; t < tptr[i + 1]; ++t) acc = cadd(acc, cmul(a[tslot[t]], xr[trow[t]]));
+    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) acc = cadd(acc, cmul(a[k], xr[col[k]]));
+    y[r * ldy + i] = acc;
+}
+
+// r = b - A x, in place into r (which holds A x on entry)
+__global__ void k_residual(int n, int nrhs, const cd* __restrict__ b, int64_t ldb, cd* __restrict__ r, int64_t ldr) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n || c >= nrhs) return;
+    r[c * ldr + i] = csub(b[c * ldb + i], r[c * ldr + i]);
+}
+
+// Per-RHS sums of |v|^2 over rows, into partials[c * nblocks + blockIdx.x]; a
+// fixed-shape tree so the result does not depend on scheduling.
+__global__ void __launch_bounds__(256) k_sumsq_partial(int n, const cd* __restrict__ v, int64_t ldv,
+                                                       double* __restrict__ partials) {
+    __shared__ double red[256];
+    const int c = blockIdx.y;
+    const cd* col = v + c * ldv;
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += col[i].re * col[i].re + col[i].im * col[i].im;
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_absmax_partial(int64_t n, const cd* __restrict__ v, double* __restrict__ partials) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256)
+        s = fmax(s, hypot(v[i].re, v[i].im));
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// b~[i] = d[perm[i]] * b[perm[i]]  (scale_rhs then P b; the order of the product
+// (b * (d, 0)) follows src/equilibration.cpp:73-82)
+__global__ void k_scale_permute_in(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ d,
+                                   const cd* __restrict__ b, int64_t ldb, cd* __restrict__ out, int64_t ldo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n || c >= nrhs) return;
+    const int p = perm[i];
+    out[c * ldo + i] = cmul(b[c * ldb + p], cd{d[p], 0.0});
+}
+
+// x[perm[i]] = d[perm[i]] * x~[i]  (P^T then recover_equilibrated_solution)
+__global__ void k_permute_scale_out(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ d,
+                                    const cd* __restrict__ xt, int64_t ldt, cd* __restrict__ x, int64_t ldx, int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= n || c >= nrhs) return;
+    const int p = perm[i];
+    const cd v = cmul(xt[c * ldt + i], cd{d[p], 0.0});
+    x[c * ldx + p] = accumulate ? cadd(x[c * ldx + p], v) : v;
+}
+
+}  // namespace aphi_dev

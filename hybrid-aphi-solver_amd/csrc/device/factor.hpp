@@ -1,0 +1,45 @@
+// Host-callable launchers for the supernodal factorization and solve kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.hpp"
+
+namespace aphi_dev {
+
+constexpr int kNB = 64;  // block-column width of the blocked front factorization and solves
+
+template <typename T> void launch_zero(hipStream_t st, int64_t n, void* p);
+template <typename T> void launch_scatter_panel(hipStream_t st, int64_t nnz, const int64_t* s2p, const cd* a, void* P);
+template <typename T> size_t small_lds_bytes(int max_m);
+template <typename T>
+void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, const FrontPlan& fp, void* P, void* CB,
+                        double pivot_floor, FactorStatsDev* stats);
+template <typename T>
+void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
+                       const FrontPlan& fp, void* P, void* CB);
+template <typename T>
+void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
+                       const FrontPlan& fp, void* P, void* CB, void* mws, double pivot_floor, FactorStatsDev* stats);
+
+// solve.hip
+template <typename T>
+void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
+                    const double* colscale, void* out);
+template <typename T>
+void launch_perm_out(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const void* xt,
+                     const double* colscale, cd* x, int64_t ldx, int accumulate);
+template <typename T>
+void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int nrhs, const FrontPlan& fp, const void* b, int n,
+                       void* W);
+template <typename T>
+void launch_fwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
+                     const FrontPlan& fp, const void* P, void* W);
+template <typename T>
+void launch_bwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
+                     const FrontPlan& fp, const void* P, const void* W, void* X, int n, void* partial);
+void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
+                         int mode);
+
+}  // namespace aphi_dev

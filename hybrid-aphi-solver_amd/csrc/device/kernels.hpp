@@ -1,0 +1,71 @@
+// Kernel declarations and the small POD structs shared between the HIP kernels
+// and their host-side launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace aphi_dev {
+
+// Per body, per frequency: nu, alpha, beta of src/element_matrix.cpp:10-25.
+struct BodyCoef {
+    double nu;
+    cd alpha;
+    cd beta;
+};
+
+// Per frequency scalars of the scatter (src/assembly.cpp:12-45), formed on the
+// host with std::complex so they are the reference's values exactly.
+struct FreqScales {
+    cd column;        // c
+    cd row_jw;        // r * j*omega
+    cd row_column;    // r * c
+    cd over_column[8];// prescribed value id -> value / c
+    cd times_row[8];  // port amplitude id -> r * I
+};
+
+// Everything the numeric factorization reads about the supernodal structure.
+struct FrontPlan {
+    const int* first;          // ns + 1
+    const int64_t* row_ptr;    // ns + 1
+    const int* rows;           // per supernode row lists
+    const int* rel;            // relative indices into the parent (aligned with rows)
+    const int* child_ptr;      // ns + 1
+    const int* child;
+    const int64_t* panel_off;  // ns + 1
+    const int64_t* cb_off;     // ns
+    const int* col_ref;        // factor-order column -> reference (analysis) column
+};
+
+struct FactorStatsDev {
+    double smallest_pivot;   // atomic min of |D|
+    double largest_pivot;    // atomic max of |D|
+    double largest_multiplier;
+    int fail_col;            // smallest reference column whose pivot failed, or INT_MAX
+    int pad;
+};
+
+// assembly.hip
+__global__ void k_element(int num_tets, const double* coords, const int* tets, const int* tet_body,
+                          const BodyCoef* coef, cd* E, int* error_flag);
+__global__ void k_assemble_reduce(int64_t num_targets, int num_slots, const int64_t* tptr, const uint32_t* entry,
+                                  const uint8_t* kind, const cd* E, FreqScales sc, cd* values, cd* rhs);
+__global__ void k_copy_cd(int64_t n, const cd* src, cd* dst);
+__global__ void k_fill_d(int64_t n, double* p, double v);
+__global__ void k_eq_rowmax(int64_t nnz, const int* row_of, const int* col, const cd* w, double* rowmax);
+__global__ void k_eq_factor(int n, double* rowmax, double* factor, double* d);
+__global__ void k_eq_scale(int64_t nnz, const int* row_of, const int* col, const double* factor, cd* w);
+__global__ void k_eq_apply(int64_t nnz, const int* row_of, const int* col, const double* d, const cd* a, cd* out);
+__global__ void k_symv(int n, int nrhs, const int* row_ptr, const int* col, const int* tptr, const int* tslot,
+                       const int* trow, const cd* a, const cd* x, int64_t ldx, cd* y, int64_t ldy);
+__global__ void k_residual(int n, int nrhs, const cd* b, int64_t ldb, cd* r, int64_t ldr);
+__global__ void k_sumsq_partial(int n, const cd* v, int64_t ldv, double* partials);
+__global__ void k_absmax_partial(int64_t n, const cd* v, double* partials);
+__global__ void k_scale_permute_in(int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
+                                   cd* out, int64_t ldo);
+__global__ void k_permute_scale_out(int n, int nrhs, const int* perm, const double* d, const cd* xt, int64_t ldt,
+                                    cd* x, int64_t ldx, int accumulate);
+
+}  // namespace aphi_dev

@@ -1,0 +1,329 @@
+// Multi-RHS supernodal triangular solves (src/factorization.cpp:254-313, organised
+// by fronts): forward through L (unit), divide by D, backward through L^T -- the
+// transpose, never the conjugate transpose.
+//
+// Forward, leaves to root: each supernode's front vector W_s (m x nrhs) takes its
+// own right-hand side rows plus its children's update vectors (children in order,
+// so the sums are deterministic), solves its unit-lower diagonal part and leaves
+// W_s[k:m] - L21 y as its own update vector. Backward, root to leaves: x_s =
+// L11^-T (D^-1 y_s - L21^T x[rows beyond]), where the rows beyond belong to
+// ancestors that are already solved. Blocks of NB columns bound the work of one
+// workgroup; row-parallel kernels carry the rest.
+#include "common.hpp"
+#include "factor.hpp"
+
+namespace aphi_dev {
+
+namespace {
+
+template <typename T>
+using C = typename cplx<T>::type;
+
+struct Sn {
+    int m, k, first;
+    int64_t rp;
+};
+
+__device__ __forceinline__ Sn sn(const FrontPlan& fp, int s) {
+    Sn x;
+    x.rp = fp.row_ptr[s];
+    x.m = static_cast<int>(fp.row_ptr[s + 1] - x.rp);
+    x.first = fp.first[s];
+    x.k = fp.first[s + 1] - x.first;
+    return x;
+}
+
+}  // namespace
+
+template <typename T>
+__global__ void k_perm_in(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ d,
+                          const cd* __restrict__ b, int64_t ldb, const double* __restrict__ colscale, C<T>* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n || r >= nrhs) return;
+    const int p = perm[i];
+    cd v = cmul(b[r * ldb + p], cd{d[p], 0.0});
+    const double s = colscale[r];
+    if (s != 1.0) v = cd{v.re * s, v.im * s};
+    out[static_cast<int64_t>(r) * n + i] = C<T>{static_cast<T>(v.re), static_cast<T>(v.im)};
+}
+
+template <typename T>
+__global__ void k_perm_out(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ d,
+                           const C<T>* __restrict__ xt, const double* __restrict__ colscale, cd* __restrict__ x,
+                           int64_t ldx, int accumulate) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n || r >= nrhs) return;
+    const int p = perm[i];
+    const C<T> t = xt[static_cast<int64_t>(r) * n + i];
+    cd v = cd{static_cast<double>(t.re), static_cast<double>(t.im)};
+    const double s = colscale[r];
+    if (s != 1.0) v = cd{v.re / s, v.im / s};
+    v = cmul(v, cd{d[p], 0.0});
+    x[r * ldx + p] = accumulate ? cadd(x[r * ldx + p], v) : v;
+}
+
+// W_s := [b rows of s ; 0] + extend-add of the children's update vectors.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_gather(const int* __restrict__ nodes, int nrhs, FrontPlan fp,
+                                                    const C<T>* __restrict__ b, int n, C<T>* W) {
+    const int s = nodes[blockIdx.x];
+    const int r = blockIdx.y;
+    const Sn x = sn(fp, s);
+    C<T>* Ws = W + x.rp * nrhs + static_cast<int64_t>(r) * x.m;
+    for (int i = threadIdx.x; i < x.m; i += blockDim.x)
+        Ws[i] = i < x.k ? b[static_cast<int64_t>(r) * n + x.first + i] : C<T>{0, 0};
+    for (int c = fp.child_ptr[s]; c < fp.child_ptr[s + 1]; ++c) {
+        __syncthreads();
+        const int ch = fp.child[c];
+        const Sn y = sn(fp, ch);
+        const int cu = y.m - y.k;
+        const C<T>* Wc = W + y.rp * nrhs + static_cast<int64_t>(r) * y.m + y.k;
+        const int* rel = fp.rel + y.rp + y.k;
+        for (int i = threadIdx.x; i < cu; i += blockDim.x) Ws[rel[i]] = gadd(Ws[rel[i]], Wc[i]);
+    }
+}
+
+// y_b := L_bb^-1 y_b for block b of every supernode with k > b.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_diag(const int* __restrict__ nodes, int b, int nrhs, FrontPlan fp,
+                                                  const C<T>* __restrict__ P, C<T>* W) {
+    extern __shared__ unsigned char smem_raw[];
+    const int s = nodes[blockIdx.x];
+    const Sn x = sn(fp, s);
+    if (x.k <= b) return;
+    const int nb = min(kNB, x.k - b);
+    C<T>* L = reinterpret_cast<C<T>*>(smem_raw);  // L[c*kNB + i]
+    C<T>* Y = L + kNB * kNB;                      // Y[r*kNB + i]
+    const C<T>* Ps = P + fp.panel_off[s];
+    C<T>* Ws = W + x.rp * nrhs;
+    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+        const int c = idx / nb, i = idx % nb;
+        L[c * kNB + i] = Ps[static_cast<int64_t>(b + c) * x.m + b + i];
+    }
+    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
+        const int r = idx / nb, i = idx % nb;
+        Y[r * kNB + i] = Ws[static_cast<int64_t>(r) * x.m + b + i];
+    }
+    for (int c = 0; c < nb; ++c) {
+        __syncthreads();
+        const int w = nb - c - 1;
+        for (int idx = threadIdx.x; idx < w * nrhs; idx += blockDim.x) {
+            const int r = idx / w, i = c + 1 + idx % w;
+            Y[r * kNB + i] = gfms(Y[r * kNB + i], L[c * kNB + i], Y[r * kNB + c]);
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
+        const int r = idx / nb, i = idx % nb;
+        Ws[static_cast<int64_t>(r) * x.m + b + i] = Y[r * kNB + i];
+    }
+}
+
+// W[i] -= L[i, b:b+nb] y_b for rows i >= b+nb, one row per thread.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ nodes, int b, int nrhs, FrontPlan fp,
+                                                    const C<T>* __restrict__ P, C<T>* W) {
+    __shared__ C<T> Y[16 * kNB];
+    const int s = nodes[blockIdx.y];
+    const Sn x = sn(fp, s);
+    if (x.k <= b) return;
+    const int nb = min(kNB, x.k - b);
+    const int i0 = b + nb + static_cast<int>(blockIdx.x) * 256;
+    if (i0 >= x.m) return;
+    const C<T>* Ps = P + fp.panel_off[s];
+    C<T>* Ws = W + x.rp * nrhs;
+    const int i = i0 + threadIdx.x;
+    for (int r0 = 0; r0 < nrhs; r0 += 16) {
+        const int nr = min(16, nrhs - r0);
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < nb * nr; idx += blockDim.x) {
+            const int r = idx / nb, c = idx % nb;
+            Y[r * kNB + c] = Ws[static_cast<int64_t>(r0 + r) * x.m + b + c];
+        }
+        __syncthreads();
+        if (i < x.m) {
+            C<T> acc[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = C<T>{0, 0};
+            for (int c = 0; c < nb; ++c) {
+                const C<T> l = Ps[static_cast<int64_t>(b + c) * x.m + i];
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if (r < nr) acc[r] = gfma(acc[r], l, Y[r * kNB + c]);
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (r < nr) {
+                    C<T>& dst = Ws[static_cast<int64_t>(r0 + r) * x.m + i];
+                    dst = gsub(dst, acc[r]);
+                }
+        }
+    }
+}
+
+// partial[slot][blk] (nb x nrhs) = sum over 64 rows of L[row, b+c] * x[row]
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nodes, int b, int nrhs, int stride,
+                                                     FrontPlan fp, const C<T>* __restrict__ P,
+                                                     const C<T>* __restrict__ X, int n, C<T>* __restrict__ partial) {
+    extern __shared__ unsigned char smem_raw[];
+    const int s = nodes[blockIdx.y];
+    const Sn x = sn(fp, s);
+    if (x.k <= b) return;
+    const int nb = min(kNB, x.k - b);
+    const int i0 = b + nb + static_cast<int>(blockIdx.x) * 64;
+    if (i0 >= x.m) return;
+    const int nrows = min(64, x.m - i0);
+    C<T>* Lt = reinterpret_cast<C<T>*>(smem_raw);  // Lt[c*64 + row]
+    C<T>* Xt = Lt + kNB * 64;                      // Xt[r*64 + row]
+    const C<T>* Ps = P + fp.panel_off[s];
+    const int* rows = fp.rows + x.rp;
+    for (int idx = threadIdx.x; idx < nb * 64; idx += blockDim.x) {
+        const int c = idx / 64, row = idx % 64;
+        Lt[c * 64 + row] = row < nrows ? Ps[static_cast<int64_t>(b + c) * x.m + i0 + row] : C<T>{0, 0};
+    }
+    for (int idx = threadIdx.x; idx < nrhs * 64; idx += blockDim.x) {
+        const int r = idx / 64, row = idx % 64;
+        Xt[r * 64 + row] = row < nrows ? X[static_cast<int64_t>(r) * n + rows[i0 + row]] : C<T>{0, 0};
+    }
+    __syncthreads();
+    C<T>* out = partial + (static_cast<int64_t>(blockIdx.y) * stride + blockIdx.x) * kNB * nrhs;
+    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
+        const int c = idx / nrhs, r = idx % nrhs;
+        C<T> acc = C<T>{0, 0};
+        for (int row = 0; row < 64; ++row) acc = gfma(acc, Lt[c * 64 + row], Xt[r * 64 + row]);
+        out[c * nrhs + r] = acc;
+    }
+}
+
+// x_b = L_bb^-T (D_b^-1 y_b - sum of partials), written to X.
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes, int b, int nrhs, int stride,
+                                                  FrontPlan fp, const C<T>* __restrict__ P, const C<T>* __restrict__ W,
+                                                  C<T>* __restrict__ X, int n, const C<T>* __restrict__ partial) {
+    extern __shared__ unsigned char smem_raw[];
+    const int s = nodes[blockIdx.x];
+    const Sn x = sn(fp, s);
+    if (x.k <= b) return;
+    const int nb = min(kNB, x.k - b);
+    const int nblk = x.m > b + nb ? (x.m - b - nb + 63) / 64 : 0;
+    C<T>* L = reinterpret_cast<C<T>*>(smem_raw);  // L[c*kNB + i]
+    C<T>* Z = L + kNB * kNB;                      // Z[r*kNB + c]
+    const C<T>* Ps = P + fp.panel_off[s];
+    const C<T>* Ws = W + x.rp * nrhs;
+    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+        const int c = idx / nb, i = idx % nb;
+        L[c * kNB + i] = Ps[static_cast<int64_t>(b + c) * x.m + b + i];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
+        const int r = idx / nb, c = idx % nb;
+        C<T> t = C<T>{0, 0};
+        const C<T>* part = partial + static_cast<int64_t>(blockIdx.x) * stride * kNB * nrhs;
+        for (int blk = 0; blk < nblk; ++blk) t = gadd(t, part[static_cast<int64_t>(blk) * kNB * nrhs + c * nrhs + r]);
+        const C<T> y = Ws[static_cast<int64_t>(r) * x.m + b + c];
+        // D^-1 y: y / D as y * (1/D)
+        Z[r * kNB + c] = gsub(gmul(y, grecip(L[c * kNB + c])), t);
+    }
+    for (int c = nb - 1; c >= 0; --c) {
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < c * nrhs; idx += blockDim.x) {
+            const int r = idx / c, q = idx % c;
+            Z[r * kNB + q] = gfms(Z[r * kNB + q], L[q * kNB + c], Z[r * kNB + c]);
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
+        const int r = idx / nb, c = idx % nb;
+        X[static_cast<int64_t>(r) * n + x.first + b + c] = Z[r * kNB + c];
+    }
+}
+
+// colscale[r] = 1 (mode 0) or 1/||b_r|| (mode 1, 1 for a zero column)
+__global__ void k_colscale_final(int nrhs, int nparts, const double* __restrict__ partials, double* __restrict__ colscale,
+                                 int mode) {
+    const int r = threadIdx.x;
+    if (r >= nrhs) return;
+    if (mode == 0) {
+        colscale[r] = 1.0;
+        return;
+    }
+    double s = 0.0;
+    for (int p = 0; p < nparts; ++p) s += partials[r * nparts + p];
+    colscale[r] = s > 0.0 ? 1.0 / sqrt(s) : 1.0;
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <typename T>
+void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
+                    const double* colscale, void* out) {
+    hipLaunchKernelGGL(k_perm_in<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, n, nrhs, perm, d, b, ldb, colscale,
+                       static_cast<C<T>*>(out));
+}
+
+template <typename T>
+void launch_perm_out(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const void* xt,
+                     const double* colscale, cd* x, int64_t ldx, int accumulate) {
+    hipLaunchKernelGGL(k_perm_out<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, n, nrhs, perm, d,
+                       static_cast<const C<T>*>(xt), colscale, x, ldx, accumulate);
+}
+
+template <typename T>
+void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int nrhs, const FrontPlan& fp, const void* b, int n,
+                       void* W) {
+    if (count <= 0) return;
+    hipLaunchKernelGGL(k_fwd_gather<T>, dim3(count, nrhs), dim3(256), 0, st, nodes, nrhs, fp,
+                       static_cast<const C<T>*>(b), n, static_cast<C<T>*>(W));
+}
+
+template <typename T>
+void launch_fwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
+                     const FrontPlan& fp, const void* P, void* W) {
+    if (count <= 0) return;
+    const size_t lds = static_cast<size_t>(kNB * kNB + kNB * nrhs) * sizeof(C<T>);
+    hipLaunchKernelGGL(k_fwd_diag<T>, dim3(count), dim3(256), lds, st, nodes, b, nrhs, fp, static_cast<const C<T>*>(P),
+                       static_cast<C<T>*>(W));
+    if (max_row_blocks > 0)
+        hipLaunchKernelGGL(k_fwd_update<T>, dim3(max_row_blocks, count), dim3(256), 0, st, nodes, b, nrhs, fp,
+                           static_cast<const C<T>*>(P), static_cast<C<T>*>(W));
+}
+
+template <typename T>
+void launch_bwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
+                     const FrontPlan& fp, const void* P, const void* W, void* X, int n, void* partial) {
+    if (count <= 0) return;
+    const int stride = max_row_blocks > 0 ? max_row_blocks : 1;
+    if (max_row_blocks > 0) {
+        const size_t lds = static_cast<size_t>(kNB * 64 + nrhs * 64) * sizeof(C<T>);
+        hipLaunchKernelGGL(k_bwd_partial<T>, dim3(max_row_blocks, count), dim3(256), lds, st, nodes, b, nrhs, stride, fp,
+                           static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(partial));
+    }
+    const size_t lds = static_cast<size_t>(kNB * kNB + kNB * nrhs) * sizeof(C<T>);
+    hipLaunchKernelGGL(k_bwd_diag<T>, dim3(count), dim3(256), lds, st, nodes, b, nrhs, stride, fp,
+                       static_cast<const C<T>*>(P), static_cast<const C<T>*>(W), static_cast<C<T>*>(X), n,
+                       static_cast<const C<T>*>(partial));
+}
+
+void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
+                         int mode) {
+    const int nparts = 64;
+    if (mode != 0)
+        hipLaunchKernelGGL(k_sumsq_partial, dim3(nparts, nrhs), dim3(256), 0, st, n, b, ldb, partials);
+    hipLaunchKernelGGL(k_colscale_final, dim3(1), dim3(64), 0, st, nrhs, nparts, partials, colscale, mode);
+}
+
+template void launch_perm_in<double>(hipStream_t, int, int, const int*, const double*, const cd*, int64_t, const double*, void*);
+template void launch_perm_in<float>(hipStream_t, int, int, const int*, const double*, const cd*, int64_t, const double*, void*);
+template void launch_perm_out<double>(hipStream_t, int, int, const int*, const double*, const void*, const double*, cd*, int64_t, int);
+template void launch_perm_out<float>(hipStream_t, int, int, const int*, const double*, const void*, const double*, cd*, int64_t, int);
+template void launch_fwd_gather<double>(hipStream_t, const int*, int, int, const FrontPlan&, const void*, int, void*);
+template void launch_fwd_gather<float>(hipStream_t, const int*, int, int, const FrontPlan&, const void*, int, void*);
+template void launch_fwd_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, void*);
+template void launch_fwd_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, void*);
+template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, void*, int, void*);
+template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, void*, int, void*);
+
+}  // namespace aphi_dev

@@ -1,0 +1,229 @@
+/* C-ABI of the MI355X A-Phi path: element assembly into the symmetric CSR, and
+ * complex-symmetric LDL^T factorization and solve, on one gfx950 GPU.
+ *
+ * Plain C types only: pointers, sizes, status codes. Complex arrays are
+ * interleaved (re, im) doubles, the layout of std::complex<double>. Arrays
+ * documented as "device" are HIP device pointers; everything else is host
+ * memory. Every call returns an aphi_status; on failure aphi_last_error()
+ * returns the message (thread-local, valid until the next call on the thread).
+ *
+ * Which reference entry point each call replaces is noted beside it
+ * (paths relative to the reference repository).
+ */
+#ifndef APHI_GPU_H
+#define APHI_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The host mirror maps them back onto the reference's error
+ * conventions: INVALID_ARGUMENT -> std::invalid_argument, LOGIC_ERROR ->
+ * std::logic_error, RUNTIME_ERROR -> std::runtime_error, PIVOT_FAILED -> the
+ * `false` return of factorize_ldlt / solve_symmetric with the stats filled. */
+typedef enum aphi_status {
+    APHI_OK = 0,
+    APHI_INVALID_ARGUMENT = 1,
+    APHI_LOGIC_ERROR = 2,
+    APHI_RUNTIME_ERROR = 3,
+    APHI_DEVICE_ERROR = 4,
+    APHI_OUT_OF_MEMORY = 5,
+    APHI_PIVOT_FAILED = 6
+} aphi_status;
+
+/* include/aphi_solver/problem.hpp: enum class Conditioning */
+typedef enum aphi_conditioning { APHI_NATURAL = 0, APHI_ROW_SCALED = 1, APHI_SCALED_PHI = 2 } aphi_conditioning;
+/* include/aphi_solver/ordering.hpp: enum class Ordering */
+typedef enum aphi_ordering { APHI_ORDER_NATURAL = 0, APHI_ORDER_RCM = 1, APHI_ORDER_AMD = 2 } aphi_ordering;
+/* Arithmetic of the numeric factorization: complex128, or complex64 (MFMA
+ * FP32) followed by FP64 iterative refinement. */
+typedef enum aphi_precision { APHI_FP64 = 0, APHI_FP32 = 1 } aphi_precision;
+
+const char* aphi_last_error(void);
+const char* aphi_version(void);
+
+/* ------------------------------------------------------------------ host symbolic problem
+ * The integer half of the path, kept on the host and bit-exact with the
+ * reference: mesh topology (src/mesh.cpp:26-142), binding and tree-cotree gauge
+ * (src/problem_binding.cpp:208-700, src/tree_cotree.cpp:89-251), DOF map
+ * (src/dof_map.cpp:8-194), upper-triangle sparsity (src/sparsity.cpp:19-136),
+ * ScatterMap (src/assembly.cpp:196-232), ordering (src/ordering.cpp:54-608) and
+ * the elimination tree / counts (src/symbolic.cpp:58-196), plus the supernodal
+ * plan the device factorization uses. */
+typedef struct aphi_problem aphi_problem;
+
+/* A Kuhn-split cube (tools/generate_cube_mesh.py:20-48) of n^3 hexes, side
+ * `side` metres, with a conductor bar of hex columns [bar_i0,bar_i1) x
+ * [bar_j0,bar_j1) spanning z; voltage ports on the bar's z=side and z=0 faces;
+ * n x A = 0 on the outer boundary; full-wave Phi. */
+typedef struct aphi_cube_spec {
+    int n;
+    double side;
+    int bar_i0, bar_i1, bar_j0, bar_j1;
+    double sigma, eps_r, mu_r;
+    double air_sigma, air_eps_r, air_mu_r;
+    double v_top_re, v_top_im, v_bottom_re, v_bottom_im;
+    int ordering;          /* aphi_ordering */
+    int relax_small;       /* supernode amalgamation: <= 0 selects the default */
+    double relax_fill;     /* <= 0 selects the default */
+} aphi_cube_spec;
+
+typedef struct aphi_problem_counts {
+    int64_t nodes, edges, faces, tets;
+    int64_t num_a, num_phi, num_ports, num_total; /* DofMap counts */
+    int64_t nnz;                                  /* stored upper-triangle entries */
+    int64_t predicted_nnz;                        /* nnz(L) incl. diagonal (SolverAnalysis) */
+    int64_t num_roots;
+    int64_t tree_edges, dirichlet_edges, free_edges;
+    int64_t num_super, num_levels, factor_entries, cb_pool_entries, contributions;
+    double flops;                                 /* 8*sum_j [m_j(m_j+1)/2 + 2 m_j] */
+    double analyze_ms, build_ms;
+} aphi_problem_counts;
+
+int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem** out);
+void aphi_problem_destroy(aphi_problem* p);
+int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out);
+/* Named integer arrays, copied into out[0..capacity); *count receives the full
+ * length (call with capacity 0 to size). Names: tet_nodes, edges, tet_edges,
+ * tet_edge_signs, tet_body, dirichlet_edge, tree_edge, edge_index, phi_index,
+ * port_index, row_ptr, col_index, perm, iperm, parent, row_count, col_count,
+ * scatter_live, scatter_pos, scatter_offset, scatter_slot, super_first,
+ * super_parent, super_row_ptr, super_rows, super_level, factor_perm,
+ * slot_to_panel. */
+int aphi_problem_get_int(const aphi_problem* p, const char* name, int64_t* out, int64_t capacity, int64_t* count);
+/* Named double arrays: coords (3 per node). */
+int aphi_problem_get_double(const aphi_problem* p, const char* name, double* out, int64_t capacity, int64_t* count);
+/* The log sweep of src/input_file.cpp:19-50. */
+int aphi_frequency_sweep(double start, double stop, int points, int logarithmic, double* out);
+
+/* ------------------------------------------------------------------ device boundary
+ * What the reference host hands the GPU once per mesh: geometry, the assembly
+ * plan (every contribution to every slot / rhs entry, in the reference's
+ * accumulation order), the pattern, and the supernodal factor plan. */
+typedef struct aphi_mesh_desc {
+    int32_t num_nodes, num_tets, num_bodies, n;
+    int64_t nnz;
+    const double* coords;          /* 3 * num_nodes, metres */
+    const int32_t* tet_nodes;      /* 4 * num_tets */
+    const int32_t* tet_body;       /* num_tets, index into the body arrays */
+    const double* body_sigma;      /* num_bodies */
+    const double* body_eps_r;
+    const double* body_mu_r;
+    /* assembly plan: contributions for target t (slot t < nnz, rhs row t - nnz)
+     * are [target_ptr[t], target_ptr[t+1]) */
+    const int64_t* target_ptr;     /* nnz + n + 1 */
+    const uint32_t* entry;         /* tet * 196 + element entry */
+    const uint8_t* kind;           /* contribution kind | sign << 4 | value id << 5 */
+    int32_t num_values;            /* <= 8 prescribed values */
+    const double* values;          /* 2 * num_values (complex) */
+    /* upper-triangle CSR of the original ordering */
+    const int32_t* row_ptr;        /* n + 1 */
+    const int32_t* col_index;      /* nnz */
+    /* supernodal plan (factorization order) */
+    const int32_t* factor_perm;    /* n: factor position -> original unknown */
+    const int32_t* col_ref;        /* n: factor position -> analysis (reference) position */
+    int32_t num_super, num_levels;
+    const int32_t* super_first;    /* num_super + 1 */
+    const int32_t* super_parent;   /* num_super */
+    const int64_t* super_row_ptr;  /* num_super + 1 */
+    const int32_t* super_rows;     /* super_row_ptr[num_super] */
+    const int32_t* super_rel;      /* aligned with super_rows */
+    const int32_t* super_level;    /* num_super */
+    const int64_t* panel_offset;   /* num_super + 1 */
+    const int64_t* cb_offset;      /* num_super */
+    int64_t cb_pool_entries;
+    const int64_t* slot_to_panel;  /* nnz */
+    double flops;                  /* algorithmic flops of one factorization */
+    int64_t predicted_nnz;
+} aphi_mesh_desc;
+
+/* Pointers into the problem's own arrays; valid while the problem lives. */
+int aphi_problem_describe(const aphi_problem* p, aphi_mesh_desc* out);
+
+typedef struct aphi_context aphi_context;
+typedef struct aphi_mesh aphi_mesh;
+
+int aphi_context_create(int device, aphi_context** out);
+void aphi_context_destroy(aphi_context* ctx);
+int aphi_context_synchronize(aphi_context* ctx);
+
+/* Uploads one mesh's plan (once per mesh; reused at every frequency). */
+int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* desc, aphi_mesh** out);
+void aphi_mesh_destroy(aphi_mesh* mesh);
+/* Device bytes currently held by this mesh (plan + numeric buffers). */
+int64_t aphi_mesh_device_bytes(const aphi_mesh* mesh);
+
+/* refill(SymmetricSystem&, bound, mesh, dofs, pattern, omega, conditioning, map)
+ * -- include/aphi_solver/assembly.hpp:197, src/assembly.cpp:468-494. Clears and
+ * assembles the upper-triangle values and the rhs on the device. Refuses a
+ * conditioning that is not symmetric and omega == 0 under a scaled one. */
+int aphi_refill(aphi_mesh* mesh, double omega, int conditioning);
+/* The assembled system, copied to host: values 2*nnz doubles, rhs 2*n. */
+int aphi_system_download(aphi_mesh* mesh, double* values, double* rhs);
+/* Replaces the device system with a caller-assembled one (host arrays). */
+int aphi_system_upload(aphi_mesh* mesh, const double* values, const double* rhs);
+/* Device pointer to the assembled rhs (n complex), for solving the port system. */
+const void* aphi_rhs_device(const aphi_mesh* mesh);
+
+/* compute_symmetric_equilibration(a, iterations) + apply (src/equilibration.cpp:94-162).
+ * iterations 0 disables scaling. d_out (host, n) may be NULL. */
+int aphi_equilibrate(aphi_mesh* mesh, int iterations, double* d_out);
+
+/* factorize_ldlt (include/aphi_solver/factorization.hpp:82, src/factorization.cpp:109-218)
+ * of the equilibrated system. Returns APHI_PIVOT_FAILED (stats->ok = 0) when a
+ * pivot is at or below pivot_floor. */
+typedef struct aphi_factor_stats {
+    int ok;
+    int columns_done;
+    double smallest_pivot, largest_pivot, largest_multiplier;
+    int64_t nnz;
+    double milliseconds;
+    double flops;
+} aphi_factor_stats;
+int aphi_factorize(aphi_mesh* mesh, int precision, double pivot_floor, aphi_factor_stats* stats);
+
+/* solve (include/aphi_solver/factorization.hpp:110) for nrhs right-hand sides
+ * with the current factor, undoing the equilibration: x = D * solve(D * b).
+ * b and x are device arrays, column-major with leading dimensions ldb, ldx. */
+int aphi_solve(aphi_mesh* mesh, int nrhs, const void* b, int64_t ldb, void* x, int64_t ldx);
+
+/* solve_symmetric (include/aphi_solver/factorization.hpp:206) extended with FP64
+ * iterative refinement: equilibrate, factor (FP64 or FP32), solve, then refine
+ * with the FP64 residual of the ORIGINAL system until every right-hand side's
+ * normwise backward error is <= tolerance. */
+typedef struct aphi_solve_options {
+    int precision;                 /* aphi_precision */
+    int equilibration_iterations;  /* 10 in the reference; 0 disables */
+    double pivot_floor;
+    int max_refinements;           /* 0: no refinement (the reference's behaviour) */
+    double tolerance;              /* backward-error target, e.g. 1e-12 */
+} aphi_solve_options;
+
+typedef struct aphi_solve_report {
+    int ok;
+    int refinement_steps;
+    double backward_error;         /* max over right-hand sides */
+    double residual;               /* max ||Ax-b||/||b|| */
+    double equilibration_min, equilibration_max;
+    double smallest_pivot, largest_pivot, largest_multiplier;
+    int64_t factor_nnz;
+    int columns_done;
+    double factorize_ms, solve_ms, total_ms;
+} aphi_solve_report;
+
+int aphi_solve_symmetric(aphi_mesh* mesh, int nrhs, const void* b, int64_t ldb, void* x, int64_t ldx,
+                         const aphi_solve_options* options, aphi_solve_report* report, double* backward_errors);
+
+/* backward_error (include/aphi_solver/factorization.hpp:140, src/factorization.cpp:401-416)
+ * and relative_residual (:315-329) per right-hand side, on the ORIGINAL system. */
+int aphi_backward_error(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx, const void* b, int64_t ldb,
+                        double* backward_errors, double* residuals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APHI_GPU_H */
