@@ -11,6 +11,8 @@ Device arrays are torch tensors (PyTorch is only the allocator here); every
 number is computed by libaphi_amd.so. There is no CPU fallback.
 """
 import ctypes as C
+import sys
+import weakref
 from dataclasses import dataclass
 
 import numpy as np
@@ -102,9 +104,9 @@ class Problem:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
-            lib().aphi_problem_destroy(h)
-            self._h = None
+        if h and not sys.is_finalizing() and _capi._lib is not None:
+            _capi._lib.aphi_problem_destroy(h)
+        self._h = None
 
     @property
     def n(self):
@@ -140,14 +142,18 @@ class Context:
         _check(lib().aphi_context_create(device, C.byref(h)))
         self._h = h
         self.device = device
+        self._meshes = weakref.WeakSet()
 
     def synchronize(self):
         _check(lib().aphi_context_synchronize(self._h))
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().aphi_context_destroy(self._h)
-            self._h = None
+        """Destroys the context; meshes created on it are destroyed first."""
+        for m in list(getattr(self, "_meshes", ())):
+            m.close()
+        if getattr(self, "_h", None) and not sys.is_finalizing() and _capi._lib is not None:
+            _capi._lib.aphi_context_destroy(self._h)
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -169,18 +175,24 @@ class Mesh:
         self.problem = problem  # keeps the described host arrays alive during upload
         desc = problem.describe()
         h = C.c_void_p()
+        if not ctx._h:
+            raise ValueError("Mesh: the context is closed")
         _check(lib().aphi_mesh_create(ctx._h, C.byref(desc), C.byref(h)))
         self._h = h
+        ctx._meshes.add(self)
         self.n = problem.n
         self.nnz = int(problem.counts["nnz"])
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().aphi_mesh_destroy(self._h)
-            self._h = None
+        if getattr(self, "_h", None) and not sys.is_finalizing() and _capi._lib is not None:
+            _capi._lib.aphi_mesh_destroy(self._h)
+        self._h = None
 
     def __del__(self):
         self.close()
+
+    def set_profiling(self, on=True):
+        _check(lib().aphi_mesh_set_profiling(self._h, int(on)))
 
     def device_bytes(self):
         return int(lib().aphi_mesh_device_bytes(self._h))
@@ -220,6 +232,12 @@ class Mesh:
         if status != _capi.APHI_PIVOT_FAILED:
             _check(status)
         return {k: getattr(st, k) for k, _ in _capi.FactorStats._fields_}
+
+    def factor_diagonal(self):
+        """D in factorization order (see aphi_factor_diagonal)."""
+        out = np.zeros(self.n, dtype=np.complex128)
+        _check(lib().aphi_factor_diagonal(self._h, out.ctypes.data_as(_capi.DP)))
+        return out
 
     # solve -- factorization.hpp:110 (device tensors, n x nrhs column-major = (nrhs, n) row-major)
     def solve(self, b, x=None):

@@ -67,7 +67,8 @@ class MeshDesc(C.Structure):
 class FactorStats(C.Structure):
     _fields_ = [("ok", C.c_int), ("columns_done", C.c_int), ("smallest_pivot", C.c_double),
                 ("largest_pivot", C.c_double), ("largest_multiplier", C.c_double), ("nnz", C.c_int64),
-                ("milliseconds", C.c_double), ("flops", C.c_double)]
+                ("milliseconds", C.c_double), ("flops", C.c_double), ("update_launches", C.c_int64),
+                ("update_ms", C.c_double), ("update_flops", C.c_double)]
 
 
 class SolveOptions(C.Structure):
@@ -76,7 +77,8 @@ class SolveOptions(C.Structure):
 
 
 class SolveReport(C.Structure):
-    _fields_ = [("ok", C.c_int), ("refinement_steps", C.c_int), ("backward_error", C.c_double),
+    _fields_ = [("ok", C.c_int), ("refinement_steps", C.c_int), ("precision_used", C.c_int),
+                ("backward_error", C.c_double), ("scaled_backward_error", C.c_double),
                 ("residual", C.c_double), ("equilibration_min", C.c_double), ("equilibration_max", C.c_double),
                 ("smallest_pivot", C.c_double), ("largest_pivot", C.c_double),
                 ("largest_multiplier", C.c_double), ("factor_nnz", C.c_int64), ("columns_done", C.c_int),
@@ -99,6 +101,7 @@ SIGNATURES = {
     "aphi_context_synchronize": (C.c_int, [C.c_void_p]),
     "aphi_mesh_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     "aphi_mesh_destroy": (None, [C.c_void_p]),
+    "aphi_mesh_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "aphi_mesh_device_bytes": (C.c_int64, [C.c_void_p]),
     "aphi_refill": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "aphi_system_download": (C.c_int, [C.c_void_p, DP, DP]),
@@ -106,6 +109,7 @@ SIGNATURES = {
     "aphi_rhs_device": (C.c_void_p, [C.c_void_p]),
     "aphi_equilibrate": (C.c_int, [C.c_void_p, C.c_int, DP]),
     "aphi_factorize": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(FactorStats)]),
+    "aphi_factor_diagonal": (C.c_int, [C.c_void_p, DP]),
     "aphi_solve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64]),
     "aphi_solve_symmetric": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                        C.POINTER(SolveOptions), C.POINTER(SolveReport), DP]),

@@ -8,6 +8,7 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -27,7 +28,16 @@ namespace {
 
 constexpr double kMu0 = 4.0e-7 * 3.14159265358979323846;
 constexpr double kEps0 = 8.8541878128e-12;
-constexpr int kSmallMax = 96;  // fronts up to this many rows are factored in LDS by one workgroup
+constexpr int kSmallMaxLimit = 96;  // LDS bound: fronts up to this many rows fit one workgroup
+
+// Fronts up to this many rows are factored in LDS by one workgroup; the rest take
+// the blocked path. APHI_SMALL_FRONT_MAX (0..96) overrides it, for tuning.
+int small_front_max() {
+    const char* e = std::getenv("APHI_SMALL_FRONT_MAX");
+    if (!e) return kSmallMaxLimit;
+    const int v = std::atoi(e);
+    return v < 0 ? 0 : (v > kSmallMaxLimit ? kSmallMaxLimit : v);
+}
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)
@@ -101,8 +111,10 @@ struct aphi_mesh {
     int solve_nrhs_cap = 0;
     bool assembled = false, equilibrated = false, factored = false;
     int factor_precision = -1;
-    double amax = 0.0, eq_min = 1.0, eq_max = 1.0;
+    double amax = 0.0, amax_eq = 1.0, eq_min = 1.0, eq_max = 1.0;
     aphi_factor_stats last_stats{};
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_events;
 
     aphi_dev::FrontPlan plan() const {
         aphi_dev::FrontPlan fp;
@@ -175,6 +187,15 @@ void equilibrate(aphi_mesh* m, int iterations, double* d_out) {
     m->eq_min = n ? *std::min_element(h.begin(), h.end()) : 1.0;
     m->eq_max = n ? *std::max_element(h.begin(), h.end()) : 1.0;
     if (d_out) std::memcpy(d_out, h.data(), h.size() * sizeof(double));
+    {
+        const int parts = 256;
+        hipLaunchKernelGGL(aphi_dev::k_absmax_partial, dim3(parts), dim3(256), 0, s, m->nnz, m->Aeq.as<cd>(),
+                           m->partials.as<double>());
+        std::vector<double> hp(parts);
+        HIPCHK(hipMemcpyAsync(hp.data(), m->partials.p, parts * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        m->amax_eq = *std::max_element(hp.begin(), hp.end());
+    }
     m->equilibrated = true;
     m->factored = false;
 }
@@ -195,6 +216,17 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     aphi_dev::launch_scatter_panel<T>(s, m->nnz, m->s2p.as<int64_t>(), m->Aeq.as<cd>(), m->P.p);
     const int* nodes = m->nodes.as<int>();
     auto* st = m->stats.as<aphi_dev::FactorStatsDev>();
+    int64_t launches = 0;
+    double update_flops = 0.0;
+    size_t ev_used = 0;
+    const auto next_event = [&]() {
+        if (ev_used == m->prof_events.size()) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            m->prof_events.push_back(e);
+        }
+        return m->prof_events[ev_used++];
+    };
     for (const Level& L : m->levels) {
         if (L.small_count)
             aphi_dev::launch_front_small<T>(s, nodes + L.off, L.small_count, L.small_max_m, fp, m->P.p, m->CB.p, pivot_floor, st);
@@ -205,6 +237,7 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
         for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
         for (int b = 0; b < maxk; b += aphi_dev::kNB) {
             int rb = 0, tiles = 0;
+            double step_flops = 0.0;
             for (int i = 0; i < L.large_count; ++i) {
                 const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
                 if (kk <= b) continue;
@@ -214,8 +247,17 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                 rb = std::max(rb, (rem + 63) / 64);
                 const int t = (rem + 63) / 64;
                 tiles = std::max(tiles, t * (t + 1) / 2);
+                step_flops += 8.0 * nb * (static_cast<double>(rem) * (rem + 1) / 2.0);
             }
-            aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, tiles, fp, m->P.p, m->CB.p, m->mws.p, pivot_floor, st);
+            const bool timed = m->profiling && tiles > 0;
+            aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, 0, fp, m->P.p, m->CB.p, m->mws.p, pivot_floor, st);
+            if (tiles > 0) {
+                if (timed) HIPCHK(hipEventRecord(next_event(), s));
+                aphi_dev::launch_update<T>(s, lg, L.large_count, b, tiles, fp, m->P.p, m->CB.p);
+                if (timed) HIPCHK(hipEventRecord(next_event(), s));
+                ++launches;
+                update_flops += step_flops;
+            }
         }
     }
     HIPCHK(hipGetLastError());
@@ -234,6 +276,17 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     r.nnz = m->predicted_nnz;
     r.milliseconds = ms;
     r.flops = m->flops;
+    r.update_launches = launches;
+    r.update_flops = update_flops;
+    if (m->profiling) {
+        double total = 0.0;
+        for (size_t e = 0; e + 1 < ev_used; e += 2) {
+            float t = 0.0f;
+            HIPCHK(hipEventElapsedTime(&t, m->prof_events[e], m->prof_events[e + 1]));
+            total += t;
+        }
+        r.update_ms = total;
+    }
     *out = r;
     m->last_stats = r;
     m->factored = r.ok != 0;
@@ -248,7 +301,7 @@ void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kNB * nrhs * cs);
     m->colscale.reserve(nrhs * sizeof(double));
     m->res.reserve(static_cast<size_t>(m->n) * nrhs * sizeof(cd));
-    m->norms.reserve(static_cast<size_t>(64) * nrhs * 3 * sizeof(double));
+    m->norms.reserve(static_cast<size_t>(64) * nrhs * 6 * sizeof(double));
     m->partials.reserve(std::max<size_t>(m->partials.bytes, static_cast<size_t>(64) * nrhs * sizeof(double)));
     m->solve_nrhs_cap = nrhs;
 }
@@ -308,8 +361,10 @@ void solve_any(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t 
     else solve_t<float>(m, nrhs, b, ldb, x, ldx, accumulate);
 }
 
-// backward error and relative residual per RHS; leaves r = b - A x in m->res
-void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int64_t ldb, double* be, double* rr) {
+// backward error and relative residual per RHS, and (when bes is given) the
+// backward error of the equilibrated system; leaves r = b - A x in m->res
+void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int64_t ldb, double* be, double* rr,
+             double* bes = nullptr) {
     hipStream_t s = m->st();
     const int n = m->n;
     ensure_solve_buffers(m, nrhs);
@@ -324,7 +379,17 @@ void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int6
     hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, res, static_cast<int64_t>(n), nrm);
     hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, x, ldx, nrm + parts * nrhs);
     hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, b, ldb, nrm + 2 * parts * nrhs);
-    std::vector<double> h(static_cast<size_t>(3 * parts * nrhs));
+    const bool scaled = bes != nullptr && m->equilibrated;
+    if (scaled) {
+        const double* dd = m->d.as<double>();
+        hipLaunchKernelGGL(aphi_dev::k_sumsq_weighted_partial, dim3(parts, nrhs), dim3(256), 0, s, n, res,
+                           static_cast<int64_t>(n), dd, 1, nrm + 3 * parts * nrhs);
+        hipLaunchKernelGGL(aphi_dev::k_sumsq_weighted_partial, dim3(parts, nrhs), dim3(256), 0, s, n, x, ldx, dd, -1,
+                           nrm + 4 * parts * nrhs);
+        hipLaunchKernelGGL(aphi_dev::k_sumsq_weighted_partial, dim3(parts, nrhs), dim3(256), 0, s, n, b, ldb, dd, 1,
+                           nrm + 5 * parts * nrhs);
+    }
+    std::vector<double> h(static_cast<size_t>((scaled ? 6 : 3) * parts * nrhs));
     HIPCHK(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
@@ -338,6 +403,18 @@ void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int6
         const double num = std::sqrt(sr), den = m->amax * std::sqrt(sx) + std::sqrt(sb);
         if (be) be[r] = den > 0.0 ? num / den : num;
         if (rr) rr[r] = sb > 0.0 ? num / std::sqrt(sb) : num;
+        if (scaled) {
+            double tr = 0, tx = 0, tb = 0;
+            for (int p = 0; p < parts; ++p) {
+                tr += h[static_cast<size_t>(3 * parts * nrhs + r * parts + p)];
+                tx += h[static_cast<size_t>(4 * parts * nrhs + r * parts + p)];
+                tb += h[static_cast<size_t>(5 * parts * nrhs + r * parts + p)];
+            }
+            const double dn = m->amax_eq * std::sqrt(tx) + std::sqrt(tb);
+            bes[r] = dn > 0.0 ? std::sqrt(tr) / dn : std::sqrt(tr);
+        } else if (bes) {
+            bes[r] = be ? be[r] : 0.0;
+        }
     }
 }
 
@@ -500,11 +577,12 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
         upload(m->child, ch.data(), ch.size());
         m->levels.assign(static_cast<size_t>(d->num_levels), Level{});
         std::vector<std::vector<int>> small(static_cast<size_t>(d->num_levels)), large(static_cast<size_t>(d->num_levels));
+        const int small_max = small_front_max();
         for (int s = 0; s < ns; ++s) {
             const int L = d->super_level[s];
             if (L < 0 || L >= d->num_levels) throw std::invalid_argument("aphi_mesh_create: supernode level out of range");
             const int64_t mm = d->super_row_ptr[s + 1] - d->super_row_ptr[s];
-            (mm <= kSmallMax ? small : large)[static_cast<size_t>(L)].push_back(s);
+            (mm <= small_max ? small : large)[static_cast<size_t>(L)].push_back(s);
         }
         std::vector<int> node_list, ea;
         for (int L = 0; L < d->num_levels; ++L) {
@@ -544,7 +622,15 @@ extern "C" void aphi_mesh_destroy(aphi_mesh* m) {
     if (!m) return;
     (void)hipSetDevice(m->ctx->device);
     (void)hipStreamSynchronize(m->ctx->stream);
+    for (hipEvent_t e : m->prof_events) (void)hipEventDestroy(e);
     delete m;
+}
+
+extern "C" int aphi_mesh_set_profiling(aphi_mesh* m, int on) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_mesh_set_profiling: null mesh");
+        m->profiling = on != 0;
+    });
 }
 
 extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
@@ -672,10 +758,43 @@ extern "C" int aphi_factorize(aphi_mesh* m, int precision, double pivot_floor, a
     });
 }
 
+extern "C" int aphi_factor_diagonal(aphi_mesh* m, double* out) {
+    return aphi_capi::guarded([&] {
+        require(m && out, "aphi_factor_diagonal: null argument");
+        require(m->factor_precision >= 0 && m->P.p, "aphi_factor_diagonal: nothing factored");
+        // D sits on each panel's diagonal: column c of supernode s at panel_off[s] + c*m + c
+        std::vector<int> first(static_cast<size_t>(m->num_super) + 1);
+        std::vector<int64_t> rp(static_cast<size_t>(m->num_super) + 1), po(static_cast<size_t>(m->num_super) + 1);
+        HIPCHK(hipMemcpy(first.data(), m->first.p, first.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(rp.data(), m->srow_ptr.p, rp.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(po.data(), m->panel_off.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+        const size_t cs = m->factor_precision == APHI_FP64 ? sizeof(cd) : sizeof(aphi_dev::cf);
+        std::vector<unsigned char> host(static_cast<size_t>(m->factor_entries) * cs);
+        HIPCHK(hipStreamSynchronize(m->st()));
+        HIPCHK(hipMemcpy(host.data(), m->P.p, host.size(), hipMemcpyDeviceToHost));
+        for (int s = 0; s < m->num_super; ++s) {
+            const int64_t mm = rp[static_cast<size_t>(s) + 1] - rp[static_cast<size_t>(s)];
+            for (int c = 0; c < first[static_cast<size_t>(s) + 1] - first[static_cast<size_t>(s)]; ++c) {
+                const int64_t at = po[static_cast<size_t>(s)] + c * mm + c;
+                const int j = first[static_cast<size_t>(s)] + c;
+                if (m->factor_precision == APHI_FP64) {
+                    const cd* v = reinterpret_cast<const cd*>(host.data()) + at;
+                    out[2 * j] = v->re;
+                    out[2 * j + 1] = v->im;
+                } else {
+                    const aphi_dev::cf* v = reinterpret_cast<const aphi_dev::cf*>(host.data()) + at;
+                    out[2 * j] = v->re;
+                    out[2 * j + 1] = v->im;
+                }
+            }
+        }
+    });
+}
+
 extern "C" int aphi_solve(aphi_mesh* m, int nrhs, const void* b, int64_t ldb, void* x, int64_t ldx) {
     return aphi_capi::guarded([&] {
         require(m && b && x, "aphi_solve: null argument");
-        require(nrhs >= 1 && ldb >= m->n && ldx >= m->n, "aphi_solve: bad nrhs or leading dimension");
+        require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_solve: nrhs must be 1..64 and ld >= n");
         solve_any(m, nrhs, static_cast<const cd*>(b), ldb, static_cast<cd*>(x), ldx, 0);
         HIPCHK(hipStreamSynchronize(m->st()));
     });
@@ -686,7 +805,7 @@ extern "C" int aphi_backward_error(aphi_mesh* m, int nrhs, const void* x, int64_
     return aphi_capi::guarded([&] {
         require(m && x && b, "aphi_backward_error: null argument");
         require(m->assembled, "aphi_backward_error: nothing assembled");
-        require(nrhs >= 1 && ldb >= m->n && ldx >= m->n, "aphi_backward_error: bad nrhs or leading dimension");
+        require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_backward_error: nrhs must be 1..64 and ld >= n");
         measure(m, nrhs, static_cast<const cd*>(x), ldx, static_cast<const cd*>(b), ldb, be, rr);
     });
 }
@@ -695,7 +814,7 @@ extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int6
                                     const aphi_solve_options* opt, aphi_solve_report* rep, double* backward_errors) {
     return aphi_capi::guarded([&] {
         require(m && bv && xv && opt && rep, "aphi_solve_symmetric: null argument");
-        require(nrhs >= 1 && ldb >= m->n && ldx >= m->n, "aphi_solve_symmetric: bad nrhs or leading dimension");
+        require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_solve_symmetric: nrhs must be 1..64 and ld >= n");
         require(opt->max_refinements >= 0, "aphi_solve_symmetric: max_refinements must be >= 0");
         const auto t0 = std::chrono::steady_clock::now();
         const cd* b = static_cast<const cd*>(bv);
@@ -718,19 +837,32 @@ extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int6
         }
         const auto s0 = std::chrono::steady_clock::now();
         solve_any(m, nrhs, b, ldb, x, ldx, 0);
-        std::vector<double> be(static_cast<size_t>(nrhs)), rr(static_cast<size_t>(nrhs));
-        int steps = 0;
+        std::vector<double> be(static_cast<size_t>(nrhs)), rr(static_cast<size_t>(nrhs)), bs(static_cast<size_t>(nrhs));
+        int steps = 0, since_factor = 0;
+        r.precision_used = opt->precision;
         for (;;) {
-            measure(m, nrhs, x, ldx, b, ldb, be.data(), rr.data());
+            measure(m, nrhs, x, ldx, b, ldb, be.data(), rr.data(), bs.data());
             if (opt->max_refinements == 0) break;
-            const double worst = *std::max_element(be.begin(), be.end());
-            if (worst <= opt->tolerance || steps >= opt->max_refinements) break;
+            const double worst = std::max(*std::max_element(be.begin(), be.end()), *std::max_element(bs.begin(), bs.end()));
+            if (worst <= opt->tolerance) break;
+            if (since_factor >= opt->max_refinements) {
+                if (m->factor_precision != APHI_FP32) break;
+                // FP32 refinement did not converge: factor in FP64 and keep refining.
+                aphi_factor_stats f64{};
+                factorize(m, APHI_FP64, opt->pivot_floor, &f64);
+                r.factorize_ms += f64.milliseconds;
+                r.precision_used = APHI_FP64;
+                since_factor = 0;
+                if (!f64.ok) break;
+            }
             solve_any(m, nrhs, m->res.as<cd>(), m->n, x, ldx, 1);
             ++steps;
+            ++since_factor;
         }
         const auto s1 = std::chrono::steady_clock::now();
         r.refinement_steps = steps;
         r.backward_error = *std::max_element(be.begin(), be.end());
+        r.scaled_backward_error = *std::max_element(bs.begin(), bs.end());
         r.residual = *std::max_element(rr.begin(), rr.end());
         r.solve_ms = std::chrono::duration<double, std::milli>(s1 - s0).count();
         r.total_ms = std::chrono::duration<double, std::milli>(s1 - t0).count();

@@ -217,6 +217,21 @@ extern "C" int aphi_problem_get_int(const aphi_problem* p, const char* name, int
             return copy_out(p->sp.level, out, capacity, count);
         } else if (k == "factor_perm") {
             return copy_out(p->sp.perm, out, capacity, count);
+        } else if (k == "tet_dofs") {
+            for (int t = 0; t < m.num_tets(); ++t) {
+                const TetDofs d = p->dofs.local_dofs(t, m, p->bound);
+                for (int i = 0; i < 6; ++i) tmp.push_back(d.edge_index[i]);
+                for (int i = 0; i < 10; ++i) tmp.push_back(d.phi_index[i]);
+            }
+        } else if (k == "body_ptr") {
+            tmp.push_back(0);
+            for (const Body& b : p->bound.bodies) tmp.push_back(tmp.back() + static_cast<int64_t>(b.tets.size()));
+        } else if (k == "body_tets") {
+            for (const Body& b : p->bound.bodies) tmp.insert(tmp.end(), b.tets.begin(), b.tets.end());
+        } else if (k == "col_ref") {
+            return copy_out(p->col_ref, out, capacity, count);
+        } else if (k == "super_rel") {
+            return copy_out(p->sp.rel, out, capacity, count);
         } else if (k == "slot_to_panel") {
             return copy_out(p->s2p, out, capacity, count);
         } else {
@@ -231,10 +246,25 @@ extern "C" int aphi_problem_get_double(const aphi_problem* p, const char* name, 
     return aphi_capi::guarded([&] {
         if (!p || !name || !count || (capacity > 0 && !out)) throw std::invalid_argument("aphi_problem_get_double: null argument");
         const std::string k(name);
-        if (k != "coords") throw std::invalid_argument("aphi_problem_get_double: unknown array '" + k + "'");
-        *count = static_cast<int64_t>(p->coords.size());
+        std::vector<double> tmp;
+        const std::vector<double>* src = &tmp;
+        if (k == "coords") {
+            src = &p->coords;
+        } else if (k == "tet_phi_fixed") {
+            for (int t = 0; t < p->mesh.num_tets(); ++t) {
+                const TetDofs d = p->dofs.local_dofs(t, p->mesh, p->bound);
+                for (int i = 0; i < 10; ++i) { tmp.push_back(d.phi_fixed[i].real()); tmp.push_back(d.phi_fixed[i].imag()); }
+            }
+        } else if (k == "body_param") {
+            for (const Body& b : p->bound.bodies) { tmp.push_back(b.sigma); tmp.push_back(b.eps_r); tmp.push_back(b.mu_r); }
+        } else if (k == "port_value") {
+            for (const Complex& v : p->dofs.port_value) { tmp.push_back(v.real()); tmp.push_back(v.imag()); }
+        } else {
+            throw std::invalid_argument("aphi_problem_get_double: unknown array '" + k + "'");
+        }
+        *count = static_cast<int64_t>(src->size());
         const int64_t n = std::min<int64_t>(capacity, *count);
-        std::memcpy(out, p->coords.data(), static_cast<size_t>(n) * sizeof(double));
+        if (n > 0) std::memcpy(out, src->data(), static_cast<size_t>(n) * sizeof(double));
     });
 }
 

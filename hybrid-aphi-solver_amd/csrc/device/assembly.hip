@@ -370,6 +370,29 @@ __global__ void __launch_bounds__(256) k_sumsq_partial(int n, const cd* __restri
     if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = red[0];
 }
 
+// Per-RHS sums of (d_i |v_i|)^2 (power = 1) or (|v_i| / d_i)^2 (power = -1): the
+// norms of the equilibrated system's residual, solution and right-hand side.
+__global__ void __launch_bounds__(256) k_sumsq_weighted_partial(int n, const cd* __restrict__ v, int64_t ldv,
+                                                                const double* __restrict__ d, int power,
+                                                                double* __restrict__ partials) {
+    __shared__ double red[256];
+    const int c = blockIdx.y;
+    const cd* col = v + c * ldv;
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double w = power > 0 ? d[i] : 1.0 / d[i];
+        const double re = col[i].re * w, im = col[i].im * w;
+        s += re * re + im * im;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = red[0];
+}
+
 __global__ void __launch_bounds__(256) k_absmax_partial(int64_t n, const cd* __restrict__ v, double* __restrict__ partials) {
     __shared__ double red[256];
     double s = 0.0;

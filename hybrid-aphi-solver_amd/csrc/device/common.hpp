@@ -77,4 +77,12 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
     atomicMax(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// Opt a kernel into more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
+template <typename K>
+inline void allow_large_lds(K kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(bytes));
+}
+
 }  // namespace aphi_dev

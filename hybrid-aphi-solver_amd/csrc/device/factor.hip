@@ -326,17 +326,22 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
 
 template <typename T> struct Mfma;
 
+// Operands: lane l supplies A[l%16][l/16] and B[l/16][l%16] for both types. The
+// accumulator layouts differ (measured on gfx950): register v of lane l holds
+// D[l/16 + 4v][l%16] for f64 and D[4(l/16) + v][l%16] for f32.
 template <> struct Mfma<double> {
     typedef double v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return lane / 16 + 4 * v; }
 };
 template <> struct Mfma<float> {
     typedef float v4 __attribute__((ext_vector_type(4)));
     static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
 };
 
 template <typename T>
@@ -420,7 +425,7 @@ __global__ void __launch_bounds__(256) k_update(const int* __restrict__ nodes, i
                 }
         }
     }
-    // D[p][q]: lane holds p = 4*(lane/16)+v (j), q = lane%16 (i)
+    // D[p][q]: lane holds p = Mfma<T>::row(lane, v) (j), q = lane%16 (i)
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -428,7 +433,7 @@ __global__ void __launch_bounds__(256) k_update(const int* __restrict__ nodes, i
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wi + x * 16 + (lane % 16);
-                const int j = j0 + wj + y * 16 + 4 * (lane / 16) + v;
+                const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
                 if (i < fr.m && j < fr.m && i >= j) {
                     C<T>* dst = fr.at(i, j);
                     C<T> cur = *dst;
@@ -464,6 +469,7 @@ template <typename T>
 void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, const FrontPlan& fp, void* P, void* CB,
                         double pivot_floor, FactorStatsDev* stats) {
     if (count <= 0) return;
+    allow_large_lds(k_front_small<T>, small_lds_bytes<T>(max_m));
     hipLaunchKernelGGL(k_front_small<T>, dim3(count), dim3(256), small_lds_bytes<T>(max_m), st, nodes, fp,
                        static_cast<C<T>*>(P), static_cast<C<T>*>(CB), pivot_floor, stats);
 }
@@ -480,16 +486,23 @@ template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, double pivot_floor, FactorStatsDev* stats) {
     const size_t diag_lds = static_cast<size_t>(2 * kNB * kNB + kNB) * sizeof(C<T>);
+    allow_large_lds(k_diag<T>, diag_lds);
     hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(256), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
                        static_cast<C<T>*>(CB), static_cast<C<T>*>(mws), pivot_floor, stats);
     if (max_row_blocks > 0) {
         const size_t panel_lds = static_cast<size_t>(64 * kNB + kNB * kNB) * sizeof(C<T>);
+        allow_large_lds(k_panel<T>, panel_lds);
         hipLaunchKernelGGL(k_panel<T>, dim3(max_row_blocks, count), dim3(256), panel_lds, st, nodes, b, fp,
                            static_cast<C<T>*>(P), static_cast<C<T>*>(CB), static_cast<const C<T>*>(mws), stats);
     }
-    if (max_tiles > 0)
-        hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(256), 0, st, nodes, b, fp, static_cast<C<T>*>(P),
-                           static_cast<C<T>*>(CB));
+    if (max_tiles > 0) launch_update<T>(st, nodes, count, b, max_tiles, fp, P, CB);
+}
+
+template <typename T>
+void launch_update(hipStream_t st, const int* nodes, int count, int b, int max_tiles, const FrontPlan& fp, void* P,
+                   void* CB) {
+    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(256), 0, st, nodes, b, fp, static_cast<C<T>*>(P),
+                       static_cast<C<T>*>(CB));
 }
 
 template void launch_zero<double>(hipStream_t, int64_t, void*);
@@ -504,6 +517,8 @@ template void launch_front_small<float>(hipStream_t, const int*, int, int, const
                                         FactorStatsDev*);
 template void launch_extend_add<double>(hipStream_t, const int*, const int*, int, int, const FrontPlan&, void*, void*);
 template void launch_extend_add<float>(hipStream_t, const int*, const int*, int, int, const FrontPlan&, void*, void*);
+template void launch_update<double>(hipStream_t, const int*, int, int, int, const FrontPlan&, void*, void*);
+template void launch_update<float>(hipStream_t, const int*, int, int, int, const FrontPlan&, void*, void*);
 template void launch_block_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,
                                         void*, double, FactorStatsDev*);
 template void launch_block_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,

@@ -20,6 +20,9 @@ template <typename T>
 void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
                        const FrontPlan& fp, void* P, void* CB);
 template <typename T>
+void launch_update(hipStream_t st, const int* nodes, int count, int b, int max_tiles, const FrontPlan& fp, void* P,
+                   void* CB);
+template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, double pivot_floor, FactorStatsDev* stats);
 

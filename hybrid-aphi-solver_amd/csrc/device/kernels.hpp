@@ -63,6 +63,7 @@ __global__ void k_symv(int n, int nrhs, const int* row_ptr, const int* col, cons
 __global__ void k_residual(int n, int nrhs, const cd* b, int64_t ldb, cd* r, int64_t ldr);
 __global__ void k_sumsq_partial(int n, const cd* v, int64_t ldv, double* partials);
 __global__ void k_absmax_partial(int64_t n, const cd* v, double* partials);
+__global__ void k_sumsq_weighted_partial(int n, const cd* v, int64_t ldv, const double* d, int power, double* partials);
 __global__ void k_scale_permute_in(int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
                                    cd* out, int64_t ldo);
 __global__ void k_permute_scale_out(int n, int nrhs, const int* perm, const double* d, const cd* xt, int64_t ldt,

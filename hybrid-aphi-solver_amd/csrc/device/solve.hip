@@ -284,6 +284,7 @@ void launch_fwd_step(hipStream_t st, const int* nodes, int count, int b, int max
                      const FrontPlan& fp, const void* P, void* W) {
     if (count <= 0) return;
     const size_t lds = static_cast<size_t>(kNB * kNB + kNB * nrhs) * sizeof(C<T>);
+    allow_large_lds(k_fwd_diag<T>, lds);
     hipLaunchKernelGGL(k_fwd_diag<T>, dim3(count), dim3(256), lds, st, nodes, b, nrhs, fp, static_cast<const C<T>*>(P),
                        static_cast<C<T>*>(W));
     if (max_row_blocks > 0)
@@ -298,10 +299,12 @@ void launch_bwd_step(hipStream_t st, const int* nodes, int count, int b, int max
     const int stride = max_row_blocks > 0 ? max_row_blocks : 1;
     if (max_row_blocks > 0) {
         const size_t lds = static_cast<size_t>(kNB * 64 + nrhs * 64) * sizeof(C<T>);
+        allow_large_lds(k_bwd_partial<T>, lds);
         hipLaunchKernelGGL(k_bwd_partial<T>, dim3(max_row_blocks, count), dim3(256), lds, st, nodes, b, nrhs, stride, fp,
                            static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(partial));
     }
     const size_t lds = static_cast<size_t>(kNB * kNB + kNB * nrhs) * sizeof(C<T>);
+    allow_large_lds(k_bwd_diag<T>, lds);
     hipLaunchKernelGGL(k_bwd_diag<T>, dim3(count), dim3(256), lds, st, nodes, b, nrhs, stride, fp,
                        static_cast<const C<T>*>(P), static_cast<const C<T>*>(W), static_cast<C<T>*>(X), n,
                        static_cast<const C<T>*>(partial));

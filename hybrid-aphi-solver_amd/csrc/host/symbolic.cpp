@@ -331,10 +331,10 @@ SupernodePlan build_supernodes(const SolverAnalysis& an, const SupernodeOptions&
     for (int s = 0; s < ns; ++s) sp.num_levels = std::max(sp.num_levels, sp.level[static_cast<size_t>(s)] + 1);
     sp.factor_entries = static_cast<size_t>(sp.panel_offset[static_cast<size_t>(ns)]);
 
-    // 7. Contribution-block workspace. A block is written when its supernode is
-    // factored (time 2*level+1) and read when the parent assembles (time
-    // 2*level(parent)); blocks whose lifetimes overlap get disjoint ranges. A
-    // simulated first-fit allocator over the level schedule assigns offsets.
+    // 7. Contribution-block workspace. A block lives from its own level to its
+    // parent's level, both inclusive; blocks whose lifetimes overlap get disjoint
+    // ranges. A simulated first-fit allocator over the level schedule assigns
+    // the offsets.
     sp.cb_offset.assign(static_cast<size_t>(ns), -1);
     {
         std::vector<std::vector<int>> born(static_cast<size_t>(sp.num_levels)), dies(static_cast<size_t>(sp.num_levels));
@@ -353,25 +353,9 @@ SupernodePlan build_supernodes(const SolverAnalysis& an, const SupernodeOptions&
             return ((c * c + 63) / 64) * 64;  // keep every block 1 KiB-aligned
         };
         for (int L = 0; L < sp.num_levels; ++L) {
-            // Blocks consumed by level L's assembly free up before level L's factorization.
-            for (int s : dies[static_cast<size_t>(L)]) {
-                int64_t off = sp.cb_offset[static_cast<size_t>(s)], sz = size_of(s);
-                if (sz == 0) continue;
-                auto it = free_blocks.emplace(off, sz).first;
-                if (it != free_blocks.begin()) {
-                    auto pv = std::prev(it);
-                    if (pv->first + pv->second == it->first) {
-                        pv->second += it->second;
-                        free_blocks.erase(it);
-                        it = pv;
-                    }
-                }
-                auto nx = std::next(it);
-                if (nx != free_blocks.end() && it->first + it->second == nx->first) {
-                    it->second += nx->second;
-                    free_blocks.erase(nx);
-                }
-            }
+            // Blocks written at level L are placed before the blocks level L reads
+            // are released: within one level the writes of some fronts overlap the
+            // reads of others, so a lifetime covers both of its end levels.
             for (int s : born[static_cast<size_t>(L)]) {
                 const int64_t sz = size_of(s);
                 int64_t off = -1;
@@ -400,6 +384,24 @@ SupernodePlan build_supernodes(const SolverAnalysis& an, const SupernodeOptions&
                     }
                 }
                 sp.cb_offset[static_cast<size_t>(s)] = off;
+            }
+            for (int s : dies[static_cast<size_t>(L)]) {
+                int64_t off = sp.cb_offset[static_cast<size_t>(s)], sz = size_of(s);
+                if (sz == 0) continue;
+                auto it = free_blocks.emplace(off, sz).first;
+                if (it != free_blocks.begin()) {
+                    auto pv = std::prev(it);
+                    if (pv->first + pv->second == it->first) {
+                        pv->second += it->second;
+                        free_blocks.erase(it);
+                        it = pv;
+                    }
+                }
+                auto nx = std::next(it);
+                if (nx != free_blocks.end() && it->first + it->second == nx->first) {
+                    it->second += nx->second;
+                    free_blocks.erase(nx);
+                }
             }
         }
         sp.cb_pool_entries = top;

@@ -91,10 +91,11 @@ int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out);
  * tet_edge_signs, tet_body, dirichlet_edge, tree_edge, edge_index, phi_index,
  * port_index, row_ptr, col_index, perm, iperm, parent, row_count, col_count,
  * scatter_live, scatter_pos, scatter_offset, scatter_slot, super_first,
- * super_parent, super_row_ptr, super_rows, super_level, factor_perm,
- * slot_to_panel. */
+ * super_parent, super_row_ptr, super_rows, super_rel, super_level, factor_perm, col_ref,
+ * slot_to_panel, tet_dofs (6 edge + 10 phi per tet), body_ptr, body_tets. */
 int aphi_problem_get_int(const aphi_problem* p, const char* name, int64_t* out, int64_t capacity, int64_t* count);
-/* Named double arrays: coords (3 per node). */
+/* Named double arrays: coords (3 per node), tet_phi_fixed (10 complex per tet),
+ * body_param (sigma, eps_r, mu_r per body), port_value (complex per port). */
 int aphi_problem_get_double(const aphi_problem* p, const char* name, double* out, int64_t capacity, int64_t* count);
 /* The log sweep of src/input_file.cpp:19-50. */
 int aphi_frequency_sweep(double start, double stop, int points, int logarithmic, double* out);
@@ -147,12 +148,16 @@ typedef struct aphi_context aphi_context;
 typedef struct aphi_mesh aphi_mesh;
 
 int aphi_context_create(int device, aphi_context** out);
+/* Meshes created on a context must be destroyed before it. */
 void aphi_context_destroy(aphi_context* ctx);
 int aphi_context_synchronize(aphi_context* ctx);
 
 /* Uploads one mesh's plan (once per mesh; reused at every frequency). */
 int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* desc, aphi_mesh** out);
 void aphi_mesh_destroy(aphi_mesh* mesh);
+/* Per-launch event timing of the factorization's trailing-update kernel (adds
+ * synchronisation-free event records around each launch; off by default). */
+int aphi_mesh_set_profiling(aphi_mesh* mesh, int on);
 /* Device bytes currently held by this mesh (plan + numeric buffers). */
 int64_t aphi_mesh_device_bytes(const aphi_mesh* mesh);
 
@@ -180,10 +185,23 @@ typedef struct aphi_factor_stats {
     int columns_done;
     double smallest_pivot, largest_pivot, largest_multiplier;
     int64_t nnz;
-    double milliseconds;
-    double flops;
+    double milliseconds;           /* whole factorization, HIP events on the library stream */
+    double flops;                  /* algorithmic: 8*sum_j [m_j(m_j+1)/2 + 2 m_j] */
+    /* Filled when profiling is on (aphi_mesh_set_profiling): the MFMA trailing
+     * update, timed launch by launch with HIP events, and the real flops those
+     * launches perform (8 per complex multiply-add over the lower triangles). */
+    int64_t update_launches;
+    double update_ms;
+    double update_flops;
 } aphi_factor_stats;
 int aphi_factorize(aphi_mesh* mesh, int precision, double pivot_floor, aphi_factor_stats* stats);
+
+/* D of the current factor, in factorization order (2*n doubles). Column j of the
+ * factorization order is column col_ref[j] of the reference's: LDL^T without
+ * pivoting is unique for a given elimination tree postorder, so this is the
+ * reference's SymmetricFactor::diagonal (include/aphi_solver/factorization.hpp:41)
+ * permuted by col_ref. */
+int aphi_factor_diagonal(aphi_mesh* mesh, double* out);
 
 /* solve (include/aphi_solver/factorization.hpp:110) for nrhs right-hand sides
  * with the current factor, undoing the equilibration: x = D * solve(D * b).
@@ -192,8 +210,11 @@ int aphi_solve(aphi_mesh* mesh, int nrhs, const void* b, int64_t ldb, void* x, i
 
 /* solve_symmetric (include/aphi_solver/factorization.hpp:206) extended with FP64
  * iterative refinement: equilibrate, factor (FP64 or FP32), solve, then refine
- * with the FP64 residual of the ORIGINAL system until every right-hand side's
- * normwise backward error is <= tolerance. */
+ * with the FP64 residual of the ORIGINAL system until, for every right-hand side,
+ * both the reference's normwise backward error and the same measure on the
+ * equilibrated system are <= tolerance. An FP32 factor whose refinement has not
+ * converged after max_refinements steps is replaced by an FP64 factor and the
+ * refinement continues (report->precision_used says which finished). */
 typedef struct aphi_solve_options {
     int precision;                 /* aphi_precision */
     int equilibration_iterations;  /* 10 in the reference; 0 disables */
@@ -205,7 +226,9 @@ typedef struct aphi_solve_options {
 typedef struct aphi_solve_report {
     int ok;
     int refinement_steps;
-    double backward_error;         /* max over right-hand sides */
+    int precision_used;            /* FP32 falls back to FP64 when refinement stalls */
+    double backward_error;         /* max over right-hand sides (reference definition) */
+    double scaled_backward_error;  /* the same measure on the equilibrated system D A D */
     double residual;               /* max ||Ax-b||/||b|| */
     double equilibration_min, equilibration_max;
     double smallest_pivot, largest_pivot, largest_multiplier;

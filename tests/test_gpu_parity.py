@@ -1,0 +1,180 @@
+"""Device path against the oracle, through the C-ABI: refill bit for bit,
+equilibration, the factor's D (unique for the ordering), solutions of the plain
+FP64 path (relative error <= 1e-10) and of the FP32 + FP64
+refinement path (backward error <= 1e-12), pivot refusal, determinism."""
+import numpy as np
+import pytest
+
+from conftest import cube_config, golden
+
+import aphi_amd as A
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+NUMERIC = ["cube2", "cube3_offcentre", "config1", "cube8"]
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).cuda()
+
+
+@pytest.fixture(scope="module", params=NUMERIC)
+def case(request, gpu_context):
+    meta, _ = golden(request.param)
+    p = A.Problem(cube_config(meta))
+    m = A.Mesh(gpu_context, p)
+    vals, rhs = O.refill(p, meta["omega"])
+    x, D, st = O.solve_symmetric(p, vals, rhs)
+    yield dict(meta=meta, p=p, m=m, vals=vals, rhs=rhs, x=x, D=D, st=st)
+    m.close()
+
+
+def test_refill_bitwise_equals_oracle(case):
+    m = case["m"]
+    m.refill(case["meta"]["omega"])
+    vals, rhs = m.system()
+    assert np.array_equal(vals.view(np.uint64), case["vals"].view(np.uint64))
+    assert np.array_equal(rhs.view(np.uint64), case["rhs"].view(np.uint64))
+
+
+def test_refill_scaled_phi_bitwise(case):
+    m, p = case["m"], case["p"]
+    omega = 2 * np.pi * 3.3e5
+    m.refill(omega, A.APHI_SCALED_PHI)
+    vals, rhs = m.system()
+    ov, orhs = O.refill(p, omega, conditioning=2)
+    assert np.array_equal(vals.view(np.uint64), ov.view(np.uint64))
+    assert np.array_equal(rhs.view(np.uint64), orhs.view(np.uint64))
+
+
+def test_refill_refusals(case):
+    m = case["m"]
+    with pytest.raises(ValueError):
+        m.refill(100.0, A.APHI_NATURAL)
+    with pytest.raises(ValueError):
+        m.refill(0.0, A.APHI_ROW_SCALED)
+
+
+def test_equilibration_matches_oracle(case):
+    m, p = case["m"], case["p"]
+    m.refill(case["meta"]["omega"])
+    d = m.equilibrate(10)
+    od = O.equilibrate(p, case["vals"], 10)
+    assert np.abs(d / od - 1).max() < 1e-14
+
+
+@pytest.mark.parametrize("precision,tol", [(A.APHI_FP64, 1e-8), (A.APHI_FP32, 1e-2)])
+def test_factor_diagonal_matches_oracle(case, precision, tol):
+    """LDL^T without pivoting is unique for the ordering; the factorization order is
+    a postorder of the reference's elimination tree, so D is the oracle's D permuted.
+    FP64: normwise to 1e-10 of max|D| and every pivot to 1e-8 relative. FP32:
+    normwise to 1e-2 of max|D| (small pivots lose digits to cancellation in single
+    precision; the refinement recovers the solution, checked below)."""
+    m, p = case["m"], case["p"]
+    m.refill(case["meta"]["omega"])
+    m.equilibrate(10)
+    st = m.factorize_ldlt(precision)
+    assert st["ok"]
+    D = m.factor_diagonal()
+    expect = case["D"][p.ints("col_ref")]
+    o = case["st"]
+    if precision == A.APHI_FP64:
+        assert np.abs(D - expect).max() / np.abs(expect).max() < 1e-10
+        assert np.max(np.abs(D - expect) / np.abs(expect)) < tol
+        for k in ("smallest_pivot", "largest_pivot", "largest_multiplier"):
+            assert abs(st[k] / o[k] - 1) < tol, k
+    else:
+        assert np.abs(D - expect).max() / np.abs(expect).max() < tol
+        assert abs(st["largest_pivot"] / o["largest_pivot"] - 1) < tol
+        assert abs(st["largest_multiplier"] / o["largest_multiplier"] - 1) < 1e-2
+    assert st["nnz"] == p.counts["predicted_nnz"]
+
+
+def test_fp64_solution_matches_oracle(case):
+    m = case["m"]
+    m.refill(case["meta"]["omega"])
+    x, rep = m.solve_symmetric(dev(case["rhs"]), precision=A.APHI_FP64)
+    xg = x.cpu().numpy()
+    assert rep["ok"]
+    assert np.linalg.norm(xg - case["x"]) / np.linalg.norm(case["x"]) <= 1e-10
+    assert rep["backward_error"] < 1e-14
+
+
+@pytest.mark.parametrize("tol,err", [(1e-12, 1e-5), (1e-15, 1e-10)])
+def test_fp32_refined_solution(case, tol, err):
+    """FP32 factor + FP64 refinement. At the north-star tolerance (backward error
+    1e-12) the solution error against complex128 is bounded by the conditioning
+    (reported, here <= 1e-5); refined to 1e-15 it matches the plain FP64 bar."""
+    m = case["m"]
+    m.refill(case["meta"]["omega"])
+    x, rep = m.solve_symmetric(dev(case["rhs"]), precision=A.APHI_FP32, max_refinements=40, tolerance=tol)
+    assert rep["ok"] and rep["backward_error"] <= tol and rep["scaled_backward_error"] <= tol
+    assert np.linalg.norm(x.cpu().numpy() - case["x"]) / np.linalg.norm(case["x"]) <= err
+
+
+def test_multi_rhs_matches_oracle(case):
+    m, p = case["m"], case["p"]
+    m.refill(case["meta"]["omega"])
+    rng = np.random.default_rng(17)
+    B = rng.normal(size=(8, p.n)) + 1j * rng.normal(size=(8, p.n))
+    X, rep = m.solve_symmetric(dev(B), precision=A.APHI_FP64)
+    ox, _, _ = O.solve_symmetric(p, case["vals"], B)
+    Xg = X.cpu().numpy()
+    for r in range(8):
+        assert np.linalg.norm(Xg[r] - ox[r]) / np.linalg.norm(ox[r]) <= 1e-10
+    be, _ = m.backward_error(X, dev(B))
+    assert np.all(be < 1e-14)
+
+
+def test_pivot_floor_refusal(case):
+    m, p = case["m"], case["p"]
+    m.refill(case["meta"]["omega"])
+    m.equilibrate(10)
+    st = m.factorize_ldlt(A.APHI_FP64, pivot_floor=1e30)
+    assert not st["ok"] and st["columns_done"] == 1
+    with pytest.raises(ValueError):
+        m.solve(dev(case["rhs"]))  # no usable factor
+
+
+def test_deterministic_bits(case):
+    m = case["m"]
+    m.refill(case["meta"]["omega"])
+    x1, _ = m.solve_symmetric(dev(case["rhs"]), precision=A.APHI_FP32, max_refinements=5)
+    m.refill(case["meta"]["omega"])
+    x2, _ = m.solve_symmetric(dev(case["rhs"]), precision=A.APHI_FP32, max_refinements=5)
+    assert np.array_equal(x1.cpu().numpy().view(np.uint64), x2.cpu().numpy().view(np.uint64))
+
+
+def test_upload_equals_refill(case):
+    m = case["m"]
+    m.upload_system(case["vals"], case["rhs"])
+    x, rep = m.solve_symmetric(dev(case["rhs"]), precision=A.APHI_FP64)
+    assert np.linalg.norm(x.cpu().numpy() - case["x"]) / np.linalg.norm(case["x"]) <= 1e-10
+
+
+@pytest.mark.parametrize("freq", [10.0, 1e4, 1e7])
+def test_config2_sweep_points(gpu_context, freq):
+    """Config 2 (24^3 hexes, 196,978 unknowns) at sweep points: the plain FP64
+    path and the FP32 + refinement path both reach backward error <= 1e-12 and
+    agree with each other."""
+    meta, _ = golden("config2")
+    p = A.Problem(cube_config(meta))
+    m = A.Mesh(gpu_context, p)
+    m.refill(2 * np.pi * freq)
+    vals, rhs = m.system()
+    rng = np.random.default_rng(5)
+    B = np.vstack([rhs[None, :], rng.normal(size=(7, p.n)) + 1j * rng.normal(size=(7, p.n))])
+    x64, r64 = m.solve_symmetric(dev(B), precision=A.APHI_FP64)
+    assert r64["backward_error"] <= 1e-12
+    x32, r32 = m.solve_symmetric(dev(B), precision=A.APHI_FP32, max_refinements=30)
+    assert r32["backward_error"] <= 1e-12 and r32["scaled_backward_error"] <= 1e-12
+    a, b = x64.cpu().numpy(), x32.cpu().numpy()
+    for r in range(B.shape[0]):
+        assert np.linalg.norm(a[r] - b[r]) / np.linalg.norm(a[r]) <= 1e-5
+    x32s, r32s = m.solve_symmetric(dev(B), precision=A.APHI_FP32, max_refinements=40, tolerance=1e-15)
+    c = x32s.cpu().numpy()
+    for r in range(B.shape[0]):
+        assert np.linalg.norm(a[r] - c[r]) / np.linalg.norm(a[r]) <= 1e-9
+    m.close()
