@@ -111,6 +111,18 @@ def main():
     st32 = mesh.factorize_ldlt(A.APHI_FP32)
     st64 = mesh.factorize_ldlt(A.APHI_FP64)
     mesh.set_profiling(False)
+    ctx.synchronize()
+    s0 = time.perf_counter()
+    mesh.solve(B[order[0]], X)
+    ctx.synchronize()
+    solve64_ms = (time.perf_counter() - s0) * 1e3
+    mesh.factorize_ldlt(A.APHI_FP32)
+    ctx.synchronize()
+    s0 = time.perf_counter()
+    mesh.solve(B[order[0]], X)
+    ctx.synchronize()
+    solve32_ms = (time.perf_counter() - s0) * 1e3
+    log(f"{a.nrhs}-RHS solve: fp64 factor {solve64_ms:.1f} ms, fp32 factor {solve32_ms:.1f} ms")
     ach32 = st32["update_flops"] / (st32["update_ms"] * 1e-3) / 1e12
     ach64 = st64["update_flops"] / (st64["update_ms"] * 1e-3) / 1e12
     log(f"factor fp32 {st32['milliseconds']:.1f} ms ({st32['flops'] / st32['milliseconds'] / 1e9:.1f} TFLOP/s alg.), "
@@ -163,6 +175,8 @@ def main():
         "speedup_vs_fp64": (value / value64) if value64 else None,
         "factor_ms_fp32": st32["milliseconds"],
         "factor_ms_fp64": st64["milliseconds"],
+        "solve_ms_fp32": solve32_ms,
+        "solve_ms_fp64": solve64_ms,
         "roofline": {"bound": "mfma", "achieved": ach32, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach32 / MFMA_FP32_PEAK_TFLOPS, "traffic": None,
                      "kernel": "k_update<float> (MFMA trailing update)",

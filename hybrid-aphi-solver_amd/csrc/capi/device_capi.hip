@@ -103,11 +103,12 @@ struct aphi_mesh {
     int64_t max_bwd_partials = 0;  // count * row blocks over levels, per NB*nrhs
     // plan on the device
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
-    DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start;
+    DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
+    int64_t xinv_entries = 0;
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
-    DevBuf P, CB, mws;
-    DevBuf bt, W, X, bpart, colscale, res, norms;
+    DevBuf P, CB, mws, Xinv;
+    DevBuf bt, W, X, Yv, bpart, colscale, res, norms;
     int solve_nrhs_cap = 0;
     bool assembled = false, equilibrated = false, factored = false;
     int factor_precision = -1;
@@ -127,6 +128,7 @@ struct aphi_mesh {
         fp.panel_off = panel_off.as<int64_t>();
         fp.cb_off = cb_off.as<int64_t>();
         fp.col_ref = col_ref.as<int>();
+        fp.xinv_off = xinv_off.as<int64_t>();
         return fp;
     }
     hipStream_t st() const { return ctx->stream; }
@@ -208,6 +210,7 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     m->P.reserve(static_cast<size_t>(m->factor_entries) * cs);
     m->CB.reserve(static_cast<size_t>(std::max<int64_t>(m->cb_entries, 1)) * cs);
     m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
+    m->Xinv.reserve(static_cast<size_t>(std::max<int64_t>(m->xinv_entries, 1)) * cs);
     m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
     aphi_dev::FactorStatsDev init{std::numeric_limits<double>::infinity(), 0.0, 0.0, INT_MAX, 0};
     HIPCHK(hipMemcpyAsync(m->stats.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
@@ -229,7 +232,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     };
     for (const Level& L : m->levels) {
         if (L.small_count)
-            aphi_dev::launch_front_small<T>(s, nodes + L.off, L.small_count, L.small_max_m, fp, m->P.p, m->CB.p, pivot_floor, st);
+            aphi_dev::launch_front_small<T>(s, nodes + L.off, L.small_count, L.small_max_m, fp, m->P.p, m->CB.p, m->Xinv.p,
+                                            pivot_floor, st);
         if (!L.large_count) continue;
         const int* lg = nodes + L.off + L.small_count;
         aphi_dev::launch_extend_add<T>(s, lg, m->ea_start.as<int>() + L.ea_off, L.large_count, L.ea_blocks, fp, m->P.p, m->CB.p);
@@ -250,7 +254,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                 step_flops += 8.0 * nb * (static_cast<double>(rem) * (rem + 1) / 2.0);
             }
             const bool timed = m->profiling && tiles > 0;
-            aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, 0, fp, m->P.p, m->CB.p, m->mws.p, pivot_floor, st);
+            aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, 0, fp, m->P.p, m->CB.p, m->mws.p, m->Xinv.p,
+                                           pivot_floor, st);
             if (tiles > 0) {
                 if (timed) HIPCHK(hipEventRecord(next_event(), s));
                 aphi_dev::launch_update<T>(s, lg, L.large_count, b, tiles, fp, m->P.p, m->CB.p);
@@ -297,6 +302,7 @@ void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     const size_t cs = sizeof(cd);  // sized for the wider type; FP32 uses the front half
     m->bt.reserve(static_cast<size_t>(m->n) * nrhs * cs);
     m->X.reserve(static_cast<size_t>(m->n) * nrhs * cs);
+    m->Yv.reserve(static_cast<size_t>(m->n) * nrhs * cs);
     m->W.reserve(static_cast<size_t>(m->rows_total) * nrhs * cs);
     m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kNB * nrhs * cs);
     m->colscale.reserve(nrhs * sizeof(double));
@@ -329,7 +335,7 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
                 const int rem = L.m[static_cast<size_t>(i)] - b0 - std::min(aphi_dev::kNB, kk - b0);
                 if (rem > 0) rb = std::max(rb, (rem + 255) / 256);
             }
-            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->W.p);
+            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->Xinv.p, m->W.p, m->Yv.p, n);
         }
     }
     for (int li = m->num_levels - 1; li >= 0; --li) {
@@ -346,7 +352,8 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
                 const int rem = L.m[static_cast<size_t>(i)] - b0 - std::min(aphi_dev::kNB, kk - b0);
                 if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
             }
-            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->W.p, m->X.p, n, m->bpart.p);
+            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->Xinv.p, m->Yv.p, m->X.p, n,
+                                         m->bpart.p);
         }
     }
     aphi_dev::launch_perm_out<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), m->X.p, m->colscale.as<double>(), x, ldx,
@@ -612,6 +619,19 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             m->max_bwd_partials = std::max<int64_t>(m->max_bwd_partials, static_cast<int64_t>(lv.count()) * std::max(maxrb, 1));
         }
         upload(m->nodes, node_list.data(), node_list.size());
+        {
+            // diagonal-block inverses: full kNB x kNB blocks, then one nb x nb
+            std::vector<int64_t> xo(static_cast<size_t>(ns));
+            int64_t off = 0;
+            for (int s = 0; s < ns; ++s) {
+                const int64_t kk = d->super_first[s + 1] - d->super_first[s];
+                xo[static_cast<size_t>(s)] = off;
+                const int64_t rest = kk % aphi_dev::kNB;
+                off += (kk / aphi_dev::kNB) * aphi_dev::kNB * aphi_dev::kNB + rest * rest;
+            }
+            m->xinv_entries = off;
+            upload(m->xinv_off, xo.data(), xo.size());
+        }
         upload(m->ea_start, ea.data(), ea.size());
         m->flag.reserve(sizeof(int));
         *out = m.release();
@@ -638,7 +658,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->Xinv, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
                            &m->bpart, &m->colscale, &m->res, &m->norms};
     int64_t total = 0;

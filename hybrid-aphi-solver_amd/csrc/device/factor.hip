@@ -99,7 +99,7 @@ __global__ void k_scatter_panel(int64_t nnz, const int64_t* __restrict__ s2p, co
 // in order), factor all k columns, write the panel and the update block back.
 template <typename T>
 __global__ void __launch_bounds__(256) k_front_small(const int* __restrict__ nodes, FrontPlan fp, C<T>* P, C<T>* CB,
-                                                     double pivot_floor, FactorStatsDev* st) {
+                                                     C<T>* __restrict__ xinv, double pivot_floor, FactorStatsDev* st) {
     extern __shared__ unsigned char smem_raw[];
     C<T>* F = reinterpret_cast<C<T>*>(smem_raw);
     const int s = nodes[blockIdx.x];
@@ -154,7 +154,28 @@ __global__ void __launch_bounds__(256) k_front_small(const int* __restrict__ nod
     }
     block_max_multiplier(st, lmax, nullptr);
 
-    for (int idx = tid; idx < m * k; idx += nt) fr.panel[idx] = F[idx];
+    // Inverses of the unit-lower diagonal blocks (kNB columns each), built in the
+    // unused strict upper triangle of F: Linv[r][c] (r > c) lives at F[r*m + c].
+    for (int bb = 0; bb < k; bb += kNB) {
+        const int nbx = min(kNB, k - bb);
+        if (tid < nbx) {
+            const int c = tid;
+            for (int r = c + 1; r < nbx; ++r) {
+                C<T> acc = F[(bb + c) * m + (bb + r)];  // q = c term: L[r][c] * 1
+                for (int q = c + 1; q < r; ++q)
+                    acc = gfma(acc, F[(bb + q) * m + (bb + r)], F[(bb + q) * m + (bb + c)]);
+                F[(bb + r) * m + (bb + c)] = C<T>{-acc.re, -acc.im};
+            }
+        }
+        __syncthreads();
+        C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(bb / kNB) * kNB * kNB;
+        for (int idx = tid; idx < nbx * nbx; idx += nt) {
+            const int q = idx / nbx, c = idx % nbx;  // column q, row c
+            Xg[idx] = c > q ? F[(bb + c) * m + (bb + q)] : C<T>{static_cast<T>(c == q ? 1 : 0), 0};
+        }
+    }
+
+    for (int idx = tid; idx < m * k; idx += nt) fr.panel[idx] = (idx % m) < (idx / m) ? C<T>{0, 0} : F[idx];
     const int u = m - k;
     for (int idx = tid; idx < u * u; idx += nt) {
         const int j = idx / u, i = idx % u;
@@ -215,7 +236,8 @@ __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ node
 // the per-front workspace for the panel step.
 template <typename T>
 __global__ void __launch_bounds__(256) k_diag(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB,
-                                              C<T>* __restrict__ mws, double pivot_floor, FactorStatsDev* st) {
+                                              C<T>* __restrict__ mws, C<T>* __restrict__ xinv, double pivot_floor,
+                                              FactorStatsDev* st) {
     extern __shared__ unsigned char smem_raw[];
     const int s = nodes[blockIdx.x];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
@@ -267,11 +289,13 @@ __global__ void __launch_bounds__(256) k_diag(const int* __restrict__ nodes, int
         }
     }
     __syncthreads();
-    // M[q][c] = X[c][q] / D[c]  (c >= q), stored mws[q*kNB + c]
+    // M[q][c] = X[c][q] / D[c]  (c >= q), stored mws[q*kNB + c]; X kept for the solves
     C<T>* M = mws + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
+    C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(b / kNB) * kNB * kNB;
     for (int idx = tid; idx < nb * nb; idx += nt) {
         const int q = idx / nb, c = idx % nb;
         M[q * kNB + c] = (c >= q) ? gmul(X[q * kNB + c], grecip(A[c * kNB + c])) : C<T>{0, 0};
+        Xg[q * nb + c] = (c >= q) ? X[q * kNB + c] : C<T>{0, 0};  // column q, row c
     }
 }
 
@@ -467,11 +491,11 @@ size_t small_lds_bytes(int max_m) {
 
 template <typename T>
 void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, const FrontPlan& fp, void* P, void* CB,
-                        double pivot_floor, FactorStatsDev* stats) {
+                        void* xinv, double pivot_floor, FactorStatsDev* stats) {
     if (count <= 0) return;
     allow_large_lds(k_front_small<T>, small_lds_bytes<T>(max_m));
     hipLaunchKernelGGL(k_front_small<T>, dim3(count), dim3(256), small_lds_bytes<T>(max_m), st, nodes, fp,
-                       static_cast<C<T>*>(P), static_cast<C<T>*>(CB), pivot_floor, stats);
+                       static_cast<C<T>*>(P), static_cast<C<T>*>(CB), static_cast<C<T>*>(xinv), pivot_floor, stats);
 }
 
 template <typename T>
@@ -484,11 +508,12 @@ void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, i
 
 template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
-                       const FrontPlan& fp, void* P, void* CB, void* mws, double pivot_floor, FactorStatsDev* stats) {
+                       const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,
+                       FactorStatsDev* stats) {
     const size_t diag_lds = static_cast<size_t>(2 * kNB * kNB + kNB) * sizeof(C<T>);
     allow_large_lds(k_diag<T>, diag_lds);
     hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(256), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
-                       static_cast<C<T>*>(CB), static_cast<C<T>*>(mws), pivot_floor, stats);
+                       static_cast<C<T>*>(CB), static_cast<C<T>*>(mws), static_cast<C<T>*>(xinv), pivot_floor, stats);
     if (max_row_blocks > 0) {
         const size_t panel_lds = static_cast<size_t>(64 * kNB + kNB * kNB) * sizeof(C<T>);
         allow_large_lds(k_panel<T>, panel_lds);
@@ -511,17 +536,17 @@ template void launch_scatter_panel<double>(hipStream_t, int64_t, const int64_t*,
 template void launch_scatter_panel<float>(hipStream_t, int64_t, const int64_t*, const cd*, void*);
 template size_t small_lds_bytes<double>(int);
 template size_t small_lds_bytes<float>(int);
-template void launch_front_small<double>(hipStream_t, const int*, int, int, const FrontPlan&, void*, void*, double,
-                                         FactorStatsDev*);
-template void launch_front_small<float>(hipStream_t, const int*, int, int, const FrontPlan&, void*, void*, double,
-                                        FactorStatsDev*);
+template void launch_front_small<double>(hipStream_t, const int*, int, int, const FrontPlan&, void*, void*, void*,
+                                         double, FactorStatsDev*);
+template void launch_front_small<float>(hipStream_t, const int*, int, int, const FrontPlan&, void*, void*, void*,
+                                        double, FactorStatsDev*);
 template void launch_extend_add<double>(hipStream_t, const int*, const int*, int, int, const FrontPlan&, void*, void*);
 template void launch_extend_add<float>(hipStream_t, const int*, const int*, int, int, const FrontPlan&, void*, void*);
 template void launch_update<double>(hipStream_t, const int*, int, int, int, const FrontPlan&, void*, void*);
 template void launch_update<float>(hipStream_t, const int*, int, int, int, const FrontPlan&, void*, void*);
 template void launch_block_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,
-                                        void*, double, FactorStatsDev*);
+                                        void*, void*, double, FactorStatsDev*);
 template void launch_block_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,
-                                       void*, double, FactorStatsDev*);
+                                       void*, void*, double, FactorStatsDev*);
 
 }  // namespace aphi_dev

@@ -15,7 +15,7 @@ template <typename T> void launch_scatter_panel(hipStream_t st, int64_t nnz, con
 template <typename T> size_t small_lds_bytes(int max_m);
 template <typename T>
 void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, const FrontPlan& fp, void* P, void* CB,
-                        double pivot_floor, FactorStatsDev* stats);
+                        void* xinv, double pivot_floor, FactorStatsDev* stats);
 template <typename T>
 void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
                        const FrontPlan& fp, void* P, void* CB);
@@ -24,7 +24,8 @@ void launch_update(hipStream_t st, const int* nodes, int count, int b, int max_t
                    void* CB);
 template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
-                       const FrontPlan& fp, void* P, void* CB, void* mws, double pivot_floor, FactorStatsDev* stats);
+                       const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,
+                       FactorStatsDev* stats);
 
 // solve.hip
 template <typename T>
@@ -38,10 +39,10 @@ void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int nrhs, co
                        void* W);
 template <typename T>
 void launch_fwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, void* W);
+                     const FrontPlan& fp, const void* P, const void* xinv, void* W, void* Y, int n);
 template <typename T>
 void launch_bwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* W, void* X, int n, void* partial);
+                     const FrontPlan& fp, const void* P, const void* xinv, const void* Y, void* X, int n, void* partial);
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
                          int mode);
 

@@ -37,6 +37,9 @@ struct FrontPlan {
     const int64_t* panel_off;  // ns + 1
     const int64_t* cb_off;     // ns
     const int* col_ref;        // factor-order column -> reference (analysis) column
+    // Inverses of the unit-lower diagonal blocks, one per kNB columns of each
+    // supernode (the last one nb x nb), column-major, kept for the solves.
+    const int64_t* xinv_off;   // ns
 };
 
 struct FactorStatsDev {

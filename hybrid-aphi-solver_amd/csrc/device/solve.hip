@@ -85,62 +85,44 @@ __global__ void __launch_bounds__(256) k_fwd_gather(const int* __restrict__ node
     }
 }
 
-// y_b := L_bb^-1 y_b for block b of every supernode with k > b.
+// Forward step b of every supernode with k > b, one launch: each workgroup forms
+// y_b = L_bb^-1 w_b from the stored block inverse (a 64x64 GEMV, recomputed per
+// workgroup instead of a serial substitution), workgroup 0 writes y_b to Y, and
+// every workgroup updates its 256 rows below the block: W[i] -= L[i, b:b+nb] y_b.
+// Rows b..b+nb of W are only read in this launch, so there is no race.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_diag(const int* __restrict__ nodes, int b, int nrhs, FrontPlan fp,
-                                                  const C<T>* __restrict__ P, C<T>* W) {
-    extern __shared__ unsigned char smem_raw[];
-    const int s = nodes[blockIdx.x];
-    const Sn x = sn(fp, s);
-    if (x.k <= b) return;
-    const int nb = min(kNB, x.k - b);
-    C<T>* L = reinterpret_cast<C<T>*>(smem_raw);  // L[c*kNB + i]
-    C<T>* Y = L + kNB * kNB;                      // Y[r*kNB + i]
-    const C<T>* Ps = P + fp.panel_off[s];
-    C<T>* Ws = W + x.rp * nrhs;
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int c = idx / nb, i = idx % nb;
-        L[c * kNB + i] = Ps[static_cast<int64_t>(b + c) * x.m + b + i];
-    }
-    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
-        const int r = idx / nb, i = idx % nb;
-        Y[r * kNB + i] = Ws[static_cast<int64_t>(r) * x.m + b + i];
-    }
-    for (int c = 0; c < nb; ++c) {
-        __syncthreads();
-        const int w = nb - c - 1;
-        for (int idx = threadIdx.x; idx < w * nrhs; idx += blockDim.x) {
-            const int r = idx / w, i = c + 1 + idx % w;
-            Y[r * kNB + i] = gfms(Y[r * kNB + i], L[c * kNB + i], Y[r * kNB + c]);
-        }
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
-        const int r = idx / nb, i = idx % nb;
-        Ws[static_cast<int64_t>(r) * x.m + b + i] = Y[r * kNB + i];
-    }
-}
-
-// W[i] -= L[i, b:b+nb] y_b for rows i >= b+nb, one row per thread.
-template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ nodes, int b, int nrhs, FrontPlan fp,
-                                                    const C<T>* __restrict__ P, C<T>* W) {
-    __shared__ C<T> Y[16 * kNB];
+__global__ void __launch_bounds__(256) k_fwd_block(const int* __restrict__ nodes, int b, int nrhs, FrontPlan fp,
+                                                   const C<T>* __restrict__ P, const C<T>* __restrict__ xinv, C<T>* W,
+                                                   C<T>* __restrict__ Y, int n) {
+    __shared__ C<T> Xl[kNB * kNB];
+    __shared__ C<T> Wb[16 * kNB];
+    __shared__ C<T> Yb[16 * kNB];
     const int s = nodes[blockIdx.y];
     const Sn x = sn(fp, s);
     if (x.k <= b) return;
     const int nb = min(kNB, x.k - b);
     const int i0 = b + nb + static_cast<int>(blockIdx.x) * 256;
-    if (i0 >= x.m) return;
+    if (blockIdx.x > 0 && i0 >= x.m) return;
     const C<T>* Ps = P + fp.panel_off[s];
+    const C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(b / kNB) * kNB * kNB;
     C<T>* Ws = W + x.rp * nrhs;
-    const int i = i0 + threadIdx.x;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < nb * nb; idx += 256) Xl[idx] = Xg[idx];  // column q, row c at q*nb + c
+    const int i = i0 + tid;
     for (int r0 = 0; r0 < nrhs; r0 += 16) {
         const int nr = min(16, nrhs - r0);
         __syncthreads();
-        for (int idx = threadIdx.x; idx < nb * nr; idx += blockDim.x) {
+        for (int idx = tid; idx < nb * nr; idx += 256) {
             const int r = idx / nb, c = idx % nb;
-            Y[r * kNB + c] = Ws[static_cast<int64_t>(r0 + r) * x.m + b + c];
+            Wb[r * kNB + c] = Ws[static_cast<int64_t>(r0 + r) * x.m + b + c];
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nb * nr; idx += 256) {
+            const int r = idx / nb, c = idx % nb;
+            C<T> acc = C<T>{0, 0};
+            for (int q = 0; q <= c; ++q) acc = gfma(acc, Xl[q * nb + c], Wb[r * kNB + q]);
+            Yb[r * kNB + c] = acc;
+            if (blockIdx.x == 0) Y[static_cast<int64_t>(r0 + r) * n + x.first + b + c] = acc;
         }
         __syncthreads();
         if (i < x.m) {
@@ -151,7 +133,7 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
                 const C<T> l = Ps[static_cast<int64_t>(b + c) * x.m + i];
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
-                    if (r < nr) acc[r] = gfma(acc[r], l, Y[r * kNB + c]);
+                    if (r < nr) acc[r] = gfma(acc[r], l, Yb[r * kNB + c]);
             }
 #pragma unroll
             for (int r = 0; r < 16; ++r)
@@ -198,46 +180,44 @@ __global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nod
     }
 }
 
-// x_b = L_bb^-T (D_b^-1 y_b - sum of partials), written to X.
+// x_b = L_bb^-T (D_b^-1 y_b - sum of partials), written to X: a GEMV with the
+// stored block inverse (L_bb^-T)[c][q] = Linv[q][c], q >= c.
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes, int b, int nrhs, int stride,
-                                                  FrontPlan fp, const C<T>* __restrict__ P, const C<T>* __restrict__ W,
-                                                  C<T>* __restrict__ X, int n, const C<T>* __restrict__ partial) {
-    extern __shared__ unsigned char smem_raw[];
+__global__ void __launch_bounds__(256) k_bwd_final(const int* __restrict__ nodes, int b, int nrhs, int stride,
+                                                   FrontPlan fp, const C<T>* __restrict__ P,
+                                                   const C<T>* __restrict__ xinv, const C<T>* __restrict__ Y,
+                                                   C<T>* __restrict__ X, int n, const C<T>* __restrict__ partial) {
+    __shared__ C<T> Xl[kNB * kNB];
+    __shared__ C<T> Z[16 * kNB];
     const int s = nodes[blockIdx.x];
     const Sn x = sn(fp, s);
     if (x.k <= b) return;
     const int nb = min(kNB, x.k - b);
     const int nblk = x.m > b + nb ? (x.m - b - nb + 63) / 64 : 0;
-    C<T>* L = reinterpret_cast<C<T>*>(smem_raw);  // L[c*kNB + i]
-    C<T>* Z = L + kNB * kNB;                      // Z[r*kNB + c]
     const C<T>* Ps = P + fp.panel_off[s];
-    const C<T>* Ws = W + x.rp * nrhs;
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int c = idx / nb, i = idx % nb;
-        L[c * kNB + i] = Ps[static_cast<int64_t>(b + c) * x.m + b + i];
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
-        const int r = idx / nb, c = idx % nb;
-        C<T> t = C<T>{0, 0};
-        const C<T>* part = partial + static_cast<int64_t>(blockIdx.x) * stride * kNB * nrhs;
-        for (int blk = 0; blk < nblk; ++blk) t = gadd(t, part[static_cast<int64_t>(blk) * kNB * nrhs + c * nrhs + r]);
-        const C<T> y = Ws[static_cast<int64_t>(r) * x.m + b + c];
-        // D^-1 y: y / D as y * (1/D)
-        Z[r * kNB + c] = gsub(gmul(y, grecip(L[c * kNB + c])), t);
-    }
-    for (int c = nb - 1; c >= 0; --c) {
+    const C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(b / kNB) * kNB * kNB;
+    const C<T>* part = partial + static_cast<int64_t>(blockIdx.x) * stride * kNB * nrhs;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < nb * nb; idx += 256) Xl[idx] = Xg[idx];
+    for (int r0 = 0; r0 < nrhs; r0 += 16) {
+        const int nr = min(16, nrhs - r0);
         __syncthreads();
-        for (int idx = threadIdx.x; idx < c * nrhs; idx += blockDim.x) {
-            const int r = idx / c, q = idx % c;
-            Z[r * kNB + q] = gfms(Z[r * kNB + q], L[q * kNB + c], Z[r * kNB + c]);
+        for (int idx = tid; idx < nb * nr; idx += 256) {
+            const int r = idx / nb, c = idx % nb;
+            C<T> t = C<T>{0, 0};
+            for (int blk = 0; blk < nblk; ++blk)
+                t = gadd(t, part[static_cast<int64_t>(blk) * kNB * nrhs + c * nrhs + (r0 + r)]);
+            const C<T> y = Y[static_cast<int64_t>(r0 + r) * n + x.first + b + c];
+            const C<T> d = Ps[static_cast<int64_t>(b + c) * x.m + b + c];
+            Z[r * kNB + c] = gsub(gmul(y, grecip(d)), t);
         }
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
-        const int r = idx / nb, c = idx % nb;
-        X[static_cast<int64_t>(r) * n + x.first + b + c] = Z[r * kNB + c];
+        __syncthreads();
+        for (int idx = tid; idx < nb * nr; idx += 256) {
+            const int r = idx / nb, c = idx % nb;
+            C<T> acc = C<T>{0, 0};
+            for (int q = c; q < nb; ++q) acc = gfma(acc, Xl[c * nb + q], Z[r * kNB + q]);
+            X[static_cast<int64_t>(r0 + r) * n + x.first + b + c] = acc;
+        }
     }
 }
 
@@ -281,20 +261,16 @@ void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int nrhs, co
 
 template <typename T>
 void launch_fwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, void* W) {
+                     const FrontPlan& fp, const void* P, const void* xinv, void* W, void* Y, int n) {
     if (count <= 0) return;
-    const size_t lds = static_cast<size_t>(kNB * kNB + kNB * nrhs) * sizeof(C<T>);
-    allow_large_lds(k_fwd_diag<T>, lds);
-    hipLaunchKernelGGL(k_fwd_diag<T>, dim3(count), dim3(256), lds, st, nodes, b, nrhs, fp, static_cast<const C<T>*>(P),
-                       static_cast<C<T>*>(W));
-    if (max_row_blocks > 0)
-        hipLaunchKernelGGL(k_fwd_update<T>, dim3(max_row_blocks, count), dim3(256), 0, st, nodes, b, nrhs, fp,
-                           static_cast<const C<T>*>(P), static_cast<C<T>*>(W));
+    hipLaunchKernelGGL(k_fwd_block<T>, dim3(max_row_blocks > 0 ? max_row_blocks : 1, count), dim3(256), 0, st, nodes,
+                       b, nrhs, fp, static_cast<const C<T>*>(P), static_cast<const C<T>*>(xinv), static_cast<C<T>*>(W),
+                       static_cast<C<T>*>(Y), n);
 }
 
 template <typename T>
 void launch_bwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* W, void* X, int n, void* partial) {
+                     const FrontPlan& fp, const void* P, const void* xinv, const void* Y, void* X, int n, void* partial) {
     if (count <= 0) return;
     const int stride = max_row_blocks > 0 ? max_row_blocks : 1;
     if (max_row_blocks > 0) {
@@ -303,11 +279,9 @@ void launch_bwd_step(hipStream_t st, const int* nodes, int count, int b, int max
         hipLaunchKernelGGL(k_bwd_partial<T>, dim3(max_row_blocks, count), dim3(256), lds, st, nodes, b, nrhs, stride, fp,
                            static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(partial));
     }
-    const size_t lds = static_cast<size_t>(kNB * kNB + kNB * nrhs) * sizeof(C<T>);
-    allow_large_lds(k_bwd_diag<T>, lds);
-    hipLaunchKernelGGL(k_bwd_diag<T>, dim3(count), dim3(256), lds, st, nodes, b, nrhs, stride, fp,
-                       static_cast<const C<T>*>(P), static_cast<const C<T>*>(W), static_cast<C<T>*>(X), n,
-                       static_cast<const C<T>*>(partial));
+    hipLaunchKernelGGL(k_bwd_final<T>, dim3(count), dim3(256), 0, st, nodes, b, nrhs, stride, fp,
+                       static_cast<const C<T>*>(P), static_cast<const C<T>*>(xinv), static_cast<const C<T>*>(Y),
+                       static_cast<C<T>*>(X), n, static_cast<const C<T>*>(partial));
 }
 
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
@@ -324,9 +298,9 @@ template void launch_perm_out<double>(hipStream_t, int, int, const int*, const d
 template void launch_perm_out<float>(hipStream_t, int, int, const int*, const double*, const void*, const double*, cd*, int64_t, int);
 template void launch_fwd_gather<double>(hipStream_t, const int*, int, int, const FrontPlan&, const void*, int, void*);
 template void launch_fwd_gather<float>(hipStream_t, const int*, int, int, const FrontPlan&, const void*, int, void*);
-template void launch_fwd_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, void*);
-template void launch_fwd_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, void*);
-template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, void*, int, void*);
-template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, void*, int, void*);
+template void launch_fwd_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int);
+template void launch_fwd_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int);
+template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, void*, int, void*);
+template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, void*, int, void*);
 
 }  // namespace aphi_dev
