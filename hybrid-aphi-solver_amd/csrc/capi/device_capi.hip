@@ -76,7 +76,7 @@ void upload(DevBuf& b, const T* src, size_t n) {
 size_t csize(int precision) { return precision == APHI_FP64 ? sizeof(cd) : sizeof(aphi_dev::cf); }
 
 struct Level {
-    int small_count = 0, small_max_m = 0, large_count = 0, ea_blocks = 0;
+    int small_count = 0, small_max_m = 0, large_count = 0, ea_blocks = 0, large_max_m = 0;
     int64_t off = 0;       // first node of this level in the node list: small ones, then large ones
     int64_t ea_off = 0;    // this level's extend-add block starts
     std::vector<int> m, k; // per node (small then large)
@@ -104,6 +104,7 @@ struct aphi_mesh {
     // plan on the device
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
+    DevBuf ea_bound_off, ea_bound;
     int64_t xinv_entries = 0;
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
@@ -129,6 +130,8 @@ struct aphi_mesh {
         fp.cb_off = cb_off.as<int64_t>();
         fp.col_ref = col_ref.as<int>();
         fp.xinv_off = xinv_off.as<int64_t>();
+        fp.ea_bound_off = ea_bound_off.as<int64_t>();
+        fp.ea_bound = ea_bound.as<int>();
         return fp;
     }
     hipStream_t st() const { return ctx->stream; }
@@ -207,7 +210,7 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     hipStream_t s = m->st();
     const aphi_dev::FrontPlan fp = m->plan();
     const size_t cs = sizeof(typename aphi_dev::cplx<T>::type);
-    m->P.reserve(static_cast<size_t>(m->factor_entries) * cs);
+    m->P.reserve(static_cast<size_t>(m->factor_entries + aphi_dev::kPanelTail) * cs);
     m->CB.reserve(static_cast<size_t>(std::max<int64_t>(m->cb_entries, 1)) * cs);
     m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
     m->Xinv.reserve(static_cast<size_t>(std::max<int64_t>(m->xinv_entries, 1)) * cs);
@@ -215,7 +218,7 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     aphi_dev::FactorStatsDev init{std::numeric_limits<double>::infinity(), 0.0, 0.0, INT_MAX, 0};
     HIPCHK(hipMemcpyAsync(m->stats.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(m->ctx->ev0, s));
-    aphi_dev::launch_zero<T>(s, m->factor_entries, m->P.p);
+    aphi_dev::launch_zero<T>(s, m->factor_entries + aphi_dev::kPanelTail, m->P.p);
     aphi_dev::launch_scatter_panel<T>(s, m->nnz, m->s2p.as<int64_t>(), m->Aeq.as<cd>(), m->P.p);
     const int* nodes = m->nodes.as<int>();
     auto* st = m->stats.as<aphi_dev::FactorStatsDev>();
@@ -236,33 +239,49 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                                             pivot_floor, st);
         if (!L.large_count) continue;
         const int* lg = nodes + L.off + L.small_count;
-        aphi_dev::launch_extend_add<T>(s, lg, m->ea_start.as<int>() + L.ea_off, L.large_count, L.ea_blocks, fp, m->P.p, m->CB.p);
+        aphi_dev::launch_extend_add<T>(s, lg, m->ea_start.as<int>() + L.ea_off, L.large_count, L.ea_blocks,
+                                       (L.large_max_m + aphi_dev::kEaRows - 1) / aphi_dev::kEaRows, fp, m->P.p, m->CB.p);
         int maxk = 0;
         for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
+        const int ts = aphi_dev::update_tile<T>();
+        // one update launch: the largest tile count over the level's fronts, and its flops
+        const auto update = [&](int c0, int width, int bo, int inner) {
+            int tiles = 0;
+            double fl = 0.0;
+            for (int i = 0; i < L.large_count; ++i) {
+                const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
+                if (kk <= c0) continue;
+                const int c1 = std::min(c0 + width, kk);
+                const int tend = inner ? std::min(bo + aphi_dev::kOuter, kk) : mm;
+                if (tend <= c1) continue;
+                const int nrt = (mm - c1 + ts - 1) / ts, nct = (tend - c1 + ts - 1) / ts;
+                tiles = std::max(tiles, nct * nrt - nct * (nct - 1) / 2);
+                const double kw = c1 - c0, w = tend - c1, h = mm - c1;
+                fl += 8.0 * kw * (w * h - w * (w - 1.0) / 2.0);
+            }
+            if (tiles == 0) return;
+            const bool timed = m->profiling;
+            if (timed) HIPCHK(hipEventRecord(next_event(), s));
+            aphi_dev::launch_update<T>(s, lg, L.large_count, c0, width, bo, inner, tiles, fp, m->P.p, m->CB.p);
+            if (timed) HIPCHK(hipEventRecord(next_event(), s));
+            ++launches;
+            update_flops += fl;
+        };
         for (int b = 0; b < maxk; b += aphi_dev::kNB) {
-            int rb = 0, tiles = 0;
-            double step_flops = 0.0;
+            int rb = 0;
             for (int i = 0; i < L.large_count; ++i) {
                 const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
                 if (kk <= b) continue;
-                const int nb = std::min(aphi_dev::kNB, kk - b);
-                const int rem = mm - b - nb;
-                if (rem <= 0) continue;
-                rb = std::max(rb, (rem + 63) / 64);
-                const int t = (rem + 63) / 64;
-                tiles = std::max(tiles, t * (t + 1) / 2);
-                step_flops += 8.0 * nb * (static_cast<double>(rem) * (rem + 1) / 2.0);
+                const int rem = mm - b - std::min(aphi_dev::kNB, kk - b);
+                if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
             }
-            const bool timed = m->profiling && tiles > 0;
             aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, 0, fp, m->P.p, m->CB.p, m->mws.p, m->Xinv.p,
                                            pivot_floor, st);
-            if (tiles > 0) {
-                if (timed) HIPCHK(hipEventRecord(next_event(), s));
-                aphi_dev::launch_update<T>(s, lg, L.large_count, b, tiles, fp, m->P.p, m->CB.p);
-                if (timed) HIPCHK(hipEventRecord(next_event(), s));
-                ++launches;
-                update_flops += step_flops;
-            }
+            // two-level blocking: a narrow rank-kNB update inside the outer block,
+            // then a rank-kOuter update of the trailing front when the block is done
+            const int bo = (b / aphi_dev::kOuter) * aphi_dev::kOuter;
+            update(b, aphi_dev::kNB, bo, 1);
+            if ((b + aphi_dev::kNB) % aphi_dev::kOuter == 0 || b + aphi_dev::kNB >= maxk) update(bo, aphi_dev::kOuter, bo, 0);
         }
     }
     HIPCHK(hipGetLastError());
@@ -608,6 +627,7 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                     lv.k.push_back(kk);
                     if (pass == 0) lv.small_max_m = std::max(lv.small_max_m, mm);
                     else {
+                        lv.large_max_m = std::max(lv.large_max_m, mm);
                         ea.push_back(blk);
                         blk += (mm + 63) / 64;
                     }
@@ -633,6 +653,30 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             upload(m->xinv_off, xo.data(), xo.size());
         }
         upload(m->ea_start, ea.data(), ea.size());
+        {
+            // extend-add bounds for the children of blocked fronts
+            std::vector<int64_t> bo(static_cast<size_t>(ns), -1);
+            std::vector<int> bv;
+            for (int c = 0; c < ns; ++c) {
+                const int p = d->super_parent[c];
+                if (p < 0) continue;
+                const int64_t pm = d->super_row_ptr[p + 1] - d->super_row_ptr[p];
+                if (pm <= small_max) continue;
+                const int ck = d->super_first[c + 1] - d->super_first[c];
+                const int32_t* rl = d->super_rel + d->super_row_ptr[c] + ck;
+                const int cu = static_cast<int>(d->super_row_ptr[c + 1] - d->super_row_ptr[c]) - ck;
+                bo[static_cast<size_t>(c)] = static_cast<int64_t>(bv.size());
+                const int nq = static_cast<int>((pm + 63) / 64);
+                int pos = 0;
+                for (int q = 0; q <= nq; ++q) {
+                    while (pos < cu && rl[pos] < 64 * q) ++pos;
+                    bv.push_back(pos);
+                }
+                if (pos != cu) throw std::invalid_argument("aphi_mesh_create: relative indices exceed the parent front");
+            }
+            upload(m->ea_bound_off, bo.data(), bo.size());
+            upload(m->ea_bound, bv.data(), bv.size());
+        }
         m->flag.reserve(sizeof(int));
         *out = m.release();
     });
@@ -658,7 +702,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->xinv_off, &m->Xinv, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->Xinv, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
                            &m->bpart, &m->colscale, &m->res, &m->norms};
     int64_t total = 0;

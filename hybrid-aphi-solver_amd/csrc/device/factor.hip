@@ -185,12 +185,16 @@ __global__ void __launch_bounds__(256) k_front_small(const int* __restrict__ nod
 
 // ------------------------------------------------------------------ large fronts
 
-// Extend-add for large fronts. Workgroup = (front, block of 64 front columns);
-// it owns those columns exclusively, clears their update-block part, then adds
-// each child's contributions in child order -- deterministic without atomics.
+// Extend-add for large fronts. Workgroup = (front, block of 64 front columns,
+// chunk of kEaRows front rows); it owns those entries exclusively, clears their
+// update-block part, then adds each child's contributions in child order --
+// deterministic without atomics. The child index ranges come from the
+// precomputed bounds; each wave takes two child columns at a time with all of
+// their rows (kEaRows / 64 per lane) loaded before any store.
 template <typename T>
 __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ nodes, const int* __restrict__ blk_start,
                                                     int nnodes, FrontPlan fp, C<T>* P, C<T>* CB) {
+    constexpr int RL = kEaRows / 64;  // rows per lane per column
     // locate the front this block belongs to
     int lo = 0, hi = nnodes;
     while (hi - lo > 1) {
@@ -200,13 +204,18 @@ __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ node
     }
     const int s = nodes[lo];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
-    const int c0 = (static_cast<int>(blockIdx.x) - blk_start[lo]) * 64;
+    const int cblk = static_cast<int>(blockIdx.x) - blk_start[lo];
+    const int c0 = cblk * 64;
     const int c1 = min(c0 + 64, fr.m);
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int r0 = max(c0, static_cast<int>(blockIdx.y) * kEaRows);
+    const int r1 = min(fr.m, (static_cast<int>(blockIdx.y) + 1) * kEaRows);
+    if (r0 >= r1) return;
+    const int q0 = r0 / 64, q1 = (r1 + 63) / 64;  // row range in 64-blocks (r0, r1 multiples of 64 or m)
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
 
-    // clear owned update-block columns (lower part)
-    for (int j = max(c0, fr.k); j < c1; ++j)
-        for (int i = j + tid; i < fr.m; i += nt) *fr.at(i, j) = C<T>{0, 0};
+    // clear owned update-block entries (lower part)
+    for (int j = max(c0, fr.k) + wave; j < c1; j += 4)
+        for (int i = max(j, r0) + lane; i < r1; i += 64) *fr.at(i, j) = C<T>{0, 0};
 
     for (int c = fp.child_ptr[s]; c < fp.child_ptr[s + 1]; ++c) {
         __syncthreads();
@@ -215,87 +224,169 @@ __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ node
         const int ck = fp.first[ch + 1] - fp.first[ch];
         const int cu = cm - ck;
         const int* rel = fp.rel + fp.row_ptr[ch] + ck;
-        // child columns mapping into [c0, c1): rel is increasing
-        int a = 0, b = cu;
-        while (a < b) { const int mid = (a + b) / 2; if (rel[mid] < c0) a = mid + 1; else b = mid; }
-        int e = a, f = cu;
-        while (e < f) { const int mid = (e + f) / 2; if (rel[mid] < c1) e = mid + 1; else f = mid; }
+        const int* bnd = fp.ea_bound + fp.ea_bound_off[ch];
+        const int a = bnd[cblk], e = bnd[cblk + 1];   // child columns landing in [c0, c1)
+        const int ia = bnd[q0], ie = bnd[q1];         // child rows landing in [r0, r1)
         const C<T>* cb = CB + fp.cb_off[ch];
-        for (int jc = a; jc < e; ++jc) {
-            const int pj = rel[jc];
-            for (int ic = jc + tid; ic < cu; ic += nt) {
-                C<T>* dst = fr.at(rel[ic], pj);
-                *dst = gadd(*dst, cb[static_cast<int64_t>(jc) * cu + ic]);
+        for (int jb = a + 2 * wave; jb < e; jb += 8) {
+            int dr[2][RL];
+            C<T> v[2][RL];
+            int pj[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int jc = jb + h;
+                pj[h] = jc < e ? rel[jc] : 0;
+                const int base = max(jc, ia);
+#pragma unroll
+                for (int u = 0; u < RL; ++u) {
+                    const int ic = base + lane + 64 * u;
+                    const bool ok = jc < e && ic < ie;
+                    dr[h][u] = ok ? rel[ic] : -1;
+                    v[h][u] = ok ? cb[static_cast<int64_t>(jc) * cu + ic] : C<T>{0, 0};
+                }
             }
+            C<T> cur[2][RL];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int u = 0; u < RL; ++u)
+                    if (dr[h][u] >= 0) cur[h][u] = *fr.at(dr[h][u], pj[h]);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int u = 0; u < RL; ++u)
+                    if (dr[h][u] >= 0) *fr.at(dr[h][u], pj[h]) = gadd(cur[h][u], v[h][u]);
         }
     }
 }
 
-// Diagonal block of step b for each front with k > b: LDL^T of the nbb x nbb
-// block in LDS, written back; then M = L_bb^-T D_bb^-1 (upper triangular) into
-// the per-front workspace for the panel step.
+// Scalar read of lane `src`'s value (src uniform): the broadcasts of the
+// register-resident diagonal factorization.
+__device__ __forceinline__ float lane_read(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_read(double v, int src) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), src);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <typename Cx>
+__device__ __forceinline__ Cx lane_read(Cx v, int src) {
+    return Cx{lane_read(v.re, src), lane_read(v.im, src)};
+}
+
+// Diagonal block of step b for each front with k > b, one wavefront per front.
+// Lane r holds row r of the nb x nb block in registers (rows and columns beyond
+// nb padded with the identity); the right-looking LDL^T takes each column value
+// it needs from the owning lane with a scalar lane read, so the elimination runs
+// without barriers or LDS traffic. Then lane c forms column c of L_bb^-1 by
+// forward substitution (L read as LDS broadcasts), and the block inverse (kept
+// for the solves) and M = L_bb^-T D_bb^-1 (for the panel step) are written out.
 template <typename T>
-__global__ void __launch_bounds__(256) k_diag(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB,
-                                              C<T>* __restrict__ mws, C<T>* __restrict__ xinv, double pivot_floor,
-                                              FactorStatsDev* st) {
+__global__ void __launch_bounds__(64) k_diag(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB,
+                                             C<T>* __restrict__ mws, C<T>* __restrict__ xinv, double pivot_floor,
+                                             FactorStatsDev* st) {
     extern __shared__ unsigned char smem_raw[];
+    constexpr int LD = kNB + 1;
+    C<T>* Ls = reinterpret_cast<C<T>*>(smem_raw);  // Ls[c*LD + r]: factored block, then L_bb^-1 by columns
+    C<T>* Dl = Ls + kNB * LD;                     // pivots
     const int s = nodes[blockIdx.x];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
     if (fr.k <= b) return;
     const int nb = min(kNB, fr.k - b);
-    C<T>* A = reinterpret_cast<C<T>*>(smem_raw);  // A[c*kNB + r]
-    C<T>* X = A + kNB * kNB;                      // L^-1, X[c*kNB + r]
-    C<T>* lcol = X + kNB * kNB;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int idx = tid; idx < nb * nb; idx += nt) {
-        const int c = idx / nb, r = idx % nb;
-        A[c * kNB + r] = (r >= c) ? *fr.at(b + r, b + c) : C<T>{0, 0};
-    }
-    __syncthreads();
+    const int lane = threadIdx.x;
+    C<T>* blk = fr.panel + static_cast<int64_t>(b) * fr.m + b;  // element (b, b) of the front
     double lmax = 0.0;
-    for (int j = 0; j < nb; ++j) {
-        const C<T> d = A[j * kNB + j];
-        if (tid == 0) record_pivot<T>(st, fp, fp.first[s] + b + j, d, pivot_floor);
-        const C<T> rd = grecip(d);
-        for (int i = j + 1 + tid; i < nb; i += nt) {
-            const C<T> l = gmul(A[j * kNB + i], rd);
-            lcol[i] = l;
-            lmax = fmax(lmax, static_cast<double>(hypot(l.re, l.im)));
+    if constexpr (sizeof(T) == 4) {
+        C<T> a[kNB];
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) {
+            const bool in = lane < nb && c < nb;
+            a[c] = in ? (c <= lane ? blk[static_cast<int64_t>(c) * fr.m + lane] : C<T>{0, 0})
+                      : C<T>{static_cast<T>(lane == c ? 1 : 0), 0};
+        }
+        // a[c] of lane r is A[r][c]; entries above the diagonal are scratch.
+#pragma unroll
+        for (int j = 0; j < kNB; ++j) {
+            const C<T> aj = a[j];
+            const C<T> l = gmul(aj, grecip(lane_read(aj, j)));
+            if (lane > j) a[j] = l;
+#pragma unroll
+            for (int c = j + 1; c < kNB; ++c) a[c] = gfms(a[c], l, lane_read(aj, c));  // A[r][c] -= l_r A[c][j]
+        }
+#pragma unroll
+        for (int c = 0; c < kNB; ++c) Ls[c * LD + lane] = a[c];
+    } else {
+        // complex double: the same elimination with row `lane` in LDS (64 complex
+        // doubles per lane would not fit the register file)
+        C<T>* U = Dl + kNB;  // the pivot column before scaling
+#pragma unroll 8
+        for (int c = 0; c < kNB; ++c) {
+            const bool in = lane < nb && c < nb;
+            Ls[c * LD + lane] = in ? (c <= lane ? blk[static_cast<int64_t>(c) * fr.m + lane] : C<T>{0, 0})
+                                   : C<T>{static_cast<T>(lane == c ? 1 : 0), 0};
         }
         __syncthreads();
-        const int w = nb - j - 1;
-        for (int idx = tid; idx < w * w; idx += nt) {
-            const int c = j + 1 + idx / w, i = j + 1 + idx % w;
-            if (i < c) continue;
-            A[c * kNB + i] = gfms(A[c * kNB + i], A[j * kNB + i], lcol[c]);
-        }
-        __syncthreads();
-        for (int i = j + 1 + tid; i < nb; i += nt) A[j * kNB + i] = lcol[i];
-        __syncthreads();
-    }
-    block_max_multiplier(st, lmax, nullptr);
-    for (int idx = tid; idx < nb * nb; idx += nt) {
-        const int c = idx / nb, r = idx % nb;
-        if (r >= c) *fr.at(b + r, b + c) = A[c * kNB + r];
-    }
-    // X = L^-1 (unit lower), one column per thread
-    if (tid < nb) {
-        const int c = tid;
-        for (int r = 0; r < nb; ++r) X[c * kNB + r] = C<T>{static_cast<T>(r == c ? 1 : 0), 0};
-        for (int r = c + 1; r < nb; ++r) {
-            C<T> acc = C<T>{0, 0};
-            for (int q = c; q < r; ++q) acc = gfma(acc, A[q * kNB + r], X[c * kNB + q]);
-            X[c * kNB + r] = C<T>{-acc.re, -acc.im};
+        for (int j = 0; j < kNB; ++j) {
+            const C<T> aj = Ls[j * LD + lane];
+            const C<T> l = gmul(aj, grecip(Ls[j * LD + j]));
+            U[lane] = aj;
+            __syncthreads();
+            if (lane > j) Ls[j * LD + lane] = l;
+#pragma unroll 4
+            for (int c = j + 1; c < kNB; ++c) Ls[c * LD + lane] = gfms(Ls[c * LD + lane], l, U[c]);
+            __syncthreads();
         }
     }
     __syncthreads();
-    // M[q][c] = X[c][q] / D[c]  (c >= q), stored mws[q*kNB + c]; X kept for the solves
+    for (int c = 0; c < nb; ++c) {
+        const C<T> v = Ls[c * LD + lane];
+        if (c < lane && lane < nb) lmax = fmax(lmax, static_cast<double>(hypot(v.re, v.im)));
+        if (c <= lane && lane < nb) blk[static_cast<int64_t>(c) * fr.m + lane] = v;
+    }
+    const C<T> dd = Ls[lane * LD + lane];
+    Dl[lane] = dd;
+    if (lane < nb) record_pivot<T>(st, fp, fp.first[s] + b + lane, dd, pivot_floor);
+    block_max_multiplier(st, lmax, nullptr);
+    // column `lane` of L_bb^-1: x[r] = delta(r, lane) - sum_{q<r} L[r][q] x[q];
+    // x in registers for complex float, in a per-lane LDS column for complex
+    // double (whose 64 entries would not fit the register file next to the rest)
+    if constexpr (sizeof(T) == 4) {
+        C<T> x[kNB];
+#pragma unroll
+        for (int r = 0; r < kNB; ++r) {
+            C<T> acc = C<T>{static_cast<T>(lane == r ? 1 : 0), 0};
+#pragma unroll
+            for (int q = 0; q < r; ++q) acc = gfms(acc, Ls[q * LD + r], x[q]);
+            x[r] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kNB; ++r) Ls[lane * LD + r] = x[r];  // Ls[q*LD + c] = Linv[c][q]
+    } else {
+        C<T>* Xs = Dl + 2 * kNB;  // Xs[r*kNB + lane] = x[r] of lane
+#pragma unroll
+        for (int r = 0; r < kNB; ++r) {
+            C<T> acc = C<T>{static_cast<T>(lane == r ? 1 : 0), 0};
+#pragma unroll 16
+            for (int q = 0; q < r; ++q) acc = gfms(acc, Ls[q * LD + r], Xs[q * kNB + lane]);
+            Xs[r * kNB + lane] = acc;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kNB; ++r) Ls[lane * LD + r] = Xs[r * kNB + lane];
+    }
+    __syncthreads();
+    // M[q][c] = Linv[c][q] / D[c] (c >= q) at mws[q*kNB + c]; the inverse, column q
+    // and row c at q*nb + c, is kept for the solves
     C<T>* M = mws + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
     C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(b / kNB) * kNB * kNB;
-    for (int idx = tid; idx < nb * nb; idx += nt) {
+    for (int idx = lane; idx < nb * nb; idx += 64) {
         const int q = idx / nb, c = idx % nb;
-        M[q * kNB + c] = (c >= q) ? gmul(X[q * kNB + c], grecip(A[c * kNB + c])) : C<T>{0, 0};
-        Xg[q * nb + c] = (c >= q) ? X[q * kNB + c] : C<T>{0, 0};  // column q, row c
+        const C<T> v = c >= q ? Ls[q * LD + c] : C<T>{0, 0};
+        Xg[idx] = v;
+        M[q * kNB + c] = c >= q ? gmul(v, grecip(Dl[c])) : C<T>{0, 0};
     }
 }
 
@@ -317,13 +408,23 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
     C<T>* Ms = Fs + 64 * kNB;                     // Ms[q*kNB + c]
     const C<T>* M = mws + static_cast<int64_t>(blockIdx.y) * kNB * kNB;
     const int tid = threadIdx.x, nt = blockDim.x;
-    for (int idx = tid; idx < nb * 64; idx += nt) {
-        const int q = idx / 64, r = idx % 64;
-        Fs[q * 64 + r] = r < nr ? fr.panel[static_cast<int64_t>(b + q) * fr.m + r0 + r] : C<T>{0, 0};
-    }
-    for (int idx = tid; idx < nb * nb; idx += nt) {
-        const int q = idx / nb, c = idx % nb;
-        Ms[q * kNB + c] = M[q * kNB + c];
+    {
+        // all loads of the row block and of M in flight before the LDS stores
+        C<T> f[64 * kNB / 256], g[kNB * kNB / 256];
+#pragma unroll
+        for (int u = 0; u < 64 * kNB / 256; ++u) {
+            const int idx = tid + 256 * u, q = idx / 64, r = idx % 64;
+            f[u] = (q < nb && r < nr) ? fr.panel[static_cast<int64_t>(b + q) * fr.m + r0 + r] : C<T>{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < kNB * kNB / 256; ++u) {
+            const int idx = tid + 256 * u;
+            g[u] = M[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < 64 * kNB / 256; ++u) Fs[tid + 256 * u] = f[u];
+#pragma unroll
+        for (int u = 0; u < kNB * kNB / 256; ++u) Ms[tid + 256 * u] = g[u];
     }
     __syncthreads();
     const int r = tid % 64;
@@ -368,103 +469,149 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
 };
 
+// Tiling of the update. A wave moves one column segment of the L panel into LDS
+// with a single 16-byte-per-lane DMA load (global_load_lds_dwordx4): 128 rows of
+// complex float or 64 rows of complex double. That segment length is the tile
+// side: 128x128 tiles for complex float (each wave a 64x64 quadrant of 4x4 MFMA
+// blocks), 64x64 for complex double (2x2 blocks per wave).
+template <typename T> struct UpdTile;
+template <> struct UpdTile<float> { static constexpr int WB = 4, KC = 16; };
+template <> struct UpdTile<double> { static constexpr int WB = 2, KC = 8; };
+
+__device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
+    __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                     (__attribute__((address_space(3))) void*)lds_dst, 16, 0, 0);
+}
+
+// F[i][j] -= sum_{c in [c0, c1)} L[i][c] D[c] L[j][c] for i >= j over the target
+// region: rows [c1, m), columns [c1, tend) with c1 = min(c0 + width, k) and
+// tend = min(bo + kOuter, k) for the narrow update inside an outer block
+// (inner = 1) or tend = m for the trailing update after it (inner = 0). One
+// workgroup per square tile of the region's lower part (column-major
+// enumeration). The tile of F is loaded into the MFMA accumulators first and
+// written back at the end (no read-modify-write epilogue). Per stage of KC
+// columns, L rows j0.. (MFMA "row" operand) and i0.. (the "column" operand,
+// multiplied by -D at read time) arrive by DMA into one of two LDS buffers
+// while the other is being consumed; one barrier per stage. Each lane's output
+// column is a front row i, so stores are contiguous. Reads past a column's end
+// (rows >= m) land in rows of the tile that are never stored, so the factor
+// storage carries a small zeroed tail for them.
 template <typename T>
-__global__ void __launch_bounds__(256) k_update(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB) {
+__global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes, int c0, int width, int bo, int inner,
+                                                   FrontPlan fp, C<T>* P, C<T>* CB) {
     using V4 = typename Mfma<T>::v4;
-    constexpr int KC = 16;  // block columns staged per iteration
-    __shared__ T Lr[KC][64], Li[KC][64], Wr[KC][64], Wi[KC][64];
-    __shared__ C<T> Dsh[kNB];
+    constexpr int WB = UpdTile<T>::WB, KC = UpdTile<T>::KC;
+    constexpr int TS = 32 * WB;                          // tile side
+    constexpr int RPL = 16 / static_cast<int>(sizeof(C<T>));  // rows per lane per DMA
+    static_assert(TS == 64 * RPL, "one wave DMA covers one tile column segment");
+    constexpr int LDR = TS + 16;                         // LDS row stride (complex)
+    __shared__ C<T> Lc[2][KC][LDR], Wc[2][KC][LDR];
+    __shared__ C<T> Dsh[kOuter];
     const int s = nodes[blockIdx.y];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
-    if (fr.k <= b) return;
-    const int nb = min(kNB, fr.k - b);
-    const int t0 = b + nb;
-    const int nrem = fr.m - t0;
-    if (nrem <= 0) return;
-    const int ntile = (nrem + 63) / 64;
+    if (fr.k <= c0) return;
+    const int c1 = min(c0 + width, fr.k);
+    const int tend = inner ? min(bo + kOuter, fr.k) : fr.m;
+    if (tend <= c1) return;
+    const int K = c1 - c0;
+    const int nrt = (fr.m - c1 + TS - 1) / TS;
+    const int nct = (tend - c1 + TS - 1) / TS;
     const int t = static_cast<int>(blockIdx.x);
-    if (t >= ntile * (ntile + 1) / 2) return;
-    // lower-triangular tile index -> (ti, tj), ti >= tj
-    int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    while (ti * (ti + 1) / 2 > t) --ti;
-    const int tj = t - ti * (ti + 1) / 2;
-    const int i0 = t0 + ti * 64, j0 = t0 + tj * 64;
+    if (t >= nct * nrt - nct * (nct - 1) / 2) return;
+    // column tile tj holds nrt - tj tiles; first(tj) = tj*nrt - tj(tj-1)/2
+    const double q2 = 2.0 * nrt + 1.0;
+    int tj = static_cast<int>((q2 - sqrt(q2 * q2 - 8.0 * t)) * 0.5);
+    tj = max(0, min(tj, nct - 1));
+    while (tj > 0 && tj * nrt - tj * (tj - 1) / 2 > t) --tj;
+    while (tj + 1 < nct && (tj + 1) * nrt - (tj + 1) * tj / 2 <= t) ++tj;
+    const int ti = tj + (t - (tj * nrt - tj * (tj - 1) / 2));
+    const int i0 = c1 + ti * TS, j0 = c1 + tj * TS;
 
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
-    const int wi = (wave & 1) * 32, wj = (wave >> 1) * 32;  // quadrant offsets (i, j)
+    const int wi = (wave & 1) * (TS / 2), wj = (wave >> 1) * (TS / 2);  // quadrant offsets (i, j)
+    const C<T>* L0 = fr.panel + static_cast<int64_t>(c0) * fr.m;
+    const int nst = (K + KC - 1) / KC;
 
-    for (int q = tid; q < nb; q += 256) Dsh[q] = fr.panel[static_cast<int64_t>(b + q) * fr.m + (b + q)];
-
-    V4 accr[2][2], acci[2][2];
+    // stage st: 2*KC column segments (L rows j0.. then rows i0..), KC/2 per wave
+    const auto issue_stage = [&](int buf, int st) {
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            accr[x][y] = V4{0, 0, 0, 0};
-            acci[x][y] = V4{0, 0, 0, 0};
+        for (int u = 0; u < 2 * KC / 4; ++u) {
+            const int jdx = wave + 4 * u;
+            const int mat = jdx / KC, q = jdx % KC;
+            const int c = min(st * KC + q, K - 1);
+            const int row = min((mat ? i0 : j0) + RPL * lane, fr.m - 1);
+            lds_dma16(L0 + static_cast<int64_t>(c) * fr.m + row, mat ? &Wc[buf][q][0] : &Lc[buf][q][0]);
         }
+    };
+    issue_stage(0, 0);
 
-    for (int kk = 0; kk < nb; kk += KC) {
-        __syncthreads();
-        // stage L rows j0.. and W rows i0.. for columns b+kk .. b+kk+KC
-        for (int idx = tid; idx < KC * 64; idx += 256) {
-            const int q = idx / 64, r = idx % 64;
-            const int c = kk + q;
-            C<T> lj = C<T>{0, 0}, li = C<T>{0, 0};
-            if (c < nb) {
-                const C<T>* col = fr.panel + static_cast<int64_t>(b + c) * fr.m;
-                if (j0 + r < fr.m) lj = col[j0 + r];
-                if (i0 + r < fr.m) li = gmul(col[i0 + r], Dsh[c]);
+    V4 accr[WB][WB], acci[WB][WB];
+#pragma unroll
+    for (int x = 0; x < WB; ++x)
+#pragma unroll
+        for (int y = 0; y < WB; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wi + x * 16 + (lane % 16);
+                const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
+                const int jj = min(j, fr.m - 1), ii = max(min(i, fr.m - 1), jj);  // a stored entry
+                const C<T> c = *fr.at(ii, jj);
+                const bool ok = i < fr.m && j < tend && i >= j;
+                accr[x][y][v] = ok ? c.re : static_cast<T>(0);
+                acci[x][y][v] = ok ? c.im : static_cast<T>(0);
             }
-            Lr[q][r] = lj.re;
-            Li[q][r] = lj.im;
-            Wr[q][r] = li.re;
-            Wi[q][r] = li.im;
-        }
-        __syncthreads();
+    for (int q = tid; q < nst * KC; q += 256) {
+        const int qq = min(q, K - 1);
+        const C<T> d = L0[static_cast<int64_t>(qq) * fr.m + c0 + qq];
+        Dsh[q] = q < K ? C<T>{-d.re, -d.im} : C<T>{0, 0};
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+
+    for (int st = 0; st < nst; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nst) issue_stage(buf ^ 1, st + 1);
 #pragma unroll
         for (int k4 = 0; k4 < KC; k4 += 4) {
             const int kq = k4 + lane / 16;
-            T ar[2], ai[2], br[2], bi[2];
+            const C<T> d = Dsh[st * KC + kq];
+            T ar[WB], ai[WB], br[WB], bi[WB];
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                ar[y] = Lr[kq][wj + y * 16 + (lane % 16)];
-                ai[y] = Li[kq][wj + y * 16 + (lane % 16)];
+            for (int y = 0; y < WB; ++y) {
+                const C<T> a = Lc[buf][kq][wj + y * 16 + (lane % 16)];
+                ar[y] = a.re;
+                ai[y] = a.im;
             }
 #pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                br[x] = Wr[kq][wi + x * 16 + (lane % 16)];
-                bi[x] = Wi[kq][wi + x * 16 + (lane % 16)];
+            for (int x = 0; x < WB; ++x) {
+                const C<T> w = gmul(Wc[buf][kq][wi + x * 16 + (lane % 16)], d);
+                br[x] = w.re;
+                bi[x] = w.im;
             }
 #pragma unroll
-            for (int x = 0; x < 2; ++x)
+            for (int x = 0; x < WB; ++x)
 #pragma unroll
-                for (int y = 0; y < 2; ++y) {
+                for (int y = 0; y < WB; ++y) {
                     accr[x][y] = Mfma<T>::mac(ar[y], br[x], accr[x][y]);
                     accr[x][y] = Mfma<T>::mac(-ai[y], bi[x], accr[x][y]);
                     acci[x][y] = Mfma<T>::mac(ar[y], bi[x], acci[x][y]);
                     acci[x][y] = Mfma<T>::mac(ai[y], br[x], acci[x][y]);
                 }
         }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
     }
     // D[p][q]: lane holds p = Mfma<T>::row(lane, v) (j), q = lane%16 (i)
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < WB; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y)
+        for (int y = 0; y < WB; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wi + x * 16 + (lane % 16);
                 const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
-                if (i < fr.m && j < fr.m && i >= j) {
-                    C<T>* dst = fr.at(i, j);
-                    C<T> cur = *dst;
-                    cur.re -= accr[x][y][v];
-                    cur.im -= acci[x][y][v];
-                    *dst = cur;
-                }
+                if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = C<T>{accr[x][y][v], acci[x][y][v]};
             }
 }
 
@@ -500,9 +647,9 @@ void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, 
 
 template <typename T>
 void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
-                       const FrontPlan& fp, void* P, void* CB) {
-    if (count <= 0 || total_blocks <= 0) return;
-    hipLaunchKernelGGL(k_extend_add<T>, dim3(total_blocks), dim3(256), 0, st, nodes, blk_start, count, fp,
+                       int row_chunks, const FrontPlan& fp, void* P, void* CB) {
+    if (count <= 0 || total_blocks <= 0 || row_chunks <= 0) return;
+    hipLaunchKernelGGL(k_extend_add<T>, dim3(total_blocks, row_chunks), dim3(256), 0, st, nodes, blk_start, count, fp,
                        static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
 }
 
@@ -510,9 +657,9 @@ template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,
                        FactorStatsDev* stats) {
-    const size_t diag_lds = static_cast<size_t>(2 * kNB * kNB + kNB) * sizeof(C<T>);
+    const size_t diag_lds = static_cast<size_t>(kNB * (kNB + 1) + 2 * kNB + (sizeof(T) == 8 ? kNB * kNB : 0)) * sizeof(C<T>);
     allow_large_lds(k_diag<T>, diag_lds);
-    hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(256), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
+    hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(64), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
                        static_cast<C<T>*>(CB), static_cast<C<T>*>(mws), static_cast<C<T>*>(xinv), pivot_floor, stats);
     if (max_row_blocks > 0) {
         const size_t panel_lds = static_cast<size_t>(64 * kNB + kNB * kNB) * sizeof(C<T>);
@@ -520,14 +667,20 @@ void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int m
         hipLaunchKernelGGL(k_panel<T>, dim3(max_row_blocks, count), dim3(256), panel_lds, st, nodes, b, fp,
                            static_cast<C<T>*>(P), static_cast<C<T>*>(CB), static_cast<const C<T>*>(mws), stats);
     }
-    if (max_tiles > 0) launch_update<T>(st, nodes, count, b, max_tiles, fp, P, CB);
+    (void)max_tiles;
 }
 
 template <typename T>
-void launch_update(hipStream_t st, const int* nodes, int count, int b, int max_tiles, const FrontPlan& fp, void* P,
-                   void* CB) {
-    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(256), 0, st, nodes, b, fp, static_cast<C<T>*>(P),
-                       static_cast<C<T>*>(CB));
+void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int bo, int inner, int max_tiles,
+                   const FrontPlan& fp, void* P, void* CB) {
+    if (count <= 0 || max_tiles <= 0) return;
+    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(256), 0, st, nodes, c0, width, bo, inner, fp,
+                       static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
+}
+
+template <typename T>
+int update_tile() {
+    return 32 * UpdTile<T>::WB;
 }
 
 template void launch_zero<double>(hipStream_t, int64_t, void*);
@@ -540,10 +693,12 @@ template void launch_front_small<double>(hipStream_t, const int*, int, int, cons
                                          double, FactorStatsDev*);
 template void launch_front_small<float>(hipStream_t, const int*, int, int, const FrontPlan&, void*, void*, void*,
                                         double, FactorStatsDev*);
-template void launch_extend_add<double>(hipStream_t, const int*, const int*, int, int, const FrontPlan&, void*, void*);
-template void launch_extend_add<float>(hipStream_t, const int*, const int*, int, int, const FrontPlan&, void*, void*);
-template void launch_update<double>(hipStream_t, const int*, int, int, int, const FrontPlan&, void*, void*);
-template void launch_update<float>(hipStream_t, const int*, int, int, int, const FrontPlan&, void*, void*);
+template void launch_extend_add<double>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
+template void launch_extend_add<float>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
+template void launch_update<double>(hipStream_t, const int*, int, int, int, int, int, int, const FrontPlan&, void*, void*);
+template void launch_update<float>(hipStream_t, const int*, int, int, int, int, int, int, const FrontPlan&, void*, void*);
+template int update_tile<double>();
+template int update_tile<float>();
 template void launch_block_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,
                                         void*, void*, double, FactorStatsDev*);
 template void launch_block_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,

@@ -8,7 +8,10 @@
 
 namespace aphi_dev {
 
-constexpr int kNB = 64;  // block-column width of the blocked front factorization and solves
+constexpr int kNB = 64;       // block-column width of the blocked front factorization and solves
+constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64)
+constexpr int kOuter = 256;   // columns per outer block: the trailing update is rank kOuter
+constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
 
 template <typename T> void launch_zero(hipStream_t st, int64_t n, void* p);
 template <typename T> void launch_scatter_panel(hipStream_t st, int64_t nnz, const int64_t* s2p, const cd* a, void* P);
@@ -18,10 +21,14 @@ void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, 
                         void* xinv, double pivot_floor, FactorStatsDev* stats);
 template <typename T>
 void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
-                       const FrontPlan& fp, void* P, void* CB);
+                       int row_chunks, const FrontPlan& fp, void* P, void* CB);
+// Rank-(c1 - c0) update with the L columns [c0, c1 = min(c0 + width, k)) of each
+// front: inner = 1 updates the columns [c1, min(bo + kOuter, k)) of the outer
+// block starting at bo, inner = 0 the whole trailing front [c1, m).
 template <typename T>
-void launch_update(hipStream_t st, const int* nodes, int count, int b, int max_tiles, const FrontPlan& fp, void* P,
-                   void* CB);
+void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int bo, int inner, int max_tiles,
+                   const FrontPlan& fp, void* P, void* CB);
+template <typename T> int update_tile();  // tile side of k_update
 template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,

@@ -40,6 +40,11 @@ struct FrontPlan {
     // Inverses of the unit-lower diagonal blocks, one per kNB columns of each
     // supernode (the last one nb x nb), column-major, kept for the solves.
     const int64_t* xinv_off;   // ns
+    // For a child of a blocked front: bound[ea_bound_off[child] + q] is the first
+    // index of the child's update rows whose parent position is >= 64 q
+    // (q = 0 .. ceil(m_parent / 64)); -1 for other supernodes.
+    const int64_t* ea_bound_off;  // ns
+    const int* ea_bound;
 };
 
 struct FactorStatsDev {

@@ -89,11 +89,14 @@ __global__ void __launch_bounds__(256) k_fwd_gather(const int* __restrict__ node
 // y_b = L_bb^-1 w_b from the stored block inverse (a 64x64 GEMV, recomputed per
 // workgroup instead of a serial substitution), workgroup 0 writes y_b to Y, and
 // every workgroup updates its 256 rows below the block: W[i] -= L[i, b:b+nb] y_b.
-// Rows b..b+nb of W are only read in this launch, so there is no race.
+// Rows b..b+nb of W are only read in this launch, so there is no race. Global
+// loads are issued in unrolled batches so that each thread has many in flight:
+// the step is latency-bound otherwise.
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_block(const int* __restrict__ nodes, int b, int nrhs, FrontPlan fp,
                                                    const C<T>* __restrict__ P, const C<T>* __restrict__ xinv, C<T>* W,
                                                    C<T>* __restrict__ Y, int n) {
+    constexpr int CH = sizeof(T) == 4 ? 32 : 16;  // L columns held in registers per batch
     __shared__ C<T> Xl[kNB * kNB];
     __shared__ C<T> Wb[16 * kNB];
     __shared__ C<T> Yb[16 * kNB];
@@ -107,45 +110,68 @@ __global__ void __launch_bounds__(256) k_fwd_block(const int* __restrict__ nodes
     const C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(b / kNB) * kNB * kNB;
     C<T>* Ws = W + x.rp * nrhs;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) Xl[idx] = Xg[idx];  // column q, row c at q*nb + c
+    {
+        C<T> t[kNB * kNB / 256];
+#pragma unroll
+        for (int u = 0; u < kNB * kNB / 256; ++u) {
+            const int idx = tid + 256 * u;
+            t[u] = idx < nb * nb ? Xg[idx] : C<T>{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < kNB * kNB / 256; ++u) Xl[tid + 256 * u] = t[u];  // column q, row c at q*nb + c
+    }
     const int i = i0 + tid;
     for (int r0 = 0; r0 < nrhs; r0 += 16) {
         const int nr = min(16, nrhs - r0);
         __syncthreads();
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int r = idx / nb, c = idx % nb;
-            Wb[r * kNB + c] = Ws[static_cast<int64_t>(r0 + r) * x.m + b + c];
+        {
+            C<T> t[16 * kNB / 256];
+#pragma unroll
+            for (int u = 0; u < 16 * kNB / 256; ++u) {
+                const int idx = tid + 256 * u, r = idx / kNB, c = idx % kNB;
+                t[u] = (r < nr && c < nb) ? Ws[static_cast<int64_t>(r0 + r) * x.m + b + c] : C<T>{0, 0};
+            }
+#pragma unroll
+            for (int u = 0; u < 16 * kNB / 256; ++u) Wb[tid + 256 * u] = t[u];
         }
         __syncthreads();
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int r = idx / nb, c = idx % nb;
+#pragma unroll
+        for (int u = 0; u < 16 * kNB / 256; ++u) {
+            const int idx = tid + 256 * u, r = idx / kNB, c = idx % kNB;
             C<T> acc = C<T>{0, 0};
-            for (int q = 0; q <= c; ++q) acc = gfma(acc, Xl[q * nb + c], Wb[r * kNB + q]);
-            Yb[r * kNB + c] = acc;
-            if (blockIdx.x == 0) Y[static_cast<int64_t>(r0 + r) * n + x.first + b + c] = acc;
+            if (r < nr && c < nb) {
+                for (int q = 0; q <= c; ++q) acc = gfma(acc, Xl[q * nb + c], Wb[r * kNB + q]);
+                if (blockIdx.x == 0) Y[static_cast<int64_t>(r0 + r) * n + x.first + b + c] = acc;
+            }
+            Yb[idx] = acc;  // zero outside nb x nr
         }
         __syncthreads();
         if (i < x.m) {
             C<T> acc[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] = C<T>{0, 0};
-            for (int c = 0; c < nb; ++c) {
-                const C<T> l = Ps[static_cast<int64_t>(b + c) * x.m + i];
+            for (int c0 = 0; c0 < nb; c0 += CH) {
+                C<T> l[CH];
 #pragma unroll
-                for (int r = 0; r < 16; ++r)
-                    if (r < nr) acc[r] = gfma(acc[r], l, Yb[r * kNB + c]);
+                for (int u = 0; u < CH; ++u)
+                    l[u] = c0 + u < nb ? Ps[static_cast<int64_t>(b + c0 + u) * x.m + i] : C<T>{0, 0};
+#pragma unroll
+                for (int u = 0; u < CH; ++u)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) acc[r] = gfma(acc[r], l[u], Yb[r * kNB + c0 + u]);
             }
+            C<T> w[16];
 #pragma unroll
             for (int r = 0; r < 16; ++r)
-                if (r < nr) {
-                    C<T>& dst = Ws[static_cast<int64_t>(r0 + r) * x.m + i];
-                    dst = gsub(dst, acc[r]);
-                }
+                if (r < nr) w[r] = Ws[static_cast<int64_t>(r0 + r) * x.m + i];
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (r < nr) Ws[static_cast<int64_t>(r0 + r) * x.m + i] = gsub(w[r], acc[r]);
         }
     }
 }
 
-// partial[slot][blk] (nb x nrhs) = sum over 64 rows of L[row, b+c] * x[row]
+// partial[slot][blk] (nb x nrhs, c-major) = sum over 64 rows of L[row, b+c] * x[row]
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nodes, int b, int nrhs, int stride,
                                                      FrontPlan fp, const C<T>* __restrict__ P,
@@ -159,35 +185,61 @@ __global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nod
     if (i0 >= x.m) return;
     const int nrows = min(64, x.m - i0);
     C<T>* Lt = reinterpret_cast<C<T>*>(smem_raw);  // Lt[c*64 + row]
-    C<T>* Xt = Lt + kNB * 64;                      // Xt[r*64 + row]
+    C<T>* Xt = Lt + kNB * 64;                      // Xt[row*nrhs + r]
     const C<T>* Ps = P + fp.panel_off[s];
     const int* rows = fp.rows + x.rp;
-    for (int idx = threadIdx.x; idx < nb * 64; idx += blockDim.x) {
-        const int c = idx / 64, row = idx % 64;
-        Lt[c * 64 + row] = row < nrows ? Ps[static_cast<int64_t>(b + c) * x.m + i0 + row] : C<T>{0, 0};
+    const int tid = threadIdx.x;
+    {
+        C<T> t[kNB * 64 / 256];
+#pragma unroll
+        for (int u = 0; u < kNB * 64 / 256; ++u) {
+            const int idx = tid + 256 * u, c = idx / 64, row = idx % 64;
+            t[u] = (c < nb && row < nrows) ? Ps[static_cast<int64_t>(b + c) * x.m + i0 + row] : C<T>{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < kNB * 64 / 256; ++u) Lt[tid + 256 * u] = t[u];
     }
-    for (int idx = threadIdx.x; idx < nrhs * 64; idx += blockDim.x) {
-        const int r = idx / 64, row = idx % 64;
-        Xt[r * 64 + row] = row < nrows ? X[static_cast<int64_t>(r) * n + rows[i0 + row]] : C<T>{0, 0};
+    {
+        // rows x nrhs gather of the solved unknowns (nrhs <= 64)
+        int g[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u, row = idx % 64;
+            g[u] = (idx < nrhs * 64 && row < nrows) ? rows[i0 + row] : -1;
+        }
+        C<T> t[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u, r = idx / 64;
+            t[u] = g[u] >= 0 ? X[static_cast<int64_t>(r) * n + g[u]] : C<T>{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int idx = tid + 256 * u, r = idx / 64, row = idx % 64;
+            if (idx < nrhs * 64) Xt[row * nrhs + r] = t[u];
+        }
     }
     __syncthreads();
     C<T>* out = partial + (static_cast<int64_t>(blockIdx.y) * stride + blockIdx.x) * kNB * nrhs;
-    for (int idx = threadIdx.x; idx < nb * nrhs; idx += blockDim.x) {
+    for (int idx = tid; idx < nb * nrhs; idx += 256) {
         const int c = idx / nrhs, r = idx % nrhs;
         C<T> acc = C<T>{0, 0};
-        for (int row = 0; row < 64; ++row) acc = gfma(acc, Lt[c * 64 + row], Xt[r * 64 + row]);
+#pragma unroll 8
+        for (int row = 0; row < 64; ++row) acc = gfma(acc, Lt[c * 64 + row], Xt[row * nrhs + r]);
         out[c * nrhs + r] = acc;
     }
 }
 
 // x_b = L_bb^-T (D_b^-1 y_b - sum of partials), written to X: a GEMV with the
-// stored block inverse (L_bb^-T)[c][q] = Linv[q][c], q >= c.
+// stored block inverse (L_bb^-T)[c][q] = Linv[q][c], q >= c. The partials are
+// summed in block order (deterministic).
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_final(const int* __restrict__ nodes, int b, int nrhs, int stride,
                                                    FrontPlan fp, const C<T>* __restrict__ P,
                                                    const C<T>* __restrict__ xinv, const C<T>* __restrict__ Y,
                                                    C<T>* __restrict__ X, int n, const C<T>* __restrict__ partial) {
-    __shared__ C<T> Xl[kNB * kNB];
+    constexpr int LD = kNB + 1;  // padded rows: lanes read along c
+    __shared__ C<T> Xl[kNB * LD];
     __shared__ C<T> Z[16 * kNB];
     const int s = nodes[blockIdx.x];
     const Sn x = sn(fp, s);
@@ -198,25 +250,44 @@ __global__ void __launch_bounds__(256) k_bwd_final(const int* __restrict__ nodes
     const C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(b / kNB) * kNB * kNB;
     const C<T>* part = partial + static_cast<int64_t>(blockIdx.x) * stride * kNB * nrhs;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += 256) Xl[idx] = Xg[idx];
+    {
+        C<T> t[kNB * kNB / 256];
+#pragma unroll
+        for (int u = 0; u < kNB * kNB / 256; ++u) {
+            const int idx = tid + 256 * u;
+            t[u] = idx < nb * nb ? Xg[idx] : C<T>{0, 0};
+        }
+#pragma unroll
+        for (int u = 0; u < kNB * kNB / 256; ++u) {
+            const int idx = tid + 256 * u;
+            if (idx < nb * nb) Xl[(idx / nb) * LD + idx % nb] = t[u];  // Xl[j*LD + i] = Linv[i][j]
+        }
+    }
     for (int r0 = 0; r0 < nrhs; r0 += 16) {
         const int nr = min(16, nrhs - r0);
         __syncthreads();
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int r = idx / nb, c = idx % nb;
-            C<T> t = C<T>{0, 0};
-            for (int blk = 0; blk < nblk; ++blk)
-                t = gadd(t, part[static_cast<int64_t>(blk) * kNB * nrhs + c * nrhs + (r0 + r)]);
-            const C<T> y = Y[static_cast<int64_t>(r0 + r) * n + x.first + b + c];
-            const C<T> d = Ps[static_cast<int64_t>(b + c) * x.m + b + c];
-            Z[r * kNB + c] = gsub(gmul(y, grecip(d)), t);
+#pragma unroll
+        for (int u = 0; u < 16 * kNB / 256; ++u) {
+            const int idx = tid + 256 * u, c = idx / 16, r = idx % 16;
+            if (c < nb && r < nr) {
+                C<T> t = C<T>{0, 0};
+                const C<T>* pp = part + c * nrhs + (r0 + r);
+#pragma unroll 8
+                for (int blk = 0; blk < nblk; ++blk) t = gadd(t, pp[static_cast<int64_t>(blk) * kNB * nrhs]);
+                const C<T> y = Y[static_cast<int64_t>(r0 + r) * n + x.first + b + c];
+                const C<T> d = Ps[static_cast<int64_t>(b + c) * x.m + b + c];
+                Z[r * kNB + c] = gsub(gmul(y, grecip(d)), t);
+            }
         }
         __syncthreads();
-        for (int idx = tid; idx < nb * nr; idx += 256) {
-            const int r = idx / nb, c = idx % nb;
-            C<T> acc = C<T>{0, 0};
-            for (int q = c; q < nb; ++q) acc = gfma(acc, Xl[c * nb + q], Z[r * kNB + q]);
-            X[static_cast<int64_t>(r0 + r) * n + x.first + b + c] = acc;
+#pragma unroll
+        for (int u = 0; u < 16 * kNB / 256; ++u) {
+            const int idx = tid + 256 * u, r = idx / kNB, c = idx % kNB;
+            if (c < nb && r < nr) {
+                C<T> acc = C<T>{0, 0};
+                for (int q = c; q < nb; ++q) acc = gfma(acc, Xl[c * LD + q], Z[r * kNB + q]);
+                X[static_cast<int64_t>(r0 + r) * n + x.first + b + c] = acc;
+            }
         }
     }
 }

@@ -23,6 +23,11 @@ def golden(name):
     return meta, arrays
 
 
+def conditioning(name):
+    """Condition numbers of a golden numeric case (tests/golden/make_conditioning.py)."""
+    return json.load(open(os.path.join(GOLDEN, "conditioning.json")))[name]
+
+
 def cube_config(meta):
     import aphi_amd as A
     return A.CubeConfig(n=meta["n"], side=meta["side"], bar=tuple(meta["bar"]), sigma=meta["sigma"])

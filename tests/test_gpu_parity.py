@@ -5,7 +5,7 @@ refinement path (backward error <= 1e-12), pivot refusal, determinism."""
 import numpy as np
 import pytest
 
-from conftest import cube_config, golden
+from conftest import conditioning, cube_config, golden
 
 import aphi_amd as A
 from oracle import oracle as O
@@ -102,16 +102,21 @@ def test_fp64_solution_matches_oracle(case):
     assert rep["backward_error"] < 1e-14
 
 
-@pytest.mark.parametrize("tol,err", [(1e-12, 1e-5), (1e-15, 1e-10)])
-def test_fp32_refined_solution(case, tol, err):
-    """FP32 factor + FP64 refinement. At the north-star tolerance (backward error
-    1e-12) the solution error against complex128 is bounded by the conditioning
-    (reported, here <= 1e-5); refined to 1e-15 it matches the plain FP64 bar."""
+@pytest.mark.parametrize("tol", [1e-12, 1e-15])
+def test_fp32_refined_solution(case, tol):
+    """FP32 factor + FP64 refinement to backward error `tol` (normwise and on the
+    equilibrated system). Its distance to the reference's complex128 solution is
+    bounded by the conditioning: kappa_2(D A D) * (tol + the reference's own
+    backward error, ~1e-15), with a factor 4 for the first-order bound and the
+    change of norm under D. kappa comes from a dense SVD of the golden matrix
+    (tests/golden/make_conditioning.py): 1.5e7 for cube8."""
     m = case["m"]
     m.refill(case["meta"]["omega"])
     x, rep = m.solve_symmetric(dev(case["rhs"]), precision=A.APHI_FP32, max_refinements=40, tolerance=tol)
     assert rep["ok"] and rep["backward_error"] <= tol and rep["scaled_backward_error"] <= tol
-    assert np.linalg.norm(x.cpu().numpy() - case["x"]) / np.linalg.norm(case["x"]) <= err
+    kappa = conditioning(case["meta"]["case"])["cond2_equilibrated"]
+    bound = 4.0 * kappa * (tol + 1e-15)
+    assert np.linalg.norm(x.cpu().numpy() - case["x"]) / np.linalg.norm(case["x"]) <= bound
 
 
 def test_multi_rhs_matches_oracle(case):
@@ -173,12 +178,18 @@ def test_config2_sweep_points(gpu_context, freq):
     a, b = x64.cpu().numpy(), x32.cpu().numpy()
     for r in range(B.shape[0]):
         assert np.linalg.norm(a[r] - b[r]) / np.linalg.norm(a[r]) <= 1e-5
-    # Driven to backward error 1e-15, both precisions converge to the same
-    # solution; the unrefined FP64 solve above carries cond(A) * 1e-16 of its own
-    # error, so the anchor here is the refined FP64 solution.
+    # Driven to backward error 1e-15, both precisions satisfy the equations to
+    # that level, checked independently by the oracle's backward error (the
+    # reference's formula, on the CPU). Their mutual distance is then bounded by
+    # kappa * 2e-15 only; config2 at high frequency is too large for a dense
+    # condition number, so the agreement bound here is the loose 1e-7.
     x64s, r64s = m.solve_symmetric(dev(B), precision=A.APHI_FP64, max_refinements=10, tolerance=1e-15)
     x32s, r32s = m.solve_symmetric(dev(B), precision=A.APHI_FP32, max_refinements=40, tolerance=1e-15)
     a, c = x64s.cpu().numpy(), x32s.cpu().numpy()
+    print("tight", freq, r64s["backward_error"], r64s["refinement_steps"], r32s["backward_error"],
+          r32s["refinement_steps"])
     for r in range(B.shape[0]):
-        assert np.linalg.norm(a[r] - c[r]) / np.linalg.norm(a[r]) <= 1e-9
+        assert O.backward_error(p, vals, a[r], B[r])[0] <= 2e-15
+        assert O.backward_error(p, vals, c[r], B[r])[0] <= 2e-15
+        assert np.linalg.norm(a[r] - c[r]) / np.linalg.norm(a[r]) <= 1e-7
     m.close()
