@@ -40,6 +40,23 @@ def lib():
     return _capi.load()
 
 
+def _alive():
+    """False while the interpreter shuts down (module globals may already be gone)."""
+    try:
+        return not sys.is_finalizing() and _capi._lib is not None
+    except Exception:
+        return False
+
+
+def _destroy(fn_name, h):
+    """Calls a destroy entry point unless the interpreter is shutting down."""
+    try:
+        if h and _alive():
+            getattr(_capi._lib, fn_name)(h)
+    except Exception:
+        pass
+
+
 def _check(status):
     if status == _capi.APHI_OK:
         return
@@ -103,9 +120,7 @@ class Problem:
         self.counts = {k: getattr(c, k) for k, _ in _capi.ProblemCounts._fields_}
 
     def __del__(self):
-        h = getattr(self, "_h", None)
-        if h and not sys.is_finalizing() and _capi._lib is not None:
-            _capi._lib.aphi_problem_destroy(h)
+        _destroy("aphi_problem_destroy", getattr(self, "_h", None))
         self._h = None
 
     @property
@@ -151,8 +166,7 @@ class Context:
         """Destroys the context; meshes created on it are destroyed first."""
         for m in list(getattr(self, "_meshes", ())):
             m.close()
-        if getattr(self, "_h", None) and not sys.is_finalizing() and _capi._lib is not None:
-            _capi._lib.aphi_context_destroy(self._h)
+        _destroy("aphi_context_destroy", getattr(self, "_h", None))
         self._h = None
 
     def __del__(self):
@@ -184,8 +198,7 @@ class Mesh:
         self.nnz = int(problem.counts["nnz"])
 
     def close(self):
-        if getattr(self, "_h", None) and not sys.is_finalizing() and _capi._lib is not None:
-            _capi._lib.aphi_mesh_destroy(self._h)
+        _destroy("aphi_mesh_destroy", getattr(self, "_h", None))
         self._h = None
 
     def __del__(self):
@@ -266,6 +279,22 @@ class Mesh:
         report = {k: getattr(rep, k) for k, _ in _capi.SolveReport._fields_}
         report["backward_errors"] = be
         return x.reshape(b.shape), report
+
+    # terminal_current + impedance -- postprocess.cpp:526-547, postprocess.hpp:356-358
+    def terminal_current(self, x, omega, length, axis=(0.0, 0.0, 1.0), voltage=1.0, conditioning=APHI_ROW_SCALED):
+        """Port current I (complex), R = Re(V/I) and L = Im(V/I)/omega of each
+        solution in x (device complex128, n per row)."""
+        x2 = x.reshape(-1, self.n)
+        k = x2.shape[0]
+        cur = np.zeros(2 * k)
+        res = np.zeros(k)
+        ind = np.zeros(k)
+        ax = np.ascontiguousarray(axis, dtype=np.float64)
+        _check(lib().aphi_terminal_current(self._h, k, _dev_ptr(x2), self.n, float(omega), int(conditioning),
+                                           ax.ctypes.data_as(_capi.DP), float(length), float(voltage),
+                                           cur.ctypes.data_as(_capi.DP), res.ctypes.data_as(_capi.DP),
+                                           ind.ctypes.data_as(_capi.DP)))
+        return cur.view(np.complex128), res, ind
 
     # backward_error / relative_residual -- factorization.hpp:140, :130
     def backward_error(self, x, b):

@@ -61,6 +61,7 @@ class MeshDesc(C.Structure):
         ("super_rel", I32P), ("super_level", I32P), ("panel_offset", I64P), ("cb_offset", I64P),
         ("cb_pool_entries", C.c_int64), ("slot_to_panel", I64P),
         ("flops", C.c_double), ("predicted_nnz", C.c_int64),
+        ("tet_dofs", I32P), ("tet_edge_signs", C.POINTER(C.c_int8)), ("tet_phi_coeff", DP), ("tet_phi_fixed", DP),
     ]
 
 
@@ -114,6 +115,8 @@ SIGNATURES = {
     "aphi_solve_symmetric": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                        C.POINTER(SolveOptions), C.POINTER(SolveReport), DP]),
     "aphi_backward_error": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, DP, DP]),
+    "aphi_terminal_current": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, DP,
+                                        C.c_double, C.c_double, DP, DP, DP]),
 }
 
 _lib = None

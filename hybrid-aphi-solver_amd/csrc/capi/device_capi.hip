@@ -104,12 +104,15 @@ struct aphi_mesh {
     // plan on the device
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
-    DevBuf ea_bound_off, ea_bound;
+    DevBuf ea_bound_off, ea_bound, x256_off, inv_pairs;
+    DevBuf tdofs, tsign, tcoef, tfixed, bsigma, tcontrib, tcurrent;  // port-current recovery
+    int64_t x256_entries = 0;
+    int num_inv_pairs = 0;
     int64_t xinv_entries = 0;
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
-    DevBuf P, CB, mws, Xinv;
-    DevBuf bt, W, X, Yv, bpart, colscale, res, norms;
+    DevBuf P, CB, mws, Xinv, X256, XT256;
+    DevBuf bt, W, X, Yv, bpart, tsum, colscale, res, norms;
     int solve_nrhs_cap = 0;
     bool assembled = false, equilibrated = false, factored = false;
     int factor_precision = -1;
@@ -132,6 +135,7 @@ struct aphi_mesh {
         fp.xinv_off = xinv_off.as<int64_t>();
         fp.ea_bound_off = ea_bound_off.as<int64_t>();
         fp.ea_bound = ea_bound.as<int>();
+        fp.x256_off = x256_off.as<int64_t>();
         return fp;
     }
     hipStream_t st() const { return ctx->stream; }
@@ -214,6 +218,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     m->CB.reserve(static_cast<size_t>(std::max<int64_t>(m->cb_entries, 1)) * cs);
     m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
     m->Xinv.reserve(static_cast<size_t>(std::max<int64_t>(m->xinv_entries, 1)) * cs);
+    m->X256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
+    m->XT256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
     m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
     aphi_dev::FactorStatsDev init{std::numeric_limits<double>::infinity(), 0.0, 0.0, INT_MAX, 0};
     HIPCHK(hipMemcpyAsync(m->stats.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
@@ -284,6 +290,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
             if ((b + aphi_dev::kNB) % aphi_dev::kOuter == 0 || b + aphi_dev::kNB >= maxk) update(bo, aphi_dev::kOuter, bo, 0);
         }
     }
+    aphi_dev::launch_inv_blocks<T>(s, m->inv_pairs.as<int2>(), m->num_inv_pairs, fp, m->P.p, m->Xinv.p, m->X256.p,
+                                   m->XT256.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(m->ctx->ev1, s));
     aphi_dev::FactorStatsDev h{};
@@ -323,7 +331,8 @@ void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     m->X.reserve(static_cast<size_t>(m->n) * nrhs * cs);
     m->Yv.reserve(static_cast<size_t>(m->n) * nrhs * cs);
     m->W.reserve(static_cast<size_t>(m->rows_total) * nrhs * cs);
-    m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kNB * nrhs * cs);
+    m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kSB * nrhs * cs);
+    m->tsum.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kSB * nrhs * cs);
     m->colscale.reserve(nrhs * sizeof(double));
     m->res.reserve(static_cast<size_t>(m->n) * nrhs * sizeof(cd));
     m->norms.reserve(static_cast<size_t>(64) * nrhs * 6 * sizeof(double));
@@ -341,21 +350,26 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
     aphi_dev::launch_column_scale(s, n, nrhs, b, ldb, m->partials.as<double>(), m->colscale.as<double>(), mode);
     aphi_dev::launch_perm_in<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), b, ldb, m->colscale.as<double>(), m->bt.p);
     const int* nodes = m->nodes.as<int>();
+    constexpr int SB = aphi_dev::kSB;
+    // rows below block B of the level's supernodes, in 64-row chunks (max over the level)
+    const auto row_chunks = [](const Level& L, int B) {
+        int rb = 0;
+        for (int i = 0; i < L.count(); ++i) {
+            const int kk = L.k[static_cast<size_t>(i)];
+            if (kk <= B) continue;
+            const int rem = L.m[static_cast<size_t>(i)] - B - std::min(aphi_dev::kSB, kk - B);
+            if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
+        }
+        return rb;
+    };
     for (const Level& L : m->levels) {
         const int cnt = L.count();
         aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, nrhs, fp, m->bt.p, n, m->W.p);
         int maxk = 0;
         for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
-        for (int b0 = 0; b0 < maxk; b0 += aphi_dev::kNB) {
-            int rb = 0;
-            for (int i = 0; i < cnt; ++i) {
-                const int kk = L.k[static_cast<size_t>(i)];
-                if (kk <= b0) continue;
-                const int rem = L.m[static_cast<size_t>(i)] - b0 - std::min(aphi_dev::kNB, kk - b0);
-                if (rem > 0) rb = std::max(rb, (rem + 255) / 256);
-            }
-            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->Xinv.p, m->W.p, m->Yv.p, n);
-        }
+        for (int B = 0; B < maxk; B += SB)
+            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), row_chunks(L, B), nrhs, fp,
+                                         m->P.p, m->X256.p, m->W.p, m->Yv.p, n);
     }
     for (int li = m->num_levels - 1; li >= 0; --li) {
         const Level& L = m->levels[static_cast<size_t>(li)];
@@ -363,17 +377,9 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         int maxk = 0;
         for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
         if (maxk == 0) continue;
-        for (int b0 = ((maxk - 1) / aphi_dev::kNB) * aphi_dev::kNB; b0 >= 0; b0 -= aphi_dev::kNB) {
-            int rb = 0;
-            for (int i = 0; i < cnt; ++i) {
-                const int kk = L.k[static_cast<size_t>(i)];
-                if (kk <= b0) continue;
-                const int rem = L.m[static_cast<size_t>(i)] - b0 - std::min(aphi_dev::kNB, kk - b0);
-                if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
-            }
-            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, b0, rb, nrhs, fp, m->P.p, m->Xinv.p, m->Yv.p, m->X.p, n,
-                                         m->bpart.p);
-        }
+        for (int B = ((maxk - 1) / SB) * SB; B >= 0; B -= SB)
+            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), row_chunks(L, B), nrhs, fp,
+                                         m->P.p, m->XT256.p, m->Yv.p, m->X.p, n, m->bpart.p, m->tsum.p);
     }
     aphi_dev::launch_perm_out<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), m->X.p, m->colscale.as<double>(), x, ldx,
                                  accumulate);
@@ -631,7 +637,7 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                         ea.push_back(blk);
                         blk += (mm + 63) / 64;
                     }
-                    maxrb = std::max(maxrb, (mm - std::min(kk, aphi_dev::kNB) + 63) / 64);
+                    maxrb = std::max(maxrb, (mm - std::min(kk, aphi_dev::kSB) + 63) / 64);
                 }
             lv.ea_blocks = blk;
             ea.push_back(blk);
@@ -651,6 +657,21 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             }
             m->xinv_entries = off;
             upload(m->xinv_off, xo.data(), xo.size());
+            // kSB-block inverses for the solves, and the (supernode, block) list that builds them
+            std::vector<int64_t> xb(static_cast<size_t>(ns));
+            std::vector<int2> pairs;
+            off = 0;
+            for (int s = 0; s < ns; ++s) {
+                const int64_t kk = d->super_first[s + 1] - d->super_first[s];
+                xb[static_cast<size_t>(s)] = off;
+                const int64_t rest = kk % aphi_dev::kSB;
+                off += (kk / aphi_dev::kSB) * aphi_dev::kSB * aphi_dev::kSB + rest * rest;
+                for (int64_t B = 0; B < kk; B += aphi_dev::kSB) pairs.push_back(int2{s, static_cast<int>(B)});
+            }
+            m->x256_entries = off;
+            m->num_inv_pairs = static_cast<int>(pairs.size());
+            upload(m->x256_off, xb.data(), xb.size());
+            upload(m->inv_pairs, pairs.data(), pairs.size());
         }
         upload(m->ea_start, ea.data(), ea.size());
         {
@@ -677,6 +698,14 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             upload(m->ea_bound_off, bo.data(), bo.size());
             upload(m->ea_bound, bv.data(), bv.size());
         }
+        if (d->tet_dofs && d->tet_edge_signs && d->tet_phi_coeff && d->tet_phi_fixed) {
+            const size_t nt = static_cast<size_t>(d->num_tets);
+            upload(m->tdofs, d->tet_dofs, nt * 16);
+            upload(m->tsign, d->tet_edge_signs, nt * 6);
+            upload(m->tcoef, d->tet_phi_coeff, nt * 10);
+            upload(m->tfixed, d->tet_phi_fixed, nt * 20);
+            upload(m->bsigma, d->body_sigma, static_cast<size_t>(d->num_bodies));
+        }
         m->flag.reserve(sizeof(int));
         *out = m.release();
     });
@@ -702,7 +731,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->Xinv, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
                            &m->bpart, &m->colscale, &m->res, &m->norms};
     int64_t total = 0;
@@ -805,6 +834,56 @@ extern "C" int aphi_system_upload(aphi_mesh* m, const double* values, const doub
 
 extern "C" const void* aphi_rhs_device(const aphi_mesh* m) { return m ? m->rhs.p : nullptr; }
 
+namespace {
+// std::complex<double> division as g++'s libgcc __divdc3 evaluates it for finite
+// operands (Smith's method, no scaling), so V / I is the reference's impedance.
+Complex divide_like_libgcc(Complex x, Complex y) {
+    const double a = x.real(), b = x.imag(), c = y.real(), d = y.imag();
+    if (std::fabs(c) < std::fabs(d)) {
+        const double ratio = c / d, denom = (c * ratio) + d;
+        return Complex(((a * ratio) + b) / denom, ((b * ratio) - a) / denom);
+    }
+    const double ratio = d / c, denom = (d * ratio) + c;
+    return Complex(((b * ratio) + a) / denom, (b - (a * ratio)) / denom);
+}
+}  // namespace
+
+extern "C" int aphi_terminal_current(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, double omega, int conditioning,
+                                     const double* axis, double length, double voltage, double* current,
+                                     double* resistance, double* inductance) {
+    return aphi_capi::guarded([&] {
+        require(m && xv && axis && current, "aphi_terminal_current: null argument");
+        require(nrhs >= 1 && nrhs <= 64 && ldx >= m->n, "aphi_terminal_current: nrhs must be 1..64 and ldx >= n");
+        require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, "aphi_terminal_current: unknown conditioning");
+        require(m->tdofs.p != nullptr, "aphi_terminal_current: the mesh was created without per-tet unknowns");
+        if (!(length > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
+        hipStream_t s = m->st();
+        const int nt = m->num_tets;
+        m->tcontrib.reserve(static_cast<size_t>(nt) * nrhs * sizeof(cd));
+        m->tcurrent.reserve(static_cast<size_t>(nrhs) * sizeof(cd));
+        aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
+                                  m->tdofs.as<int>(), m->tsign.as<int8_t>(), m->tcoef.as<double>(), m->tfixed.as<cd>()};
+        // -jw and the ScaledPhi substitution factor (Solution::phi_scale, src/postprocess.cpp:35-40)
+        const cd minus_jw{-0.0, -omega};
+        const cd scale = conditioning == APHI_SCALED_PHI ? cd{0.0, omega} : cd{1.0, 0.0};
+        hipLaunchKernelGGL(aphi_dev::k_tet_current, dim3(blocks_for(nt), nrhs), dim3(256), 0, s, nt, tp, nrhs,
+                           static_cast<const cd*>(xv), ldx, minus_jw, scale, axis[0], axis[1], axis[2], m->tcontrib.as<cd>());
+        hipLaunchKernelGGL(aphi_dev::k_current_sum, dim3(nrhs), dim3(64), 0, s, nt, tp, m->tcontrib.as<cd>(), length,
+                           m->tcurrent.as<cd>());
+        std::vector<cd> h(static_cast<size_t>(nrhs));
+        HIPCHK(hipMemcpyAsync(h.data(), m->tcurrent.p, h.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipGetLastError());
+        for (int r = 0; r < nrhs; ++r) {
+            current[2 * r] = h[static_cast<size_t>(r)].re;
+            current[2 * r + 1] = h[static_cast<size_t>(r)].im;
+            const Complex z = divide_like_libgcc(Complex(voltage, 0.0), Complex(h[static_cast<size_t>(r)].re, h[static_cast<size_t>(r)].im));
+            if (resistance) resistance[r] = z.real();
+            if (inductance) inductance[r] = z.imag() / omega;
+        }
+    });
+}
+
 extern "C" int aphi_equilibrate(aphi_mesh* m, int iterations, double* d_out) {
     return aphi_capi::guarded([&] {
         require(m != nullptr, "aphi_equilibrate: null mesh");
@@ -904,20 +983,33 @@ extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int6
         std::vector<double> be(static_cast<size_t>(nrhs)), rr(static_cast<size_t>(nrhs)), bs(static_cast<size_t>(nrhs));
         int steps = 0, since_factor = 0;
         r.precision_used = opt->precision;
+        // largest entry, +inf when any is NaN (so a non-finite solution never passes)
+        const auto worst_of = [](const std::vector<double>& v) {
+            double w = 0.0;
+            for (double e : v) w = std::isnan(e) ? std::numeric_limits<double>::infinity() : std::max(w, e);
+            return w;
+        };
         for (;;) {
             measure(m, nrhs, x, ldx, b, ldb, be.data(), rr.data(), bs.data());
-            if (opt->max_refinements == 0) break;
-            const double worst = std::max(*std::max_element(be.begin(), be.end()), *std::max_element(bs.begin(), bs.end()));
+            const double worst = std::max(worst_of(be), worst_of(bs));
+            const bool finite = std::isfinite(worst);
+            if (opt->max_refinements == 0 && finite) break;
             if (worst <= opt->tolerance) break;
-            if (since_factor >= opt->max_refinements) {
+            if (since_factor >= opt->max_refinements || !finite) {
                 if (m->factor_precision != APHI_FP32) break;
-                // FP32 refinement did not converge: factor in FP64 and keep refining.
+                // FP32 refinement did not converge (or overflowed): factor in FP64
+                // and keep refining; a non-finite iterate restarts from b.
                 aphi_factor_stats f64{};
                 factorize(m, APHI_FP64, opt->pivot_floor, &f64);
                 r.factorize_ms += f64.milliseconds;
                 r.precision_used = APHI_FP64;
                 since_factor = 0;
                 if (!f64.ok) break;
+                if (!finite) {
+                    solve_any(m, nrhs, b, ldb, x, ldx, 0);
+                    continue;
+                }
+                if (opt->max_refinements == 0) continue;
             }
             solve_any(m, nrhs, m->res.as<cd>(), m->n, x, ldx, 1);
             ++steps;
@@ -925,12 +1017,12 @@ extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int6
         }
         const auto s1 = std::chrono::steady_clock::now();
         r.refinement_steps = steps;
-        r.backward_error = *std::max_element(be.begin(), be.end());
-        r.scaled_backward_error = *std::max_element(bs.begin(), bs.end());
-        r.residual = *std::max_element(rr.begin(), rr.end());
+        r.backward_error = worst_of(be);
+        r.scaled_backward_error = worst_of(bs);
+        r.residual = worst_of(rr);
         r.solve_ms = std::chrono::duration<double, std::milli>(s1 - s0).count();
         r.total_ms = std::chrono::duration<double, std::milli>(s1 - t0).count();
-        r.ok = 1;
+        r.ok = std::isfinite(r.backward_error) ? 1 : 0;
         if (backward_errors) std::memcpy(backward_errors, be.data(), be.size() * sizeof(double));
         *rep = r;
     });

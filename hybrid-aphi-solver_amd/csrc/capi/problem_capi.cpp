@@ -28,6 +28,9 @@ struct aphi_problem {
     std::vector<double> coords;
     std::vector<int32_t> tet_nodes, tet_body, col_ref, super_level;
     std::vector<double> body_sigma, body_eps_r, body_mu_r, values;
+    std::vector<int32_t> tet_dofs;
+    std::vector<int8_t> tet_signs;
+    std::vector<double> tet_phi_coeff, tet_phi_fixed;
     double analyze_ms = 0.0, build_ms = 0.0;
 };
 
@@ -102,6 +105,26 @@ extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem
         for (const Complex& v : p->plan.fixed_value) {
             p->values.push_back(v.real());
             p->values.push_back(v.imag());
+        }
+        {
+            const size_t nt = static_cast<size_t>(m.num_tets());
+            p->tet_dofs.resize(nt * 16);
+            p->tet_signs.resize(nt * 6);
+            p->tet_phi_coeff.resize(nt * 10);
+            p->tet_phi_fixed.resize(nt * 20);
+            for (size_t t = 0; t < nt; ++t) {
+                const TetDofs d = p->dofs.local_dofs(static_cast<int>(t), m, p->bound);
+                for (int i = 0; i < 6; ++i) {
+                    p->tet_dofs[16 * t + i] = d.edge_index[i];
+                    p->tet_signs[6 * t + i] = m.tet_edge_signs[t][static_cast<size_t>(i)];
+                }
+                for (int i = 0; i < 10; ++i) {
+                    p->tet_dofs[16 * t + 6 + i] = d.phi_index[i];
+                    p->tet_phi_coeff[10 * t + i] = d.phi_coeff[i];
+                    p->tet_phi_fixed[20 * t + 2 * i] = d.phi_fixed[i].real();
+                    p->tet_phi_fixed[20 * t + 2 * i + 1] = d.phi_fixed[i].imag();
+                }
+            }
         }
         const int n = p->sp.n;
         p->col_ref.resize(static_cast<size_t>(n));
@@ -250,6 +273,8 @@ extern "C" int aphi_problem_get_double(const aphi_problem* p, const char* name, 
         const std::vector<double>* src = &tmp;
         if (k == "coords") {
             src = &p->coords;
+        } else if (k == "tet_phi_coeff") {
+            src = &p->tet_phi_coeff;
         } else if (k == "tet_phi_fixed") {
             for (int t = 0; t < p->mesh.num_tets(); ++t) {
                 const TetDofs d = p->dofs.local_dofs(t, p->mesh, p->bound);
@@ -314,5 +339,9 @@ extern "C" int aphi_problem_describe(const aphi_problem* p, aphi_mesh_desc* d) {
         d->slot_to_panel = p->s2p.data();
         d->flops = p->sp.flops;
         d->predicted_nnz = static_cast<int64_t>(p->analysis.predicted_nnz);
+        d->tet_dofs = p->tet_dofs.data();
+        d->tet_edge_signs = p->tet_signs.data();
+        d->tet_phi_coeff = p->tet_phi_coeff.data();
+        d->tet_phi_fixed = p->tet_phi_fixed.data();
     });
 }

@@ -99,6 +99,49 @@ __device__ __forceinline__ double dot3(const double a[3], const double b[3]) {
     return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
+// compute_tet_geometry (src/basis_functions.cpp:15-85) of tet t: barycentric
+// gradients and |signed_tet_volume| (src/mesh.cpp:151-161). False for a
+// degenerate tet.
+__device__ bool tet_geometry(const double* __restrict__ coords, const int* __restrict__ tets, int t, Geo& G) {
+    double m[4][4], inv[4][4];
+    double p[4][3];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int v = tets[4 * t + j];
+        p[j][0] = coords[3 * v + 0];
+        p[j][1] = coords[3 * v + 1];
+        p[j][2] = coords[3 * v + 2];
+        m[j][0] = 1.0;
+        m[j][1] = p[j][0];
+        m[j][2] = p[j][1];
+        m[j][3] = p[j][2];
+    }
+    if (!invert4(m, inv)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) G.g[i][0] = G.g[i][1] = G.g[i][2] = 0.0;
+        G.vol = 0.0;
+        return false;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        G.g[i][0] = inv[1][i];
+        G.g[i][1] = inv[2][i];
+        G.g[i][2] = inv[3][i];
+    }
+    double a[3], b[3], c[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        a[k] = p[1][k] - p[0][k];
+        b[k] = p[2][k] - p[0][k];
+        c[k] = p[3][k] - p[0][k];
+    }
+    const double x = a[1] * b[2] - a[2] * b[1];
+    const double y = a[2] * b[0] - a[0] * b[2];
+    const double z = a[0] * b[1] - a[1] * b[0];
+    G.vol = fabs((x * c[0] + y * c[1] + z * c[2]) / 6.0);
+    return true;
+}
+
 }  // namespace
 
 // One 16-lane group per tet: lane 0 builds the geometry into LDS, then the 16
@@ -121,45 +164,8 @@ __global__ void __launch_bounds__(256) k_element(int num_tets, const double* __r
             for (int i = 0; i < 4; ++i) qp[k][i] = (i == k) ? a : b;
     }
     if (lane == 0 && t < num_tets) {
-        double m[4][4], inv[4][4];
-        double p[4][3];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int v = tets[4 * t + j];
-            p[j][0] = coords[3 * v + 0];
-            p[j][1] = coords[3 * v + 1];
-            p[j][2] = coords[3 * v + 2];
-            m[j][0] = 1.0;
-            m[j][1] = p[j][0];
-            m[j][2] = p[j][1];
-            m[j][3] = p[j][2];
-        }
         Geo G;
-        if (!invert4(m, inv)) {
-            atomicOr(error_flag, 1);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) G.g[i][0] = G.g[i][1] = G.g[i][2] = 0.0;
-            G.vol = 0.0;
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                G.g[i][0] = inv[1][i];
-                G.g[i][1] = inv[2][i];
-                G.g[i][2] = inv[3][i];
-            }
-            // |signed_tet_volume|, src/mesh.cpp:151-161
-            double a[3], b[3], c[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                a[k] = p[1][k] - p[0][k];
-                b[k] = p[2][k] - p[0][k];
-                c[k] = p[3][k] - p[0][k];
-            }
-            const double x = a[1] * b[2] - a[2] * b[1];
-            const double y = a[2] * b[0] - a[0] * b[2];
-            const double z = a[0] * b[1] - a[1] * b[0];
-            G.vol = fabs((x * c[0] + y * c[1] + z * c[2]) / 6.0);
-        }
+        if (!tet_geometry(coords, tets, t, G)) atomicOr(error_flag, 1);
         geo[local] = G;
     }
     __syncthreads();
@@ -427,6 +433,84 @@ __global__ void k_permute_scale_out(int n, int nrhs, const int* __restrict__ per
     const int p = perm[i];
     const cd v = cmul(xt[c * ldt + i], cd{d[p], 0.0});
     x[c * ldx + p] = accumulate ? cadd(x[c * ldx + p], v) : v;
+}
+
+// ------------------------------------------------------------------ port current
+
+// Per tet and right-hand side: the centroid current density of compute_fields
+// (src/postprocess.cpp:311-400) -- A from the edge unknowns with the global
+// edge signs, Phi from the Phi unknowns (scaled back under ScaledPhi) or the
+// prescribed values, E = -j w A - grad Phi, J = sigma E -- and its flux term of
+// terminal_current (src/postprocess.cpp:538-544): (J . axis) |V|. Written in the
+// reference's operation order (contraction off), so each term is the
+// reference's bit for bit. Non-conducting tets contribute nothing.
+__global__ void __launch_bounds__(256) k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd* __restrict__ x,
+                                                     int64_t ldx, cd minus_jw, cd scale, double ax, double ay, double az,
+                                                     cd* __restrict__ contrib) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (t >= num_tets || r >= nrhs) return;
+    const double sigma = tp.body_sigma[tp.tet_body[t]];
+    cd out = cd{0.0, 0.0};
+    if (sigma > 0.0) {
+        Geo G;
+        tet_geometry(tp.coords, tp.tets, t, G);
+        const cd* xr = x + static_cast<int64_t>(r) * ldx;
+        const int* d = tp.dofs + 16 * static_cast<int64_t>(t);
+        const double L[4] = {0.25, 0.25, 0.25, 0.25};
+        cd av[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const cd ae = d[k] >= 0 ? xr[d[k]] : cd{0.0, 0.0};
+            double w[3];
+            whitney(G, k, L, w);
+            const double sg = static_cast<double>(tp.signs[6 * static_cast<int64_t>(t) + k]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) av[c] = cadd(av[c], cscale(ae, w[c] * sg));
+        }
+        cd gp[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+        for (int b = 0; b < 10; ++b) {
+            const int ib = d[6 + b];
+            cd pn;
+            if (ib >= 0) pn = cmul(cscale(scale, tp.phi_coeff[10 * static_cast<int64_t>(t) + b]), xr[ib]);
+            else pn = tp.phi_fixed[10 * static_cast<int64_t>(t) + b];
+            double gr[3];
+            p2grad(G, b, L, gr);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gp[c] = cadd(gp[c], cscale(pn, gr[c]));
+        }
+        cd j[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) j[c] = cscale(csub(cmul(minus_jw, av[c]), gp[c]), sigma);
+        out = cscale(cadd(cadd(cscale(j[0], ax), cscale(j[1], ay)), cscale(j[2], az)), G.vol);
+    }
+    contrib[static_cast<int64_t>(r) * num_tets + t] = out;
+}
+
+// current[r] = -(sum over conducting tets, in tet order, of the flux terms) / length:
+// one wavefront per right-hand side adds 64 terms at a time lane by lane, so the
+// sum is the reference's sequential one.
+__global__ void __launch_bounds__(64) k_current_sum(int num_tets, TetFieldPlan tp, const cd* __restrict__ contrib,
+                                                    double length, cd* __restrict__ current) {
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
+    const cd* cr = contrib + static_cast<int64_t>(r) * num_tets;
+    double fre = 0.0, fim = 0.0;
+    for (int t0 = 0; t0 < num_tets; t0 += 64) {
+        const int t = t0 + lane;
+        const bool on = t < num_tets && tp.body_sigma[tp.tet_body[t]] > 0.0;
+        const cd v = on ? cr[t] : cd{0.0, 0.0};
+        const unsigned long long mask = __ballot(on);
+#pragma unroll
+        for (int l = 0; l < 64; ++l) {
+            if ((mask >> l) & 1ull) {
+                fre = fre + lane_read(v.re, l);
+                fim = fim + lane_read(v.im, l);
+            }
+        }
+    }
+    if (lane == 0) current[r] = cd{-fre / length, -fim / length};
 }
 
 }  // namespace aphi_dev

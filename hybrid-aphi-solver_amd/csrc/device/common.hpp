@@ -72,6 +72,70 @@ template <typename C> __device__ __forceinline__ C grecip(C a) {
 }
 template <typename C> __device__ __forceinline__ auto gabs(C a) -> decltype(a.re) { return hypot(a.re, a.im); }
 
+template <typename T> struct Mfma;
+
+// Operands: lane l supplies A[l%16][l/16] and B[l/16][l%16] for both types. The
+// accumulator layouts differ (measured on gfx950): register v of lane l holds
+// D[l/16 + 4v][l%16] for f64 and D[4(l/16) + v][l%16] for f32.
+template <> struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return lane / 16 + 4 * v; }
+};
+template <> struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
+};
+
+// One wave: acc (a 16x16 complex tile in the MFMA accumulator layout, real and
+// imaginary parts apart) += A (16 x K) B (K x 16), with A and B given as element
+// functors a_at(p, k) and b_at(k, q). K runs in chunks of 64: the 16 A values a
+// lane needs for a chunk are loaded together (one memory latency per chunk),
+// then 64 MFMAs consume them. Lane l supplies A[l%16][k0 + l/16] and
+// B[k0 + l/16][l%16]; a complex product is four real MFMAs. b_at must return
+// zeros for K <= k < K rounded up to 64, and a_at finite values there.
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ void cmma16(typename Mfma<T>::v4& accr, typename Mfma<T>::v4& acci, int K, FA a_at,
+                                       FB b_at) {
+    const int lane = static_cast<int>(threadIdx.x % 64);
+    const int p = lane % 16, kq = lane / 16;
+    for (int k0 = 0; k0 < K; k0 += 64) {
+        decltype(a_at(0, 0)) a[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) a[u] = a_at(p, k0 + 4 * u + kq);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const auto b = b_at(k0 + 4 * u + kq, p);
+            accr = Mfma<T>::mac(a[u].re, b.re, accr);
+            accr = Mfma<T>::mac(-a[u].im, b.im, accr);
+            acci = Mfma<T>::mac(a[u].re, b.im, acci);
+            acci = Mfma<T>::mac(a[u].im, b.re, acci);
+        }
+    }
+}
+
+// Scalar read of lane `src`'s value (src uniform): the broadcasts of the
+// register-resident diagonal factorization and
+// the ordered reductions.
+__device__ __forceinline__ float lane_read(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ __forceinline__ double lane_read(double v, int src) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffLL), src);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), src);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <typename Cx>
+__device__ __forceinline__ Cx lane_read(Cx v, int src) {
+    return Cx{lane_read(v.re, src), lane_read(v.im, src)};
+}
+
 // Order-preserving bit patterns of non-negative doubles, for atomic max.
 __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
     atomicMax(reinterpret_cast<unsigned long long*>(addr), static_cast<unsigned long long>(__double_as_longlong(v)));

@@ -260,22 +260,6 @@ __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ node
     }
 }
 
-// Scalar read of lane `src`'s value (src uniform): the broadcasts of the
-// register-resident diagonal factorization.
-__device__ __forceinline__ float lane_read(float v, int src) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
-}
-__device__ __forceinline__ double lane_read(double v, int src) {
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffLL), src);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), src);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-template <typename Cx>
-__device__ __forceinline__ Cx lane_read(Cx v, int src) {
-    return Cx{lane_read(v.re, src), lane_read(v.im, src)};
-}
-
 // Diagonal block of step b for each front with k > b, one wavefront per front.
 // Lane r holds row r of the nb x nb block in registers (rows and columns beyond
 // nb padded with the identity); the right-looking LDL^T takes each column value
@@ -448,26 +432,6 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
 // blocks with separate real/imaginary accumulators (complex product as four real
 // products). The MFMA "row" operand is L[j] and the "column" operand W[i] =
 // L[i] D, so each lane's output column is a front row i: stores are contiguous.
-
-template <typename T> struct Mfma;
-
-// Operands: lane l supplies A[l%16][l/16] and B[l/16][l%16] for both types. The
-// accumulator layouts differ (measured on gfx950): register v of lane l holds
-// D[l/16 + 4v][l%16] for f64 and D[4(l/16) + v][l%16] for f32.
-template <> struct Mfma<double> {
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return lane / 16 + 4 * v; }
-};
-template <> struct Mfma<float> {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
-};
 
 // Tiling of the update. A wave moves one column segment of the L panel into LDS
 // with a single 16-byte-per-lane DMA load (global_load_lds_dwordx4): 128 rows of

@@ -12,6 +12,7 @@ constexpr int kNB = 64;       // block-column width of the blocked front factori
 constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64)
 constexpr int kOuter = 256;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
+constexpr int kSB = 256;        // solve block: columns per forward/backward step (multiple of kNB)
 
 template <typename T> void launch_zero(hipStream_t st, int64_t n, void* p);
 template <typename T> void launch_scatter_panel(hipStream_t st, int64_t nnz, const int64_t* s2p, const cd* a, void* P);
@@ -44,12 +45,18 @@ void launch_perm_out(hipStream_t st, int n, int nrhs, const int* perm, const dou
 template <typename T>
 void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int nrhs, const FrontPlan& fp, const void* b, int n,
                        void* W);
+// inverses of the kSB diagonal blocks (column-major X, row-major XT), from the
+// kNB-block inverses; pairs = (supernode, first column of the block)
 template <typename T>
-void launch_fwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* xinv, void* W, void* Y, int n);
+void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const FrontPlan& fp, const void* P,
+                       const void* xinv, void* X, void* XT);
 template <typename T>
-void launch_bwd_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* xinv, const void* Y, void* X, int n, void* partial);
+void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_row_blocks, int nrhs,
+                     const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n);
+template <typename T>
+void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_row_blocks, int nrhs,
+                     const FrontPlan& fp, const void* P, const void* XT256, const void* Y, void* X, int n,
+                     void* partial, void* tsum);
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
                          int mode);
 

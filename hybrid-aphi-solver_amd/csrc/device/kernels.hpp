@@ -45,6 +45,8 @@ struct FrontPlan {
     // (q = 0 .. ceil(m_parent / 64)); -1 for other supernodes.
     const int64_t* ea_bound_off;  // ns
     const int* ea_bound;
+    // Inverses of the kSB-column diagonal blocks (the last one nb x nb), for the solves.
+    const int64_t* x256_off;   // ns
 };
 
 struct FactorStatsDev {
@@ -76,5 +78,20 @@ __global__ void k_scale_permute_in(int n, int nrhs, const int* perm, const doubl
                                    cd* out, int64_t ldo);
 __global__ void k_permute_scale_out(int n, int nrhs, const int* perm, const double* d, const cd* xt, int64_t ldt,
                                     cd* x, int64_t ldx, int accumulate);
+
+// Per-tet data of the field recovery (compute_fields, src/postprocess.cpp:312-340).
+struct TetFieldPlan {
+    const double* coords;
+    const int* tets;
+    const int* tet_body;
+    const double* body_sigma;
+    const int* dofs;          // 16 per tet: 6 edge unknowns, 10 Phi unknowns
+    const int8_t* signs;      // 6 per tet
+    const double* phi_coeff;  // 10 per tet
+    const cd* phi_fixed;      // 10 per tet
+};
+__global__ void k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd* x, int64_t ldx, cd minus_jw, cd scale,
+                              double ax, double ay, double az, cd* contrib);
+__global__ void k_current_sum(int num_tets, TetFieldPlan tp, const cd* contrib, double length, cd* current);
 
 }  // namespace aphi_dev

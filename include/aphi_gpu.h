@@ -139,6 +139,12 @@ typedef struct aphi_mesh_desc {
     const int64_t* slot_to_panel;  /* nnz */
     double flops;                  /* algorithmic flops of one factorization */
     int64_t predicted_nnz;
+    /* per-tet unknowns for the field recovery of compute_fields
+     * (src/postprocess.cpp:312-340): DofMap::local_dofs of every tet */
+    const int32_t* tet_dofs;       /* 16 * num_tets: 6 edge unknowns, 10 Phi unknowns (-1: eliminated) */
+    const int8_t* tet_edge_signs;  /* 6 * num_tets: Mesh::tet_edge_signs */
+    const double* tet_phi_coeff;   /* 10 * num_tets */
+    const double* tet_phi_fixed;   /* 20 * num_tets: prescribed Phi (complex) where the unknown is -1 */
 } aphi_mesh_desc;
 
 /* Pointers into the problem's own arrays; valid while the problem lives. */
@@ -172,6 +178,19 @@ int aphi_system_download(aphi_mesh* mesh, double* values, double* rhs);
 int aphi_system_upload(aphi_mesh* mesh, const double* values, const double* rhs);
 /* Device pointer to the assembled rhs (n complex), for solving the port system. */
 const void* aphi_rhs_device(const aphi_mesh* mesh);
+
+/* Port current and impedance of each of nrhs solutions x (device, n complex per
+ * column, leading dimension ldx): terminal_current (src/postprocess.cpp:526-547)
+ * over the centroid current density J = sigma E of compute_fields
+ * (src/postprocess.cpp:311-400, E = -j omega A - grad Phi), and impedance
+ * (include/aphi_solver/postprocess.hpp:356-358). current: 2 * nrhs doubles
+ * (complex I); resistance / inductance (may be null): Re(V/I) and Im(V/I)/omega
+ * for the drive voltage V. The sum over tets runs in tet order, as the
+ * reference's does, so a given x yields the reference's I bit for bit. Refuses
+ * length <= 0 like the reference. */
+int aphi_terminal_current(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx, double omega, int conditioning,
+                          const double* axis, double length, double voltage, double* current, double* resistance,
+                          double* inductance);
 
 /* compute_symmetric_equilibration(a, iterations) + apply (src/equilibration.cpp:94-162).
  * iterations 0 disables scaling. d_out (host, n) may be NULL. */

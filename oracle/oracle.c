@@ -694,3 +694,90 @@ void oracle_factor_sample(int n, const int32_t* row_ptr, const int32_t* col, con
     free(as);
     free(d);
 }
+
+/* ------------------------------------------------------------------ port current */
+
+/* compute_fields' centroid current density (src/postprocess.cpp:311-400: A from
+ * the edge unknowns with the global edge signs, Phi from the Phi unknowns scaled
+ * by Solution::phi_scale (:35-40) or the prescribed values, E = -jw A - grad Phi,
+ * J = sigma E) and terminal_current (:526-547): I = -(1/length) sum over
+ * conducting tets, in tet order, of (J . axis) |V|; impedance Z = V / I
+ * (include/aphi_solver/postprocess.hpp:356-358), R = Re Z, L = Im Z / omega.
+ * Complex products are written out as the reference's std::complex evaluates
+ * them for finite values. out: I_re, I_im, R, L; terms (optional): the flux term
+ * of each tet (2 doubles, zero for non-conducting tets). Returns 0 for a bad
+ * length, like the reference's refusal. */
+int oracle_terminal_current(int num_tets, const double* coords, const int32_t* tet_nodes, const int32_t* tet_body,
+                            const double* body_sigma, const int32_t* tet_dofs, const double* tet_signs,
+                            const double* tet_phi_coeff, const double* tet_phi_fixed, const double* x, double omega,
+                            int conditioning, const double* axis, double length, double voltage, double* out,
+                            double* terms) {
+    if (!(length > 0.0)) return 0;
+    const double mjw_re = -0.0, mjw_im = -omega;                    /* -jw */
+    const double sc_re = conditioning == 2 ? 0.0 : 1.0, sc_im = conditioning == 2 ? omega : 0.0;
+    const double L[4] = {0.25, 0.25, 0.25, 0.25};
+    double f_re = 0.0, f_im = 0.0;
+    for (int t = 0; t < num_tets; ++t) {
+        const double sigma = body_sigma[tet_body[t]];
+        if (terms) { terms[2 * t] = 0.0; terms[2 * t + 1] = 0.0; }
+        if (!(sigma > 0.0)) continue;
+        double p[4][3], g[4][3], vol;
+        for (int j = 0; j < 4; ++j)
+            for (int c = 0; c < 3; ++c) p[j][c] = coords[3 * tet_nodes[4 * t + j] + c];
+        if (!tet_geometry(p, g, &vol)) return 0;
+        const int32_t* d = tet_dofs + 16 * (int64_t)t;
+        double av_re[3] = {0.0, 0.0, 0.0}, av_im[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 6; ++k) {
+            double ae_re = 0.0, ae_im = 0.0;
+            if (d[k] >= 0) { ae_re = x[2 * d[k]]; ae_im = x[2 * d[k] + 1]; }
+            double w[3];
+            whitney(g, k, L, w);
+            const double sg = tet_signs[6 * (int64_t)t + k];
+            for (int c = 0; c < 3; ++c) {
+                const double wc = w[c] * sg;
+                av_re[c] = av_re[c] + ae_re * wc;
+                av_im[c] = av_im[c] + ae_im * wc;
+            }
+        }
+        double gp_re[3] = {0.0, 0.0, 0.0}, gp_im[3] = {0.0, 0.0, 0.0};
+        for (int b = 0; b < 10; ++b) {
+            const int ib = d[6 + b];
+            double pn_re, pn_im;
+            if (ib >= 0) {
+                const double cf = tet_phi_coeff[10 * (int64_t)t + b];
+                const double s_re = sc_re * cf, s_im = sc_im * cf;          /* scale * coeff */
+                const double xr = x[2 * ib], xi = x[2 * ib + 1];
+                pn_re = s_re * xr - s_im * xi;                              /* (scale * coeff) * x */
+                pn_im = s_re * xi + s_im * xr;
+            } else {
+                pn_re = tet_phi_fixed[20 * (int64_t)t + 2 * b];
+                pn_im = tet_phi_fixed[20 * (int64_t)t + 2 * b + 1];
+            }
+            double gr[3];
+            p2grad(g, b, L, gr);
+            for (int c = 0; c < 3; ++c) {
+                gp_re[c] = gp_re[c] + pn_re * gr[c];
+                gp_im[c] = gp_im[c] + pn_im * gr[c];
+            }
+        }
+        double j_re[3], j_im[3];
+        for (int c = 0; c < 3; ++c) {
+            const double e_re = (mjw_re * av_re[c] - mjw_im * av_im[c]) - gp_re[c];
+            const double e_im = (mjw_re * av_im[c] + mjw_im * av_re[c]) - gp_im[c];
+            j_re[c] = sigma * e_re;
+            j_im[c] = sigma * e_im;
+        }
+        const double t_re = ((j_re[0] * axis[0] + j_re[1] * axis[1]) + j_re[2] * axis[2]) * vol;
+        const double t_im = ((j_im[0] * axis[0] + j_im[1] * axis[1]) + j_im[2] * axis[2]) * vol;
+        if (terms) { terms[2 * t] = t_re; terms[2 * t + 1] = t_im; }
+        f_re = f_re + t_re;
+        f_im = f_im + t_im;
+    }
+    const cplx cur = CMPLX(-f_re / length, -f_im / length);
+    const cplx Z = CMPLX(voltage, 0.0) / cur;
+    out[0] = creal(cur);
+    out[1] = cimag(cur);
+    out[2] = creal(Z);
+    out[3] = cimag(Z) / omega;
+    return 1;
+}

@@ -170,3 +170,29 @@ def factor_sample(problem, values, budget_s):
     lib().oracle_factor_sample(C.c_int(n), _p(rp, C.c_int32), _p(ci, C.c_int32), _p(v.view(np.float64), C.c_double),
                                _p(pm, C.c_int32), C.c_double(budget_s), _p(out, C.c_double))
     return dict(seconds=out[0], rows=int(out[1]), flops_done=out[2], flops_total=out[3], equilibrate_s=out[4])
+
+
+def terminal_current(problem, x, omega, length, axis=(0.0, 0.0, 1.0), voltage=1.0, conditioning=1, terms=False):
+    """Port current I and R, L of one solution x (n complex); with terms=True also
+    the per-tet flux terms (complex, zero outside the conductor)."""
+    nt = int(problem.counts["tets"])
+    bp = _f64(problem.doubles("body_param")).reshape(-1, 3)
+    sig = np.ascontiguousarray(bp[:, 0])
+    arrays = [_f64(problem.doubles("coords")), _i32(problem.ints("tet_nodes")), _i32(problem.ints("tet_body")), sig,
+              _i32(problem.ints("tet_dofs")), _f64(problem.ints("tet_edge_signs")),
+              _f64(problem.doubles("tet_phi_coeff")), _f64(problem.doubles("tet_phi_fixed"))]
+    types = [C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double, C.c_double]
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    ax = _f64(axis)
+    out = np.zeros(4)
+    tr = np.zeros(2 * nt) if terms else None
+    ok = lib().oracle_terminal_current(C.c_int(nt), *[_p(a, t) for a, t in zip(arrays, types)],
+                                       _p(xx.view(np.float64), C.c_double), C.c_double(omega), C.c_int(conditioning),
+                                       _p(ax, C.c_double), C.c_double(length), C.c_double(voltage),
+                                       _p(out, C.c_double), _p(tr, C.c_double) if terms else None)
+    if not ok:
+        raise ValueError("terminal_current: length must be positive (or a tet is degenerate)")
+    I = complex(out[0], out[1])
+    if terms:
+        return I, out[2], out[3], tr.view(np.complex128)
+    return I, out[2], out[3]

@@ -152,6 +152,62 @@ def test_deterministic_bits(case):
     assert np.array_equal(x1.cpu().numpy().view(np.uint64), x2.cpu().numpy().view(np.uint64))
 
 
+def test_precision_switch(case):
+    """FP64 then FP32 then FP64 factorizations on one mesh reuse its device
+    buffers; every solve must be finite and as accurate as a fresh one."""
+    import torch
+    m, p = case["m"], case["p"]
+    m.refill(case["meta"]["omega"])
+    m.equilibrate(10)
+    B = dev(np.tile(case["rhs"], (3, 1)))
+    errs = {}
+    for prec in (A.APHI_FP64, A.APHI_FP32, A.APHI_FP64, A.APHI_FP32):
+        X = torch.zeros_like(B)
+        m.factorize_ldlt(prec)
+        m.solve(B, X)
+        x = X.cpu().numpy()
+        assert np.isfinite(x).all()
+        e = max(np.linalg.norm(x[r] - case["x"]) / np.linalg.norm(case["x"]) for r in range(3))
+        errs.setdefault(prec, []).append(e)
+    assert errs[A.APHI_FP64][0] == errs[A.APHI_FP64][1] and errs[A.APHI_FP32][0] == errs[A.APHI_FP32][1]
+    assert errs[A.APHI_FP64][0] <= 1e-10
+
+
+def test_terminal_current_bitwise(case):
+    """Device port current / R / L on the reference's own solution: bit for bit
+    the reference's (the per-tet terms and the tet-order sum are both exact)."""
+    m = case["m"]
+    _, arrays = golden(case["meta"]["case"])
+    meta = case["meta"]
+    I, R, L = m.terminal_current(dev(arrays["x"][None, :]), meta["omega"], meta["side"])
+    assert I[0] == complex(meta["I_re"], meta["I_im"])
+    assert R[0] == meta["R"] and L[0] == meta["L"]
+    with pytest.raises(ValueError):
+        m.terminal_current(dev(arrays["x"][None, :]), meta["omega"], -1.0)
+
+
+def test_terminal_current_after_device_solve(case):
+    """I, R and L from the device FP64 solution (several right-hand sides at
+    once) against the reference's. The solution matches to 1e-10, and I is a
+    linear functional of it, so I agrees to 1e-8 relative; Z = V/I then agrees
+    to the same relative error in |Z|, which bounds R = Re Z and omega L = Im Z
+    componentwise (R alone can be tiny next to |Z|: for cube8 Re I / |I| is
+    2.5e-4, so R's own relative error is larger)."""
+    m = case["m"]
+    meta = case["meta"]
+    m.refill(meta["omega"])
+    B = np.tile(case["rhs"], (3, 1))
+    x, rep = m.solve_symmetric(dev(B), precision=A.APHI_FP64)
+    I, R, L = m.terminal_current(x, meta["omega"], meta["side"])
+    I_ref = complex(meta["I_re"], meta["I_im"])
+    Z_ref = 1.0 / I_ref
+    w = meta["omega"]
+    for r in range(3):
+        assert abs(I[r] - I_ref) <= 1e-8 * abs(I_ref)
+        assert abs(R[r] - meta["R"]) <= 1e-8 * abs(Z_ref)
+        assert abs(w * L[r] - w * meta["L"]) <= 1e-8 * abs(Z_ref)
+
+
 def test_upload_equals_refill(case):
     m = case["m"]
     m.upload_system(case["vals"], case["rhs"])

@@ -58,6 +58,34 @@ def unit_tet(scale=1.0):
     return np.array([0, 0, 0, scale, 0, 0, 0, scale, 0, 0, 0, scale], dtype=float)
 
 
+def test_terminal_current_bitwise_equals_reference(case):
+    """compute_fields + terminal_current + impedance on the reference's own x
+    (src/postprocess.cpp:311-400, 526-547): I, R and L bit for bit."""
+    meta, arrays, p, _, _ = case
+    I, R, L = O.terminal_current(p, arrays["x"], meta["omega"], meta["side"])
+    assert I == complex(meta["I_re"], meta["I_im"])
+    assert R == meta["R"] and L == meta["L"]
+
+
+def test_terminal_current_linear_and_refusals(case):
+    """The current is linear in the free unknowns once the prescribed potentials
+    are zero: I(x1 + x2) - I(x1) - I(x2) vanishes to rounding; a non-positive
+    length is refused as in src/postprocess.cpp:528-530."""
+    meta, arrays, p, _, _ = case
+    x = arrays["x"]
+    rng = np.random.default_rng(3)
+    y = rng.normal(size=x.shape) + 1j * rng.normal(size=x.shape)
+    _, _, _, t1 = O.terminal_current(p, x, meta["omega"], meta["side"], terms=True)
+    _, _, _, t2 = O.terminal_current(p, y, meta["omega"], meta["side"], terms=True)
+    _, _, _, t12 = O.terminal_current(p, x + y, meta["omega"], meta["side"], terms=True)
+    _, _, _, t0 = O.terminal_current(p, np.zeros_like(x), meta["omega"], meta["side"], terms=True)
+    # affine: terms(x) = M x + terms(0)
+    lhs = t12 - t1 - t2 + t0
+    assert np.max(np.abs(lhs)) <= 1e-9 * max(np.max(np.abs(t1)), np.max(np.abs(t2)))
+    with pytest.raises(ValueError):
+        O.terminal_current(p, x, meta["omega"], 0.0)
+
+
 def test_element_curl_curl_hand_values():
     dots = np.array([[8, -4, -4, 4, 0, 4], [-4, 8, -4, -4, 4, 0], [-4, -4, 8, 0, -4, -4],
                      [4, -4, 0, 4, 0, 0], [0, 4, -4, 0, 4, 0], [4, 0, -4, 0, 0, 4]], dtype=float)
