@@ -4,9 +4,12 @@ Workload: the mixed-precision configuration -- a 32^3-hex Kuhn cube (196,608 tet
 464,910 unknowns) with an off-centre 6x3 conductor bar, 64 log-spaced frequencies
 from 1 Hz to 100 MHz (full-wave), 16 seeded N(0,1) complex right-hand sides per
 frequency. One step = one frequency point: device refill, equilibration,
-complex64 LDL^T on MFMA, 16-RHS solve, FP64 iterative refinement until every
-right-hand side's backward error is <= 1e-12. The plain FP64 path (complex128
-factor, no refinement) is timed on the same frequency points for comparison.
+complex64 LDL^T on MFMA, 16-RHS solve, FP64 GMRES refinement (the FP32 factor as
+preconditioner) until every right-hand side's backward error is <= 1e-12. Where
+the points already solved show the FP32 factor would need more refinement
+solves than a complex128 factor costs, the point is factored in complex128
+instead (aphi_amd.PrecisionPolicy); the counts are reported. The plain FP64 path
+(complex128 factor, no refinement) is timed on the same frequency points.
 
 Prints one JSON line. Inputs are resident in HBM before the timed region.
 """
@@ -40,6 +43,8 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling (0 skips)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device copy of known size (HBM counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep's frequency points are independent and would shard across GPUs "
@@ -51,6 +56,14 @@ def main():
 
     if not torch.cuda.is_available():
         sys.exit("bench.py: no HIP device")
+    if a.copy_calibration:
+        # 1 GiB read + 1 GiB written by one copy kernel: the scale for FETCH_SIZE / WRITE_SIZE
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
+        return
     bar = (6, 12, 20, 23) if a.n == 32 else (a.n // 5, a.n // 5 + 6, (2 * a.n) // 3, (2 * a.n) // 3 + 3)
     cfg = A.CubeConfig(n=a.n, side=0.1, bar=bar, sigma=5.8e7)
     t0 = time.time()
@@ -70,18 +83,29 @@ def main():
     def step(fi, precision):
         mesh.refill(2 * np.pi * freqs[fi])
         _, rep = mesh.solve_symmetric(B[fi], X, precision=precision, equilibration_iterations=10,
-                                      max_refinements=(20 if precision == A.APHI_FP32 else 0), tolerance=1e-12)
+                                      max_refinements=(30 if precision == A.APHI_FP32 else 0), tolerance=1e-12)
+        return rep
+
+    # mixed-precision sweep: FP32 factor + GMRES refinement, FP64 factor at the
+    # frequencies where the points solved so far show FP32 would be slower
+    policy = A.PrecisionPolicy()
+
+    def step_mixed(fi):
+        prec = policy.choose(freqs[fi])
+        rep = step(fi, prec)
+        policy.record(freqs[fi], prec, rep)
+        rep["requested"] = prec
         return rep
 
     for i in range(a.warmup):
-        step(order[i], A.APHI_FP32)
+        step_mixed(order[i])
     ctx.synchronize()
     torch.cuda.synchronize()
     lat, reps = [], []
     t_start = time.perf_counter()
     for i in range(a.steps):
         s0 = time.perf_counter()
-        reps.append(step(order[a.warmup + i], A.APHI_FP32))
+        reps.append(step_mixed(order[a.warmup + i]))
         lat.append(time.perf_counter() - s0)
     ctx.synchronize()
     torch.cuda.synchronize()
@@ -102,6 +126,21 @@ def main():
     value64 = a.fp64_steps / (time.perf_counter() - t64) if a.fp64_steps else None
     if value64:
         log(f"fp64: {value64:.3f} points/s  max be {max(r['backward_error'] for r in reps64):.2e}")
+
+    # error of the FP32+IR solution against the complex128 path, per frequency
+    # (untimed; both solved again on the same points)
+    err_vs_fp64 = []
+    for i in range(min(a.fp64_steps, a.steps)):
+        fi = order[a.warmup + i]
+        mesh.refill(2 * np.pi * freqs[fi])
+        x64, _ = mesh.solve_symmetric(B[fi], precision=A.APHI_FP64, equilibration_iterations=10)
+        mesh.refill(2 * np.pi * freqs[fi])
+        x32, _ = mesh.solve_symmetric(B[fi], precision=A.APHI_FP32, equilibration_iterations=10,
+                                      max_refinements=20, tolerance=1e-12)
+        e = torch.linalg.vector_norm(x32 - x64, dim=1) / torch.linalg.vector_norm(x64, dim=1)
+        err_vs_fp64.append(float(e.max()))
+    if err_vs_fp64:
+        log("relative error of FP32+IR vs complex128 per frequency:", " ".join(f"{e:.2e}" for e in err_vs_fp64))
 
     # roofline of the dominant kernel (MFMA trailing update), measured live with
     # HIP events around each of its launches on the library's stream
@@ -155,10 +194,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 (FP32 MFMA) factor + c128 refinement",
+        "dtype": "c64 (FP32 MFMA) factor + c128 GMRES refinement; c128 factor where FP32 is predicted slower",
         "data": "synthetic: seeded N(0,1) complex right-hand sides; generated Kuhn-cube mesh",
         "config": {"workload": f"{a.n}^3-hex Kuhn cube, 6x3 off-centre bar, 64 log frequencies 1 Hz-100 MHz, "
-                               f"{a.nrhs} RHS per point, FP32-MFMA LDL^T + FP64 IR to 1e-12",
+                               f"{a.nrhs} RHS per point, FP32-MFMA LDL^T + FP64 GMRES-IR to 1e-12",
                    "unknowns": n, "nnz_L": int(prob.counts["predicted_nnz"]),
                    "factor_flops": prob.counts["flops"], "frequencies_timed": [float(freqs[order[a.warmup + i]])
                                                                                for i in range(a.steps)]},
@@ -168,10 +207,13 @@ def main():
         "backward_error_max": max(r["backward_error"] for r in reps),
         "scaled_backward_error_max": max(r["scaled_backward_error"] for r in reps),
         "refinement_steps": [r["refinement_steps"] for r in reps],
+        "precision_requested": [r["requested"] for r in reps],
         "precision_used": [r["precision_used"] for r in reps],
+        "fp32_points": sum(1 for r in reps if r["precision_used"] == A.APHI_FP32),
         "fp64_value": value64,
         "fp64_p99_ms": float(np.percentile(lat64, 99) * 1e3) if lat64 else None,
         "fp64_backward_error_max": max(r["backward_error"] for r in reps64) if reps64 else None,
+        "error_vs_complex128": err_vs_fp64,
         "speedup_vs_fp64": (value / value64) if value64 else None,
         "factor_ms_fp32": st32["milliseconds"],
         "factor_ms_fp64": st64["milliseconds"],

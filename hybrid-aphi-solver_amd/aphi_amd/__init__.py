@@ -24,8 +24,11 @@ from ._capi import (APHI_FP32, APHI_FP64, APHI_NATURAL, APHI_ORDER_AMD, APHI_ORD
 __all__ = [
     "LogicError", "DeviceError", "CubeConfig", "Problem", "Context", "Mesh", "frequency_sweep",
     "APHI_FP32", "APHI_FP64", "APHI_NATURAL", "APHI_ROW_SCALED", "APHI_SCALED_PHI",
-    "APHI_ORDER_AMD", "APHI_ORDER_NATURAL", "APHI_ORDER_RCM", "lib",
+    "APHI_ORDER_AMD", "APHI_ORDER_NATURAL", "APHI_ORDER_RCM", "lib", "PrecisionPolicy",
 ]
+
+
+from .sweep import PrecisionPolicy  # noqa: E402
 
 
 class LogicError(RuntimeError):

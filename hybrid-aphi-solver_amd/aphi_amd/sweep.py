@@ -1,0 +1,74 @@
+"""Precision choice across the points of a frequency sweep.
+
+The FP32 factor plus GMRES refinement reaches the backward-error target in a few
+solves at most frequencies, but where the equilibrated matrix is badly
+conditioned (here: high frequencies, skin depth far below the mesh size) the
+FP32 factor is too inaccurate a preconditioner and refinement needs tens of
+solves, or stalls and falls back to an FP64 factor. The reference's sweep
+(tools/solve_mesh.cpp:152-259) factors every point in complex128; this policy
+keeps that as the choice wherever the points already solved nearby show the
+mixed-precision path would be slower, and uses FP32 everywhere else.
+"""
+import math
+
+from ._capi import APHI_FP32, APHI_FP64
+
+
+class PrecisionPolicy:
+    """Chooses APHI_FP32 or APHI_FP64 for a frequency from the points solved so far.
+
+    Cost model per point: FP32 = factor32 + (1 + solves) * solve32, FP64 =
+    factor64 + solve64, with the times taken from the solve reports as they come
+    in (factor64 starts as twice factor32, the FLOP-rate ratio of the two MFMA
+    paths). The refinement solves expected at a new frequency are the larger of
+    those recorded at its nearest neighbours below and above in log-frequency; a
+    point where FP32 refinement did not converge counts as infinitely many.
+    """
+
+    def __init__(self, fp64_factor_ratio=2.0):
+        self.points = {}          # log10(f) -> refinement solves (inf: FP32 not viable)
+        self.factor32 = None
+        self.factor64 = None
+        self.solve32 = None
+        self.solve64 = None
+        self.ratio = fp64_factor_ratio
+
+    def expected_solves(self, freq):
+        if not self.points:
+            return 0.0
+        x = math.log10(freq)
+        below = [k for k in self.points if k <= x]
+        above = [k for k in self.points if k >= x]
+        near = []
+        if below:
+            near.append(self.points[max(below)])
+        if above:
+            near.append(self.points[min(above)])
+        return max(near)
+
+    def choose(self, freq):
+        k = self.expected_solves(freq)
+        if self.factor32 is None or self.solve32 is None:
+            return APHI_FP32
+        if math.isinf(k):
+            return APHI_FP64
+        f64 = self.factor64 if self.factor64 is not None else self.ratio * self.factor32
+        s64 = self.solve64 if self.solve64 is not None else self.solve32
+        return APHI_FP32 if self.factor32 + (1.0 + k) * self.solve32 <= f64 + s64 else APHI_FP64
+
+    def record(self, freq, precision, report):
+        """Feeds back one solve_symmetric report (precision = the one requested)."""
+        x = math.log10(freq)
+        steps = report["refinement_steps"]
+        solve_each = report["solve_ms"] / max(1, steps + 1)
+        if precision == APHI_FP32:
+            if report["precision_used"] == APHI_FP32:
+                self.points[x] = float(steps)
+                self.factor32 = report["factorize_ms"]
+                self.solve32 = solve_each
+            else:
+                self.points[x] = math.inf
+        else:
+            # the FP64 timings only: what FP32 would have needed here stays unknown
+            self.factor64 = report["factorize_ms"]
+            self.solve64 = solve_each

@@ -8,6 +8,7 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -88,7 +89,9 @@ struct Level {
 struct aphi_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // second stream of the factorization (default priority)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t main_done = nullptr, side_done = nullptr;  // ordering only (no timing)
 };
 
 struct aphi_mesh {
@@ -113,6 +116,7 @@ struct aphi_mesh {
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
     DevBuf P, CB, mws, Xinv, X256, XT256;
     DevBuf bt, W, X, Yv, bpart, tsum, colscale, res, norms;
+    DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
     int solve_nrhs_cap = 0;
     bool assembled = false, equilibrated = false, factored = false;
     int factor_precision = -1;
@@ -251,43 +255,67 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
         for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
         const int ts = aphi_dev::update_tile<T>();
         // one update launch: the largest tile count over the level's fronts, and its flops
-        const auto update = [&](int c0, int width, int bo, int inner) {
+        // one update launch on stream `on`: the largest tile count over the level's
+        // fronts, and its flops (region semantics of launch_update)
+        const auto update = [&](hipStream_t on, int c0, int width, int tlo, int thi) {
             int tiles = 0;
             double fl = 0.0;
             for (int i = 0; i < L.large_count; ++i) {
                 const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
                 if (kk <= c0) continue;
                 const int c1 = std::min(c0 + width, kk);
-                const int tend = inner ? std::min(bo + aphi_dev::kOuter, kk) : mm;
-                if (tend <= c1) continue;
-                const int nrt = (mm - c1 + ts - 1) / ts, nct = (tend - c1 + ts - 1) / ts;
+                const int t0 = std::max(c1, std::min(tlo, kk));
+                const int tend = thi < 0 ? mm : std::min(thi, kk);
+                if (tend <= t0) continue;
+                const int nrt = (mm - t0 + ts - 1) / ts, nct = (tend - t0 + ts - 1) / ts;
                 tiles = std::max(tiles, nct * nrt - nct * (nct - 1) / 2);
-                const double kw = c1 - c0, w = tend - c1, h = mm - c1;
+                const double kw = c1 - c0, w = tend - t0, h = mm - t0;
                 fl += 8.0 * kw * (w * h - w * (w - 1.0) / 2.0);
             }
             if (tiles == 0) return;
             const bool timed = m->profiling;
-            if (timed) HIPCHK(hipEventRecord(next_event(), s));
-            aphi_dev::launch_update<T>(s, lg, L.large_count, c0, width, bo, inner, tiles, fp, m->P.p, m->CB.p);
-            if (timed) HIPCHK(hipEventRecord(next_event(), s));
+            if (timed) HIPCHK(hipEventRecord(next_event(), on));
+            aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, fp, m->P.p, m->CB.p);
+            if (timed) HIPCHK(hipEventRecord(next_event(), on));
             ++launches;
             update_flops += fl;
         };
-        for (int b = 0; b < maxk; b += aphi_dev::kNB) {
-            int rb = 0;
-            for (int i = 0; i < L.large_count; ++i) {
-                const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
-                if (kk <= b) continue;
-                const int rem = mm - b - std::min(aphi_dev::kNB, kk - b);
-                if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
+        // Outer blocks of kOuter columns. Inside a block, per kNB columns: the
+        // diagonal block, the panel, and a narrow update of the block's remaining
+        // columns. After a block, its rank-kOuter update is split: the next
+        // block's columns on the main stream, the rest of the trailing front on
+        // the side stream, so that the next block's factorization (main) runs
+        // while the bulk of the update (side) does. The next split waits for the
+        // side stream's previous update, whose targets overlap.
+        const int nouter = (maxk + aphi_dev::kOuter - 1) / aphi_dev::kOuter;
+        bool side_busy = false;
+        for (int ob = 0; ob < nouter; ++ob) {
+            const int bo = ob * aphi_dev::kOuter;
+            const int be = std::min(bo + aphi_dev::kOuter, maxk);
+            for (int b = bo; b < be; b += aphi_dev::kNB) {
+                int rb = 0;
+                for (int i = 0; i < L.large_count; ++i) {
+                    const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
+                    if (kk <= b) continue;
+                    const int rem = mm - b - std::min(aphi_dev::kNB, kk - b);
+                    if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
+                }
+                aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, 0, fp, m->P.p, m->CB.p, m->mws.p, m->Xinv.p,
+                                               pivot_floor, st);
+                update(s, b, aphi_dev::kNB, 0, bo + aphi_dev::kOuter);
             }
-            aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, 0, fp, m->P.p, m->CB.p, m->mws.p, m->Xinv.p,
-                                           pivot_floor, st);
-            // two-level blocking: a narrow rank-kNB update inside the outer block,
-            // then a rank-kOuter update of the trailing front when the block is done
-            const int bo = (b / aphi_dev::kOuter) * aphi_dev::kOuter;
-            update(b, aphi_dev::kNB, bo, 1);
-            if ((b + aphi_dev::kNB) % aphi_dev::kOuter == 0 || b + aphi_dev::kNB >= maxk) update(bo, aphi_dev::kOuter, bo, 0);
+            if (side_busy) HIPCHK(hipStreamWaitEvent(s, m->ctx->side_done, 0));
+            if (ob + 1 < nouter) {
+                update(s, bo, aphi_dev::kOuter, 0, bo + 2 * aphi_dev::kOuter);
+                HIPCHK(hipEventRecord(m->ctx->main_done, s));
+                HIPCHK(hipStreamWaitEvent(m->ctx->side, m->ctx->main_done, 0));
+                update(m->ctx->side, bo, aphi_dev::kOuter, bo + 2 * aphi_dev::kOuter, -1);
+                HIPCHK(hipEventRecord(m->ctx->side_done, m->ctx->side));
+                side_busy = true;
+            } else {
+                update(s, bo, aphi_dev::kOuter, 0, -1);
+                side_busy = false;
+            }
         }
     }
     aphi_dev::launch_inv_blocks<T>(s, m->inv_pairs.as<int2>(), m->num_inv_pairs, fp, m->P.p, m->Xinv.p, m->X256.p,
@@ -396,7 +424,7 @@ void solve_any(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t 
 // backward error and relative residual per RHS, and (when bes is given) the
 // backward error of the equilibrated system; leaves r = b - A x in m->res
 void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int64_t ldb, double* be, double* rr,
-             double* bes = nullptr) {
+             double* bes = nullptr, double* scaled_den = nullptr) {
     hipStream_t s = m->st();
     const int n = m->n;
     ensure_solve_buffers(m, nrhs);
@@ -444,9 +472,193 @@ void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int6
             }
             const double dn = m->amax_eq * std::sqrt(tx) + std::sqrt(tb);
             bes[r] = dn > 0.0 ? std::sqrt(tr) / dn : std::sqrt(tr);
+            if (scaled_den) scaled_den[r] = dn;
         } else if (bes) {
             bes[r] = be ? be[r] : 0.0;
         }
+    }
+}
+
+// Right-preconditioned restarted GMRES on the equilibrated system
+// (D A D)(D^-1 x) = D b with the current factor as the preconditioner, in FP64:
+// it minimises ||D (b - A x)||, the residual of the equilibrated backward error,
+// over the Krylov space that classical refinement (one correction per solve)
+// explores one direction at a time, so it never needs more solves and it keeps
+// converging where the FP32 factor is too inaccurate for classical refinement.
+// Classical Gram-Schmidt with one reorthogonalization; partial sums in a fixed
+// order, so repeated runs give the same bits. Improves x in place, leaves the
+// measured backward errors of the final x in be/rr/bs and returns the number of
+// preconditioner solves.
+constexpr int kGmresRestart = 40;
+
+// APHI_TRACE_REFINEMENT=1 prints the refinement's residual history to stderr.
+bool trace_refinement() {
+    static const bool on = [] {
+        const char* e = std::getenv("APHI_TRACE_REFINEMENT");
+        return e != nullptr && e[0] == '1';
+    }();
+    return on;
+}
+int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ldx, double tol, int max_solves,
+                 std::vector<double>& be, std::vector<double>& rr, std::vector<double>& bs) {
+    using aphi_dev::cd;
+    const int n = m->n;
+    hipStream_t s = m->st();
+    const int RS = kGmresRestart;
+    const int64_t vs = static_cast<int64_t>(nrhs) * n;
+    const int parts = 64;
+    m->gV.reserve(static_cast<size_t>(RS + 1) * vs * sizeof(cd));
+    m->gZ.reserve(static_cast<size_t>(RS) * vs * sizeof(cd));
+    m->gW.reserve(static_cast<size_t>(vs) * sizeof(cd));
+    m->gU.reserve(static_cast<size_t>(vs) * sizeof(cd));
+    m->gDots.reserve(static_cast<size_t>(RS + 1) * nrhs * parts * sizeof(cd));
+    m->gSmall.reserve(static_cast<size_t>(RS + 1) * nrhs * sizeof(cd));
+    cd* V = m->gV.as<cd>();
+    cd* Z = m->gZ.as<cd>();
+    cd* w = m->gW.as<cd>();
+    cd* u = m->gU.as<cd>();
+    cd* dots = m->gDots.as<cd>();
+    const double* d = m->d.as<double>();
+    const dim3 grid(blocks_for(n), nrhs);
+    std::vector<double> den(static_cast<size_t>(nrhs));
+    std::vector<Complex> H(static_cast<size_t>(nrhs) * (RS + 1) * RS), cs(static_cast<size_t>(nrhs) * RS),
+        g(static_cast<size_t>(nrhs) * (RS + 1));
+    std::vector<double> cr(static_cast<size_t>(nrhs) * RS);
+    std::vector<double> hn(static_cast<size_t>(nrhs)), inv(static_cast<size_t>(nrhs));
+    std::vector<cd> hup(static_cast<size_t>(RS + 1) * nrhs), hdot(static_cast<size_t>(RS + 1) * nrhs * parts);
+    const auto worst = [&]() {
+        double wv = 0.0;
+        for (int r = 0; r < nrhs; ++r) {
+            const double e = std::max(be[static_cast<size_t>(r)], bs[static_cast<size_t>(r)]);
+            wv = std::isnan(e) ? std::numeric_limits<double>::infinity() : std::max(wv, e);
+        }
+        return wv;
+    };
+    // per-RHS 2-norms of w (host sums of fixed partials)
+    const auto norms = [&](const cd* v, std::vector<double>& out) {
+        double* nrm = m->norms.as<double>();
+        hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, v, static_cast<int64_t>(n), nrm);
+        std::vector<double> h(static_cast<size_t>(parts) * nrhs);
+        HIPCHK(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int r = 0; r < nrhs; ++r) {
+            double t = 0.0;
+            for (int p = 0; p < parts; ++p) t += h[static_cast<size_t>(r * parts + p)];
+            out[static_cast<size_t>(r)] = std::sqrt(t);
+        }
+    };
+    const auto scale_cols = [&](const cd* src, const std::vector<double>& f, cd* dst) {
+        HIPCHK(hipMemcpyAsync(m->gSmall.p, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(aphi_dev::k_scale_cols, grid, dim3(256), 0, s, n, nrhs, src, m->gSmall.as<double>(), dst);
+    };
+    int solves = 0;
+    for (;;) {
+        measure(m, nrhs, x, ldx, b, ldb, be.data(), rr.data(), bs.data(), den.data());
+        const double wv = worst();
+        if (trace_refinement()) std::fprintf(stderr, "gmres restart after %d solves: worst backward error %.3e\n", solves, wv);
+        if (wv <= tol || !std::isfinite(wv) || solves >= max_solves) return solves;
+        // V_0 = D r / ||D r||
+        hipLaunchKernelGGL(aphi_dev::k_scale_d, grid, dim3(256), 0, s, n, nrhs, d, 1, m->res.as<cd>(),
+                           static_cast<int64_t>(n), V, static_cast<int64_t>(n));
+        std::vector<double> beta(static_cast<size_t>(nrhs));
+        norms(V, beta);
+        for (int r = 0; r < nrhs; ++r) inv[static_cast<size_t>(r)] = beta[static_cast<size_t>(r)] > 0.0 ? 1.0 / beta[static_cast<size_t>(r)] : 0.0;
+        scale_cols(V, inv, V);
+        std::fill(H.begin(), H.end(), Complex(0.0, 0.0));
+        std::fill(g.begin(), g.end(), Complex(0.0, 0.0));
+        for (int r = 0; r < nrhs; ++r) g[static_cast<size_t>(r) * (RS + 1)] = Complex(beta[static_cast<size_t>(r)], 0.0);
+        int k = 0;
+        for (int j = 0; j < RS && solves < max_solves; ++j) {
+            cd* Vj = V + static_cast<int64_t>(j) * vs;
+            cd* Zj = Z + static_cast<int64_t>(j) * vs;
+            hipLaunchKernelGGL(aphi_dev::k_scale_d, grid, dim3(256), 0, s, n, nrhs, d, -1, Vj, static_cast<int64_t>(n), u,
+                               static_cast<int64_t>(n));
+            solve_any(m, nrhs, u, n, Zj, n, 0);
+            ++solves;
+            hipLaunchKernelGGL(aphi_dev::k_symv, grid, dim3(256), 0, s, n, nrhs, m->row_ptr.as<int>(), m->col.as<int>(),
+                               m->ttptr.as<int>(), m->tslot.as<int>(), m->trow.as<int>(), m->A.as<cd>(), Zj,
+                               static_cast<int64_t>(n), w, static_cast<int64_t>(n));
+            hipLaunchKernelGGL(aphi_dev::k_scale_d, grid, dim3(256), 0, s, n, nrhs, d, 1, w, static_cast<int64_t>(n), w,
+                               static_cast<int64_t>(n));
+            for (int pass = 0; pass < 2; ++pass) {
+                hipLaunchKernelGGL(aphi_dev::k_dot_partial, dim3(parts, j + 1, nrhs), dim3(256), 0, s, n, nrhs, V, vs, w, dots);
+                HIPCHK(hipMemcpyAsync(hdot.data(), dots, static_cast<size_t>(j + 1) * nrhs * parts * sizeof(cd),
+                                      hipMemcpyDeviceToHost, s));
+                HIPCHK(hipStreamSynchronize(s));
+                for (int v = 0; v <= j; ++v)
+                    for (int r = 0; r < nrhs; ++r) {
+                        double tr = 0.0, ti = 0.0;
+                        for (int p = 0; p < parts; ++p) {
+                            const cd& c = hdot[(static_cast<size_t>(v) * nrhs + r) * parts + p];
+                            tr += c.re;
+                            ti += c.im;
+                        }
+                        hup[static_cast<size_t>(v) * nrhs + r] = cd{tr, ti};
+                        H[(static_cast<size_t>(r) * (RS + 1) + v) * RS + j] += Complex(tr, ti);
+                    }
+                HIPCHK(hipMemcpyAsync(m->gSmall.p, hup.data(), static_cast<size_t>(j + 1) * nrhs * sizeof(cd),
+                                      hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(aphi_dev::k_sub_basis, grid, dim3(256), 0, s, n, nrhs, j + 1, V, vs, m->gSmall.as<cd>(), w);
+            }
+            norms(w, hn);
+            for (int r = 0; r < nrhs; ++r) inv[static_cast<size_t>(r)] = hn[static_cast<size_t>(r)] > 0.0 ? 1.0 / hn[static_cast<size_t>(r)] : 0.0;
+            scale_cols(w, inv, V + static_cast<int64_t>(j + 1) * vs);
+            // Givens rotations per right-hand side; |g_{j+1}| = ||D r|| of the updated x
+            double ratio = 0.0;
+            for (int r = 0; r < nrhs; ++r) {
+                Complex* Hr = &H[static_cast<size_t>(r) * (RS + 1) * RS];
+                Complex* gr = &g[static_cast<size_t>(r) * (RS + 1)];
+                Hr[static_cast<size_t>(j + 1) * RS + j] = Complex(hn[static_cast<size_t>(r)], 0.0);
+                for (int i = 0; i < j; ++i) {
+                    const double c = cr[static_cast<size_t>(r) * RS + i];
+                    const Complex sn = cs[static_cast<size_t>(r) * RS + i];
+                    const Complex a = Hr[static_cast<size_t>(i) * RS + j], bb = Hr[static_cast<size_t>(i + 1) * RS + j];
+                    Hr[static_cast<size_t>(i) * RS + j] = c * a + sn * bb;
+                    Hr[static_cast<size_t>(i + 1) * RS + j] = -std::conj(sn) * a + c * bb;
+                }
+                const Complex a = Hr[static_cast<size_t>(j) * RS + j], bb = Hr[static_cast<size_t>(j + 1) * RS + j];
+                const double rho = std::sqrt(std::norm(a) + std::norm(bb));
+                double c = 1.0;
+                Complex sn(0.0, 0.0);
+                if (rho > 0.0) {
+                    if (std::abs(a) == 0.0) {
+                        c = 0.0;
+                        sn = std::conj(bb) / rho;
+                    } else {
+                        c = std::abs(a) / rho;
+                        sn = (a / std::abs(a)) * std::conj(bb) / rho;
+                    }
+                }
+                cr[static_cast<size_t>(r) * RS + j] = c;
+                cs[static_cast<size_t>(r) * RS + j] = sn;
+                Hr[static_cast<size_t>(j) * RS + j] = c * a + sn * bb;
+                Hr[static_cast<size_t>(j + 1) * RS + j] = Complex(0.0, 0.0);
+                const Complex gj = gr[j];
+                gr[j] = c * gj;
+                gr[j + 1] = -std::conj(sn) * gj;
+                const double dr = den[static_cast<size_t>(r)];
+                ratio = std::max(ratio, dr > 0.0 ? std::abs(gr[j + 1]) / dr : std::abs(gr[j + 1]));
+            }
+            k = j + 1;
+            if (trace_refinement()) std::fprintf(stderr, "  gmres %2d: estimated scaled backward error %.3e\n", solves, ratio);
+            if (!(ratio > 0.5 * tol)) break;
+        }
+        // y = R^-1 g per right-hand side, then x += Z y
+        for (int r = 0; r < nrhs; ++r) {
+            const Complex* Hr = &H[static_cast<size_t>(r) * (RS + 1) * RS];
+            const Complex* gr = &g[static_cast<size_t>(r) * (RS + 1)];
+            std::vector<Complex> y(static_cast<size_t>(k));
+            for (int i = k - 1; i >= 0; --i) {
+                Complex t = gr[i];
+                for (int q = i + 1; q < k; ++q) t -= Hr[static_cast<size_t>(i) * RS + q] * y[static_cast<size_t>(q)];
+                const Complex dg = Hr[static_cast<size_t>(i) * RS + i];
+                y[static_cast<size_t>(i)] = std::abs(dg) > 0.0 ? t / dg : Complex(0.0, 0.0);
+            }
+            for (int i = 0; i < k; ++i)
+                hup[static_cast<size_t>(i) * nrhs + r] = cd{y[static_cast<size_t>(i)].real(), y[static_cast<size_t>(i)].imag()};
+        }
+        HIPCHK(hipMemcpyAsync(m->gSmall.p, hup.data(), static_cast<size_t>(k) * nrhs * sizeof(cd), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(aphi_dev::k_add_basis, grid, dim3(256), 0, s, n, nrhs, k, Z, vs, m->gSmall.as<cd>(), x, ldx);
     }
 }
 
@@ -480,8 +692,11 @@ extern "C" int aphi_context_create(int device, aphi_context** out) {
         auto* c = new aphi_context;
         c->device = device;
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&c->ev0));
         HIPCHK(hipEventCreate(&c->ev1));
+        HIPCHK(hipEventCreateWithFlags(&c->main_done, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -490,8 +705,12 @@ extern "C" void aphi_context_destroy(aphi_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->side);
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
+    (void)hipEventDestroy(c->main_done);
+    (void)hipEventDestroy(c->side_done);
+    (void)hipStreamDestroy(c->side);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -731,7 +950,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
                            &m->bpart, &m->colscale, &m->res, &m->norms};
     int64_t total = 0;
@@ -983,6 +1202,12 @@ extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int6
         std::vector<double> be(static_cast<size_t>(nrhs)), rr(static_cast<size_t>(nrhs)), bs(static_cast<size_t>(nrhs));
         int steps = 0, since_factor = 0;
         r.precision_used = opt->precision;
+        if (opt->precision == APHI_FP32 && opt->max_refinements > 0) {
+            // FP32 factor: GMRES refinement; if it does not converge within the
+            // budget, the loop below refactors in FP64
+            steps = gmres_refine(m, nrhs, b, ldb, x, ldx, opt->tolerance, opt->max_refinements, be, rr, bs);
+            since_factor = opt->max_refinements;
+        }
         // largest entry, +inf when any is NaN (so a non-finite solution never passes)
         const auto worst_of = [](const std::vector<double>& v) {
             double w = 0.0;

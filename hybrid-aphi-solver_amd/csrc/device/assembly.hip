@@ -435,6 +435,77 @@ __global__ void k_permute_scale_out(int n, int nrhs, const int* __restrict__ per
     x[c * ldx + p] = accumulate ? cadd(x[c * ldx + p], v) : v;
 }
 
+// ------------------------------------------------------------------ GMRES refinement
+//
+// Vectors are complex128, n per right-hand side; basis vector v of RHS r lives at
+// V + v * vstride + r * n. Reductions go through fixed-shape partial sums.
+
+// out[r][i] = in[r][i] * d[i] (power = 1) or in[r][i] / d[i] (power = -1)
+__global__ void k_scale_d(int n, int nrhs, const double* __restrict__ d, int power, const cd* __restrict__ in,
+                          int64_t ldi, cd* __restrict__ out, int64_t ldo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n || r >= nrhs) return;
+    const cd v = in[r * ldi + i];
+    out[r * ldo + i] = power > 0 ? cscale(v, d[i]) : cd{v.re / d[i], v.im / d[i]};
+}
+
+// partials[(v * nrhs + r) * gridDim.x + blk] = sum over the block's rows of conj(V_v[r]) w[r]
+__global__ void __launch_bounds__(256) k_dot_partial(int n, int nrhs, const cd* __restrict__ V, int64_t vstride,
+                                                     const cd* __restrict__ w, cd* __restrict__ partials) {
+    __shared__ double rre[256], rim[256];
+    const int v = blockIdx.y, r = blockIdx.z;
+    const cd* a = V + v * vstride + static_cast<int64_t>(r) * n;
+    const cd* b = w + static_cast<int64_t>(r) * n;
+    double sr = 0.0, si = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const cd x = a[i], y = b[i];
+        sr += x.re * y.re + x.im * y.im;
+        si += x.re * y.im - x.im * y.re;
+    }
+    rre[threadIdx.x] = sr;
+    rim[threadIdx.x] = si;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            rre[threadIdx.x] += rre[threadIdx.x + s];
+            rim[threadIdx.x] += rim[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(static_cast<int64_t>(v) * nrhs + r) * gridDim.x + blockIdx.x] = cd{rre[0], rim[0]};
+}
+
+// w[r][i] -= sum_v h[v * nrhs + r] V_v[r][i]
+__global__ void k_sub_basis(int n, int nrhs, int nvec, const cd* __restrict__ V, int64_t vstride,
+                            const cd* __restrict__ h, cd* __restrict__ w) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n || r >= nrhs) return;
+    cd acc = w[static_cast<int64_t>(r) * n + i];
+    for (int v = 0; v < nvec; ++v) acc = csub(acc, cmul(h[v * nrhs + r], V[v * vstride + static_cast<int64_t>(r) * n + i]));
+    w[static_cast<int64_t>(r) * n + i] = acc;
+}
+
+// out[r][i] = w[r][i] * s[r]
+__global__ void k_scale_cols(int n, int nrhs, const cd* __restrict__ w, const double* __restrict__ s, cd* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n || r >= nrhs) return;
+    out[static_cast<int64_t>(r) * n + i] = cscale(w[static_cast<int64_t>(r) * n + i], s[r]);
+}
+
+// x[r][i] += sum_v y[v * nrhs + r] Z_v[r][i]
+__global__ void k_add_basis(int n, int nrhs, int nvec, const cd* __restrict__ Z, int64_t zstride,
+                            const cd* __restrict__ y, cd* __restrict__ x, int64_t ldx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= n || r >= nrhs) return;
+    cd acc = x[r * ldx + i];
+    for (int v = 0; v < nvec; ++v) acc = cadd(acc, cmul(y[v * nrhs + r], Z[v * zstride + static_cast<int64_t>(r) * n + i]));
+    x[r * ldx + i] = acc;
+}
+
 // ------------------------------------------------------------------ port current
 
 // Per tet and right-hand side: the centroid current density of compute_fields

@@ -448,11 +448,14 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
 }
 
 // F[i][j] -= sum_{c in [c0, c1)} L[i][c] D[c] L[j][c] for i >= j over the target
-// region: rows [c1, m), columns [c1, tend) with c1 = min(c0 + width, k) and
-// tend = min(bo + kOuter, k) for the narrow update inside an outer block
-// (inner = 1) or tend = m for the trailing update after it (inner = 0). One
-// workgroup per square tile of the region's lower part (column-major
-// enumeration). The tile of F is loaded into the MFMA accumulators first and
+// region: columns [t0, t1) and rows [t0, m), with c1 = min(c0 + width, k),
+// t0 = max(c1, min(tlo, k)) and t1 = min(thi, k) (thi < 0: t1 = m). The factor
+// loop uses it for the narrow update inside an outer block (t1 = the block's
+// end), the next block's columns of an outer block's update (t1 = the next
+// block's end), and the rest of the trailing front (t0 = the next block's end,
+// t1 = m), the last two on separate streams so that the next block's
+// factorization overlaps the bulk of the update. One workgroup per square tile
+// of the region's lower part (column-major enumeration). The tile of F is loaded into the MFMA accumulators first and
 // written back at the end (no read-modify-write epilogue). Per stage of KC
 // columns, L rows j0.. (MFMA "row" operand) and i0.. (the "column" operand,
 // multiplied by -D at read time) arrive by DMA into one of two LDS buffers
@@ -461,7 +464,7 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
 // (rows >= m) land in rows of the tile that are never stored, so the factor
 // storage carries a small zeroed tail for them.
 template <typename T>
-__global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes, int c0, int width, int bo, int inner,
+__global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi,
                                                    FrontPlan fp, C<T>* P, C<T>* CB) {
     using V4 = typename Mfma<T>::v4;
     constexpr int WB = UpdTile<T>::WB, KC = UpdTile<T>::KC;
@@ -475,11 +478,12 @@ __global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
     if (fr.k <= c0) return;
     const int c1 = min(c0 + width, fr.k);
-    const int tend = inner ? min(bo + kOuter, fr.k) : fr.m;
-    if (tend <= c1) return;
+    const int t0 = max(c1, min(tlo, fr.k));
+    const int tend = thi < 0 ? fr.m : min(thi, fr.k);
+    if (tend <= t0) return;
     const int K = c1 - c0;
-    const int nrt = (fr.m - c1 + TS - 1) / TS;
-    const int nct = (tend - c1 + TS - 1) / TS;
+    const int nrt = (fr.m - t0 + TS - 1) / TS;
+    const int nct = (tend - t0 + TS - 1) / TS;
     const int t = static_cast<int>(blockIdx.x);
     if (t >= nct * nrt - nct * (nct - 1) / 2) return;
     // column tile tj holds nrt - tj tiles; first(tj) = tj*nrt - tj(tj-1)/2
@@ -489,7 +493,7 @@ __global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes
     while (tj > 0 && tj * nrt - tj * (tj - 1) / 2 > t) --tj;
     while (tj + 1 < nct && (tj + 1) * nrt - (tj + 1) * tj / 2 <= t) ++tj;
     const int ti = tj + (t - (tj * nrt - tj * (tj - 1) / 2));
-    const int i0 = c1 + ti * TS, j0 = c1 + tj * TS;
+    const int i0 = t0 + ti * TS, j0 = t0 + tj * TS;
 
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
@@ -635,10 +639,10 @@ void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int m
 }
 
 template <typename T>
-void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int bo, int inner, int max_tiles,
+void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
                    const FrontPlan& fp, void* P, void* CB) {
     if (count <= 0 || max_tiles <= 0) return;
-    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(256), 0, st, nodes, c0, width, bo, inner, fp,
+    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(256), 0, st, nodes, c0, width, tlo, thi, fp,
                        static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
 }
 

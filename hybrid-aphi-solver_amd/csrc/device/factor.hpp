@@ -24,10 +24,10 @@ template <typename T>
 void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
                        int row_chunks, const FrontPlan& fp, void* P, void* CB);
 // Rank-(c1 - c0) update with the L columns [c0, c1 = min(c0 + width, k)) of each
-// front: inner = 1 updates the columns [c1, min(bo + kOuter, k)) of the outer
-// block starting at bo, inner = 0 the whole trailing front [c1, m).
+// front, of the columns [max(c1, min(tlo, k)), thi < 0 ? m : min(thi, k)) and
+// every row below them.
 template <typename T>
-void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int bo, int inner, int max_tiles,
+void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
                    const FrontPlan& fp, void* P, void* CB);
 template <typename T> int update_tile();  // tile side of k_update
 template <typename T>

@@ -79,6 +79,13 @@ __global__ void k_scale_permute_in(int n, int nrhs, const int* perm, const doubl
 __global__ void k_permute_scale_out(int n, int nrhs, const int* perm, const double* d, const cd* xt, int64_t ldt,
                                     cd* x, int64_t ldx, int accumulate);
 
+// GMRES refinement (FP64)
+__global__ void k_scale_d(int n, int nrhs, const double* d, int power, const cd* in, int64_t ldi, cd* out, int64_t ldo);
+__global__ void k_dot_partial(int n, int nrhs, const cd* V, int64_t vstride, const cd* w, cd* partials);
+__global__ void k_sub_basis(int n, int nrhs, int nvec, const cd* V, int64_t vstride, const cd* h, cd* w);
+__global__ void k_scale_cols(int n, int nrhs, const cd* w, const double* s, cd* out);
+__global__ void k_add_basis(int n, int nrhs, int nvec, const cd* Z, int64_t zstride, const cd* y, cd* x, int64_t ldx);
+
 // Per-tet data of the field recovery (compute_fields, src/postprocess.cpp:312-340).
 struct TetFieldPlan {
     const double* coords;

@@ -1,0 +1,41 @@
+"""The sweep's precision choice (aphi_amd.sweep.PrecisionPolicy), host logic only."""
+import math
+
+import aphi_amd as A
+
+
+def rep(steps, used, factor_ms, solve_ms):
+    return {"refinement_steps": steps, "precision_used": used, "factorize_ms": factor_ms, "solve_ms": solve_ms}
+
+
+def test_starts_with_fp32_and_learns_costs():
+    p = A.PrecisionPolicy()
+    assert p.choose(10.0) == A.APHI_FP32
+    p.record(10.0, A.APHI_FP32, rep(3, A.APHI_FP32, 800.0, 4 * 25.0))
+    assert p.factor32 == 800.0 and p.solve32 == 25.0
+    # a neighbour that needed 3 solves: FP32 (800 + 4*25) beats the FP64 prior (1600 + 25)
+    assert p.choose(20.0) == A.APHI_FP32
+
+
+def test_fallback_marks_neighbourhood_fp64_and_bounds_it():
+    p = A.PrecisionPolicy()
+    p.record(1e3, A.APHI_FP32, rep(3, A.APHI_FP32, 800.0, 100.0))
+    p.record(1e7, A.APHI_FP32, rep(30, A.APHI_FP64, 2500.0, 900.0))
+    assert math.isinf(p.points[7.0])
+    assert p.choose(5e7) == A.APHI_FP64       # beyond the stalled point
+    assert p.choose(3e6) == A.APHI_FP64       # between an easy and a stalled point: the worse neighbour
+    assert p.choose(100.0) == A.APHI_FP32     # next to easy points only
+    # an FP64 point records timings, not an FP32 solve count
+    p.record(5e7, A.APHI_FP64, rep(0, A.APHI_FP64, 1700.0, 36.0))
+    assert p.factor64 == 1700.0 and 7.69 not in p.points
+
+
+def test_cost_model_threshold():
+    p = A.PrecisionPolicy()
+    p.record(1e3, A.APHI_FP32, rep(3, A.APHI_FP32, 800.0, 100.0))        # 25 ms per solve
+    p.record(1e5, A.APHI_FP64, rep(0, A.APHI_FP64, 1650.0, 35.0))
+    p.record(1e6, A.APHI_FP32, rep(30, A.APHI_FP32, 800.0, 31 * 25.0))   # slow but converged
+    # 800 + 31*25 = 1575 <= 1650 + 35: still FP32 next to it
+    assert p.choose(2e6) == A.APHI_FP32
+    p.record(2e6, A.APHI_FP32, rep(36, A.APHI_FP32, 800.0, 37 * 25.0))  # 800 + 37*25 = 1725 > 1685
+    assert p.choose(3e6) == A.APHI_FP64
