@@ -63,12 +63,14 @@ template <typename C> __device__ __forceinline__ C gfma(C a, C b, C c) {
     return C{a.re + b.re * c.re - b.im * c.im, a.im + b.re * c.im + b.im * c.re};
 }
 template <typename C> __device__ __forceinline__ C grecip(C a) {
-    // 1/a with a scaling that keeps |a|^2 in range
+    // 1/a = conj(a)/|a|^2, with a scaled by the power of two 2^-e that brings
+    // max(|re|, |im|) into [0.5, 1) (exact), so |a|^2 stays in range; one division
     using T = decltype(a.re);
-    const T s = fabs(a.re) > fabs(a.im) ? fabs(a.re) : fabs(a.im);
-    const T ar = a.re / s, ai = a.im / s;
-    const T den = (ar * ar + ai * ai) * s;
-    return C{ar / den, -ai / den};
+    int e;
+    frexp(fmax(fabs(a.re), fabs(a.im)), &e);
+    const T ar = ldexp(a.re, -e), ai = ldexp(a.im, -e);
+    const T inv = T(1) / (ar * ar + ai * ai);
+    return C{ldexp(ar * inv, -e), ldexp(-ai * inv, -e)};
 }
 template <typename C> __device__ __forceinline__ auto gabs(C a) -> decltype(a.re) { return hypot(a.re, a.im); }
 

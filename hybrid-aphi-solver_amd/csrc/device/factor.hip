@@ -17,6 +17,8 @@
 // loop: extend-add, then per NB-column block a diagonal factorization, a panel
 // multiply by (L_bb^-T D_bb^-1), and an MFMA trailing update of the remaining
 // front (the only O(m^2 k) step).
+#include <climits>
+
 #include "common.hpp"
 #include "factor.hpp"
 
@@ -64,6 +66,28 @@ __device__ __forceinline__ void record_pivot(FactorStatsDev* st, const FrontPlan
     stat_min(&st->smallest_pivot, mag);
     atomic_max_nonneg(&st->largest_pivot, mag);
     if (!(mag > floor_)) atomicMin(&st->fail_col, fp.col_ref[col]);
+}
+
+// Pivot statistics of up to 64 pivots held one per lane by one wavefront (lanes
+// with active = false hold none): reduced across the wave first, then one
+// atomic per statistic, since atomics from every lane on the same address
+// serialise.
+template <typename T>
+__device__ __forceinline__ void wave_pivot_stats(FactorStatsDev* st, const FrontPlan& fp, int col, C<T> d, bool active,
+                                                 double floor_) {
+    const double mag = active ? hypot(static_cast<double>(d.re), static_cast<double>(d.im)) : 0.0;
+    double mn = active ? mag : __builtin_inf(), mx = mag;
+    int fail = (active && !(mag > floor_)) ? fp.col_ref[col] : INT_MAX;
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = fmin(mn, __shfl_xor(mn, off));
+        mx = fmax(mx, __shfl_xor(mx, off));
+        fail = min(fail, __shfl_xor(fail, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (mn < __builtin_inf()) stat_min(&st->smallest_pivot, mn);
+        atomic_max_nonneg(&st->largest_pivot, mx);
+        if (fail != INT_MAX) atomicMin(&st->fail_col, fail);
+    }
 }
 
 // Block-wide max of a non-negative value into the stats.
@@ -260,115 +284,139 @@ __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ node
     }
 }
 
-// Diagonal block of step b for each front with k > b, one wavefront per front.
-// Lane r holds row r of the nb x nb block in registers (rows and columns beyond
-// nb padded with the identity); the right-looking LDL^T takes each column value
-// it needs from the owning lane with a scalar lane read, so the elimination runs
-// without barriers or LDS traffic. Then lane c forms column c of L_bb^-1 by
-// forward substitution (L read as LDS broadcasts), and the block inverse (kept
-// for the solves) and M = L_bb^-T D_bb^-1 (for the panel step) are written out.
+// Diagonal block of step b for each front with k > b, one 1024-thread workgroup
+// per front. Items: the 2080 lower-triangle entries of the 64 x 64 block (rows
+// and columns beyond nb padded with the identity) and the 2016 strict-lower
+// entries of X = L_bb^-1, four per thread, kept in registers. Step j does both
+// eliminations with one barrier: A[i][c] -= (A[i][j]/A[j][j]) A[c][j] for
+// j < c <= i, and X[i][c] -= L[i][j] X[j][c] for c < j < i (with X[i][j] set to
+// -L[i][j] at step j), reading the pivot column of A and row j of X from LDS;
+// their owners publish column j+1 and row j+1 into the other buffer of each
+// pair. A column of L is final (and written out) at its own step. Then the
+// inverse (kept for the solves) and M = L_bb^-T D_bb^-1 (for the panel step)
+// are written out.
 template <typename T>
-__global__ void __launch_bounds__(64) k_diag(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB,
-                                             C<T>* __restrict__ mws, C<T>* __restrict__ xinv, double pivot_floor,
-                                             FactorStatsDev* st) {
+__global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB,
+                                               C<T>* __restrict__ mws, C<T>* __restrict__ xinv, double pivot_floor,
+                                               FactorStatsDev* st) {
     extern __shared__ unsigned char smem_raw[];
     constexpr int LD = kNB + 1;
-    C<T>* Ls = reinterpret_cast<C<T>*>(smem_raw);  // Ls[c*LD + r]: factored block, then L_bb^-1 by columns
-    C<T>* Dl = Ls + kNB * LD;                     // pivots
+    constexpr int NA = kNB * (kNB + 1) / 2;      // A items (lower triangle with the diagonal)
+    constexpr int NX = kNB * (kNB - 1) / 2;      // X items (strict lower triangle)
+    constexpr int NT = 1024;
+    constexpr int NE = (NA + NX + NT - 1) / NT;  // items per thread
+    C<T>* Xs = reinterpret_cast<C<T>*>(smem_raw); // Xs[c*LD + r] = X[r][c]
+    C<T>* colA = Xs + kNB * LD;                   // two pivot-column buffers
+    C<T>* rowX = colA + 2 * kNB;                  // two pivot-row buffers
+    C<T>* Dl = rowX + 2 * kNB;
     const int s = nodes[blockIdx.x];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
     if (fr.k <= b) return;
     const int nb = min(kNB, fr.k - b);
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
     C<T>* blk = fr.panel + static_cast<int64_t>(b) * fr.m + b;  // element (b, b) of the front
+
+    // item -> (row, column, is_x); columns of A start at c*kNB - c(c-1)/2, of X at c*(kNB-1) - c(c-1)/2
+    int ir[NE], ic[NE];
+    bool isx[NE];
+    T vre[NE], vim[NE];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int item = tid + NT * u;
+        const bool x = item >= NA;
+        const int e = x ? item - NA : item;
+        const int len = x ? kNB - 1 : kNB;       // entries of column 0
+        const int total = x ? NX : NA;
+        const float q = 2.0f * len + 1.0f;
+        int c = static_cast<int>((q - sqrtf(q * q - 8.0f * static_cast<float>(min(e, total - 1)))) * 0.5f);
+        c = max(0, min(c, len - 1));
+        while (c > 0 && c * len - c * (c - 1) / 2 > e) --c;
+        while (c + 1 < len && (c + 1) * len - (c + 1) * c / 2 <= e) ++c;
+        const bool valid = e < total;
+        const int r = c + (x ? 1 : 0) + (e - (c * len - c * (c - 1) / 2));
+        ir[u] = valid ? r : 0;
+        ic[u] = valid ? c : kNB + 1;  // past every step, never published
+        isx[u] = x;
+    }
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        const int r = ir[u], c = ic[u];
+        const bool load = !isx[u] && c < nb && r < nb;
+        const C<T> v = blk[static_cast<int64_t>(load ? c : 0) * fr.m + (load ? r : 0)];
+        vre[u] = load ? v.re : static_cast<T>(!isx[u] && r == c ? 1 : 0);
+        vim[u] = load ? v.im : static_cast<T>(0);
+        if (!isx[u] && c == 0) colA[r] = C<T>{vre[u], vim[u]};
+    }
+    __syncthreads();
+    // Steps at which each item changes: A[r][c] at j <= c (updated for j < c,
+    // turned into L[r][c] at j = c), X[r][c] at c <= j < r. Each wave skips an item
+    // slot outside the union of its lanes' ranges (a wave-uniform branch).
+    int lo[NE], hi[NE];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+        int l0 = isx[u] ? ic[u] : 0, h0 = isx[u] ? ir[u] - 1 : ic[u];
+        for (int off = 32; off > 0; off >>= 1) {
+            l0 = min(l0, __shfl_xor(l0, off));
+            h0 = max(h0, __shfl_xor(h0, off));
+        }
+        lo[u] = __builtin_amdgcn_readfirstlane(l0);
+        hi[u] = __builtin_amdgcn_readfirstlane(h0);
+    }
+    for (int j = 0; j < kNB; ++j) {
+        const C<T>* cj = colA + (j & 1) * kNB;
+        C<T>* cn = colA + ((j + 1) & 1) * kNB;
+        const C<T>* rj = rowX + (j & 1) * kNB;
+        C<T>* rn = rowX + ((j + 1) & 1) * kNB;
+        const C<T> rd = grecip(cj[j]);
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            if (j < lo[u] || j > hi[u]) continue;
+            const int r = ir[u], c = min(ic[u], kNB - 1);
+            const C<T> ar = cj[r];
+            const C<T> ac = isx[u] ? rj[c] : cj[c];
+            const C<T> lr = gmul(ar, rd);  // L[r][j]
+            const C<T> cur = C<T>{vre[u], vim[u]};
+            const C<T> upd = gfms(cur, lr, ac);
+            // A: update for c > j; column j becomes L below the diagonal (D stays on it).
+            // X: X[r][j] = -L[r][j]; update for c < j < r.
+            const bool do_upd = isx[u] ? (c < j && r > j) : (c > j);
+            const bool do_set = isx[u] ? (c == j) : (c == j && r > j);
+            const T sgn = isx[u] ? T(-1) : T(1);
+            const T nre = do_upd ? upd.re : (do_set ? sgn * lr.re : cur.re);
+            const T nim = do_upd ? upd.im : (do_set ? sgn * lr.im : cur.im);
+            vre[u] = nre;
+            vim[u] = nim;
+            const bool pub = isx[u] ? (r == j + 1) : (ic[u] == j + 1);
+            if (pub) (isx[u] ? rn[c] : cn[r]) = C<T>{nre, nim};
+        }
+        __syncthreads();
+    }
+    // L and D out to the front (stores kept out of the loop so no step waits on
+    // them), X into LDS (column c, row r at Xs[c*LD + r]; diagonal 1, upper unused)
     double lmax = 0.0;
-    if constexpr (sizeof(T) == 4) {
-        C<T> a[kNB];
+    for (int idx = tid; idx < kNB; idx += NT) Xs[idx * LD + idx] = C<T>{1, 0};
 #pragma unroll
-        for (int c = 0; c < kNB; ++c) {
-            const bool in = lane < nb && c < nb;
-            a[c] = in ? (c <= lane ? blk[static_cast<int64_t>(c) * fr.m + lane] : C<T>{0, 0})
-                      : C<T>{static_cast<T>(lane == c ? 1 : 0), 0};
-        }
-        // a[c] of lane r is A[r][c]; entries above the diagonal are scratch.
-#pragma unroll
-        for (int j = 0; j < kNB; ++j) {
-            const C<T> aj = a[j];
-            const C<T> l = gmul(aj, grecip(lane_read(aj, j)));
-            if (lane > j) a[j] = l;
-#pragma unroll
-            for (int c = j + 1; c < kNB; ++c) a[c] = gfms(a[c], l, lane_read(aj, c));  // A[r][c] -= l_r A[c][j]
-        }
-#pragma unroll
-        for (int c = 0; c < kNB; ++c) Ls[c * LD + lane] = a[c];
-    } else {
-        // complex double: the same elimination with row `lane` in LDS (64 complex
-        // doubles per lane would not fit the register file)
-        C<T>* U = Dl + kNB;  // the pivot column before scaling
-#pragma unroll 8
-        for (int c = 0; c < kNB; ++c) {
-            const bool in = lane < nb && c < nb;
-            Ls[c * LD + lane] = in ? (c <= lane ? blk[static_cast<int64_t>(c) * fr.m + lane] : C<T>{0, 0})
-                                   : C<T>{static_cast<T>(lane == c ? 1 : 0), 0};
-        }
-        __syncthreads();
-        for (int j = 0; j < kNB; ++j) {
-            const C<T> aj = Ls[j * LD + lane];
-            const C<T> l = gmul(aj, grecip(Ls[j * LD + j]));
-            U[lane] = aj;
-            __syncthreads();
-            if (lane > j) Ls[j * LD + lane] = l;
-#pragma unroll 4
-            for (int c = j + 1; c < kNB; ++c) Ls[c * LD + lane] = gfms(Ls[c * LD + lane], l, U[c]);
-            __syncthreads();
+    for (int u = 0; u < NE; ++u) {
+        const int r = ir[u], c = ic[u];
+        const C<T> v = C<T>{vre[u], vim[u]};
+        if (isx[u]) {
+            if (c < kNB) Xs[c * LD + r] = v;
+        } else {
+            if (r == c) Dl[r] = v;
+            if (r > c && r < nb) lmax = fmax(lmax, static_cast<double>(hypot(v.re, v.im)));
+            if (r < nb && c < nb) blk[static_cast<int64_t>(c) * fr.m + r] = v;
         }
     }
     __syncthreads();
-    for (int c = 0; c < nb; ++c) {
-        const C<T> v = Ls[c * LD + lane];
-        if (c < lane && lane < nb) lmax = fmax(lmax, static_cast<double>(hypot(v.re, v.im)));
-        if (c <= lane && lane < nb) blk[static_cast<int64_t>(c) * fr.m + lane] = v;
-    }
-    const C<T> dd = Ls[lane * LD + lane];
-    Dl[lane] = dd;
-    if (lane < nb) record_pivot<T>(st, fp, fp.first[s] + b + lane, dd, pivot_floor);
+    if (tid < 64) wave_pivot_stats<T>(st, fp, fp.first[s] + b + min(tid, nb - 1), Dl[tid], tid < nb, pivot_floor);
     block_max_multiplier(st, lmax, nullptr);
-    // column `lane` of L_bb^-1: x[r] = delta(r, lane) - sum_{q<r} L[r][q] x[q];
-    // x in registers for complex float, in a per-lane LDS column for complex
-    // double (whose 64 entries would not fit the register file next to the rest)
-    if constexpr (sizeof(T) == 4) {
-        C<T> x[kNB];
-#pragma unroll
-        for (int r = 0; r < kNB; ++r) {
-            C<T> acc = C<T>{static_cast<T>(lane == r ? 1 : 0), 0};
-#pragma unroll
-            for (int q = 0; q < r; ++q) acc = gfms(acc, Ls[q * LD + r], x[q]);
-            x[r] = acc;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kNB; ++r) Ls[lane * LD + r] = x[r];  // Ls[q*LD + c] = Linv[c][q]
-    } else {
-        C<T>* Xs = Dl + 2 * kNB;  // Xs[r*kNB + lane] = x[r] of lane
-#pragma unroll
-        for (int r = 0; r < kNB; ++r) {
-            C<T> acc = C<T>{static_cast<T>(lane == r ? 1 : 0), 0};
-#pragma unroll 16
-            for (int q = 0; q < r; ++q) acc = gfms(acc, Ls[q * LD + r], Xs[q * kNB + lane]);
-            Xs[r * kNB + lane] = acc;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kNB; ++r) Ls[lane * LD + r] = Xs[r * kNB + lane];
-    }
-    __syncthreads();
     // M[q][c] = Linv[c][q] / D[c] (c >= q) at mws[q*kNB + c]; the inverse, column q
     // and row c at q*nb + c, is kept for the solves
     C<T>* M = mws + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
     C<T>* Xg = xinv + fp.xinv_off[s] + static_cast<int64_t>(b / kNB) * kNB * kNB;
-    for (int idx = lane; idx < nb * nb; idx += 64) {
+    for (int idx = tid; idx < nb * nb; idx += NT) {
         const int q = idx / nb, c = idx % nb;
-        const C<T> v = c >= q ? Ls[q * LD + c] : C<T>{0, 0};
+        const C<T> v = c >= q ? Xs[q * LD + c] : C<T>{0, 0};
         Xg[idx] = v;
         M[q * kNB + c] = c >= q ? gmul(v, grecip(Dl[c])) : C<T>{0, 0};
     }
@@ -426,21 +474,26 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
 
 // ------------------------------------------------------------------ trailing update (MFMA)
 //
-// F[i][j] -= sum_c L[i][c] D[c] L[j][c] over the block column c in [b, b+nb),
-// for the lower triangle of rows/columns [b+nb, m). One workgroup per 64x64 tile
-// (ti >= tj) of one front; four waves, each a 32x32 quadrant of 2x2 MFMA 16x16
-// blocks with separate real/imaginary accumulators (complex product as four real
-// products). The MFMA "row" operand is L[j] and the "column" operand W[i] =
-// L[i] D, so each lane's output column is a front row i: stores are contiguous.
+// F[i][j] -= sum_c L[i][c] D[c] L[j][c] over a range of columns c, for the lower
+// triangle of a region of the trailing front. One workgroup per square tile
+// (ti >= tj) of one front; each wave owns a block of 16 x 16 MFMA tiles. The MFMA
+// "row" operand is L[j] and the "column" operand W[i] = -L[i] D, so each lane's
+// output column is a front row i: stores are contiguous.
 
 // Tiling of the update. A wave moves one column segment of the L panel into LDS
 // with a single 16-byte-per-lane DMA load (global_load_lds_dwordx4): 128 rows of
 // complex float or 64 rows of complex double. That segment length is the tile
-// side: 128x128 tiles for complex float (each wave a 64x64 quadrant of 4x4 MFMA
-// blocks), 64x64 for complex double (2x2 blocks per wave).
+// side: 128 x 128 tiles for complex float with eight waves of 32 x 64 (three
+// accumulator sets, see k_update), 64 x 64 for complex double with four waves of
+// 32 x 32 (two sets). The per-wave accumulators set the register budget: 96 or 64
+// registers of accumulators per lane, two waves per SIMD.
 template <typename T> struct UpdTile;
-template <> struct UpdTile<float> { static constexpr int WB = 4, KC = 16; };
-template <> struct UpdTile<double> { static constexpr int WB = 2, KC = 8; };
+template <> struct UpdTile<float> {
+    static constexpr int TS = 128, WX = 2, WY = 4, GX = 4, GY = 2, KC = 32, NW = GX * GY, MINB = 1;
+};
+template <> struct UpdTile<double> {
+    static constexpr int TS = 64, WX = 2, WY = 2, GX = 2, GY = 2, KC = 8, NW = GX * GY, MINB = 2;
+};
 
 __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
     __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
@@ -464,11 +517,13 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
 // (rows >= m) land in rows of the tile that are never stored, so the factor
 // storage carries a small zeroed tail for them.
 template <typename T>
-__global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi,
+__global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB) k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi,
                                                    FrontPlan fp, C<T>* P, C<T>* CB) {
     using V4 = typename Mfma<T>::v4;
-    constexpr int WB = UpdTile<T>::WB, KC = UpdTile<T>::KC;
-    constexpr int TS = 32 * WB;                          // tile side
+    using UT = UpdTile<T>;
+    constexpr int TS = UT::TS, WX = UT::WX, WY = UT::WY, KC = UT::KC, NW = UT::NW;
+    static_assert(UT::GX * WX * 16 == TS && UT::GY * WY * 16 == TS, "waves tile the update tile");
+    static_assert((2 * KC) % NW == 0, "stage DMAs divide among the waves");
     constexpr int RPL = 16 / static_cast<int>(sizeof(C<T>));  // rows per lane per DMA
     static_assert(TS == 64 * RPL, "one wave DMA covers one tile column segment");
     constexpr int LDR = TS + 16;                         // LDS row stride (complex)
@@ -497,15 +552,15 @@ __global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes
 
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
-    const int wi = (wave & 1) * (TS / 2), wj = (wave >> 1) * (TS / 2);  // quadrant offsets (i, j)
+    const int wi = (wave % UT::GX) * (16 * WX), wj = (wave / UT::GX) * (16 * WY);  // wave offsets (i, j)
     const C<T>* L0 = fr.panel + static_cast<int64_t>(c0) * fr.m;
     const int nst = (K + KC - 1) / KC;
 
-    // stage st: 2*KC column segments (L rows j0.. then rows i0..), KC/2 per wave
+    // stage st: 2*KC column segments (L rows j0.. then rows i0..), 2*KC/NW per wave
     const auto issue_stage = [&](int buf, int st) {
 #pragma unroll
-        for (int u = 0; u < 2 * KC / 4; ++u) {
-            const int jdx = wave + 4 * u;
+        for (int u = 0; u < 2 * KC / NW; ++u) {
+            const int jdx = wave + NW * u;
             const int mat = jdx / KC, q = jdx % KC;
             const int c = min(st * KC + q, K - 1);
             const int row = min((mat ? i0 : j0) + RPL * lane, fr.m - 1);
@@ -514,11 +569,18 @@ __global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes
     };
     issue_stage(0, 0);
 
-    V4 accr[WB][WB], acci[WB][WB];
+    // FP32 tiles use the three-product form of the complex product (a = ar + i ai,
+    // b = br + i bi): k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi),
+    // re = k1 - k3, im = k1 + k2, with S1 = sum k1 from 0, S2 = sum k2 from Fi and
+    // S3 = sum k3 from -Fr (then re = S1 - S3 and im = S1 + S2 include F): three
+    // MFMAs per product instead of four. FP64 tiles use the four-product form
+    // with Fr and Fi as the accumulators.
+    constexpr bool k3 = std::is_same<T, float>::value;
+    V4 accr[WX][WY], acci[WX][WY], accs[k3 ? WX : 1][k3 ? WY : 1];
 #pragma unroll
-    for (int x = 0; x < WB; ++x)
+    for (int x = 0; x < WX; ++x)
 #pragma unroll
-        for (int y = 0; y < WB; ++y)
+        for (int y = 0; y < WY; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wi + x * 16 + (lane % 16);
@@ -526,10 +588,11 @@ __global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes
                 const int jj = min(j, fr.m - 1), ii = max(min(i, fr.m - 1), jj);  // a stored entry
                 const C<T> c = *fr.at(ii, jj);
                 const bool ok = i < fr.m && j < tend && i >= j;
-                accr[x][y][v] = ok ? c.re : static_cast<T>(0);
+                accr[x][y][v] = ok ? (k3 ? -c.re : c.re) : static_cast<T>(0);
                 acci[x][y][v] = ok ? c.im : static_cast<T>(0);
+                if constexpr (k3) accs[x][y][v] = static_cast<T>(0);
             }
-    for (int q = tid; q < nst * KC; q += 256) {
+    for (int q = tid; q < nst * KC; q += 64 * NW) {
         const int qq = min(q, K - 1);
         const C<T> d = L0[static_cast<int64_t>(qq) * fr.m + c0 + qq];
         Dsh[q] = q < K ? C<T>{-d.re, -d.im} : C<T>{0, 0};
@@ -544,42 +607,63 @@ __global__ void __launch_bounds__(256, 2) k_update(const int* __restrict__ nodes
         for (int k4 = 0; k4 < KC; k4 += 4) {
             const int kq = k4 + lane / 16;
             const C<T> d = Dsh[st * KC + kq];
-            T ar[WB], ai[WB], br[WB], bi[WB];
+            T ar[WY], ai[WY], br[WX], bi[WX];
 #pragma unroll
-            for (int y = 0; y < WB; ++y) {
+            for (int y = 0; y < WY; ++y) {
                 const C<T> a = Lc[buf][kq][wj + y * 16 + (lane % 16)];
                 ar[y] = a.re;
                 ai[y] = a.im;
             }
 #pragma unroll
-            for (int x = 0; x < WB; ++x) {
+            for (int x = 0; x < WX; ++x) {
                 const C<T> w = gmul(Wc[buf][kq][wi + x * 16 + (lane % 16)], d);
                 br[x] = w.re;
                 bi[x] = w.im;
             }
+            if constexpr (k3) {
+                T as[WY], bd[WX], bs[WX];
 #pragma unroll
-            for (int x = 0; x < WB; ++x)
+                for (int y = 0; y < WY; ++y) as[y] = ar[y] + ai[y];
 #pragma unroll
-                for (int y = 0; y < WB; ++y) {
-                    accr[x][y] = Mfma<T>::mac(ar[y], br[x], accr[x][y]);
-                    accr[x][y] = Mfma<T>::mac(-ai[y], bi[x], accr[x][y]);
-                    acci[x][y] = Mfma<T>::mac(ar[y], bi[x], acci[x][y]);
-                    acci[x][y] = Mfma<T>::mac(ai[y], br[x], acci[x][y]);
+                for (int x = 0; x < WX; ++x) {
+                    bd[x] = bi[x] - br[x];
+                    bs[x] = br[x] + bi[x];
                 }
+#pragma unroll
+                for (int x = 0; x < WX; ++x)
+#pragma unroll
+                    for (int y = 0; y < WY; ++y) {
+                        accs[x][y] = Mfma<T>::mac(as[y], br[x], accs[x][y]);
+                        acci[x][y] = Mfma<T>::mac(ar[y], bd[x], acci[x][y]);
+                        accr[x][y] = Mfma<T>::mac(ai[y], bs[x], accr[x][y]);
+                    }
+            } else {
+#pragma unroll
+                for (int x = 0; x < WX; ++x)
+#pragma unroll
+                    for (int y = 0; y < WY; ++y) {
+                        accr[x][y] = Mfma<T>::mac(ar[y], br[x], accr[x][y]);
+                        accr[x][y] = Mfma<T>::mac(-ai[y], bi[x], accr[x][y]);
+                        acci[x][y] = Mfma<T>::mac(ar[y], bi[x], acci[x][y]);
+                        acci[x][y] = Mfma<T>::mac(ai[y], br[x], acci[x][y]);
+                    }
+            }
         }
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
     // D[p][q]: lane holds p = Mfma<T>::row(lane, v) (j), q = lane%16 (i)
 #pragma unroll
-    for (int x = 0; x < WB; ++x)
+    for (int x = 0; x < WX; ++x)
 #pragma unroll
-        for (int y = 0; y < WB; ++y)
+        for (int y = 0; y < WY; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wi + x * 16 + (lane % 16);
                 const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
-                if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = C<T>{accr[x][y][v], acci[x][y][v]};
+                C<T> out = C<T>{accr[x][y][v], acci[x][y][v]};
+                if constexpr (k3) out = C<T>{accs[x][y][v] - accr[x][y][v], accs[x][y][v] + acci[x][y][v]};
+                if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
             }
 }
 
@@ -625,9 +709,9 @@ template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,
                        FactorStatsDev* stats) {
-    const size_t diag_lds = static_cast<size_t>(kNB * (kNB + 1) + 2 * kNB + (sizeof(T) == 8 ? kNB * kNB : 0)) * sizeof(C<T>);
+    const size_t diag_lds = static_cast<size_t>(kNB * (kNB + 1) + 5 * kNB) * sizeof(C<T>);
     allow_large_lds(k_diag<T>, diag_lds);
-    hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(64), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
+    hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(1024), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
                        static_cast<C<T>*>(CB), static_cast<C<T>*>(mws), static_cast<C<T>*>(xinv), pivot_floor, stats);
     if (max_row_blocks > 0) {
         const size_t panel_lds = static_cast<size_t>(64 * kNB + kNB * kNB) * sizeof(C<T>);
@@ -642,13 +726,13 @@ template <typename T>
 void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
                    const FrontPlan& fp, void* P, void* CB) {
     if (count <= 0 || max_tiles <= 0) return;
-    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(256), 0, st, nodes, c0, width, tlo, thi, fp,
+    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(64 * UpdTile<T>::NW), 0, st, nodes, c0, width, tlo, thi, fp,
                        static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
 }
 
 template <typename T>
 int update_tile() {
-    return 32 * UpdTile<T>::WB;
+    return UpdTile<T>::TS;
 }
 
 template void launch_zero<double>(hipStream_t, int64_t, void*);
