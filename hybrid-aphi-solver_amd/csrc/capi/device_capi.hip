@@ -30,6 +30,9 @@ namespace {
 constexpr double kMu0 = 4.0e-7 * 3.14159265358979323846;
 constexpr double kEps0 = 8.8541878128e-12;
 constexpr int kSmallMaxLimit = 96;  // LDS bound: fronts up to this many rows fit one workgroup
+// the solve's forward gather reads extend-add bounds for fronts of more than one
+// chunk of rows; those exist for every front that is not small
+static_assert(kSmallMaxLimit < aphi_dev::kGatherRows, "gather chunks need extend-add bounds");
 
 // Fronts up to this many rows are factored in LDS by one workgroup; the rest take
 // the blocked path. APHI_SMALL_FRONT_MAX (0..96) overrides it, for tuning.
@@ -103,7 +106,7 @@ struct aphi_mesh {
     std::vector<Complex> value_tab;
     std::vector<Level> levels;
     int max_level_count = 0;
-    int64_t max_bwd_partials = 0;  // count * row blocks over levels, per NB*nrhs
+    int64_t max_bwd_partials = 0;  // count * kBwdRows-row chunks over levels, per kSB * padded nrhs
     // plan on the device
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
@@ -275,7 +278,7 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
             if (tiles == 0) return;
             const bool timed = m->profiling;
             if (timed) HIPCHK(hipEventRecord(next_event(), on));
-            aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, fp, m->P.p, m->CB.p);
+            aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, 0, fp, m->P.p, m->CB.p);
             if (timed) HIPCHK(hipEventRecord(next_event(), on));
             ++launches;
             update_flops += fl;
@@ -355,12 +358,13 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
 void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     if (nrhs <= m->solve_nrhs_cap) return;
     const size_t cs = sizeof(cd);  // sized for the wider type; FP32 uses the front half
-    m->bt.reserve(static_cast<size_t>(m->n) * nrhs * cs);
-    m->X.reserve(static_cast<size_t>(m->n) * nrhs * cs);
-    m->Yv.reserve(static_cast<size_t>(m->n) * nrhs * cs);
-    m->W.reserve(static_cast<size_t>(m->rows_total) * nrhs * cs);
-    m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kSB * nrhs * cs);
-    m->tsum.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kSB * nrhs * cs);
+    const size_t q = static_cast<size_t>(aphi_dev::rhs_groups(nrhs)) * aphi_dev::kQ;  // padded right-hand sides
+    m->bt.reserve(static_cast<size_t>(m->n) * q * cs);
+    m->X.reserve(static_cast<size_t>(m->n) * q * cs);
+    m->Yv.reserve(static_cast<size_t>(m->n) * q * cs);
+    m->W.reserve(static_cast<size_t>(m->rows_total) * q * cs);
+    m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kSB * q * cs);
+    m->tsum.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kSB * q * cs);
     m->colscale.reserve(nrhs * sizeof(double));
     m->res.reserve(static_cast<size_t>(m->n) * nrhs * sizeof(cd));
     m->norms.reserve(static_cast<size_t>(64) * nrhs * 6 * sizeof(double));
@@ -379,25 +383,25 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
     aphi_dev::launch_perm_in<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), b, ldb, m->colscale.as<double>(), m->bt.p);
     const int* nodes = m->nodes.as<int>();
     constexpr int SB = aphi_dev::kSB;
-    // rows below block B of the level's supernodes, in 64-row chunks (max over the level)
-    const auto row_chunks = [](const Level& L, int B) {
+    // rows below block B of the level's supernodes (max over the level)
+    const auto rows_below = [](const Level& L, int B) {
         int rb = 0;
         for (int i = 0; i < L.count(); ++i) {
             const int kk = L.k[static_cast<size_t>(i)];
             if (kk <= B) continue;
-            const int rem = L.m[static_cast<size_t>(i)] - B - std::min(aphi_dev::kSB, kk - B);
-            if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
+            rb = std::max(rb, L.m[static_cast<size_t>(i)] - B - std::min(aphi_dev::kSB, kk - B));
         }
         return rb;
     };
     for (const Level& L : m->levels) {
         const int cnt = L.count();
-        aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, nrhs, fp, m->bt.p, n, m->W.p);
+        aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, std::max(L.small_max_m, L.large_max_m), nrhs, fp, m->bt.p,
+                                       n, m->rows_total, m->W.p);
         int maxk = 0;
         for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
         for (int B = 0; B < maxk; B += SB)
-            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), row_chunks(L, B), nrhs, fp,
-                                         m->P.p, m->X256.p, m->W.p, m->Yv.p, n);
+            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), rows_below(L, B), nrhs, fp,
+                                         m->P.p, m->X256.p, m->W.p, m->Yv.p, n, m->rows_total);
     }
     for (int li = m->num_levels - 1; li >= 0; --li) {
         const Level& L = m->levels[static_cast<size_t>(li)];
@@ -406,7 +410,7 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
         if (maxk == 0) continue;
         for (int B = ((maxk - 1) / SB) * SB; B >= 0; B -= SB)
-            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), row_chunks(L, B), nrhs, fp,
+            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), rows_below(L, B), nrhs, fp,
                                          m->P.p, m->XT256.p, m->Yv.p, m->X.p, n, m->bpart.p, m->tsum.p);
     }
     aphi_dev::launch_perm_out<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), m->X.p, m->colscale.as<double>(), x, ldx,
@@ -856,7 +860,7 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                         ea.push_back(blk);
                         blk += (mm + 63) / 64;
                     }
-                    maxrb = std::max(maxrb, (mm - std::min(kk, aphi_dev::kSB) + 63) / 64);
+                    maxrb = std::max(maxrb, (mm - std::min(kk, aphi_dev::kSB) + aphi_dev::kBwdRows - 1) / aphi_dev::kBwdRows);
                 }
             lv.ea_blocks = blk;
             ea.push_back(blk);

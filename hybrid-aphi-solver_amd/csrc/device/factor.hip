@@ -17,6 +17,7 @@
 // loop: extend-add, then per NB-column block a diagonal factorization, a panel
 // multiply by (L_bb^-T D_bb^-1), and an MFMA trailing update of the remaining
 // front (the only O(m^2 k) step).
+#include <algorithm>
 #include <climits>
 
 #include "common.hpp"
@@ -422,51 +423,59 @@ __global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, in
     }
 }
 
-// Panel step: rows i in [b+nb, m) of block column b become L = F * M. One
-// workgroup per (64-row block, front); its rows are staged in LDS first, so the
-// in-place write is safe.
+// Panel step: rows i in [b+nb, m) of block column b become L = F * M (M upper
+// triangular, nb x nb, zero-padded to 64 x 64 here). One workgroup per (64-row
+// block, front), wave w the 16 rows r0 + 16 w, on MFMA: 16 values of F per lane
+// in registers, then the four 16-column tiles of M read from L2 one after the
+// other. Each wave reads all of its rows before writing them, so the in-place
+// write needs no staging, and the kernel uses no LDS: it fits beside the
+// trailing-update workgroups that fill the device meanwhile.
 template <typename T>
 __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB,
                                                const C<T>* __restrict__ mws, FactorStatsDev* st) {
-    extern __shared__ unsigned char smem_raw[];
+    using V4 = typename Mfma<T>::v4;
     const int s = nodes[blockIdx.y];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
     if (fr.k <= b) return;
     const int nb = min(kNB, fr.k - b);
-    const int r0 = b + nb + static_cast<int>(blockIdx.x) * 64;
-    if (r0 >= fr.m) return;
-    const int nr = min(64, fr.m - r0);
-    C<T>* Fs = reinterpret_cast<C<T>*>(smem_raw);  // Fs[q*64 + row]
-    C<T>* Ms = Fs + 64 * kNB;                     // Ms[q*kNB + c]
+    const int rw = b + nb + static_cast<int>(blockIdx.x) * 64 + 16 * (static_cast<int>(threadIdx.x) / 64);
     const C<T>* M = mws + static_cast<int64_t>(blockIdx.y) * kNB * kNB;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    {
-        // all loads of the row block and of M in flight before the LDS stores
-        C<T> f[64 * kNB / 256], g[kNB * kNB / 256];
-#pragma unroll
-        for (int u = 0; u < 64 * kNB / 256; ++u) {
-            const int idx = tid + 256 * u, q = idx / 64, r = idx % 64;
-            f[u] = (q < nb && r < nr) ? fr.panel[static_cast<int64_t>(b + q) * fr.m + r0 + r] : C<T>{0, 0};
-        }
-#pragma unroll
-        for (int u = 0; u < kNB * kNB / 256; ++u) {
-            const int idx = tid + 256 * u;
-            g[u] = M[idx];
-        }
-#pragma unroll
-        for (int u = 0; u < 64 * kNB / 256; ++u) Fs[tid + 256 * u] = f[u];
-#pragma unroll
-        for (int u = 0; u < kNB * kNB / 256; ++u) Ms[tid + 256 * u] = g[u];
-    }
-    __syncthreads();
-    const int r = tid % 64;
+    const int lane = threadIdx.x % 64, p = lane % 16, kq = lane / 16;
     double lmax = 0.0;
-    for (int c = tid / 64; c < nb; c += nt / 64) {
-        C<T> acc = C<T>{0, 0};
-        for (int q = 0; q <= c; ++q) acc = gfma(acc, Fs[q * 64 + r], Ms[q * kNB + c]);
-        if (r < nr) {
-            fr.panel[static_cast<int64_t>(b + c) * fr.m + r0 + r] = acc;
-            lmax = fmax(lmax, static_cast<double>(hypot(acc.re, acc.im)));
+    if (rw < fr.m) {
+        C<T> a[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int q = 4 * u + kq;
+            a[u] = q < nb ? fr.panel[static_cast<int64_t>(b + q) * fr.m + min(rw + p, fr.m - 1)] : C<T>{0, 0};
+        }
+#pragma unroll 1
+        for (int ct = 0; ct < kNB; ct += 16) {
+            if (ct >= nb) break;
+            C<T> g[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int q = 4 * u + kq, c = ct + p;
+                g[u] = (q < nb && c < nb) ? M[q * kNB + c] : C<T>{0, 0};
+            }
+            V4 accr = V4{0, 0, 0, 0}, acci = V4{0, 0, 0, 0};
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                accr = Mfma<T>::mac(a[u].re, g[u].re, accr);
+                accr = Mfma<T>::mac(-a[u].im, g[u].im, accr);
+                acci = Mfma<T>::mac(a[u].re, g[u].im, acci);
+                acci = Mfma<T>::mac(a[u].im, g[u].re, acci);
+            }
+            // D[row][col]: row = Mfma<T>::row(lane, v) (front row), col = lane % 16 (column)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rw + Mfma<T>::row(lane, v), c = ct + p;
+                if (i < fr.m && c < nb) {
+                    const C<T> l = C<T>{accr[v], acci[v]};
+                    fr.panel[static_cast<int64_t>(b + c) * fr.m + i] = l;
+                    lmax = fmax(lmax, static_cast<double>(hypot(l.re, l.im)));
+                }
+            }
         }
     }
     block_max_multiplier(st, lmax, nullptr);
@@ -489,10 +498,10 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
 // registers of accumulators per lane, two waves per SIMD.
 template <typename T> struct UpdTile;
 template <> struct UpdTile<float> {
-    static constexpr int TS = 128, WX = 2, WY = 4, GX = 4, GY = 2, KC = 32, NW = GX * GY, MINB = 1;
+    static constexpr int TS = 128, WX = 2, WY = 4, GX = 4, GY = 2, KC = 32, NW = GX * GY, MINB = 1, WPC = 1;
 };
 template <> struct UpdTile<double> {
-    static constexpr int TS = 64, WX = 2, WY = 2, GX = 2, GY = 2, KC = 8, NW = GX * GY, MINB = 2;
+    static constexpr int TS = 64, WX = 2, WY = 2, GX = 2, GY = 2, KC = 8, NW = GX * GY, MINB = 2, WPC = 3;
 };
 
 __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
@@ -507,18 +516,24 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
 // end), the next block's columns of an outer block's update (t1 = the next
 // block's end), and the rest of the trailing front (t0 = the next block's end,
 // t1 = m), the last two on separate streams so that the next block's
-// factorization overlaps the bulk of the update. One workgroup per square tile
-// of the region's lower part (column-major enumeration). The tile of F is loaded into the MFMA accumulators first and
-// written back at the end (no read-modify-write epilogue). Per stage of KC
-// columns, L rows j0.. (MFMA "row" operand) and i0.. (the "column" operand,
-// multiplied by -D at read time) arrive by DMA into one of two LDS buffers
-// while the other is being consumed; one barrier per stage. Each lane's output
-// column is a front row i, so stores are contiguous. Reads past a column's end
-// (rows >= m) land in rows of the tile that are never stored, so the factor
-// storage carries a small zeroed tail for them.
+// factorization overlaps the bulk of the update.
+//
+// Square tiles of the region's lower part, column-major enumeration; a
+// persistent grid of gridDim.x workgroups per front walks them. Workgroups are
+// placed on the XCDs round-robin by their linear index, so the tiles are split
+// into eight contiguous ranges, one per XCD (neighbouring tiles share L panel
+// rows in that XCD's L2). Per stage of KC columns, L rows j0.. (MFMA "row"
+// operand) and i0.. (the "column" operand, multiplied by -D at read time) arrive
+// by DMA into one of two LDS buffers while the other is being consumed; one
+// barrier per stage. The stage numbering runs on across tiles, so the first
+// stage of the next tile is in flight during the last stage of this one; the F
+// tile is read in the epilogue and the sums stored. Each lane's output column is a front
+// row i, so stores are contiguous. Reads past a column's end (rows >= m) land in
+// rows of the tile that are never stored, so the factor storage carries a small
+// zeroed tail for them.
 template <typename T>
-__global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB) k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi,
-                                                   FrontPlan fp, C<T>* P, C<T>* CB) {
+__global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
+    k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi, FrontPlan fp, C<T>* P, C<T>* CB) {
     using V4 = typename Mfma<T>::v4;
     using UT = UpdTile<T>;
     constexpr int TS = UT::TS, WX = UT::WX, WY = UT::WY, KC = UT::KC, NW = UT::NW;
@@ -539,16 +554,31 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB) k_updat
     const int K = c1 - c0;
     const int nrt = (fr.m - t0 + TS - 1) / TS;
     const int nct = (tend - t0 + TS - 1) / TS;
-    const int t = static_cast<int>(blockIdx.x);
-    if (t >= nct * nrt - nct * (nct - 1) / 2) return;
-    // column tile tj holds nrt - tj tiles; first(tj) = tj*nrt - tj(tj-1)/2
-    const double q2 = 2.0 * nrt + 1.0;
-    int tj = static_cast<int>((q2 - sqrt(q2 * q2 - 8.0 * t)) * 0.5);
-    tj = max(0, min(tj, nct - 1));
-    while (tj > 0 && tj * nrt - tj * (tj - 1) / 2 > t) --tj;
-    while (tj + 1 < nct && (tj + 1) * nrt - (tj + 1) * tj / 2 <= t) ++tj;
-    const int ti = tj + (t - (tj * nrt - tj * (tj - 1) / 2));
-    const int i0 = t0 + ti * TS, j0 = t0 + tj * TS;
+    const int ntiles = nct * nrt - nct * (nct - 1) / 2;
+    // this workgroup's tiles: the workgroups fall into ng = min(8, G) groups by
+    // blockIdx.x % ng (the XCDs, when G is a multiple of 8); group x takes a
+    // contiguous range of tiles in proportion to its size, shared round-robin
+    const int G = static_cast<int>(gridDim.x);
+    const int ng = min(8, G);
+    const int xcd = static_cast<int>(blockIdx.x) % ng, lidx = static_cast<int>(blockIdx.x) / ng;
+    const int nloc = (G - xcd + ng - 1) / ng;
+    int before = 0;
+    for (int x = 0; x < xcd; ++x) before += (G - x + ng - 1) / ng;
+    const int tfirst = static_cast<int>(static_cast<int64_t>(ntiles) * before / G);
+    const int tlast = static_cast<int>(static_cast<int64_t>(ntiles) * (before + nloc) / G);
+    int t = tfirst + lidx;
+    if (t >= tlast) return;
+    // tile index -> (i0, j0); column tile tj holds nrt - tj tiles, first(tj) = tj*nrt - tj(tj-1)/2
+    const auto tile_origin = [&](int tt, int& i0, int& j0) {
+        const double q2 = 2.0 * nrt + 1.0;
+        int tj = static_cast<int>((q2 - sqrt(q2 * q2 - 8.0 * tt)) * 0.5);
+        tj = max(0, min(tj, nct - 1));
+        while (tj > 0 && tj * nrt - tj * (tj - 1) / 2 > tt) --tj;
+        while (tj + 1 < nct && (tj + 1) * nrt - (tj + 1) * tj / 2 <= tt) ++tj;
+        const int ti = tj + (tt - (tj * nrt - tj * (tj - 1) / 2));
+        i0 = t0 + ti * TS;
+        j0 = t0 + tj * TS;
+    };
 
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
@@ -556,8 +586,8 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB) k_updat
     const C<T>* L0 = fr.panel + static_cast<int64_t>(c0) * fr.m;
     const int nst = (K + KC - 1) / KC;
 
-    // stage st: 2*KC column segments (L rows j0.. then rows i0..), 2*KC/NW per wave
-    const auto issue_stage = [&](int buf, int st) {
+    // stage st of the tile at (i0, j0): 2*KC column segments (L rows j0.. then rows i0..), 2*KC/NW per wave
+    const auto issue_stage = [&](int buf, int st, int i0, int j0) {
 #pragma unroll
         for (int u = 0; u < 2 * KC / NW; ++u) {
             const int jdx = wave + NW * u;
@@ -567,31 +597,10 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB) k_updat
             lds_dma16(L0 + static_cast<int64_t>(c) * fr.m + row, mat ? &Wc[buf][q][0] : &Lc[buf][q][0]);
         }
     };
-    issue_stage(0, 0);
-
-    // FP32 tiles use the three-product form of the complex product (a = ar + i ai,
-    // b = br + i bi): k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi),
-    // re = k1 - k3, im = k1 + k2, with S1 = sum k1 from 0, S2 = sum k2 from Fi and
-    // S3 = sum k3 from -Fr (then re = S1 - S3 and im = S1 + S2 include F): three
-    // MFMAs per product instead of four. FP64 tiles use the four-product form
-    // with Fr and Fi as the accumulators.
-    constexpr bool k3 = std::is_same<T, float>::value;
-    V4 accr[WX][WY], acci[WX][WY], accs[k3 ? WX : 1][k3 ? WY : 1];
-#pragma unroll
-    for (int x = 0; x < WX; ++x)
-#pragma unroll
-        for (int y = 0; y < WY; ++y)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = i0 + wi + x * 16 + (lane % 16);
-                const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
-                const int jj = min(j, fr.m - 1), ii = max(min(i, fr.m - 1), jj);  // a stored entry
-                const C<T> c = *fr.at(ii, jj);
-                const bool ok = i < fr.m && j < tend && i >= j;
-                accr[x][y][v] = ok ? (k3 ? -c.re : c.re) : static_cast<T>(0);
-                acci[x][y][v] = ok ? c.im : static_cast<T>(0);
-                if constexpr (k3) accs[x][y][v] = static_cast<T>(0);
-            }
+    int i0, j0;
+    tile_origin(t, i0, j0);
+    int gs = 0;  // stage counter across tiles: LDS buffer gs & 1
+    issue_stage(0, 0, i0, j0);
     for (int q = tid; q < nst * KC; q += 64 * NW) {
         const int qq = min(q, K - 1);
         const C<T> d = L0[static_cast<int64_t>(qq) * fr.m + c0 + qq];
@@ -600,71 +609,113 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB) k_updat
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
-    for (int st = 0; st < nst; ++st) {
-        const int buf = st & 1;
-        if (st + 1 < nst) issue_stage(buf ^ 1, st + 1);
+    // FP32 tiles use the three-product form of the complex product (a = ar + i ai,
+    // b = br + i bi): k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi),
+    // re = k1 - k3, im = k1 + k2, summed in S1 (accs), S2 (acci) and S3 (accr):
+    // three MFMAs per product instead of four. FP64 tiles use the four-product
+    // form (accr, acci).
+    constexpr bool k3 = std::is_same<T, float>::value;
+    for (;;) {
+        V4 accr[WX][WY], acci[WX][WY], accs[k3 ? WX : 1][k3 ? WY : 1];
 #pragma unroll
-        for (int k4 = 0; k4 < KC; k4 += 4) {
-            const int kq = k4 + lane / 16;
-            const C<T> d = Dsh[st * KC + kq];
-            T ar[WY], ai[WY], br[WX], bi[WX];
+        for (int x = 0; x < WX; ++x)
 #pragma unroll
             for (int y = 0; y < WY; ++y) {
-                const C<T> a = Lc[buf][kq][wj + y * 16 + (lane % 16)];
-                ar[y] = a.re;
-                ai[y] = a.im;
+                accr[x][y] = V4{0, 0, 0, 0};
+                acci[x][y] = V4{0, 0, 0, 0};
+                if constexpr (k3) accs[x][y] = V4{0, 0, 0, 0};
             }
+        const int tn = t + nloc;  // this workgroup's next tile
+        int ni0 = 0, nj0 = 0;
+        if (tn < tlast) tile_origin(tn, ni0, nj0);
+        for (int st = 0; st < nst; ++st, ++gs) {
+            const int buf = gs & 1;
+            if (st + 1 < nst) issue_stage(buf ^ 1, st + 1, i0, j0);
+            else if (tn < tlast) issue_stage(buf ^ 1, 0, ni0, nj0);
 #pragma unroll
-            for (int x = 0; x < WX; ++x) {
-                const C<T> w = gmul(Wc[buf][kq][wi + x * 16 + (lane % 16)], d);
-                br[x] = w.re;
-                bi[x] = w.im;
-            }
-            if constexpr (k3) {
-                T as[WY], bd[WX], bs[WX];
+            for (int k4 = 0; k4 < KC; k4 += 4) {
+                const int kq = k4 + lane / 16;
+                const C<T> d = Dsh[st * KC + kq];
+                T ar[WY], ai[WY], br[WX], bi[WX];
 #pragma unroll
-                for (int y = 0; y < WY; ++y) as[y] = ar[y] + ai[y];
-#pragma unroll
-                for (int x = 0; x < WX; ++x) {
-                    bd[x] = bi[x] - br[x];
-                    bs[x] = br[x] + bi[x];
+                for (int y = 0; y < WY; ++y) {
+                    const C<T> a = Lc[buf][kq][wj + y * 16 + (lane % 16)];
+                    ar[y] = a.re;
+                    ai[y] = a.im;
                 }
 #pragma unroll
-                for (int x = 0; x < WX; ++x)
+                for (int x = 0; x < WX; ++x) {
+                    const C<T> w = gmul(Wc[buf][kq][wi + x * 16 + (lane % 16)], d);
+                    br[x] = w.re;
+                    bi[x] = w.im;
+                }
+                if constexpr (k3) {
+                    T as[WY], bd[WX], bs[WX];
 #pragma unroll
-                    for (int y = 0; y < WY; ++y) {
-                        accs[x][y] = Mfma<T>::mac(as[y], br[x], accs[x][y]);
-                        acci[x][y] = Mfma<T>::mac(ar[y], bd[x], acci[x][y]);
-                        accr[x][y] = Mfma<T>::mac(ai[y], bs[x], accr[x][y]);
+                    for (int y = 0; y < WY; ++y) as[y] = ar[y] + ai[y];
+#pragma unroll
+                    for (int x = 0; x < WX; ++x) {
+                        bd[x] = bi[x] - br[x];
+                        bs[x] = br[x] + bi[x];
                     }
-            } else {
 #pragma unroll
-                for (int x = 0; x < WX; ++x)
+                    for (int x = 0; x < WX; ++x)
 #pragma unroll
-                    for (int y = 0; y < WY; ++y) {
-                        accr[x][y] = Mfma<T>::mac(ar[y], br[x], accr[x][y]);
-                        accr[x][y] = Mfma<T>::mac(-ai[y], bi[x], accr[x][y]);
-                        acci[x][y] = Mfma<T>::mac(ar[y], bi[x], acci[x][y]);
-                        acci[x][y] = Mfma<T>::mac(ai[y], br[x], acci[x][y]);
-                    }
+                        for (int y = 0; y < WY; ++y) {
+                            accs[x][y] = Mfma<T>::mac(as[y], br[x], accs[x][y]);
+                            acci[x][y] = Mfma<T>::mac(ar[y], bd[x], acci[x][y]);
+                            accr[x][y] = Mfma<T>::mac(ai[y], bs[x], accr[x][y]);
+                        }
+                } else {
+#pragma unroll
+                    for (int x = 0; x < WX; ++x)
+#pragma unroll
+                        for (int y = 0; y < WY; ++y) {
+                            accr[x][y] = Mfma<T>::mac(ar[y], br[x], accr[x][y]);
+                            accr[x][y] = Mfma<T>::mac(-ai[y], bi[x], accr[x][y]);
+                            acci[x][y] = Mfma<T>::mac(ar[y], bi[x], acci[x][y]);
+                            acci[x][y] = Mfma<T>::mac(ai[y], br[x], acci[x][y]);
+                        }
+                }
             }
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
         }
-        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        __syncthreads();
+        // D[p][q]: lane holds p = Mfma<T>::row(lane, v) (j), q = lane%16 (i); all
+        // the F loads first, then the sums and stores
+        C<T> f[WX][WY][4];
+#pragma unroll
+        for (int x = 0; x < WX; ++x)
+#pragma unroll
+            for (int y = 0; y < WY; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = i0 + wi + x * 16 + (lane % 16);
+                    const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
+                    const int jj = min(j, fr.m - 1), ii = max(min(i, fr.m - 1), jj);  // a stored entry
+                    f[x][y][v] = *fr.at(ii, jj);
+                }
+#pragma unroll
+        for (int x = 0; x < WX; ++x)
+#pragma unroll
+            for (int y = 0; y < WY; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = i0 + wi + x * 16 + (lane % 16);
+                    const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
+                    const C<T> fv = f[x][y][v];
+                    C<T> out;
+                    if constexpr (k3)
+                        out = C<T>{fv.re + (accs[x][y][v] - accr[x][y][v]), fv.im + (accs[x][y][v] + acci[x][y][v])};
+                    else
+                        out = C<T>{fv.re + accr[x][y][v], fv.im + acci[x][y][v]};
+                    if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
+                }
+        if (tn >= tlast) break;
+        t = tn;
+        i0 = ni0;
+        j0 = nj0;
     }
-    // D[p][q]: lane holds p = Mfma<T>::row(lane, v) (j), q = lane%16 (i)
-#pragma unroll
-    for (int x = 0; x < WX; ++x)
-#pragma unroll
-        for (int y = 0; y < WY; ++y)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = i0 + wi + x * 16 + (lane % 16);
-                const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
-                C<T> out = C<T>{accr[x][y][v], acci[x][y][v]};
-                if constexpr (k3) out = C<T>{accs[x][y][v] - accr[x][y][v], accs[x][y][v] + acci[x][y][v]};
-                if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
-            }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -714,19 +765,39 @@ void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int m
     hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(1024), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
                        static_cast<C<T>*>(CB), static_cast<C<T>*>(mws), static_cast<C<T>*>(xinv), pivot_floor, stats);
     if (max_row_blocks > 0) {
-        const size_t panel_lds = static_cast<size_t>(64 * kNB + kNB * kNB) * sizeof(C<T>);
-        allow_large_lds(k_panel<T>, panel_lds);
-        hipLaunchKernelGGL(k_panel<T>, dim3(max_row_blocks, count), dim3(256), panel_lds, st, nodes, b, fp,
+        hipLaunchKernelGGL(k_panel<T>, dim3(max_row_blocks, count), dim3(256), 0, st, nodes, b, fp,
                            static_cast<C<T>*>(P), static_cast<C<T>*>(CB), static_cast<const C<T>*>(mws), stats);
     }
     (void)max_tiles;
 }
 
+namespace {
+// compute units of the current device (cached per device)
+int device_cus() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (cus[dev] == 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || v <= 0) v = 256;
+        cus[dev] = v;
+    }
+    return cus[dev];
+}
+}  // namespace
+
 template <typename T>
 void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
-                   const FrontPlan& fp, void* P, void* CB) {
+                   int max_wgs, const FrontPlan& fp, void* P, void* CB) {
     if (count <= 0 || max_tiles <= 0) return;
-    hipLaunchKernelGGL(k_update<T>, dim3(max_tiles, count), dim3(64 * UpdTile<T>::NW), 0, st, nodes, c0, width, tlo, thi, fp,
+    // workgroups per front: one per tile, or (max_wgs > 0) a persistent grid of
+    // about max_wgs workgroups over all fronts, a multiple of 8 (one share per
+    // XCD). A persistent grid holds its compute units until it ends, so the
+    // update that runs beside the factorization of the next block takes one
+    // tile per workgroup and lets the other stream's work in between tiles.
+    int g = max_tiles;
+    if (max_wgs > 0) g = std::min(max_tiles, std::max(8, (max_wgs / count) / 8 * 8));
+    hipLaunchKernelGGL(k_update<T>, dim3(g, count), dim3(64 * UpdTile<T>::NW), 0, st, nodes, c0, width, tlo, thi, fp,
                        static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
 }
 
@@ -747,8 +818,11 @@ template void launch_front_small<float>(hipStream_t, const int*, int, int, const
                                         double, FactorStatsDev*);
 template void launch_extend_add<double>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
 template void launch_extend_add<float>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
-template void launch_update<double>(hipStream_t, const int*, int, int, int, int, int, int, const FrontPlan&, void*, void*);
-template void launch_update<float>(hipStream_t, const int*, int, int, int, int, int, int, const FrontPlan&, void*, void*);
+template void launch_update<double>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*);
+template void launch_update<float>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*);
+template <typename T> int update_resident_wgs() { return device_cus() * UpdTile<T>::WPC; }
+template int update_resident_wgs<double>();
+template int update_resident_wgs<float>();
 template int update_tile<double>();
 template int update_tile<float>();
 template void launch_block_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,

@@ -13,6 +13,9 @@ constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64
 constexpr int kOuter = 256;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
 constexpr int kSB = 256;        // solve block: columns per forward/backward step (multiple of kNB)
+constexpr int kQ = 16;          // right-hand sides per interleaved solve group
+constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a multiple of 64)
+constexpr int kBwdRows = 256;     // rows per backward partial sum
 
 template <typename T> void launch_zero(hipStream_t st, int64_t n, void* p);
 template <typename T> void launch_scatter_panel(hipStream_t st, int64_t nnz, const int64_t* s2p, const cd* a, void* P);
@@ -26,16 +29,21 @@ void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, i
 // Rank-(c1 - c0) update with the L columns [c0, c1 = min(c0 + width, k)) of each
 // front, of the columns [max(c1, min(tlo, k)), thi < 0 ? m : min(thi, k)) and
 // every row below them.
+// max_wgs = 0: one workgroup per tile; > 0: a persistent grid of about that many
+// workgroups (for an update that has the device to itself)
 template <typename T>
 void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
-                   const FrontPlan& fp, void* P, void* CB);
-template <typename T> int update_tile();  // tile side of k_update
+                   int max_wgs, const FrontPlan& fp, void* P, void* CB);
+template <typename T> int update_tile();          // tile side of k_update
+template <typename T> int update_resident_wgs();  // k_update workgroups the device holds at once
 template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,
                        FactorStatsDev* stats);
 
-// solve.hip
+// solve.hip: vectors in groups of kQ right-hand sides, entry (i, q) of group g at
+// [(g * rows + i) * kQ + q] (rows = n, or the total front row count for W)
+int rhs_groups(int nrhs);
 template <typename T>
 void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
                     const double* colscale, void* out);
@@ -43,18 +51,21 @@ template <typename T>
 void launch_perm_out(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const void* xt,
                      const double* colscale, cd* x, int64_t ldx, int accumulate);
 template <typename T>
-void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int nrhs, const FrontPlan& fp, const void* b, int n,
-                       void* W);
+void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int max_m, int nrhs, const FrontPlan& fp,
+                       const void* b, int n, int64_t R, void* W);
 // inverses of the kSB diagonal blocks (column-major X, row-major XT), from the
 // kNB-block inverses; pairs = (supernode, first column of the block)
 template <typename T>
 void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const FrontPlan& fp, const void* P,
                        const void* xinv, void* X, void* XT);
+// max_rows_below: the most rows below block B among the level's supernodes
 template <typename T>
-void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n);
+void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
+                     const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R);
+// partial: count * ceil(max_rows_below / kBwdRows) * kSB * kQ * rhs_groups(nrhs) entries;
+// tsum: count * kSB * kQ * rhs_groups(nrhs)
 template <typename T>
-void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_row_blocks, int nrhs,
+void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
                      const FrontPlan& fp, const void* P, const void* XT256, const void* Y, void* X, int n,
                      void* partial, void* tsum);
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,

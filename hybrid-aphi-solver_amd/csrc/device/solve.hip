@@ -11,6 +11,16 @@
 // columns whose diagonal-block inverses are formed once after the
 // factorization, so a block costs two (forward) or three (backward) parallel
 // launches instead of a chain of substitutions.
+//
+// Vectors are kept in groups of kQ = 16 right-hand sides, interleaved: entry
+// (row i, right-hand side q) of group g at [(g * rows + i) * 16 + q], with rows = n
+// for the permuted right-hand side, y and x, and the total front row count for
+// the front vectors W. A 16-row by 16-column tile of any of them is then one
+// contiguous 16-complex segment per row, which is the MFMA operand shape: every
+// step kernel is a set of 16 x 16 complex tiles, one wave per (tile, 64-wide
+// slice of the inner dimension), the slices summed in LDS in a fixed order.
+#include <algorithm>
+
 #include "common.hpp"
 #include "factor.hpp"
 
@@ -35,30 +45,80 @@ __device__ __forceinline__ Sn sn(const FrontPlan& fp, int s) {
     return x;
 }
 
+template <typename T>
+using V4 = typename Mfma<T>::v4;
+
+// One wave: acc += A (16 x 64) B (64 x 16) for one 64-wide slice of the inner
+// dimension, all 32 operand loads issued before the first MFMA. Lane l supplies
+// A[l%16][l/16 + 4u] and B[l/16 + 4u][l%16], u = 0..15; a complex product is four
+// real MFMAs.
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ void cmma64(V4<T>& accr, V4<T>& acci, FA a_at, FB b_at) {
+    const int lane = static_cast<int>(threadIdx.x % 64);
+    const int p = lane % 16, kq = lane / 16;
+    C<T> a[16], b[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) a[u] = a_at(p, 4 * u + kq);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) b[u] = b_at(4 * u + kq, p);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        accr = Mfma<T>::mac(a[u].re, b[u].re, accr);
+        accr = Mfma<T>::mac(-a[u].im, b[u].im, accr);
+        acci = Mfma<T>::mac(a[u].re, b[u].im, acci);
+        acci = Mfma<T>::mac(a[u].im, b[u].re, acci);
+    }
+}
+
+// The four waves' partial 16 x 16 tiles summed in wave order; out(p, q, v) is
+// called by thread (p = tid / 16, q = tid % 16) with the sum.
+template <typename T, typename FO>
+__device__ __forceinline__ void reduce4(C<T> (*part)[256], const V4<T>& accr, const V4<T>& acci, FO out) {
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part[wave][Mfma<T>::row(lane, v) * 16 + lane % 16] = C<T>{accr[v], acci[v]};
+    __syncthreads();
+    C<T> t = part[0][tid];
+    t = gadd(t, part[1][tid]);
+    t = gadd(t, part[2][tid]);
+    t = gadd(t, part[3][tid]);
+    out(tid / 16, tid % 16, t);
+}
+
 }  // namespace
 
+// b in the caller's layout (column r at b + r * ldb) -> permuted, equilibrated,
+// column-scaled and interleaved; one thread per (row, right-hand side)
 template <typename T>
-__global__ void k_perm_in(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ d,
-                          const cd* __restrict__ b, int64_t ldb, const double* __restrict__ colscale, C<T>* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
-    if (i >= n || r >= nrhs) return;
-    const int p = perm[i];
-    cd v = cmul(b[r * ldb + p], cd{d[p], 0.0});
-    const double s = colscale[r];
-    if (s != 1.0) v = cd{v.re * s, v.im * s};
-    out[static_cast<int64_t>(r) * n + i] = C<T>{static_cast<T>(v.re), static_cast<T>(v.im)};
+__global__ void __launch_bounds__(256) k_perm_in(int n, int nrhs, const int* __restrict__ perm,
+                                                 const double* __restrict__ d, const cd* __restrict__ b, int64_t ldb,
+                                                 const double* __restrict__ colscale, C<T>* __restrict__ out) {
+    const int i = static_cast<int>(blockIdx.x) * 16 + static_cast<int>(threadIdx.x) / kQ;
+    const int q = static_cast<int>(threadIdx.x) % kQ, g = static_cast<int>(blockIdx.y);
+    const int r = g * kQ + q;
+    if (i >= n) return;
+    C<T> o = C<T>{0, 0};
+    if (r < nrhs) {
+        const int p = perm[i];
+        cd v = cmul(b[r * ldb + p], cd{d[p], 0.0});
+        const double s = colscale[r];
+        if (s != 1.0) v = cd{v.re * s, v.im * s};
+        o = C<T>{static_cast<T>(v.re), static_cast<T>(v.im)};
+    }
+    out[(static_cast<int64_t>(g) * n + i) * kQ + q] = o;
 }
 
 template <typename T>
-__global__ void k_perm_out(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ d,
-                           const C<T>* __restrict__ xt, const double* __restrict__ colscale, cd* __restrict__ x,
-                           int64_t ldx, int accumulate) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
+__global__ void __launch_bounds__(256) k_perm_out(int n, int nrhs, const int* __restrict__ perm,
+                                                  const double* __restrict__ d, const C<T>* __restrict__ xt,
+                                                  const double* __restrict__ colscale, cd* __restrict__ x, int64_t ldx,
+                                                  int accumulate) {
+    const int i = static_cast<int>(blockIdx.x) * 16 + static_cast<int>(threadIdx.x) / kQ;
+    const int q = static_cast<int>(threadIdx.x) % kQ, g = static_cast<int>(blockIdx.y);
+    const int r = g * kQ + q;
     if (i >= n || r >= nrhs) return;
     const int p = perm[i];
-    const C<T> t = xt[static_cast<int64_t>(r) * n + i];
+    const C<T> t = xt[(static_cast<int64_t>(g) * n + i) * kQ + q];
     cd v = cd{static_cast<double>(t.re), static_cast<double>(t.im)};
     const double s = colscale[r];
     if (s != 1.0) v = cd{v.re / s, v.im / s};
@@ -66,25 +126,47 @@ __global__ void k_perm_out(int n, int nrhs, const int* __restrict__ perm, const 
     x[r * ldx + p] = accumulate ? cadd(x[r * ldx + p], v) : v;
 }
 
-// W_s := [b rows of s ; 0] + extend-add of the children's update vectors.
+// Rows [r0, r0 + kGatherRows) of W_s := [b rows of s ; 0] + the children's update
+// vectors, children in order, accumulated in LDS. A child's update rows land at
+// increasing parent positions (rel), so the ones in this chunk are a contiguous
+// range, read from the extend-add bounds for fronts of more than one chunk.
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_gather(const int* __restrict__ nodes, int nrhs, FrontPlan fp,
-                                                    const C<T>* __restrict__ b, int n, C<T>* W) {
-    const int s = nodes[blockIdx.x];
-    const int r = blockIdx.y;
+__global__ void __launch_bounds__(256) k_fwd_gather(const int* __restrict__ nodes, FrontPlan fp,
+                                                    const C<T>* __restrict__ bt, int n, int64_t R, C<T>* W) {
+    extern __shared__ unsigned char smem_raw[];
+    C<T>* acc = reinterpret_cast<C<T>*>(smem_raw);  // [kGatherRows][kQ]
+    const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z);
     const Sn x = sn(fp, s);
-    C<T>* Ws = W + x.rp * nrhs + static_cast<int64_t>(r) * x.m;
-    for (int i = threadIdx.x; i < x.m; i += blockDim.x)
-        Ws[i] = i < x.k ? b[static_cast<int64_t>(r) * n + x.first + i] : C<T>{0, 0};
+    const int r0 = static_cast<int>(blockIdx.x) * kGatherRows;
+    if (r0 >= x.m) return;
+    const int rows = min(kGatherRows, x.m - r0);
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < rows * kQ; idx += 256) {
+        const int i = r0 + idx / kQ;
+        acc[idx] = i < x.k ? bt[(static_cast<int64_t>(g) * n + x.first + i) * kQ + idx % kQ] : C<T>{0, 0};
+    }
     for (int c = fp.child_ptr[s]; c < fp.child_ptr[s + 1]; ++c) {
-        __syncthreads();
         const int ch = fp.child[c];
         const Sn y = sn(fp, ch);
-        const int cu = y.m - y.k;
-        const C<T>* Wc = W + y.rp * nrhs + static_cast<int64_t>(r) * y.m + y.k;
+        int lo = 0, hi = y.m - y.k;
+        if (x.m > kGatherRows) {
+            const int* bnd = fp.ea_bound + fp.ea_bound_off[ch];
+            lo = bnd[r0 / 64];
+            hi = bnd[min((r0 + kGatherRows) / 64, (x.m + 63) / 64)];
+        }
         const int* rel = fp.rel + y.rp + y.k;
-        for (int i = threadIdx.x; i < cu; i += blockDim.x) Ws[rel[i]] = gadd(Ws[rel[i]], Wc[i]);
+        const C<T>* Wc = W + (static_cast<int64_t>(g) * R + y.rp + y.k) * kQ;
+        __syncthreads();
+        for (int idx = tid; idx < (hi - lo) * kQ; idx += 256) {
+            const int i = lo + idx / kQ, q = idx % kQ;
+            C<T>& a = acc[(rel[i] - r0) * kQ + q];
+            a = gadd(a, Wc[static_cast<int64_t>(i) * kQ + q]);
+        }
     }
+    __syncthreads();
+    C<T>* Ws = W + (static_cast<int64_t>(g) * R + x.rp + r0) * kQ;
+    for (int idx = tid; idx < rows * kQ; idx += 256) Ws[idx] = acc[idx];
 }
 
 // ------------------------------------------------------------------ diagonal-block inverses
@@ -187,249 +269,174 @@ __global__ void __launch_bounds__(256) k_inv_blocks(const int2* __restrict__ pai
 // ------------------------------------------------------------------ forward steps
 //
 // Step B of every supernode with k > B: y_B = X_B w_B (X = L_BB^-1), then the
-// rows below the block take W[i] -= L[i, B:B+nb] y_B. Right-hand sides go in
-// groups of 16.
+// rows below the block take W[i] -= L[i, B:B+nb] y_B.
 
-// The step kernels are small complex GEMMs: each wave owns 16x16 output tiles
-// (16 rows of the block or front x 16 right-hand sides) accumulated on MFMA,
-// with the right-hand-side operand staged in LDS as [row][rhs] (zero-padded).
+// y rows [16 bx, 16 bx + 16) of the block: y = X w (X lower triangular, stored
+// with zeros above the diagonal); wave w takes the columns [64 w, 64 w + 64)
 template <typename T>
-using V4 = typename Mfma<T>::v4;
-
-// y rows [64 bx, 64 bx + 64) of the block: y = X w; wave = 16 rows
-template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_diag(const int* __restrict__ nodes, int B, int nrhs, FrontPlan fp,
+__global__ void __launch_bounds__(256) k_fwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
                                                   const C<T>* __restrict__ X256, const C<T>* __restrict__ W,
-                                                  C<T>* __restrict__ Y, int n) {
-    __shared__ C<T> wl[kSB * 16];  // wl[q*16 + r]
+                                                  C<T>* __restrict__ Y, int n, int64_t R) {
+    __shared__ C<T> part[4][256];
     const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z);
     const Sn x = sn(fp, s);
     if (x.k <= B) return;
     const int nb = min(kSB, x.k - B);
-    const int c0 = static_cast<int>(blockIdx.x) * 64;
+    const int c0 = static_cast<int>(blockIdx.x) * 16;
     if (c0 >= nb) return;
-    const int qmax = min(nb, c0 + 64);  // X[c][q] = 0 for q > c
-    const int K4 = (qmax + 3) & ~3;
+    const int kmax = min(nb, c0 + 16);  // X[c][k] = 0 for k > c
+    const int k0 = 64 * (static_cast<int>(threadIdx.x) / 64);
     const C<T>* Xs = X256 + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
-    const C<T>* Ws = W + x.rp * nrhs;
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const int cw = c0 + 16 * wave;
-    for (int r0 = 0; r0 < nrhs; r0 += 16) {
-        const int nr = min(16, nrhs - r0);
-        __syncthreads();
-        {
-            C<T> t[kSB * 16 / 256];  // all loads in flight before the LDS stores
-#pragma unroll
-            for (int u = 0; u < kSB * 16 / 256; ++u) {
-                const int idx = tid + 256 * u, q = idx / 16, r = idx % 16;
-                t[u] = (q < qmax && r < nr) ? Ws[static_cast<int64_t>(r0 + r) * x.m + B + q] : C<T>{0, 0};
-            }
-#pragma unroll
-            for (int u = 0; u < kSB * 16 / 256; ++u) wl[tid + 256 * u] = t[u];
-        }
-        __syncthreads();
-        if (cw < nb) {
-            V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-            cmma16<T>(accr, acci, K4,
-                      [&](int pp, int k) { return Xs[static_cast<int64_t>(min(k, nb - 1)) * nb + min(cw + pp, nb - 1)]; },
-                      [&](int k, int q) { return wl[k * 16 + q]; });
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = cw + Mfma<T>::row(lane, v), r = lane % 16;
-                if (c < nb && r < nr) Y[static_cast<int64_t>(r0 + r) * n + x.first + B + c] = C<T>{accr[v], acci[v]};
-            }
-        }
-    }
+    const C<T>* Ws = W + (static_cast<int64_t>(g) * R + x.rp + B) * kQ;
+    V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
+    if (k0 < kmax)
+        cmma64<T>(accr, acci,
+                  [&](int p, int k) { return Xs[static_cast<int64_t>(min(k0 + k, nb - 1)) * nb + min(c0 + p, nb - 1)]; },
+                  [&](int k, int q) { return k0 + k < kmax ? Ws[(k0 + k) * kQ + q] : C<T>{0, 0}; });
+    C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
+    reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
+        if (c0 + p < nb) Yg[(c0 + p) * kQ + q] = t;
+    });
 }
 
-// W[i] -= L[i, B:B+nb] y_B for 64 rows per workgroup; wave = 16 rows
+// W[i] -= L[i, B:B+nb] y_B for the 16 rows i0.. below the block; wave w takes the
+// block columns [64 w, 64 w + 64)
 template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ nodes, int B, int nrhs, FrontPlan fp,
+__global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ nodes, int B, FrontPlan fp,
                                                     const C<T>* __restrict__ P, C<T>* W, const C<T>* __restrict__ Y,
-                                                    int n) {
-    __shared__ C<T> yl[kSB * 16];  // yl[c*16 + r]
+                                                    int n, int64_t R) {
+    __shared__ C<T> part[4][256];
     const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z);
     const Sn x = sn(fp, s);
     if (x.k <= B) return;
     const int nb = min(kSB, x.k - B);
-    const int i0 = B + nb + static_cast<int>(blockIdx.x) * 64;
+    const int i0 = B + nb + static_cast<int>(blockIdx.x) * 16;
     if (i0 >= x.m) return;
-    const int K4 = (nb + 3) & ~3;
+    const int k0 = 64 * (static_cast<int>(threadIdx.x) / 64);
     const C<T>* Ps = P + fp.panel_off[s];
-    C<T>* Ws = W + x.rp * nrhs;
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const int iw = i0 + 16 * wave;
-    for (int r0 = 0; r0 < nrhs; r0 += 16) {
-        const int nr = min(16, nrhs - r0);
-        __syncthreads();
-        {
-            C<T> t[kSB * 16 / 256];
-#pragma unroll
-            for (int u = 0; u < kSB * 16 / 256; ++u) {
-                const int idx = tid + 256 * u, c = idx / 16, r = idx % 16;
-                t[u] = (c < nb && r < nr) ? Y[static_cast<int64_t>(r0 + r) * n + x.first + B + c] : C<T>{0, 0};
-            }
-#pragma unroll
-            for (int u = 0; u < kSB * 16 / 256; ++u) yl[tid + 256 * u] = t[u];
+    const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
+    V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
+    if (k0 < nb)
+        cmma64<T>(accr, acci,
+                  [&](int p, int k) {
+                      return Ps[static_cast<int64_t>(B + min(k0 + k, nb - 1)) * x.m + min(i0 + p, x.m - 1)];
+                  },
+                  [&](int k, int q) { return k0 + k < nb ? Yg[(k0 + k) * kQ + q] : C<T>{0, 0}; });
+    C<T>* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
+    reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
+        if (i0 + p < x.m) {
+            C<T>& dst = Wg[static_cast<int64_t>(i0 + p) * kQ + q];
+            dst = gsub(dst, t);
         }
-        __syncthreads();
-        if (iw < x.m) {
-            V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-            cmma16<T>(accr, acci, K4,
-                      [&](int pp, int k) {
-                          return Ps[static_cast<int64_t>(B + min(k, nb - 1)) * x.m + min(iw + pp, x.m - 1)];
-                      },
-                      [&](int k, int q) { return yl[k * 16 + q]; });
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = iw + Mfma<T>::row(lane, v), r = lane % 16;
-                if (i < x.m && r < nr) {
-                    C<T>& dst = Ws[static_cast<int64_t>(r0 + r) * x.m + i];
-                    dst = C<T>{dst.re - accr[v], dst.im - acci[v]};
-                }
-            }
-        }
-    }
+    });
 }
 
 // ------------------------------------------------------------------ backward steps
 //
 // Step B (blocks in decreasing order): t_B = L[rows below, B:B+nb]^T x[rows below]
-// in 64-row partial sums, reduced in chunk order, then x_B = X_B^T (D_B^-1 y_B - t_B).
+// in partial sums over kBwdRows rows, reduced in chunk order, then
+// x_B = X_B^T (D_B^-1 y_B - t_B).
 
-// partial[(slot*stride + bx)][r][c] over 64 rows; wave = 16-column tiles w, w+4, ...
+// partial[((f * stride + chunk) * G + g)][c][q] for the 16 block columns c0.. over
+// the rows [i0, i0 + kBwdRows) below the block; wave w takes 64 of the rows
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nodes, int B, int nrhs, int stride,
+__global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nodes, int B, int stride, int ctiles,
                                                      FrontPlan fp, const C<T>* __restrict__ P,
                                                      const C<T>* __restrict__ X, int n, C<T>* __restrict__ partial) {
-    __shared__ C<T> xt[64 * 16];  // xt[row*16 + r]
+    __shared__ C<T> part[4][256];
     const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
     if (x.k <= B) return;
     const int nb = min(kSB, x.k - B);
-    const int i0 = B + nb + static_cast<int>(blockIdx.x) * 64;
-    if (i0 >= x.m) return;
-    const int nrows = min(64, x.m - i0);
+    const int chunk = static_cast<int>(blockIdx.x) / ctiles, c0 = (static_cast<int>(blockIdx.x) % ctiles) * 16;
+    const int i0 = B + nb + chunk * kBwdRows;
+    if (i0 >= x.m || c0 >= nb) return;
+    const int iw = i0 + 64 * (static_cast<int>(threadIdx.x) / 64);
     const C<T>* Ps = P + fp.panel_off[s];
     const int* rows = fp.rows + x.rp;
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    C<T>* out = partial + (static_cast<int64_t>(blockIdx.y) * stride + blockIdx.x) * kSB * nrhs;
-    for (int r0 = 0; r0 < nrhs; r0 += 16) {
-        const int nr = min(16, nrhs - r0);
-        __syncthreads();
-        {
-            int g[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int row = (tid + 256 * u) / 16;
-                g[u] = row < nrows ? rows[i0 + row] : 0;
-            }
-            C<T> t[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int idx = tid + 256 * u, row = idx / 16, r = idx % 16;
-                t[u] = (row < nrows && r < nr) ? X[static_cast<int64_t>(r0 + r) * n + g[u]] : C<T>{0, 0};
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) xt[tid + 256 * u] = t[u];
-        }
-        __syncthreads();
-        for (int ct = 16 * wave; ct < nb; ct += 64) {
-            V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-            cmma16<T>(accr, acci, 64,
-                      [&](int pp, int k) {
-                          return Ps[static_cast<int64_t>(B + min(ct + pp, nb - 1)) * x.m + min(i0 + k, x.m - 1)];
-                      },
-                      [&](int k, int q) { return xt[k * 16 + q]; });
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = ct + Mfma<T>::row(lane, v), r = lane % 16;
-                if (c < nb && r < nr) out[static_cast<int64_t>(r0 + r) * kSB + c] = C<T>{accr[v], acci[v]};
-            }
-        }
-    }
+    const C<T>* Xg = X + static_cast<int64_t>(g) * n * kQ;
+    V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
+    if (iw < x.m)
+        cmma64<T>(accr, acci,
+                  [&](int p, int k) {
+                      return Ps[static_cast<int64_t>(B + min(c0 + p, nb - 1)) * x.m + min(iw + k, x.m - 1)];
+                  },
+                  [&](int k, int q) {
+                      return iw + k < x.m ? Xg[static_cast<int64_t>(rows[iw + k]) * kQ + q] : C<T>{0, 0};
+                  });
+    C<T>* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
+    reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
+        if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;
+    });
 }
 
-// tsum[slot][r][c] = sum over chunks of partial, for 16 columns and one RHS per
-// workgroup: 16 summers per output, combined in a fixed order
+// tsum[(f * G + g)][c][q] = sum over chunks of partial, one block column c per
+// workgroup: 16 summers per right-hand side, combined in a fixed order
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_reduce(const int* __restrict__ nodes, int B, int nrhs, int stride,
-                                                    int cblocks, FrontPlan fp, const C<T>* __restrict__ partial,
-                                                    C<T>* __restrict__ tsum) {
+__global__ void __launch_bounds__(256) k_bwd_reduce(const int* __restrict__ nodes, int B, int stride, FrontPlan fp,
+                                                    const C<T>* __restrict__ partial, C<T>* __restrict__ tsum) {
     __shared__ C<T> red[16][17];
     const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
     if (x.k <= B) return;
     const int nb = min(kSB, x.k - B);
-    const int nch = x.m > B + nb ? (x.m - B - nb + 63) / 64 : 0;
-    const int cblk = static_cast<int>(blockIdx.x) % cblocks, r = static_cast<int>(blockIdx.x) / cblocks;
-    if (cblk * 16 >= nb || r >= nrhs) return;
-    const int tid = threadIdx.x, o = tid % 16, p = tid / 16;
-    const int c = cblk * 16 + o;
-    const C<T>* pp = partial + static_cast<int64_t>(blockIdx.y) * stride * kSB * nrhs + static_cast<int64_t>(r) * kSB + c;
+    const int nch = x.m > B + nb ? (x.m - B - nb + kBwdRows - 1) / kBwdRows : 0;
+    const int c = static_cast<int>(blockIdx.x);
+    if (c >= nb) return;
+    const int tid = threadIdx.x, q = tid % 16, p = tid / 16;
+    const int64_t cstride = static_cast<int64_t>(G) * kSB * kQ;
+    const C<T>* pp = partial + (static_cast<int64_t>(blockIdx.y) * stride * G + g) * kSB * kQ + c * kQ + q;
     C<T> t = C<T>{0, 0};
 #pragma unroll 8
-    for (int ch = p; ch < nch; ch += 16) t = gadd(t, pp[static_cast<int64_t>(ch) * kSB * nrhs]);
-    red[p][o] = t;
+    for (int ch = p; ch < nch; ch += 16) t = gadd(t, pp[ch * cstride]);
+    red[p][q] = t;
     __syncthreads();
-    if (p == 0 && c < nb) {
-        C<T> v = red[0][o];
-        for (int q = 1; q < 16; ++q) v = gadd(v, red[q][o]);
-        tsum[(static_cast<int64_t>(blockIdx.y) * nrhs + r) * kSB + c] = v;
+    if (p == 0) {
+        C<T> v = red[0][q];
+        for (int j = 1; j < 16; ++j) v = gadd(v, red[j][q]);
+        tsum[((static_cast<int64_t>(blockIdx.y) * G + g) * kSB + c) * kQ + q] = v;
     }
 }
 
-// x rows [64 bx, 64 bx + 64) of the block: x[c] = sum_{q >= c} X[q][c] z[q],
-// z = y D^-1 - t; wave = 16 rows
+// x rows [16 bx, 16 bx + 16) of the block: x[c] = sum_{k >= c} X[k][c] z[k],
+// z = y D^-1 - t; wave w takes k in [c0 + 64 w, c0 + 64 w + 64)
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes, int B, int nrhs, FrontPlan fp,
+__global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
                                                   const C<T>* __restrict__ P, const C<T>* __restrict__ XT256,
                                                   const C<T>* __restrict__ Y, const C<T>* __restrict__ tsum,
                                                   C<T>* __restrict__ X, int n) {
-    __shared__ C<T> zl[kSB * 16];  // zl[q*16 + r], q counted from c0
+    __shared__ C<T> part[4][256];
     const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
     if (x.k <= B) return;
     const int nb = min(kSB, x.k - B);
-    const int c0 = static_cast<int>(blockIdx.x) * 64;
+    const int c0 = static_cast<int>(blockIdx.x) * 16;
     if (c0 >= nb) return;
-    const int K4 = (nb - c0 + 3) & ~3;
+    const int k0 = c0 + 64 * (static_cast<int>(threadIdx.x) / 64);
     const bool below = x.m > B + nb;
     const C<T>* Ps = P + fp.panel_off[s];
     const C<T>* XTs = XT256 + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const int cw = c0 + 16 * wave;
-    for (int r0 = 0; r0 < nrhs; r0 += 16) {
-        const int nr = min(16, nrhs - r0);
-        __syncthreads();
-        {
-            C<T> yv[kSB * 16 / 256], dv[kSB * 16 / 256], tv[kSB * 16 / 256];
-#pragma unroll
-            for (int u = 0; u < kSB * 16 / 256; ++u) {
-                const int idx = tid + 256 * u, q = c0 + idx / 16, r = idx % 16;
-                const bool ok = q < nb && r < nr;
-                yv[u] = ok ? Y[static_cast<int64_t>(r0 + r) * n + x.first + B + q] : C<T>{0, 0};
-                dv[u] = ok ? Ps[static_cast<int64_t>(B + q) * x.m + B + q] : C<T>{1, 0};
-                tv[u] = (ok && below) ? tsum[(static_cast<int64_t>(blockIdx.y) * nrhs + r0 + r) * kSB + q] : C<T>{0, 0};
-            }
-#pragma unroll
-            for (int u = 0; u < kSB * 16 / 256; ++u) zl[tid + 256 * u] = gsub(gmul(yv[u], grecip(dv[u])), tv[u]);
-        }
-        __syncthreads();
-        if (cw < nb) {
-            V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-            cmma16<T>(accr, acci, K4,
-                      [&](int pp, int k) {
-                          return XTs[static_cast<int64_t>(min(c0 + k, nb - 1)) * nb + min(cw + pp, nb - 1)];
-                      },
-                      [&](int k, int q) { return zl[k * 16 + q]; });
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = cw + Mfma<T>::row(lane, v), r = lane % 16;
-                if (c < nb && r < nr) X[static_cast<int64_t>(r0 + r) * n + x.first + B + c] = C<T>{accr[v], acci[v]};
-            }
-        }
-    }
+    const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
+    const C<T>* tg = tsum + (static_cast<int64_t>(blockIdx.y) * G + g) * kSB * kQ;
+    V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
+    if (k0 < nb)
+        cmma64<T>(accr, acci,
+                  [&](int p, int k) { return XTs[static_cast<int64_t>(min(k0 + k, nb - 1)) * nb + min(c0 + p, nb - 1)]; },
+                  [&](int k, int q) {
+                      const int kk = k0 + k;
+                      if (kk >= nb) return C<T>{0, 0};
+                      const C<T> z = gmul(Yg[kk * kQ + q], grecip(Ps[static_cast<int64_t>(B + kk) * x.m + B + kk]));
+                      return below ? gsub(z, tg[kk * kQ + q]) : z;
+                  });
+    C<T>* Xg = X + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
+    reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
+        if (c0 + p < nb) Xg[(c0 + p) * kQ + q] = t;
+    });
 }
 
 // colscale[r] = 1 (mode 0) or 1/||b_r|| (mode 1, 1 for a zero column)
@@ -448,26 +455,30 @@ __global__ void k_colscale_final(int nrhs, int nparts, const double* __restrict_
 
 // ------------------------------------------------------------------ launchers
 
+int rhs_groups(int nrhs) { return (nrhs + kQ - 1) / kQ; }
+
 template <typename T>
 void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
                     const double* colscale, void* out) {
-    hipLaunchKernelGGL(k_perm_in<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, n, nrhs, perm, d, b, ldb, colscale,
-                       static_cast<C<T>*>(out));
+    hipLaunchKernelGGL(k_perm_in<T>, dim3((n + 15) / 16, rhs_groups(nrhs)), dim3(256), 0, st, n, nrhs, perm, d, b, ldb,
+                       colscale, static_cast<C<T>*>(out));
 }
 
 template <typename T>
 void launch_perm_out(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const void* xt,
                      const double* colscale, cd* x, int64_t ldx, int accumulate) {
-    hipLaunchKernelGGL(k_perm_out<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, n, nrhs, perm, d,
+    hipLaunchKernelGGL(k_perm_out<T>, dim3((n + 15) / 16, rhs_groups(nrhs)), dim3(256), 0, st, n, nrhs, perm, d,
                        static_cast<const C<T>*>(xt), colscale, x, ldx, accumulate);
 }
 
 template <typename T>
-void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int nrhs, const FrontPlan& fp, const void* b, int n,
-                       void* W) {
-    if (count <= 0) return;
-    hipLaunchKernelGGL(k_fwd_gather<T>, dim3(count, nrhs), dim3(256), 0, st, nodes, nrhs, fp,
-                       static_cast<const C<T>*>(b), n, static_cast<C<T>*>(W));
+void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int max_m, int nrhs, const FrontPlan& fp,
+                       const void* b, int n, int64_t R, void* W) {
+    if (count <= 0 || max_m <= 0) return;
+    const size_t lds = static_cast<size_t>(kGatherRows) * kQ * sizeof(C<T>);
+    allow_large_lds(k_fwd_gather<T>, lds);
+    hipLaunchKernelGGL(k_fwd_gather<T>, dim3((max_m + kGatherRows - 1) / kGatherRows, count, rhs_groups(nrhs)),
+                       dim3(256), lds, st, nodes, fp, static_cast<const C<T>*>(b), n, R, static_cast<C<T>*>(W));
 }
 
 template <typename T>
@@ -482,30 +493,32 @@ void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const Fron
 }
 
 template <typename T>
-void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_row_blocks, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n) {
+void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
+                     const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R) {
     if (count <= 0 || max_nb <= 0) return;
-    hipLaunchKernelGGL(k_fwd_diag<T>, dim3((max_nb + 63) / 64, count), dim3(256), 0, st, nodes, B, nrhs, fp,
-                       static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n);
-    if (max_row_blocks > 0) {
-        hipLaunchKernelGGL(k_fwd_update<T>, dim3(max_row_blocks, count), dim3(256), 0, st, nodes, B, nrhs, fp,
-                           static_cast<const C<T>*>(P), static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n);
-    }
+    const int G = rhs_groups(nrhs);
+    hipLaunchKernelGGL(k_fwd_diag<T>, dim3((max_nb + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
+                       static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
+    if (max_rows_below > 0)
+        hipLaunchKernelGGL(k_fwd_update<T>, dim3((max_rows_below + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
+                           static_cast<const C<T>*>(P), static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n, R);
 }
 
 template <typename T>
-void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_row_blocks, int nrhs,
+void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
                      const FrontPlan& fp, const void* P, const void* XT256, const void* Y, void* X, int n,
                      void* partial, void* tsum) {
     if (count <= 0 || max_nb <= 0) return;
-    const int stride = max_row_blocks > 0 ? max_row_blocks : 1;
-    if (max_row_blocks > 0) {
-        hipLaunchKernelGGL(k_bwd_partial<T>, dim3(max_row_blocks, count), dim3(256), 0, st, nodes, B, nrhs, stride,
+    const int G = rhs_groups(nrhs);
+    const int ctiles = (max_nb + 15) / 16;
+    const int stride = std::max(1, (max_rows_below + kBwdRows - 1) / kBwdRows);
+    if (max_rows_below > 0) {
+        hipLaunchKernelGGL(k_bwd_partial<T>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride, ctiles,
                            fp, static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(partial));
-        hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(((max_nb + 15) / 16) * nrhs, count), dim3(256), 0, st, nodes, B, nrhs,
-                           stride, (max_nb + 15) / 16, fp, static_cast<const C<T>*>(partial), static_cast<C<T>*>(tsum));
+        hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(max_nb, count, G), dim3(256), 0, st, nodes, B, stride, fp,
+                           static_cast<const C<T>*>(partial), static_cast<C<T>*>(tsum));
     }
-    hipLaunchKernelGGL(k_bwd_diag<T>, dim3((max_nb + 63) / 64, count), dim3(256), 0, st, nodes, B, nrhs, fp,
+    hipLaunchKernelGGL(k_bwd_diag<T>, dim3(ctiles, count, G), dim3(256), 0, st, nodes, B, fp,
                        static_cast<const C<T>*>(P), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
                        static_cast<const C<T>*>(tsum), static_cast<C<T>*>(X), n);
 }
@@ -522,10 +535,10 @@ template void launch_perm_in<double>(hipStream_t, int, int, const int*, const do
 template void launch_perm_in<float>(hipStream_t, int, int, const int*, const double*, const cd*, int64_t, const double*, void*);
 template void launch_perm_out<double>(hipStream_t, int, int, const int*, const double*, const void*, const double*, cd*, int64_t, int);
 template void launch_perm_out<float>(hipStream_t, int, int, const int*, const double*, const void*, const double*, cd*, int64_t, int);
-template void launch_fwd_gather<double>(hipStream_t, const int*, int, int, const FrontPlan&, const void*, int, void*);
-template void launch_fwd_gather<float>(hipStream_t, const int*, int, int, const FrontPlan&, const void*, int, void*);
-template void launch_fwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int);
-template void launch_fwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int);
+template void launch_fwd_gather<double>(hipStream_t, const int*, int, int, int, const FrontPlan&, const void*, int, int64_t, void*);
+template void launch_fwd_gather<float>(hipStream_t, const int*, int, int, int, const FrontPlan&, const void*, int, int64_t, void*);
+template void launch_fwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int, int64_t);
+template void launch_fwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int, int64_t);
 template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, void*, int, void*, void*);
 template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, void*, int, void*, void*);
 template void launch_inv_blocks<double>(hipStream_t, const int2*, int, const FrontPlan&, const void*, const void*, void*, void*);
