@@ -118,7 +118,7 @@ struct aphi_mesh {
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
     DevBuf P, CB, mws, Xinv, X256, XT256;
-    DevBuf bt, W, X, Yv, bpart, tsum, colscale, res, norms;
+    DevBuf bt, W, X, Yv, bpart, tsum, colscale, res, norms, xil;
     DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
     int solve_nrhs_cap = 0;
     bool assembled = false, equilibrated = false, factored = false;
@@ -363,6 +363,7 @@ void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     m->X.reserve(static_cast<size_t>(m->n) * q * cs);
     m->Yv.reserve(static_cast<size_t>(m->n) * q * cs);
     m->W.reserve(static_cast<size_t>(m->rows_total) * q * cs);
+    m->xil.reserve(static_cast<size_t>(m->n) * q * sizeof(cd));
     m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kSB * q * cs);
     m->tsum.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kSB * q * cs);
     m->colscale.reserve(nrhs * sizeof(double));
@@ -425,6 +426,16 @@ void solve_any(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t 
     else solve_t<float>(m, nrhs, b, ldb, x, ldx, accumulate);
 }
 
+// y = A x (original A, caller layouts), through the interleaved copy of x
+void symv(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, cd* y, int64_t ldy) {
+    hipStream_t s = m->st();
+    const int n = m->n;
+    const dim3 grid((n + 15) / 16, aphi_dev::rhs_groups(nrhs));
+    hipLaunchKernelGGL(aphi_dev::k_interleave16, grid, dim3(256), 0, s, n, nrhs, x, ldx, m->xil.as<cd>());
+    hipLaunchKernelGGL(aphi_dev::k_symv16, grid, dim3(256), 0, s, n, nrhs, m->row_ptr.as<int>(), m->col.as<int>(),
+                       m->ttptr.as<int>(), m->tslot.as<int>(), m->trow.as<int>(), m->A.as<cd>(), m->xil.as<cd>(), y, ldy);
+}
+
 // backward error and relative residual per RHS, and (when bes is given) the
 // backward error of the equilibrated system; leaves r = b - A x in m->res
 void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int64_t ldb, double* be, double* rr,
@@ -433,9 +444,7 @@ void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int6
     const int n = m->n;
     ensure_solve_buffers(m, nrhs);
     cd* res = m->res.as<cd>();
-    hipLaunchKernelGGL(aphi_dev::k_symv, dim3(blocks_for(n), nrhs), dim3(256), 0, s, n, nrhs, m->row_ptr.as<int>(),
-                       m->col.as<int>(), m->ttptr.as<int>(), m->tslot.as<int>(), m->trow.as<int>(), m->A.as<cd>(), x, ldx,
-                       res, static_cast<int64_t>(n));
+    symv(m, nrhs, x, ldx, res, static_cast<int64_t>(n));
     hipLaunchKernelGGL(aphi_dev::k_residual, dim3(blocks_for(n), nrhs), dim3(256), 0, s, n, nrhs, b, ldb, res,
                        static_cast<int64_t>(n));
     const int parts = 64;
@@ -566,6 +575,11 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
                            static_cast<int64_t>(n), V, static_cast<int64_t>(n));
         std::vector<double> beta(static_cast<size_t>(nrhs));
         norms(V, beta);
+        double start = 0.0;  // estimated worst scaled backward error at this restart
+        for (int r = 0; r < nrhs; ++r) {
+            const double dr = den[static_cast<size_t>(r)];
+            start = std::max(start, dr > 0.0 ? beta[static_cast<size_t>(r)] / dr : beta[static_cast<size_t>(r)]);
+        }
         for (int r = 0; r < nrhs; ++r) inv[static_cast<size_t>(r)] = beta[static_cast<size_t>(r)] > 0.0 ? 1.0 / beta[static_cast<size_t>(r)] : 0.0;
         scale_cols(V, inv, V);
         std::fill(H.begin(), H.end(), Complex(0.0, 0.0));
@@ -579,9 +593,7 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
                                static_cast<int64_t>(n));
             solve_any(m, nrhs, u, n, Zj, n, 0);
             ++solves;
-            hipLaunchKernelGGL(aphi_dev::k_symv, grid, dim3(256), 0, s, n, nrhs, m->row_ptr.as<int>(), m->col.as<int>(),
-                               m->ttptr.as<int>(), m->tslot.as<int>(), m->trow.as<int>(), m->A.as<cd>(), Zj,
-                               static_cast<int64_t>(n), w, static_cast<int64_t>(n));
+            symv(m, nrhs, Zj, static_cast<int64_t>(n), w, static_cast<int64_t>(n));
             hipLaunchKernelGGL(aphi_dev::k_scale_d, grid, dim3(256), 0, s, n, nrhs, d, 1, w, static_cast<int64_t>(n), w,
                                static_cast<int64_t>(n));
             for (int pass = 0; pass < 2; ++pass) {
@@ -646,6 +658,16 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
             k = j + 1;
             if (trace_refinement()) std::fprintf(stderr, "  gmres %2d: estimated scaled backward error %.3e\n", solves, ratio);
             if (!(ratio > 0.5 * tol)) break;
+            // Stalled: after five steps of this cycle, the mean reduction per step
+            // projects past 1.5 times the budget. Stop here (the caller refactors in
+            // FP64) rather than spend the rest of the budget.
+            if (j >= 4 && start > 0.0 && ratio > 0.0) {
+                const double rate = std::pow(ratio / start, 1.0 / (j + 1));
+                if (!(rate < 1.0) || solves + std::log(0.5 * tol / ratio) / std::log(rate) > 1.5 * max_solves) {
+                    if (trace_refinement()) std::fprintf(stderr, "  gmres stalled (rate %.3f per step)\n", rate);
+                    return solves;
+                }
+            }
         }
         // y = R^-1 g per right-hand side, then x += Z y
         for (int r = 0; r < nrhs; ++r) {
@@ -956,7 +978,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
                            &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
-                           &m->bpart, &m->colscale, &m->res, &m->norms};
+                           &m->bpart, &m->colscale, &m->res, &m->norms, &m->xil};
     int64_t total = 0;
     for (const DevBuf* b : all) total += static_cast<int64_t>(b->bytes);
     return total;

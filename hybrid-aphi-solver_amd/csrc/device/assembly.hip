@@ -334,20 +334,38 @@ __global__ void k_eq_apply(int64_t nnz, const int* __restrict__ row_of, const in
 
 // ------------------------------------------------------------------ operator
 
+// x (column r at x + r * ldx) -> groups of 16 right-hand sides interleaved,
+// xi[(g * n + i) * 16 + q] = x column 16 g + q (zero past nrhs); one thread per
+// (row, right-hand side)
+__global__ void __launch_bounds__(256) k_interleave16(int n, int nrhs, const cd* __restrict__ x, int64_t ldx,
+                                                      cd* __restrict__ xi) {
+    const int i = static_cast<int>(blockIdx.x) * 16 + static_cast<int>(threadIdx.x) / 16;
+    const int q = static_cast<int>(threadIdx.x) % 16, g = static_cast<int>(blockIdx.y);
+    const int r = 16 * g + q;
+    if (i >= n) return;
+    xi[(static_cast<int64_t>(g) * n + i) * 16 + q] = r < nrhs ? x[r * ldx + i] : cd{0.0, 0.0};
+}
+
 // y = A x for the symmetric upper-stored A (include/aphi_solver/sparse_symmetric.hpp:116-131),
 // accumulated per row in the reference's order: the mirrored entries (r, i), r < i
-// ascending, then row i's own entries. Multi-RHS, column-major with leading dim ld.
-__global__ void k_symv(int n, int nrhs, const int* __restrict__ row_ptr, const int* __restrict__ col,
-                       const int* __restrict__ tptr, const int* __restrict__ tslot, const int* __restrict__ trow,
-                       const cd* __restrict__ a, const cd* __restrict__ x, int64_t ldx, cd* __restrict__ y, int64_t ldy) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
-    if (i >= n || r >= nrhs) return;
-    const cd* xr = x + r * ldx;
+// ascending, then row i's own entries. x interleaved by k_interleave16, y
+// column-major with leading dim ldy. Sixteen consecutive threads take one row
+// and the sixteen right-hand sides of a group: each matrix entry is read once
+// per group and the x values it multiplies are one contiguous 256-byte segment.
+__global__ void __launch_bounds__(256) k_symv16(int n, int nrhs, const int* __restrict__ row_ptr,
+                                                const int* __restrict__ col, const int* __restrict__ tptr,
+                                                const int* __restrict__ tslot, const int* __restrict__ trow,
+                                                const cd* __restrict__ a, const cd* __restrict__ xi,
+                                                cd* __restrict__ y, int64_t ldy) {
+    const int i = static_cast<int>(blockIdx.x) * 16 + static_cast<int>(threadIdx.x) / 16;
+    const int q = static_cast<int>(threadIdx.x) % 16, g = static_cast<int>(blockIdx.y);
+    const int r = 16 * g + q;
+    if (i >= n) return;
+    const cd* xg = xi + static_cast<int64_t>(g) * n * 16 + q;
     cd acc = {0.0, 0.0};
-    for (int t = tptr[i]; t < tptr[i + 1]; ++t) acc = cadd(acc, cmul(a[tslot[t]], xr[trow[t]]));
-    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) acc = cadd(acc, cmul(a[k], xr[col[k]]));
-    y[r * ldy + i] = acc;
+    for (int t = tptr[i]; t < tptr[i + 1]; ++t) acc = cadd(acc, cmul(a[tslot[t]], xg[static_cast<int64_t>(trow[t]) * 16]));
+    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) acc = cadd(acc, cmul(a[k], xg[static_cast<int64_t>(col[k]) * 16]));
+    if (r < nrhs) y[r * ldy + i] = acc;
 }
 
 // r = b - A x, in place into r (which holds A x on entry)

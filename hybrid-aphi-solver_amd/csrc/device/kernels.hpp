@@ -68,8 +68,9 @@ __global__ void k_eq_rowmax(int64_t nnz, const int* row_of, const int* col, cons
 __global__ void k_eq_factor(int n, double* rowmax, double* factor, double* d);
 __global__ void k_eq_scale(int64_t nnz, const int* row_of, const int* col, const double* factor, cd* w);
 __global__ void k_eq_apply(int64_t nnz, const int* row_of, const int* col, const double* d, const cd* a, cd* out);
-__global__ void k_symv(int n, int nrhs, const int* row_ptr, const int* col, const int* tptr, const int* tslot,
-                       const int* trow, const cd* a, const cd* x, int64_t ldx, cd* y, int64_t ldy);
+__global__ void k_interleave16(int n, int nrhs, const cd* x, int64_t ldx, cd* xi);
+__global__ void k_symv16(int n, int nrhs, const int* row_ptr, const int* col, const int* tptr, const int* tslot,
+                         const int* trow, const cd* a, const cd* xi, cd* y, int64_t ldy);
 __global__ void k_residual(int n, int nrhs, const cd* b, int64_t ldb, cd* r, int64_t ldr);
 __global__ void k_sumsq_partial(int n, const cd* v, int64_t ldv, double* partials);
 __global__ void k_absmax_partial(int64_t n, const cd* v, double* partials);
