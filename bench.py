@@ -34,6 +34,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def update_traffic(n):
+    """HBM bytes per k_update<float> launch on this workload, from the committed
+    FETCH_SIZE / WRITE_SIZE counter runs (profiles/k_update_f32_traffic.json),
+    each scaled by the same counter's reading for a copy of known size; None for
+    other mesh sizes or without the file."""
+    path = os.path.join(ROOT, "profiles", "k_update_f32_traffic.json")
+    if n != 32 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -45,6 +57,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling (0 skips)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (HBM counter calibration) and exit")
+    ap.add_argument("--factor-only", choices=("fp32", "fp64"), default=None,
+                    help="only refill, equilibrate and factor the workload once at one frequency (counter runs) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep's frequency points are independent and would shard across GPUs "
@@ -73,6 +87,15 @@ def main():
     ctx = A.Context(0)
     mesh = A.Mesh(ctx, prob)
     freqs = A.frequency_sweep(1.0, 1e8, 64)
+    if a.factor_only:
+        prec = A.APHI_FP32 if a.factor_only == "fp32" else A.APHI_FP64
+        mesh.refill(2 * np.pi * freqs[37])
+        mesh.equilibrate(10)
+        st = mesh.factorize_ldlt(prec)
+        ctx.synchronize()
+        print(json.dumps({"factor_only": a.factor_only, "frequency": float(freqs[37]),
+                          "milliseconds": st["milliseconds"], "ok": bool(st["ok"])}))
+        return
     n = prob.n
     gen = torch.Generator(device="cuda").manual_seed(20261203)
     B = torch.complex(torch.randn(len(freqs), a.nrhs, n, generator=gen, device="cuda", dtype=torch.float64),
@@ -220,7 +243,7 @@ def main():
         "solve_ms_fp32": solve32_ms,
         "solve_ms_fp64": solve64_ms,
         "roofline": {"bound": "mfma", "achieved": ach32, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach32 / MFMA_FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": ach32 / MFMA_FP32_PEAK_TFLOPS, "traffic": update_traffic(a.n),
                      "kernel": "k_update<float> (MFMA trailing update)",
                      "launches": int(st32["update_launches"]),
                      "avg_launch_ms": st32["update_ms"] / max(1, st32["update_launches"]),

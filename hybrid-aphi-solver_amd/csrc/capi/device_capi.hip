@@ -106,7 +106,7 @@ struct aphi_mesh {
     std::vector<Complex> value_tab;
     std::vector<Level> levels;
     int max_level_count = 0;
-    int64_t max_bwd_partials = 0;  // count * kBwdRows-row chunks over levels, per kSB * padded nrhs
+    int64_t max_bwd_partials = 0;  // count * backward row chunks (aphi_dev::bwd_chunks) over levels, per kSB * padded nrhs
     // plan on the device
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
@@ -117,7 +117,7 @@ struct aphi_mesh {
     int64_t xinv_entries = 0;
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
-    DevBuf P, CB, mws, Xinv, X256, XT256;
+    DevBuf P, CB, mws, Xinv, X256, XT256, dinv;
     DevBuf bt, W, X, Yv, bpart, tsum, colscale, res, norms, xil;
     DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
     int solve_nrhs_cap = 0;
@@ -225,6 +225,7 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     m->CB.reserve(static_cast<size_t>(std::max<int64_t>(m->cb_entries, 1)) * cs);
     m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
     m->Xinv.reserve(static_cast<size_t>(std::max<int64_t>(m->xinv_entries, 1)) * cs);
+    m->dinv.reserve(static_cast<size_t>(m->n) * cs);
     m->X256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
     m->XT256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
     m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
@@ -323,6 +324,7 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     }
     aphi_dev::launch_inv_blocks<T>(s, m->inv_pairs.as<int2>(), m->num_inv_pairs, fp, m->P.p, m->Xinv.p, m->X256.p,
                                    m->XT256.p);
+    aphi_dev::launch_dinv<T>(s, m->num_super, fp, m->P.p, m->dinv.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(m->ctx->ev1, s));
     aphi_dev::FactorStatsDev h{};
@@ -412,7 +414,7 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         if (maxk == 0) continue;
         for (int B = ((maxk - 1) / SB) * SB; B >= 0; B -= SB)
             aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), rows_below(L, B), nrhs, fp,
-                                         m->P.p, m->XT256.p, m->Yv.p, m->X.p, n, m->bpart.p, m->tsum.p);
+                                         m->P.p, m->dinv.p, m->XT256.p, m->Yv.p, m->X.p, n, m->bpart.p, m->tsum.p);
     }
     aphi_dev::launch_perm_out<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), m->X.p, m->colscale.as<double>(), x, ldx,
                                  accumulate);
@@ -882,7 +884,7 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                         ea.push_back(blk);
                         blk += (mm + 63) / 64;
                     }
-                    maxrb = std::max(maxrb, (mm - std::min(kk, aphi_dev::kSB) + aphi_dev::kBwdRows - 1) / aphi_dev::kBwdRows);
+                    maxrb = std::max(maxrb, aphi_dev::bwd_chunks(mm - std::min(kk, aphi_dev::kSB)));
                 }
             lv.ea_blocks = blk;
             ea.push_back(blk);
@@ -978,7 +980,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
                            &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
-                           &m->bpart, &m->colscale, &m->res, &m->norms, &m->xil};
+                           &m->bpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv};
     int64_t total = 0;
     for (const DevBuf* b : all) total += static_cast<int64_t>(b->bytes);
     return total;

@@ -44,6 +44,7 @@ void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int m
 // solve.hip: vectors in groups of kQ right-hand sides, entry (i, q) of group g at
 // [(g * rows + i) * kQ + q] (rows = n, or the total front row count for W)
 int rhs_groups(int nrhs);
+int bwd_chunks(int rows_below);  // row chunks of a backward step's partial sums
 template <typename T>
 void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
                     const double* colscale, void* out);
@@ -62,12 +63,14 @@ void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const Fron
 template <typename T>
 void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
                      const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R);
-// partial: count * ceil(max_rows_below / kBwdRows) * kSB * kQ * rhs_groups(nrhs) entries;
+// partial: count * bwd_chunks(max_rows_below) * kSB * kQ * rhs_groups(nrhs) entries;
 // tsum: count * kSB * kQ * rhs_groups(nrhs)
 template <typename T>
 void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* XT256, const void* Y, void* X, int n,
-                     void* partial, void* tsum);
+                     const FrontPlan& fp, const void* P, const void* dinv, const void* XT256, const void* Y, void* X,
+                     int n, void* partial, void* tsum);
+// dinv[j] = 1 / D[j] for every column (n entries), after a factorization
+template <typename T> void launch_dinv(hipStream_t st, int ns, const FrontPlan& fp, const void* P, void* dinv);
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
                          int mode);
 

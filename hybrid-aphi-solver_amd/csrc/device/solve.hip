@@ -266,6 +266,15 @@ __global__ void __launch_bounds__(256) k_inv_blocks(const int2* __restrict__ pai
     }
 }
 
+// dinv[first(s) + j] = 1 / D[j] of every supernode s (one workgroup per supernode)
+template <typename T>
+__global__ void __launch_bounds__(64) k_dinv(FrontPlan fp, const C<T>* __restrict__ P, C<T>* __restrict__ dinv) {
+    const int s = static_cast<int>(blockIdx.x);
+    const Sn x = sn(fp, s);
+    const C<T>* Ps = P + fp.panel_off[s];
+    for (int j = threadIdx.x; j < x.k; j += 64) dinv[x.first + j] = grecip(Ps[static_cast<int64_t>(j) * x.m + j]);
+}
+
 // ------------------------------------------------------------------ forward steps
 //
 // Step B of every supernode with k > B: y_B = X_B w_B (X = L_BB^-1), then the
@@ -403,10 +412,10 @@ __global__ void __launch_bounds__(256) k_bwd_reduce(const int* __restrict__ node
 }
 
 // x rows [16 bx, 16 bx + 16) of the block: x[c] = sum_{k >= c} X[k][c] z[k],
-// z = y D^-1 - t; wave w takes k in [c0 + 64 w, c0 + 64 w + 64)
+// z = y D^-1 - t (D^-1 from k_dinv); wave w takes k in [c0 + 64 w, c0 + 64 w + 64)
 template <typename T>
 __global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
-                                                  const C<T>* __restrict__ P, const C<T>* __restrict__ XT256,
+                                                  const C<T>* __restrict__ dinv, const C<T>* __restrict__ XT256,
                                                   const C<T>* __restrict__ Y, const C<T>* __restrict__ tsum,
                                                   C<T>* __restrict__ X, int n) {
     __shared__ C<T> part[4][256];
@@ -419,7 +428,7 @@ __global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes,
     if (c0 >= nb) return;
     const int k0 = c0 + 64 * (static_cast<int>(threadIdx.x) / 64);
     const bool below = x.m > B + nb;
-    const C<T>* Ps = P + fp.panel_off[s];
+    const C<T>* dv = dinv + x.first + B;
     const C<T>* XTs = XT256 + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
     const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
     const C<T>* tg = tsum + (static_cast<int64_t>(blockIdx.y) * G + g) * kSB * kQ;
@@ -430,7 +439,7 @@ __global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes,
                   [&](int k, int q) {
                       const int kk = k0 + k;
                       if (kk >= nb) return C<T>{0, 0};
-                      const C<T> z = gmul(Yg[kk * kQ + q], grecip(Ps[static_cast<int64_t>(B + kk) * x.m + B + kk]));
+                      const C<T> z = gmul(Yg[kk * kQ + q], dv[kk]);
                       return below ? gsub(z, tg[kk * kQ + q]) : z;
                   });
     C<T>* Xg = X + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
@@ -456,6 +465,7 @@ __global__ void k_colscale_final(int nrhs, int nparts, const double* __restrict_
 // ------------------------------------------------------------------ launchers
 
 int rhs_groups(int nrhs) { return (nrhs + kQ - 1) / kQ; }
+int bwd_chunks(int rows_below) { return std::max(1, (rows_below + kBwdRows - 1) / kBwdRows); }
 
 template <typename T>
 void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
@@ -506,12 +516,12 @@ void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max
 
 template <typename T>
 void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* XT256, const void* Y, void* X, int n,
-                     void* partial, void* tsum) {
+                     const FrontPlan& fp, const void* P, const void* dinv, const void* XT256, const void* Y, void* X,
+                     int n, void* partial, void* tsum) {
     if (count <= 0 || max_nb <= 0) return;
     const int G = rhs_groups(nrhs);
     const int ctiles = (max_nb + 15) / 16;
-    const int stride = std::max(1, (max_rows_below + kBwdRows - 1) / kBwdRows);
+    const int stride = bwd_chunks(max_rows_below);
     if (max_rows_below > 0) {
         hipLaunchKernelGGL(k_bwd_partial<T>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride, ctiles,
                            fp, static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(partial));
@@ -519,7 +529,7 @@ void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max
                            static_cast<const C<T>*>(partial), static_cast<C<T>*>(tsum));
     }
     hipLaunchKernelGGL(k_bwd_diag<T>, dim3(ctiles, count, G), dim3(256), 0, st, nodes, B, fp,
-                       static_cast<const C<T>*>(P), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
+                       static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
                        static_cast<const C<T>*>(tsum), static_cast<C<T>*>(X), n);
 }
 
@@ -539,8 +549,15 @@ template void launch_fwd_gather<double>(hipStream_t, const int*, int, int, int, 
 template void launch_fwd_gather<float>(hipStream_t, const int*, int, int, int, const FrontPlan&, const void*, int, int64_t, void*);
 template void launch_fwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int, int64_t);
 template void launch_fwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int, int64_t);
-template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, void*, int, void*, void*);
-template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, void*, int, void*, void*);
+template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, const void*, void*, int, void*, void*);
+template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, const void*, void*, int, void*, void*);
+template <typename T>
+void launch_dinv(hipStream_t st, int ns, const FrontPlan& fp, const void* P, void* dinv) {
+    if (ns <= 0) return;
+    hipLaunchKernelGGL(k_dinv<T>, dim3(ns), dim3(64), 0, st, fp, static_cast<const C<T>*>(P), static_cast<C<T>*>(dinv));
+}
+template void launch_dinv<double>(hipStream_t, int, const FrontPlan&, const void*, void*);
+template void launch_dinv<float>(hipStream_t, int, const FrontPlan&, const void*, void*);
 template void launch_inv_blocks<double>(hipStream_t, const int2*, int, const FrontPlan&, const void*, const void*, void*, void*);
 template void launch_inv_blocks<float>(hipStream_t, const int2*, int, const FrontPlan&, const void*, const void*, void*, void*);
 
