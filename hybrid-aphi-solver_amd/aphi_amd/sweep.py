@@ -20,18 +20,21 @@ class PrecisionPolicy:
     Cost model per point: FP32 = factor32 + (1 + solves) * solve32, FP64 =
     factor64 + solve64, with the times taken from the solve reports as they come
     in (factor64 starts as twice factor32, the FLOP-rate ratio of the two MFMA
-    paths). The refinement solves expected at a new frequency are the larger of
-    those recorded at its nearest neighbours below and above in log-frequency; a
-    point where FP32 refinement did not converge counts as infinitely many.
+    paths). The refinement solves expected at a new frequency are interpolated
+    between its nearest solved neighbours below and above, linearly in
+    log-frequency on log(1 + solves); a point where FP32 refinement did not
+    converge counts as twice the refinement budget, and a frequency with no
+    converged FP32 neighbour is left to FP64 once a neighbour has failed.
     """
 
-    def __init__(self, fp64_factor_ratio=2.0):
+    def __init__(self, fp64_factor_ratio=2.0, budget=30):
         self.points = {}          # log10(f) -> refinement solves (inf: FP32 not viable)
         self.factor32 = None
         self.factor64 = None
         self.solve32 = None
         self.solve64 = None
         self.ratio = fp64_factor_ratio
+        self.budget = budget
 
     def expected_solves(self, freq):
         if not self.points:
@@ -41,10 +44,20 @@ class PrecisionPolicy:
         above = [k for k in self.points if k >= x]
         near = []
         if below:
-            near.append(self.points[max(below)])
+            near.append(max(below))
         if above:
-            near.append(self.points[min(above)])
-        return max(near)
+            near.append(min(above))
+        vals = [self.points[k] for k in near]
+        if all(math.isinf(v) for v in vals):
+            return math.inf
+
+        def level(v):
+            return math.log1p(2.0 * self.budget if math.isinf(v) else v)
+
+        if len(near) == 1 or near[0] == near[-1]:
+            return math.expm1(level(vals[0]))
+        t = (x - near[0]) / (near[1] - near[0])
+        return math.expm1((1.0 - t) * level(vals[0]) + t * level(vals[1]))
 
     def choose(self, freq):
         k = self.expected_solves(freq)

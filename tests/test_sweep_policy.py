@@ -39,3 +39,20 @@ def test_cost_model_threshold():
     assert p.choose(2e6) == A.APHI_FP32
     p.record(2e6, A.APHI_FP32, rep(36, A.APHI_FP32, 800.0, 37 * 25.0))  # 800 + 37*25 = 1725 > 1685
     assert p.choose(3e6) == A.APHI_FP64
+
+
+def test_interpolates_between_an_easy_and_a_failed_neighbour():
+    p = A.PrecisionPolicy(budget=30)
+    p.record(10 ** 5.97, A.APHI_FP32, rep(7, A.APHI_FP32, 670.0, 8 * 23.0))     # 23 ms per solve
+    p.record(10 ** 7.24, A.APHI_FP32, rep(6, A.APHI_FP64, 2100.0, 600.0))      # stalled, fell back
+    p.record(10 ** 7.62, A.APHI_FP64, rep(0, A.APHI_FP64, 1415.0, 30.0))
+    # log(1 + k) interpolated: 7 solves at 5.97, 60 (twice the budget) at 7.24
+    k = p.expected_solves(10 ** 6.35)
+    lo, hi = math.log1p(7), math.log1p(60)
+    t = (6.35 - 5.97) / (7.24 - 5.97)
+    assert abs(k - math.expm1(lo + t * (hi - lo))) < 1e-9
+    assert 7 < k < 60
+    assert p.choose(10 ** 6.35) == A.APHI_FP32   # 670 + (1 + k) * 23 < 1415 + 30
+    assert p.choose(10 ** 7.1) == A.APHI_FP64    # close to the failed point
+    assert math.isinf(p.expected_solves(10 ** 7.5))  # only failed (or no FP32) neighbours
+    assert p.choose(10 ** 7.5) == A.APHI_FP64
