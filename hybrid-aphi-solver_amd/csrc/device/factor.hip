@@ -632,20 +632,32 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
             const int buf = gs & 1;
             if (st + 1 < nst) issue_stage(buf ^ 1, st + 1, i0, j0);
             else if (tn < tlast) issue_stage(buf ^ 1, 0, ni0, nj0);
+            // the LDS operands of a step are read one step ahead, so a step's MFMAs
+            // do not wait on its own LDS reads
+            C<T> ra[2][WY], rw[2][WX], rd[2];
+            const auto fetch = [&](int slot, int k4) {
+                const int kq = k4 + lane / 16;
+                rd[slot] = Dsh[st * KC + kq];
+#pragma unroll
+                for (int y = 0; y < WY; ++y) ra[slot][y] = Lc[buf][kq][wj + y * 16 + (lane % 16)];
+#pragma unroll
+                for (int x = 0; x < WX; ++x) rw[slot][x] = Wc[buf][kq][wi + x * 16 + (lane % 16)];
+            };
+            fetch(0, 0);
 #pragma unroll
             for (int k4 = 0; k4 < KC; k4 += 4) {
-                const int kq = k4 + lane / 16;
-                const C<T> d = Dsh[st * KC + kq];
+                const int cur = (k4 / 4) & 1;
+                if (k4 + 4 < KC) fetch(cur ^ 1, k4 + 4);
+                __builtin_amdgcn_sched_barrier(0);  // keep the next step's reads ahead of this step's MFMAs
                 T ar[WY], ai[WY], br[WX], bi[WX];
 #pragma unroll
                 for (int y = 0; y < WY; ++y) {
-                    const C<T> a = Lc[buf][kq][wj + y * 16 + (lane % 16)];
-                    ar[y] = a.re;
-                    ai[y] = a.im;
+                    ar[y] = ra[cur][y].re;
+                    ai[y] = ra[cur][y].im;
                 }
 #pragma unroll
                 for (int x = 0; x < WX; ++x) {
-                    const C<T> w = gmul(Wc[buf][kq][wi + x * 16 + (lane % 16)], d);
+                    const C<T> w = gmul(rw[cur][x], rd[cur]);
                     br[x] = w.re;
                     bi[x] = w.im;
                 }
