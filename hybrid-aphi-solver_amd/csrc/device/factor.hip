@@ -611,19 +611,93 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
 
     // FP32 tiles use the three-product form of the complex product (a = ar + i ai,
     // b = br + i bi): k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi),
-    // re = k1 - k3, im = k1 + k2, summed in S1 (accs), S2 (acci) and S3 (accr):
-    // three MFMAs per product instead of four. FP64 tiles use the four-product
-    // form (accr, acci).
+    // re = k1 - k3, im = k1 + k2, summed in S1, S2 and S3: three MFMAs per product
+    // instead of four. FP64 tiles use the four-product form (accr, acci).
     constexpr bool k3 = std::is_same<T, float>::value;
+    if constexpr (k3) {
+        // FP32: each wave's 32 x 64 block as two 32 x 32 v_mfma_f32_32x32x2f32 tiles
+        // (16 passes: half the MFMA instructions of 16x16x4 for the same flops).
+        // Operands: lane l supplies A[l%32][l/32] and B[l/32][l%32]; register e
+        // of the accumulator holds D[8(e/4) + 4(l/32) + e%4][l%32].
+        typedef float v16 __attribute__((ext_vector_type(16)));
+        for (;;) {
+            v16 s1[2], s2[2], s3[2];
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) s1[y][e] = s2[y][e] = s3[y][e] = 0.0f;
+            const int tn = t + nloc;  // this workgroup's next tile
+            int ni0 = 0, nj0 = 0;
+            if (tn < tlast) tile_origin(tn, ni0, nj0);
+            const int r32 = lane % 32, k2q = lane / 32;
+            for (int st = 0; st < nst; ++st, ++gs) {
+                const int buf = gs & 1;
+                if (st + 1 < nst) issue_stage(buf ^ 1, st + 1, i0, j0);
+                else if (tn < tlast) issue_stage(buf ^ 1, 0, ni0, nj0);
+                // the LDS operands of a step are read one step ahead, so a step's
+                // MFMAs do not wait on its own LDS reads
+                C<T> ra[2][2], rw[2], rd[2];
+                const auto fetch = [&](int slot, int k2) {
+                    const int kq = k2 + k2q;
+                    rd[slot] = Dsh[st * KC + kq];
+                    ra[slot][0] = Lc[buf][kq][wj + r32];
+                    ra[slot][1] = Lc[buf][kq][wj + 32 + r32];
+                    rw[slot] = Wc[buf][kq][wi + r32];
+                };
+                fetch(0, 0);
+#pragma unroll
+                for (int k2 = 0; k2 < KC; k2 += 2) {
+                    const int cur = (k2 / 2) & 1;
+                    if (k2 + 2 < KC) fetch(cur ^ 1, k2 + 2);
+                    __builtin_amdgcn_sched_barrier(0);  // keep the next step's reads ahead of this step's MFMAs
+                    const C<T> w = gmul(rw[cur], rd[cur]);
+                    const float br = w.re, bd = w.im - w.re, bs = w.re + w.im;
+#pragma unroll
+                    for (int y = 0; y < 2; ++y) {
+                        const float ar = ra[cur][y].re, ai = ra[cur][y].im;
+                        s1[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar + ai, br, s1[y], 0, 0, 0);
+                        s2[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bd, s2[y], 0, 0, 0);
+                        s3[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, bs, s3[y], 0, 0, 0);
+                    }
+                }
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                __syncthreads();
+            }
+            // D[p][q]: p (j) = 8(e/4) + 4(lane/32) + e%4, q (i) = lane%32; all the F
+            // loads first, then the sums and stores
+            const int i = i0 + wi + r32;
+            C<T> f[2][16];
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int j = j0 + wj + 32 * y + 8 * (e / 4) + 4 * k2q + e % 4;
+                    const int jj = min(j, fr.m - 1), ii = max(min(i, fr.m - 1), jj);  // a stored entry
+                    f[y][e] = *fr.at(ii, jj);
+                }
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) {
+                    const int j = j0 + wj + 32 * y + 8 * (e / 4) + 4 * k2q + e % 4;
+                    const C<T> out = C<T>{f[y][e].re + (s1[y][e] - s3[y][e]), f[y][e].im + (s1[y][e] + s2[y][e])};
+                    if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
+                }
+            if (tn >= tlast) break;
+            t = tn;
+            i0 = ni0;
+            j0 = nj0;
+        }
+        return;
+    }
     for (;;) {
-        V4 accr[WX][WY], acci[WX][WY], accs[k3 ? WX : 1][k3 ? WY : 1];
+        V4 accr[WX][WY], acci[WX][WY];
 #pragma unroll
         for (int x = 0; x < WX; ++x)
 #pragma unroll
             for (int y = 0; y < WY; ++y) {
                 accr[x][y] = V4{0, 0, 0, 0};
                 acci[x][y] = V4{0, 0, 0, 0};
-                if constexpr (k3) accs[x][y] = V4{0, 0, 0, 0};
             }
         const int tn = t + nloc;  // this workgroup's next tile
         int ni0 = 0, nj0 = 0;
@@ -661,34 +735,15 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
                     br[x] = w.re;
                     bi[x] = w.im;
                 }
-                if constexpr (k3) {
-                    T as[WY], bd[WX], bs[WX];
 #pragma unroll
-                    for (int y = 0; y < WY; ++y) as[y] = ar[y] + ai[y];
+                for (int x = 0; x < WX; ++x)
 #pragma unroll
-                    for (int x = 0; x < WX; ++x) {
-                        bd[x] = bi[x] - br[x];
-                        bs[x] = br[x] + bi[x];
+                    for (int y = 0; y < WY; ++y) {
+                        accr[x][y] = Mfma<T>::mac(ar[y], br[x], accr[x][y]);
+                        accr[x][y] = Mfma<T>::mac(-ai[y], bi[x], accr[x][y]);
+                        acci[x][y] = Mfma<T>::mac(ar[y], bi[x], acci[x][y]);
+                        acci[x][y] = Mfma<T>::mac(ai[y], br[x], acci[x][y]);
                     }
-#pragma unroll
-                    for (int x = 0; x < WX; ++x)
-#pragma unroll
-                        for (int y = 0; y < WY; ++y) {
-                            accs[x][y] = Mfma<T>::mac(as[y], br[x], accs[x][y]);
-                            acci[x][y] = Mfma<T>::mac(ar[y], bd[x], acci[x][y]);
-                            accr[x][y] = Mfma<T>::mac(ai[y], bs[x], accr[x][y]);
-                        }
-                } else {
-#pragma unroll
-                    for (int x = 0; x < WX; ++x)
-#pragma unroll
-                        for (int y = 0; y < WY; ++y) {
-                            accr[x][y] = Mfma<T>::mac(ar[y], br[x], accr[x][y]);
-                            accr[x][y] = Mfma<T>::mac(-ai[y], bi[x], accr[x][y]);
-                            acci[x][y] = Mfma<T>::mac(ar[y], bi[x], acci[x][y]);
-                            acci[x][y] = Mfma<T>::mac(ai[y], br[x], acci[x][y]);
-                        }
-                }
             }
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
@@ -715,12 +770,7 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
                 for (int v = 0; v < 4; ++v) {
                     const int i = i0 + wi + x * 16 + (lane % 16);
                     const int j = j0 + wj + y * 16 + Mfma<T>::row(lane, v);
-                    const C<T> fv = f[x][y][v];
-                    C<T> out;
-                    if constexpr (k3)
-                        out = C<T>{fv.re + (accs[x][y][v] - accr[x][y][v]), fv.im + (accs[x][y][v] + acci[x][y][v])};
-                    else
-                        out = C<T>{fv.re + accr[x][y][v], fv.im + acci[x][y][v]};
+                    const C<T> out = C<T>{f[x][y][v].re + accr[x][y][v], f[x][y][v].im + acci[x][y][v]};
                     if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
                 }
         if (tn >= tlast) break;
