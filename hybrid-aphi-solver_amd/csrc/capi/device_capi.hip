@@ -97,6 +97,15 @@ struct aphi_context {
     hipEvent_t main_done = nullptr, side_done = nullptr;  // ordering only (no timing)
 };
 
+// The solve's per-level launches (about 1,200 at 465k unknowns) captured once
+// and replayed: they touch only the mesh's own buffers, whose addresses form the
+// key, and depend on the right-hand sides only through the group count.
+struct SolveGraph {
+    hipGraphExec_t exec = nullptr;
+    int precision = -1, groups = 0;
+    std::vector<const void*> key;
+};
+
 struct aphi_mesh {
     aphi_context* ctx = nullptr;
     int n = 0, num_nodes = 0, num_tets = 0, num_bodies = 0, num_super = 0, num_levels = 0, num_values = 0;
@@ -121,6 +130,7 @@ struct aphi_mesh {
     DevBuf bt, W, X, Yv, bpart, tsum, colscale, res, norms, xil;
     DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
     int solve_nrhs_cap = 0;
+    SolveGraph sgraph;
     bool assembled = false, equilibrated = false, factored = false;
     int factor_precision = -1;
     double amax = 0.0, amax_eq = 1.0, eq_min = 1.0, eq_max = 1.0;
@@ -357,6 +367,13 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     m->factored = r.ok != 0;
 }
 
+// APHI_SOLVE_GRAPHS=0 launches the solve's level loops one by one instead of
+// replaying their captured graph
+bool solve_graphs() {
+    const char* e = std::getenv("APHI_SOLVE_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
 void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     if (nrhs <= m->solve_nrhs_cap) return;
     const size_t cs = sizeof(cd);  // sized for the wider type; FP32 uses the front half
@@ -396,6 +413,7 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         }
         return rb;
     };
+    const auto levels = [&]() {
     for (const Level& L : m->levels) {
         const int cnt = L.count();
         aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, std::max(L.small_max_m, L.large_max_m), nrhs, fp, m->bt.p,
@@ -415,6 +433,37 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         for (int B = ((maxk - 1) / SB) * SB; B >= 0; B -= SB)
             aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), rows_below(L, B), nrhs, fp,
                                          m->P.p, m->dinv.p, m->XT256.p, m->Yv.p, m->X.p, n, m->bpart.p, m->tsum.p);
+    }
+    };
+    if (solve_graphs()) {
+        const int prec = std::is_same<T, float>::value ? APHI_FP32 : APHI_FP64;
+        const int groups = aphi_dev::rhs_groups(nrhs);
+        const std::vector<const void*> key = {m->P.p,  m->X256.p, m->XT256.p, m->dinv.p, m->bt.p, m->W.p,
+                                              m->Yv.p, m->X.p,    m->bpart.p, m->tsum.p, m->nodes.p};
+        SolveGraph& sg = m->sgraph;
+        if (!(sg.exec && sg.precision == prec && sg.groups == groups && sg.key == key)) {
+            if (sg.exec) HIPCHK(hipGraphExecDestroy(sg.exec));
+            sg.exec = nullptr;
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+            try {
+                levels();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(s, &g));
+            const hipError_t ie = hipGraphInstantiate(&sg.exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(ie);
+            sg.precision = prec;
+            sg.groups = groups;
+            sg.key = key;
+        }
+        HIPCHK(hipGraphLaunch(sg.exec, s));
+    } else {
+        levels();
     }
     aphi_dev::launch_perm_out<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), m->X.p, m->colscale.as<double>(), x, ldx,
                                  accumulate);
@@ -963,6 +1012,7 @@ extern "C" void aphi_mesh_destroy(aphi_mesh* m) {
     (void)hipSetDevice(m->ctx->device);
     (void)hipStreamSynchronize(m->ctx->stream);
     for (hipEvent_t e : m->prof_events) (void)hipEventDestroy(e);
+    if (m->sgraph.exec) (void)hipGraphExecDestroy(m->sgraph.exec);
     delete m;
 }
 

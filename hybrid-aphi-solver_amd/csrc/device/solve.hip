@@ -61,6 +61,7 @@ __device__ __forceinline__ void cmma64(V4<T>& accr, V4<T>& acci, FA a_at, FB b_a
     for (int u = 0; u < 16; ++u) a[u] = a_at(p, 4 * u + kq);
 #pragma unroll
     for (int u = 0; u < 16; ++u) b[u] = b_at(4 * u + kq, p);
+    __builtin_amdgcn_sched_barrier(0);  // every operand load in flight before the first MFMA
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
         accr = Mfma<T>::mac(a[u].re, b[u].re, accr);
@@ -302,7 +303,10 @@ __global__ void __launch_bounds__(256) k_fwd_diag(const int* __restrict__ nodes,
     if (k0 < kmax)
         cmma64<T>(accr, acci,
                   [&](int p, int k) { return Xs[static_cast<int64_t>(min(k0 + k, nb - 1)) * nb + min(c0 + p, nb - 1)]; },
-                  [&](int k, int q) { return k0 + k < kmax ? Ws[(k0 + k) * kQ + q] : C<T>{0, 0}; });
+                  [&](int k, int q) {
+                      const C<T> v = Ws[min(k0 + k, kmax - 1) * kQ + q];  // clamped read, then select
+                      return k0 + k < kmax ? v : C<T>{0, 0};
+                  });
     C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
     reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
         if (c0 + p < nb) Yg[(c0 + p) * kQ + q] = t;
@@ -332,7 +336,10 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
                   [&](int p, int k) {
                       return Ps[static_cast<int64_t>(B + min(k0 + k, nb - 1)) * x.m + min(i0 + p, x.m - 1)];
                   },
-                  [&](int k, int q) { return k0 + k < nb ? Yg[(k0 + k) * kQ + q] : C<T>{0, 0}; });
+                  [&](int k, int q) {
+                      const C<T> v = Yg[min(k0 + k, nb - 1) * kQ + q];  // clamped read, then select
+                      return k0 + k < nb ? v : C<T>{0, 0};
+                  });
     C<T>* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
     reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
         if (i0 + p < x.m) {
@@ -368,14 +375,28 @@ __global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nod
     const int* rows = fp.rows + x.rp;
     const C<T>* Xg = X + static_cast<int64_t>(g) * n * kQ;
     V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-    if (iw < x.m)
-        cmma64<T>(accr, acci,
-                  [&](int p, int k) {
-                      return Ps[static_cast<int64_t>(B + min(c0 + p, nb - 1)) * x.m + min(iw + k, x.m - 1)];
-                  },
-                  [&](int k, int q) {
-                      return iw + k < x.m ? Xg[static_cast<int64_t>(rows[iw + k]) * kQ + q] : C<T>{0, 0};
-                  });
+    if (iw < x.m) {
+        // the row indices first (one wait), then every operand load before the MFMAs
+        const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
+        int gr[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) gr[u] = rows[min(iw + 4 * u + kq, x.m - 1)];
+        C<T> a[16], b[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            a[u] = Ps[static_cast<int64_t>(B + min(c0 + p, nb - 1)) * x.m + min(iw + 4 * u + kq, x.m - 1)];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) b[u] = Xg[static_cast<int64_t>(gr[u]) * kQ + p];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const C<T> bb = iw + 4 * u + kq < x.m ? b[u] : C<T>{0, 0};
+            accr = Mfma<T>::mac(a[u].re, bb.re, accr);
+            accr = Mfma<T>::mac(-a[u].im, bb.im, accr);
+            acci = Mfma<T>::mac(a[u].re, bb.im, acci);
+            acci = Mfma<T>::mac(a[u].im, bb.re, acci);
+        }
+    }
     C<T>* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
     reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
         if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;
@@ -433,15 +454,29 @@ __global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes,
     const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
     const C<T>* tg = tsum + (static_cast<int64_t>(blockIdx.y) * G + g) * kSB * kQ;
     V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-    if (k0 < nb)
-        cmma64<T>(accr, acci,
-                  [&](int p, int k) { return XTs[static_cast<int64_t>(min(k0 + k, nb - 1)) * nb + min(c0 + p, nb - 1)]; },
-                  [&](int k, int q) {
-                      const int kk = k0 + k;
-                      if (kk >= nb) return C<T>{0, 0};
-                      const C<T> z = gmul(Yg[kk * kQ + q], dv[kk]);
-                      return below ? gsub(z, tg[kk * kQ + q]) : z;
-                  });
+    if (k0 < nb) {
+        // every load (clamped indices) issued before the first wait, then z, then the MFMAs
+        const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
+        C<T> a[16], yv[16], dw[16], tv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int kk = min(k0 + 4 * u + kq, nb - 1);
+            a[u] = XTs[static_cast<int64_t>(kk) * nb + min(c0 + p, nb - 1)];
+            yv[u] = Yg[kk * kQ + p];
+            dw[u] = dv[kk];
+            tv[u] = below ? tg[kk * kQ + p] : C<T>{0, 0};
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const C<T> z = gsub(gmul(yv[u], dw[u]), tv[u]);
+            const C<T> b = k0 + 4 * u + kq < nb ? z : C<T>{0, 0};
+            accr = Mfma<T>::mac(a[u].re, b.re, accr);
+            accr = Mfma<T>::mac(-a[u].im, b.im, accr);
+            acci = Mfma<T>::mac(a[u].re, b.im, acci);
+            acci = Mfma<T>::mac(a[u].im, b.re, acci);
+        }
+    }
     C<T>* Xg = X + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
     reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
         if (c0 + p < nb) Xg[(c0 + p) * kQ + q] = t;

@@ -249,3 +249,32 @@ def test_config2_sweep_points(gpu_context, freq):
         assert O.backward_error(p, vals, c[r], B[r])[0] <= 2e-15
         assert np.linalg.norm(a[r] - c[r]) / np.linalg.norm(a[r]) <= 1e-7
     m.close()
+
+
+@pytest.mark.parametrize("precision", [A.APHI_FP32, A.APHI_FP64])
+def test_solve_graph_replays_match_eager(case, precision, monkeypatch):
+    """The solve's level loops run as a captured graph; every replay, with other
+    solves (other right-hand-side counts) and synchronisations in between, equals
+    the launch-by-launch solve bit for bit."""
+    import torch
+    m, n = case["m"], case["p"].n
+    m.refill(case["meta"]["omega"])
+    m.equilibrate(10)
+    assert m.factorize_ldlt(precision)["ok"]
+    gen = np.random.default_rng(7)
+    b1 = dev(gen.standard_normal((16, n)) + 1j * gen.standard_normal((16, n)))
+    b2 = dev(gen.standard_normal((3, n)) + 1j * gen.standard_normal((3, n)))
+    b3 = dev(gen.standard_normal((20, n)) + 1j * gen.standard_normal((20, n)))
+    monkeypatch.setenv("APHI_SOLVE_GRAPHS", "0")
+    eager1 = m.solve(b1).cpu().numpy()
+    eager3 = m.solve(b3).cpu().numpy()
+    monkeypatch.delenv("APHI_SOLVE_GRAPHS")
+    for other in (None, b2, b3, b2):
+        x1 = m.solve(b1)
+        torch.cuda.synchronize()
+        assert np.array_equal(x1.cpu().numpy().view(np.uint64), eager1.view(np.uint64))
+        if other is not None:
+            xo = m.solve(other)
+            torch.cuda.synchronize()
+            if other is b3:
+                assert np.array_equal(xo.cpu().numpy().view(np.uint64), eager3.view(np.uint64))
