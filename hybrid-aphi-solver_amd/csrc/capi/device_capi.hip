@@ -709,10 +709,10 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
             k = j + 1;
             if (trace_refinement()) std::fprintf(stderr, "  gmres %2d: estimated scaled backward error %.3e\n", solves, ratio);
             if (!(ratio > 0.5 * tol)) break;
-            // Stalled: after five steps of this cycle, the mean reduction per step
+            // Stalled: after three steps of this cycle, the mean reduction per step
             // projects past 1.5 times the budget. Stop here (the caller refactors in
             // FP64) rather than spend the rest of the budget.
-            if (j >= 4 && start > 0.0 && ratio > 0.0) {
+            if (j >= 2 && start > 0.0 && ratio > 0.0) {
                 const double rate = std::pow(ratio / start, 1.0 / (j + 1));
                 if (!(rate < 1.0) || solves + std::log(0.5 * tol / ratio) / std::log(rate) > 1.5 * max_solves) {
                     if (trace_refinement()) std::fprintf(stderr, "  gmres stalled (rate %.3f per step)\n", rate);

@@ -615,17 +615,21 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
     // instead of four. FP64 tiles use the four-product form (accr, acci).
     constexpr bool k3 = std::is_same<T, float>::value;
     if constexpr (k3) {
-        // FP32: each wave's 32 x 64 block as two 32 x 32 v_mfma_f32_32x32x2f32 tiles
-        // (16 passes: half the MFMA instructions of 16x16x4 for the same flops).
-        // Operands: lane l supplies A[l%32][l/32] and B[l/32][l%32]; register e
-        // of the accumulator holds D[8(e/4) + 4(l/32) + e%4][l%32].
+        // FP32: each wave's (16 WX) x (16 WY) block as XB x YB tiles of 32 x 32 on
+        // v_mfma_f32_32x32x2f32 (16 passes: half the MFMA instructions of 16x16x4
+        // for the same flops). Operands: lane l supplies A[l%32][l/32] and
+        // B[l/32][l%32]; register e of the accumulator holds
+        // D[8(e/4) + 4(l/32) + e%4][l%32].
         typedef float v16 __attribute__((ext_vector_type(16)));
+        constexpr int XB = WX / 2, YB = WY / 2;
         for (;;) {
-            v16 s1[2], s2[2], s3[2];
+            v16 s1[XB][YB], s2[XB][YB], s3[XB][YB];
 #pragma unroll
-            for (int y = 0; y < 2; ++y)
+            for (int x = 0; x < XB; ++x)
 #pragma unroll
-                for (int e = 0; e < 16; ++e) s1[y][e] = s2[y][e] = s3[y][e] = 0.0f;
+                for (int y = 0; y < YB; ++y)
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) s1[x][y][e] = s2[x][y][e] = s3[x][y][e] = 0.0f;
             const int tn = t + nloc;  // this workgroup's next tile
             int ni0 = 0, nj0 = 0;
             if (tn < tlast) tile_origin(tn, ni0, nj0);
@@ -636,13 +640,14 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
                 else if (tn < tlast) issue_stage(buf ^ 1, 0, ni0, nj0);
                 // the LDS operands of a step are read one step ahead, so a step's
                 // MFMAs do not wait on its own LDS reads
-                C<T> ra[2][2], rw[2], rd[2];
+                C<T> ra[2][YB], rw[2][XB], rd[2];
                 const auto fetch = [&](int slot, int k2) {
                     const int kq = k2 + k2q;
                     rd[slot] = Dsh[st * KC + kq];
-                    ra[slot][0] = Lc[buf][kq][wj + r32];
-                    ra[slot][1] = Lc[buf][kq][wj + 32 + r32];
-                    rw[slot] = Wc[buf][kq][wi + r32];
+#pragma unroll
+                    for (int y = 0; y < YB; ++y) ra[slot][y] = Lc[buf][kq][wj + 32 * y + r32];
+#pragma unroll
+                    for (int x = 0; x < XB; ++x) rw[slot][x] = Wc[buf][kq][wi + 32 * x + r32];
                 };
                 fetch(0, 0);
 #pragma unroll
@@ -650,39 +655,50 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
                     const int cur = (k2 / 2) & 1;
                     if (k2 + 2 < KC) fetch(cur ^ 1, k2 + 2);
                     __builtin_amdgcn_sched_barrier(0);  // keep the next step's reads ahead of this step's MFMAs
-                    const C<T> w = gmul(rw[cur], rd[cur]);
-                    const float br = w.re, bd = w.im - w.re, bs = w.re + w.im;
+                    float br[XB], bd[XB], bs[XB];
 #pragma unroll
-                    for (int y = 0; y < 2; ++y) {
-                        const float ar = ra[cur][y].re, ai = ra[cur][y].im;
-                        s1[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar + ai, br, s1[y], 0, 0, 0);
-                        s2[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bd, s2[y], 0, 0, 0);
-                        s3[y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, bs, s3[y], 0, 0, 0);
+                    for (int x = 0; x < XB; ++x) {
+                        const C<T> w = gmul(rw[cur][x], rd[cur]);
+                        br[x] = w.re;
+                        bd[x] = w.im - w.re;
+                        bs[x] = w.re + w.im;
+                    }
+#pragma unroll
+                    for (int y = 0; y < YB; ++y) {
+                        const float ar = ra[cur][y].re, ai = ra[cur][y].im, as = ar + ai;
+#pragma unroll
+                        for (int x = 0; x < XB; ++x) {
+                            s1[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(as, br[x], s1[x][y], 0, 0, 0);
+                            s2[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bd[x], s2[x][y], 0, 0, 0);
+                            s3[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, bs[x], s3[x][y], 0, 0, 0);
+                        }
                     }
                 }
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 __syncthreads();
             }
-            // D[p][q]: p (j) = 8(e/4) + 4(lane/32) + e%4, q (i) = lane%32; all the F
-            // loads first, then the sums and stores
-            const int i = i0 + wi + r32;
-            C<T> f[2][16];
+            // D[p][q]: p (j) = 8(e/4) + 4(lane/32) + e%4, q (i) = lane%32; per tile all
+            // the F loads first, then the sums and stores
 #pragma unroll
-            for (int y = 0; y < 2; ++y)
+            for (int x = 0; x < XB; ++x) {
+                const int i = i0 + wi + 32 * x + r32;
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const int j = j0 + wj + 32 * y + 8 * (e / 4) + 4 * k2q + e % 4;
-                    const int jj = min(j, fr.m - 1), ii = max(min(i, fr.m - 1), jj);  // a stored entry
-                    f[y][e] = *fr.at(ii, jj);
+                for (int y = 0; y < YB; ++y) {
+                    C<T> f[16];
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int j = j0 + wj + 32 * y + 8 * (e / 4) + 4 * k2q + e % 4;
+                        const int jj = min(j, fr.m - 1), ii = max(min(i, fr.m - 1), jj);  // a stored entry
+                        f[e] = *fr.at(ii, jj);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int j = j0 + wj + 32 * y + 8 * (e / 4) + 4 * k2q + e % 4;
+                        const C<T> out = C<T>{f[e].re + (s1[x][y][e] - s3[x][y][e]), f[e].im + (s1[x][y][e] + s2[x][y][e])};
+                        if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
+                    }
                 }
-#pragma unroll
-            for (int y = 0; y < 2; ++y)
-#pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    const int j = j0 + wj + 32 * y + 8 * (e / 4) + 4 * k2q + e % 4;
-                    const C<T> out = C<T>{f[y][e].re + (s1[y][e] - s3[y][e]), f[y][e].im + (s1[y][e] + s2[y][e])};
-                    if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
-                }
+            }
             if (tn >= tlast) break;
             t = tn;
             i0 = ni0;
