@@ -10,7 +10,7 @@ namespace aphi_dev {
 
 constexpr int kNB = 64;       // block-column width of the blocked front factorization and solves
 constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64)
-constexpr int kOuter = 256;   // columns per outer block: the trailing update is rank kOuter
+constexpr int kOuter = 512;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
 constexpr int kSB = 256;        // solve block: columns per forward/backward step (multiple of kNB)
 constexpr int kQ = 16;          // right-hand sides per interleaved solve group
