@@ -3,13 +3,15 @@
 Workload: the mixed-precision configuration -- a 32^3-hex Kuhn cube (196,608 tets,
 464,910 unknowns) with an off-centre 6x3 conductor bar, 64 log-spaced frequencies
 from 1 Hz to 100 MHz (full-wave), 16 seeded N(0,1) complex right-hand sides per
-frequency. One step = one frequency point: device refill, equilibration,
+frequency, swept in ascending order like the reference's solve_mesh loop. One
+step = one frequency point: device refill, equilibration,
 complex64 LDL^T on MFMA, 16-RHS solve, FP64 GMRES refinement (the FP32 factor as
 preconditioner) until every right-hand side's backward error is <= 1e-12. Where
 the points already solved show the FP32 factor would need more refinement
 solves than a complex128 factor costs, the point is factored in complex128
 instead (aphi_amd.PrecisionPolicy); the counts are reported. The plain FP64 path
-(complex128 factor, no refinement) is timed on the same frequency points.
+(complex128 factor, no refinement) is timed on --fp64-steps of the sweep's points,
+spread over the range (its cost does not depend on the frequency).
 
 Prints one JSON line. Inputs are resident in HBM before the timed region.
 """
@@ -49,7 +51,7 @@ def update_traffic(n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=4)
     ap.add_argument("--n", type=int, default=32)
@@ -101,7 +103,11 @@ def main():
     B = torch.complex(torch.randn(len(freqs), a.nrhs, n, generator=gen, device="cuda", dtype=torch.float64),
                       torch.randn(len(freqs), a.nrhs, n, generator=gen, device="cuda", dtype=torch.float64))
     X = torch.empty(a.nrhs, n, dtype=torch.complex128, device="cuda")
-    order = [(i * 37) % len(freqs) for i in range(max(a.steps, a.fp64_steps) + a.warmup)]
+    # the sweep in ascending frequency, as the reference's solve_mesh loop runs it
+    # (tools/solve_mesh.cpp:207); --steps below 64 takes every (64/steps)-th point
+    nsteps = max(1, min(a.steps, len(freqs)))
+    order = [int(round(i * len(freqs) / nsteps)) for i in range(nsteps)]
+    order64 = [order[int(round(i * nsteps / a.fp64_steps))] for i in range(a.fp64_steps)] if a.fp64_steps else []
 
     def step(fi, precision):
         mesh.refill(2 * np.pi * freqs[fi])
@@ -120,20 +126,24 @@ def main():
         rep["requested"] = prec
         return rep
 
+    # warm-up points: the sweep's first frequencies, through a policy of their own
+    # so the timed sweep starts from nothing learnt
     for i in range(a.warmup):
-        step_mixed(order[i])
+        fi = order[i % len(order)]
+        warm = A.PrecisionPolicy()
+        step(fi, warm.choose(freqs[fi]))
     ctx.synchronize()
     torch.cuda.synchronize()
     lat, reps = [], []
     t_start = time.perf_counter()
-    for i in range(a.steps):
+    for fi in order:
         s0 = time.perf_counter()
-        reps.append(step_mixed(order[a.warmup + i]))
+        reps.append(step_mixed(fi))
         lat.append(time.perf_counter() - s0)
     ctx.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    value = a.steps / elapsed
+    value = len(order) / elapsed
     log(f"fp32+IR: {value:.3f} points/s  p50 {np.percentile(lat, 50) * 1e3:.0f} ms  "
         f"p99 {np.percentile(lat, 99) * 1e3:.0f} ms  max be {max(r['backward_error'] for r in reps):.2e}")
 
@@ -141,9 +151,9 @@ def main():
     lat64, reps64 = [], []
     ctx.synchronize()
     t64 = time.perf_counter()
-    for i in range(a.fp64_steps):
+    for fi in order64:
         s0 = time.perf_counter()
-        reps64.append(step(order[a.warmup + i], A.APHI_FP64))
+        reps64.append(step(fi, A.APHI_FP64))
         lat64.append(time.perf_counter() - s0)
     ctx.synchronize()
     value64 = a.fp64_steps / (time.perf_counter() - t64) if a.fp64_steps else None
@@ -153,8 +163,7 @@ def main():
     # error of the FP32+IR solution against the complex128 path, per frequency
     # (untimed; both solved again on the same points)
     err_vs_fp64 = []
-    for i in range(min(a.fp64_steps, a.steps)):
-        fi = order[a.warmup + i]
+    for fi in order64:
         mesh.refill(2 * np.pi * freqs[fi])
         x64, _ = mesh.solve_symmetric(B[fi], precision=A.APHI_FP64, equilibration_iterations=10)
         mesh.refill(2 * np.pi * freqs[fi])
@@ -200,7 +209,7 @@ def main():
         est = s["flops_total"] / rate + s["equilibrate_s"]
         cpu = {"value": 1.0 / est, "unit": "frequency points/s", "cores": 1, "kind": "port",
                "sample": f"the oracle's restatement of the reference's scalar up-looking LDL^T on this workload's "
-                         f"equilibrated matrix at {freqs[order[-1]]:.3g} Hz, run {s['seconds']:.1f} s "
+                         f"equilibrated matrix at {freqs[order[0]]:.3g} Hz, run {s['seconds']:.1f} s "
                          f"({s['rows']} of {n} rows, {100 * s['flops_done'] / s['flops_total']:.2f}% of its "
                          f"{s['flops_total']:.3e} flops at {rate / 1e9:.2f} GFLOP/s); whole-factorization time "
                          f"extrapolated at that rate: {est:.0f} s per frequency point (solve excluded)"}
@@ -211,19 +220,19 @@ def main():
         "value": value,
         "unit": f"frequency points/s (each: refill + factor + {a.nrhs}-RHS solve to backward error <= 1e-12)",
         "n_gpus": 1,
-        "steps": a.steps,
+        "steps": len(order),
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * elapsed / a.steps,
+        "ms_per_step": 1e3 * elapsed / len(order),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 (FP32 MFMA) factor + c128 GMRES refinement; c128 factor where FP32 is predicted slower",
         "data": "synthetic: seeded N(0,1) complex right-hand sides; generated Kuhn-cube mesh",
-        "config": {"workload": f"{a.n}^3-hex Kuhn cube, 6x3 off-centre bar, 64 log frequencies 1 Hz-100 MHz, "
-                               f"{a.nrhs} RHS per point, FP32-MFMA LDL^T + FP64 GMRES-IR to 1e-12",
+        "config": {"workload": f"{a.n}^3-hex Kuhn cube, 6x3 off-centre bar, 64 log frequencies 1 Hz-100 MHz swept in "
+                               f"ascending order ({len(order)} timed), {a.nrhs} RHS per point, FP32-MFMA LDL^T + "
+                               f"FP64 GMRES-IR to 1e-12",
                    "unknowns": n, "nnz_L": int(prob.counts["predicted_nnz"]),
-                   "factor_flops": prob.counts["flops"], "frequencies_timed": [float(freqs[order[a.warmup + i]])
-                                                                               for i in range(a.steps)]},
+                   "factor_flops": prob.counts["flops"], "frequencies_timed": [float(freqs[fi]) for fi in order]},
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
         "rhs_solves_per_s": value * a.nrhs,

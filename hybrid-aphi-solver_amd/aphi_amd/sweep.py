@@ -22,9 +22,11 @@ class PrecisionPolicy:
     in (factor64 starts as twice factor32, the FLOP-rate ratio of the two MFMA
     paths). The refinement solves expected at a new frequency are interpolated
     between its nearest solved neighbours below and above, linearly in
-    log-frequency on log(1 + solves); a point where FP32 refinement did not
-    converge counts as twice the refinement budget, and a frequency with no
-    converged FP32 neighbour is left to FP64 once a neighbour has failed.
+    log-frequency on log(1 + solves); with solved points on one side only (a
+    sweep in frequency order) the line through the two nearest ones is carried
+    on when it rises towards the new frequency. A point where FP32 refinement did
+    not converge counts as twice the refinement budget, and a frequency whose
+    nearest neighbours all failed is left to FP64.
     """
 
     def __init__(self, fp64_factor_ratio=2.0, budget=30):
@@ -55,7 +57,16 @@ class PrecisionPolicy:
             return math.log1p(2.0 * self.budget if math.isinf(v) else v)
 
         if len(near) == 1 or near[0] == near[-1]:
-            return math.expm1(level(vals[0]))
+            # one side only (a sweep in frequency order): carry the trend of the
+            # two nearest points on that side, when it rises towards x
+            side = sorted(below if below else above, key=lambda k: abs(k - x))
+            v0 = self.points[side[0]]
+            if len(side) >= 2 and not math.isinf(v0) and not math.isinf(self.points[side[1]]):
+                x0, x1 = side[0], side[1]
+                slope = (level(v0) - level(self.points[x1])) / (x0 - x1)
+                if slope * (x - x0) > 0:
+                    return math.expm1(min(level(v0) + slope * (x - x0), level(math.inf)))
+            return math.expm1(level(v0))
         t = (x - near[0]) / (near[1] - near[0])
         return math.expm1((1.0 - t) * level(vals[0]) + t * level(vals[1]))
 

@@ -35,8 +35,9 @@ def test_cost_model_threshold():
     p.record(1e3, A.APHI_FP32, rep(3, A.APHI_FP32, 800.0, 100.0))        # 25 ms per solve
     p.record(1e5, A.APHI_FP64, rep(0, A.APHI_FP64, 1650.0, 35.0))
     p.record(1e6, A.APHI_FP32, rep(30, A.APHI_FP32, 800.0, 31 * 25.0))   # slow but converged
-    # 800 + 31*25 = 1575 <= 1650 + 35: still FP32 next to it
-    assert p.choose(2e6) == A.APHI_FP32
+    # just above it the rising trend (3 solves at 1e3, 30 at 1e6) predicts about
+    # 31.5: 800 + 32.5*25 = 1612 <= 1650 + 35, still FP32
+    assert p.choose(1.2e6) == A.APHI_FP32
     p.record(2e6, A.APHI_FP32, rep(36, A.APHI_FP32, 800.0, 37 * 25.0))  # 800 + 37*25 = 1725 > 1685
     assert p.choose(3e6) == A.APHI_FP64
 
@@ -56,3 +57,20 @@ def test_interpolates_between_an_easy_and_a_failed_neighbour():
     assert p.choose(10 ** 7.1) == A.APHI_FP64    # close to the failed point
     assert math.isinf(p.expected_solves(10 ** 7.5))  # only failed (or no FP32) neighbours
     assert p.choose(10 ** 7.5) == A.APHI_FP64
+
+
+def test_extrapolates_a_rising_trend_in_an_ascending_sweep():
+    p = A.PrecisionPolicy(budget=30)
+    p.record(10 ** 5.0, A.APHI_FP32, rep(3, A.APHI_FP32, 610.0, 4 * 21.0))
+    p.record(10 ** 6.0, A.APHI_FP32, rep(9, A.APHI_FP32, 610.0, 10 * 21.0))
+    lo, hi = math.log1p(3), math.log1p(9)
+    k = p.expected_solves(10 ** 6.5)
+    assert abs(k - math.expm1(hi + 0.5 * (hi - lo))) < 1e-9 and k > 9
+    assert p.choose(10 ** 6.5) == A.APHI_FP32              # 610 + (1 + k) * 21 < 2 * 610 + 21
+    assert p.choose(10 ** 7.5) == A.APHI_FP64              # the trend passes the break-even point
+    assert p.expected_solves(10 ** 9) == math.expm1(math.log1p(60))  # capped at a failure
+    # a falling trend is not carried on: the nearest point's count stands
+    q = A.PrecisionPolicy(budget=30)
+    q.record(10 ** 5.0, A.APHI_FP32, rep(9, A.APHI_FP32, 610.0, 10 * 21.0))
+    q.record(10 ** 6.0, A.APHI_FP32, rep(3, A.APHI_FP32, 610.0, 4 * 21.0))
+    assert q.expected_solves(10 ** 6.5) == 3.0
