@@ -118,6 +118,32 @@ class Problem:
         spec = config.spec()
         _check(lib().aphi_problem_create_cube(C.byref(spec), C.byref(h)))
         self._h = h
+        self._read_counts()
+
+    @classmethod
+    def from_csr(cls, n, row_ptr, col_index, ordering=APHI_ORDER_AMD, max_factor_nnz=0, relax_small=0,
+                 relax_fill=0.0):
+        """A problem from a matrix pattern alone: SparseSymmetric::from_pattern
+        (include/aphi_solver/sparse_symmetric.hpp:38-83) + analyze(pattern, ordering,
+        max_factor_nnz) (include/aphi_solver/symbolic.hpp:104). Its device mesh takes
+        values through ``Mesh.upload_system``; ``refill`` refuses it."""
+        self = cls.__new__(cls)
+        self.config = None
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col_index, dtype=np.int32)
+        if rp.size != int(n) + 1:
+            raise ValueError("SparseSymmetric::from_pattern: row_ptr must have rows + 1 entries")
+        if rp[-1] != ci.size:
+            raise ValueError("SparseSymmetric::from_pattern: row_ptr must run from 0 to col_index.size()")
+        h = C.c_void_p()
+        _check(lib().aphi_problem_create_csr(int(n), rp.ctypes.data_as(_capi.I32P), ci.ctypes.data_as(_capi.I32P),
+                                             int(ordering), int(max_factor_nnz), int(relax_small), float(relax_fill),
+                                             C.byref(h)))
+        self._h = h
+        self._read_counts()
+        return self
+
+    def _read_counts(self):
         c = _capi.ProblemCounts()
         _check(lib().aphi_problem_counts_get(self._h, C.byref(c)))
         self.counts = {k: getattr(c, k) for k, _ in _capi.ProblemCounts._fields_}

@@ -91,6 +91,8 @@ SIGNATURES = {
     "aphi_last_error": (C.c_char_p, []),
     "aphi_version": (C.c_char_p, []),
     "aphi_problem_create_cube": (C.c_int, [C.POINTER(CubeSpec), C.POINTER(C.c_void_p)]),
+    "aphi_problem_create_csr": (C.c_int, [C.c_int32, I32P, I32P, C.c_int, C.c_int64, C.c_int, C.c_double,
+                                          C.POINTER(C.c_void_p)]),
     "aphi_problem_destroy": (None, [C.c_void_p]),
     "aphi_problem_counts_get": (C.c_int, [C.c_void_p, C.POINTER(ProblemCounts)]),
     "aphi_problem_get_int": (C.c_int, [C.c_void_p, C.c_char_p, I64P, C.c_int64, I64P]),

@@ -132,6 +132,7 @@ struct aphi_mesh {
     int solve_nrhs_cap = 0;
     SolveGraph sgraph;
     bool assembled = false, equilibrated = false, factored = false;
+    bool has_plan = true;  // false for a problem made from a bare pattern (no refill)
     int factor_precision = -1;
     double amax = 0.0, amax_eq = 1.0, eq_min = 1.0, eq_max = 1.0;
     aphi_factor_stats last_stats{};
@@ -803,7 +804,12 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
     return aphi_capi::guarded([&] {
         require(ctx && d && out, "aphi_mesh_create: null argument");
         *out = nullptr;
-        require(d->n > 0 && d->nnz > 0 && d->num_tets > 0 && d->num_super > 0, "aphi_mesh_create: empty mesh");
+        require(d->n > 0 && d->nnz > 0 && d->num_super > 0, "aphi_mesh_create: empty mesh");
+        // without an assembly plan (a problem made from a bare pattern) the system
+        // arrives through aphi_system_upload instead of aphi_refill
+        const bool plan = d->target_ptr != nullptr;
+        require(!plan || (d->num_tets > 0 && d->entry && d->kind && d->coords && d->tet_nodes && d->tet_body),
+                "aphi_mesh_create: an assembly plan needs the mesh's tets, coordinates and contributions");
         require(d->num_values >= 0 && d->num_values <= 8, "aphi_mesh_create: at most 8 prescribed values");
         HIPCHK(hipSetDevice(ctx->device));
         auto m = std::make_unique<aphi_mesh>();
@@ -816,7 +822,8 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
         m->num_super = d->num_super;
         m->num_levels = d->num_levels;
         m->num_values = d->num_values;
-        m->num_targets = d->nnz + d->n;
+        m->has_plan = plan;
+        m->num_targets = plan ? d->nnz + d->n : 0;
         m->cb_entries = d->cb_pool_entries;
         m->flops = d->flops;
         m->predicted_nnz = d->predicted_nnz;
@@ -825,23 +832,24 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
         m->mu_r.assign(d->body_mu_r, d->body_mu_r + d->num_bodies);
         for (int i = 0; i < d->num_values; ++i) m->value_tab.emplace_back(d->values[2 * i], d->values[2 * i + 1]);
         const int n = d->n, ns = d->num_super;
-        const int64_t nc = d->target_ptr[m->num_targets];
-        for (int t = 0; t < d->num_tets; ++t)
-            if (d->tet_body[t] < 0 || d->tet_body[t] >= d->num_bodies)
-                throw std::invalid_argument("aphi_mesh_create: a tet's body index is out of range");
-
-        upload(m->coords, d->coords, static_cast<size_t>(d->num_nodes) * 3);
-        upload(m->tets, d->tet_nodes, static_cast<size_t>(d->num_tets) * 4);
-        upload(m->tet_body, d->tet_body, static_cast<size_t>(d->num_tets));
-        upload(m->tptr, d->target_ptr, static_cast<size_t>(m->num_targets) + 1);
-        upload(m->entry, d->entry, static_cast<size_t>(nc));
-        upload(m->kind, d->kind, static_cast<size_t>(nc));
-        for (int64_t c = 0; c < nc; ++c) {
-            const int kk = d->kind[c] & 0x0f;
-            if (kk <= 5 && d->entry[c] >= static_cast<uint32_t>(d->num_tets) * 196u)
-                throw std::invalid_argument("aphi_mesh_create: an assembly contribution names a tet outside the mesh");
-            if ((d->kind[c] >> 5) >= std::max(d->num_values, 1) && (kk == 4 || kk == 5 || kk == 7 || kk == 8))
-                throw std::invalid_argument("aphi_mesh_create: a contribution names an unknown prescribed value");
+        if (plan) {
+            const int64_t nc = d->target_ptr[m->num_targets];
+            for (int t = 0; t < d->num_tets; ++t)
+                if (d->tet_body[t] < 0 || d->tet_body[t] >= d->num_bodies)
+                    throw std::invalid_argument("aphi_mesh_create: a tet's body index is out of range");
+            upload(m->coords, d->coords, static_cast<size_t>(d->num_nodes) * 3);
+            upload(m->tets, d->tet_nodes, static_cast<size_t>(d->num_tets) * 4);
+            upload(m->tet_body, d->tet_body, static_cast<size_t>(d->num_tets));
+            upload(m->tptr, d->target_ptr, static_cast<size_t>(m->num_targets) + 1);
+            upload(m->entry, d->entry, static_cast<size_t>(nc));
+            upload(m->kind, d->kind, static_cast<size_t>(nc));
+            for (int64_t c = 0; c < nc; ++c) {
+                const int kk = d->kind[c] & 0x0f;
+                if (kk <= 5 && d->entry[c] >= static_cast<uint32_t>(d->num_tets) * 196u)
+                    throw std::invalid_argument("aphi_mesh_create: an assembly contribution names a tet outside the mesh");
+                if ((d->kind[c] >> 5) >= std::max(d->num_values, 1) && (kk == 4 || kk == 5 || kk == 7 || kk == 8))
+                    throw std::invalid_argument("aphi_mesh_create: a contribution names an unknown prescribed value");
+            }
         }
         upload(m->row_ptr, d->row_ptr, static_cast<size_t>(n) + 1);
         upload(m->col, d->col_index, static_cast<size_t>(d->nnz));
@@ -994,7 +1002,7 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             upload(m->ea_bound_off, bo.data(), bo.size());
             upload(m->ea_bound, bv.data(), bv.size());
         }
-        if (d->tet_dofs && d->tet_edge_signs && d->tet_phi_coeff && d->tet_phi_fixed) {
+        if (plan && d->tet_dofs && d->tet_edge_signs && d->tet_phi_coeff && d->tet_phi_fixed) {
             const size_t nt = static_cast<size_t>(d->num_tets);
             upload(m->tdofs, d->tet_dofs, nt * 16);
             upload(m->tsign, d->tet_edge_signs, nt * 6);
@@ -1039,6 +1047,9 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
 extern "C" int aphi_refill(aphi_mesh* m, double omega, int conditioning) {
     return aphi_capi::guarded([&] {
         require(m != nullptr, "aphi_refill: null mesh");
+        if (!m->has_plan)
+            throw std::logic_error("aphi_refill: the mesh was created from a bare pattern and has no assembly plan; "
+                                   "hand its values over with aphi_system_upload");
         HIPCHK(hipSetDevice(m->ctx->device));
         // formulation_scales + the symmetric guard of src/assembly.cpp:12-45, 484-491
         const Complex jw(0.0, omega);

@@ -14,6 +14,7 @@
 using namespace aphi_amd;
 
 struct aphi_problem {
+    bool from_csr = false;  // a bare pattern: no mesh, no assembly plan
     CubeSpec spec;
     Mesh mesh;
     BoundProblem bound;
@@ -45,6 +46,30 @@ void clear_error() { g_error.clear(); }
 extern "C" const char* aphi_last_error(void) { return aphi_capi::g_error.c_str(); }
 extern "C" const char* aphi_version(void) { return "aphi-mi355x 0.1"; }
 
+namespace {
+
+// analyze + supernodes + slot map, shared by both ways of making a problem
+void finish_symbolic(aphi_problem* p, int ordering, size_t max_factor_nnz, int relax_small, double relax_fill) {
+    const auto t1 = std::chrono::steady_clock::now();
+    p->analysis = analyze(p->pattern, static_cast<Ordering>(ordering), max_factor_nnz);
+    const auto t2 = std::chrono::steady_clock::now();
+    SupernodeOptions opt;
+    if (relax_small > 0) opt.relax_small = relax_small;
+    if (relax_fill > 0.0) opt.relax_fill = relax_fill;
+    p->sp = build_supernodes(p->analysis, opt);
+    p->s2p = slot_to_panel(p->pattern, p->analysis, p->sp);
+    const auto t3 = std::chrono::steady_clock::now();
+    p->analyze_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    p->build_ms += std::chrono::duration<double, std::milli>(t3 - t2).count();
+    const int n = p->sp.n;
+    p->col_ref.resize(static_cast<size_t>(n));
+    for (int j = 0; j < n; ++j)
+        p->col_ref[static_cast<size_t>(j)] = p->analysis.permutation.iperm[static_cast<size_t>(p->sp.perm[static_cast<size_t>(j)])];
+    p->super_level.assign(p->sp.level.begin(), p->sp.level.end());
+}
+
+}  // namespace
+
 extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem** out) {
     return aphi_capi::guarded([&] {
         if (!spec || !out) throw std::invalid_argument("aphi_problem_create_cube: null argument");
@@ -74,17 +99,8 @@ extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem
         p->pattern = build_sparsity(p->dofs, p->bound, p->mesh, true);
         p->map = build_scatter_map(p->pattern, p->dofs, p->mesh, p->bound);
         p->plan = build_assembly_plan(p->pattern, p->map, p->dofs, p->mesh, p->bound);
-        const auto t1 = std::chrono::steady_clock::now();
-        p->analysis = analyze(p->pattern, static_cast<Ordering>(spec->ordering));
-        const auto t2 = std::chrono::steady_clock::now();
-        SupernodeOptions opt;
-        if (spec->relax_small > 0) opt.relax_small = spec->relax_small;
-        if (spec->relax_fill > 0.0) opt.relax_fill = spec->relax_fill;
-        p->sp = build_supernodes(p->analysis, opt);
-        p->s2p = slot_to_panel(p->pattern, p->analysis, p->sp);
-        const auto t3 = std::chrono::steady_clock::now();
-        p->analyze_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
-        p->build_ms = std::chrono::duration<double, std::milli>((t1 - t0) + (t3 - t2)).count();
+        p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        finish_symbolic(p.get(), spec->ordering, 0, spec->relax_small, spec->relax_fill);
 
         const Mesh& m = p->mesh;
         p->coords.resize(static_cast<size_t>(m.num_nodes()) * 3);
@@ -131,6 +147,46 @@ extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem
         for (int j = 0; j < n; ++j)
             p->col_ref[static_cast<size_t>(j)] = p->analysis.permutation.iperm[static_cast<size_t>(p->sp.perm[static_cast<size_t>(j)])];
         p->super_level.assign(p->sp.level.begin(), p->sp.level.end());
+        *out = p.release();
+    });
+}
+
+extern "C" int aphi_problem_create_csr(int32_t n, const int32_t* row_ptr, const int32_t* col_index, int ordering,
+                                       int64_t max_factor_nnz, int relax_small, double relax_fill,
+                                       aphi_problem** out) {
+    return aphi_capi::guarded([&] {
+        if (!out || !row_ptr || (n > 0 && !col_index && row_ptr[n] > 0))
+            throw std::invalid_argument("aphi_problem_create_csr: null argument");
+        *out = nullptr;
+        // the checks of SparseSymmetric::from_pattern (include/aphi_solver/sparse_symmetric.hpp:38-83)
+        if (n < 0) throw std::invalid_argument("SparseSymmetric::from_pattern: negative rows");
+        if (n == 0) throw std::invalid_argument("aphi_problem_create_csr: the matrix has no rows");
+        if (row_ptr[0] != 0) throw std::invalid_argument("SparseSymmetric::from_pattern: row_ptr must run from 0 to col_index.size()");
+        for (int r = 0; r < n; ++r) {
+            const int begin = row_ptr[r], end = row_ptr[r + 1];
+            if (end < begin) throw std::invalid_argument("SparseSymmetric::from_pattern: row_ptr is not non-decreasing");
+            for (int k = begin; k < end; ++k) {
+                const int c = col_index[k];
+                if (c < 0 || c >= n) throw std::invalid_argument("SparseSymmetric::from_pattern: a column index is out of range");
+                if (c < r)
+                    throw std::invalid_argument("SparseSymmetric::from_pattern: entry (" + std::to_string(r) + ", " +
+                                                std::to_string(c) +
+                                                ") is below the diagonal. Only the upper triangle is stored; a "
+                                                "lower-triangle entry would never be read, since matvec mirrors.");
+                if (k > begin && c <= col_index[k - 1])
+                    throw std::invalid_argument("SparseSymmetric::from_pattern: each row's columns must be strictly ascending");
+            }
+        }
+        if (ordering < 0 || ordering > 2) throw std::invalid_argument("aphi_problem_create_csr: unknown ordering");
+        if (max_factor_nnz < 0) throw std::invalid_argument("aphi_problem_create_csr: negative factor budget");
+        auto p = std::make_unique<aphi_problem>();
+        p->from_csr = true;
+        p->pattern.rows = p->pattern.cols = n;
+        p->pattern.upper_only = true;
+        p->pattern.row_ptr.assign(row_ptr, row_ptr + n + 1);
+        p->pattern.col_index.assign(col_index, col_index + row_ptr[n]);
+        p->dofs.num_total = n;
+        finish_symbolic(p.get(), ordering, static_cast<size_t>(max_factor_nnz), relax_small, relax_fill);
         *out = p.release();
     });
 }
@@ -316,9 +372,10 @@ extern "C" int aphi_problem_describe(const aphi_problem* p, aphi_mesh_desc* d) {
         d->body_sigma = p->body_sigma.data();
         d->body_eps_r = p->body_eps_r.data();
         d->body_mu_r = p->body_mu_r.data();
-        d->target_ptr = p->plan.target_ptr.data();
-        d->entry = p->plan.entry.data();
-        d->kind = p->plan.kind.data();
+        // a problem made from a bare pattern has no mesh and no assembly plan
+        d->target_ptr = p->from_csr ? nullptr : p->plan.target_ptr.data();
+        d->entry = p->from_csr ? nullptr : p->plan.entry.data();
+        d->kind = p->from_csr ? nullptr : p->plan.kind.data();
         d->num_values = static_cast<int32_t>(p->plan.fixed_value.size());
         d->values = p->values.data();
         d->row_ptr = p->pattern.row_ptr.data();
@@ -339,9 +396,11 @@ extern "C" int aphi_problem_describe(const aphi_problem* p, aphi_mesh_desc* d) {
         d->slot_to_panel = p->s2p.data();
         d->flops = p->sp.flops;
         d->predicted_nnz = static_cast<int64_t>(p->analysis.predicted_nnz);
-        d->tet_dofs = p->tet_dofs.data();
-        d->tet_edge_signs = p->tet_signs.data();
-        d->tet_phi_coeff = p->tet_phi_coeff.data();
-        d->tet_phi_fixed = p->tet_phi_fixed.data();
+        if (!p->from_csr) {
+            d->tet_dofs = p->tet_dofs.data();
+            d->tet_edge_signs = p->tet_signs.data();
+            d->tet_phi_coeff = p->tet_phi_coeff.data();
+            d->tet_phi_fixed = p->tet_phi_fixed.data();
+        }
     });
 }

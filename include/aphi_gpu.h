@@ -84,6 +84,18 @@ typedef struct aphi_problem_counts {
 } aphi_problem_counts;
 
 int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem** out);
+/* A problem from a caller's matrix pattern alone: the upper-triangle CSR of an
+ * n x n complex-symmetric matrix (row_ptr n+1 entries from 0, columns strictly
+ * ascending per row, none below the diagonal), refused with the messages of
+ * SparseSymmetric::from_pattern (include/aphi_solver/sparse_symmetric.hpp:38-83).
+ * Runs analyze(pattern, ordering, max_factor_nnz) (include/aphi_solver/symbolic.hpp:104,
+ * src/symbolic.cpp:126-196; max_factor_nnz 0 = no budget) and builds the
+ * supernodal plan. The problem has no mesh and no assembly plan: a device mesh
+ * made from it takes its values through aphi_system_upload, and aphi_refill /
+ * aphi_terminal_current refuse it. This is the path of any other caller of
+ * solve_symmetric, e.g. the gauge-free potential (src/gauge_free_potential.cpp:177). */
+int aphi_problem_create_csr(int32_t n, const int32_t* row_ptr, const int32_t* col_index, int ordering,
+                            int64_t max_factor_nnz, int relax_small, double relax_fill, aphi_problem** out);
 void aphi_problem_destroy(aphi_problem* p);
 int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out);
 /* Named integer arrays, copied into out[0..capacity); *count receives the full
