@@ -25,6 +25,7 @@ __all__ = [
     "LogicError", "DeviceError", "CubeConfig", "Problem", "Context", "Mesh", "frequency_sweep",
     "APHI_FP32", "APHI_FP64", "APHI_NATURAL", "APHI_ROW_SCALED", "APHI_SCALED_PHI",
     "APHI_ORDER_AMD", "APHI_ORDER_NATURAL", "APHI_ORDER_RCM", "lib", "PrecisionPolicy",
+    "recover_gauge_free_potential",
 ]
 
 
@@ -142,6 +143,25 @@ class Problem:
         self._h = h
         self._read_counts()
         return self
+
+    @classmethod
+    def gauge_free(cls, cube, ordering=APHI_ORDER_AMD):
+        """The gauge-free potential's P1 system on the cube problem's mesh
+        (src/gauge_free_potential.cpp:40-164): vertex map, pattern, ordering and
+        supernodal plan, and the per-tet term plan of its assembly."""
+        self = cls.__new__(cls)
+        self.config = None
+        self.cube = cube
+        h = C.c_void_p()
+        _check(lib().aphi_problem_create_gauge_free(cube._h, int(ordering), C.byref(h)))
+        self._h = h
+        self._read_counts()
+        return self
+
+    def describe_gauge_free(self):
+        d = _capi.GaugeFreeDesc()
+        _check(lib().aphi_problem_describe_gauge_free(self._h, C.byref(d)))
+        return d
 
     def _read_counts(self):
         c = _capi.ProblemCounts()
@@ -262,6 +282,39 @@ class Mesh:
         vals, rhs = self.system()
         return torch.from_numpy(rhs).to(f"cuda:{self.ctx.device}")
 
+    # compute_fields' e_tet (src/postprocess.cpp:311-391)
+    def tet_fields(self, x, omega, conditioning=APHI_ROW_SCALED):
+        """E at every tet's centroid for one solution x: (num_tets, 3) complex device tensor."""
+        import torch
+        nt = int(self.problem.counts["tets"])
+        xx = x.reshape(-1)
+        if xx.numel() != self.n:
+            raise ValueError("tet_fields: x must hold n values")
+        e = torch.empty((nt, 3), dtype=torch.complex128, device=xx.device)
+        _check(lib().aphi_tet_fields(self._h, _dev_ptr(xx.contiguous()), float(omega), int(conditioning), _dev_ptr(e)))
+        self.ctx.synchronize()
+        return e
+
+    def attach_gauge_free(self, problem):
+        """Hands a mesh made from Problem.gauge_free(...) its assembly plan."""
+        self._gf_desc = problem.describe_gauge_free()
+        _check(lib().aphi_gauge_free_attach(self._h, C.byref(self._gf_desc)))
+        self.num_vertices = int(self._gf_desc.num_nodes)
+        self.num_ports = int(self._gf_desc.num_ports)
+
+    def gauge_free_refill(self, e_tet, omega):
+        """Values and rhs of the gauge-free system from e_tet (device, (num_tets, 3))."""
+        _check(lib().aphi_gauge_free_refill(self._h, _dev_ptr(e_tet.contiguous()), float(omega)))
+
+    def gauge_free_potential(self, x):
+        """(phi_vertex device tensor, port_potential numpy) from the system's solution x."""
+        import torch
+        phi = torch.empty(self.num_vertices, dtype=torch.complex128, device=x.device)
+        ports = np.zeros(max(self.num_ports, 1), dtype=np.complex128)
+        _check(lib().aphi_gauge_free_potential(self._h, _dev_ptr(x.reshape(-1).contiguous()), _dev_ptr(phi),
+                                               ports.view(np.float64).ctypes.data_as(_capi.DP)))
+        return phi, ports[:self.num_ports]
+
     def equilibrate(self, iterations=10):
         d = np.zeros(self.n, dtype=np.float64)
         _check(lib().aphi_equilibrate(self._h, int(iterations), d.ctypes.data_as(_capi.DP)))
@@ -334,3 +387,28 @@ class Mesh:
         _check(lib().aphi_backward_error(self._h, b2.shape[0], _dev_ptr(x2), self.n, _dev_ptr(b2), self.n,
                                          be.ctypes.data_as(_capi.DP), rr.ctypes.data_as(_capi.DP)))
         return be, rr
+
+
+def recover_gauge_free_potential(source, gf, x, omega, conditioning=APHI_ROW_SCALED, precision=APHI_FP64,
+                                 max_refinements=0, tolerance=1e-12):
+    """recover_gauge_free_potential (src/gauge_free_potential.cpp:87-177) on the
+    device: E of every tet from the A-Phi solution x of mesh ``source``
+    (compute_fields), the P1 system assembled into ``gf`` (a mesh of
+    Problem.gauge_free(...) with its plan attached), solve_symmetric with 10
+    equilibration passes (the reference's options; FP32 + refinement on request),
+    and the vertex / port potentials. Returns the fields of GaugeFreePotential."""
+    import time
+    t0 = time.perf_counter()
+    e = source.tet_fields(x, omega, conditioning)
+    gf.gauge_free_refill(e, omega)
+    _, rhs = gf.system()
+    import torch
+    b = torch.from_numpy(rhs).to(x.device)
+    t1 = time.perf_counter()
+    xs, rep = gf.solve_symmetric(b, precision=precision, equilibration_iterations=10, max_refinements=max_refinements,
+                                 tolerance=tolerance)
+    t2 = time.perf_counter()
+    phi, ports = gf.gauge_free_potential(xs)
+    return dict(phi_vertex=phi, port_potential=ports, unknowns=gf.n, converged=bool(rep["ok"]),
+                residual=rep["residual"], backward_error=rep["backward_error"], assemble_ms=(t1 - t0) * 1e3,
+                solve_ms=(t2 - t1) * 1e3, report=rep)

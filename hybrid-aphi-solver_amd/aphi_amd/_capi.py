@@ -65,6 +65,18 @@ class MeshDesc(C.Structure):
     ]
 
 
+class GaugeFreeDesc(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_int32), ("num_tets", C.c_int32), ("num_bodies", C.c_int32), ("unknowns", C.c_int32),
+        ("nnz", C.c_int64),
+        ("coords", DP), ("tet_nodes", I32P), ("tet_body", I32P), ("body_sigma", DP), ("body_eps_r", DP),
+        ("tet_on", I32P), ("column", I32P), ("prescribed", DP),
+        ("num_ports", C.c_int32), ("port_column", I32P), ("port_value", DP),
+        ("target_ptr", I64P), ("entry", C.POINTER(C.c_uint32)), ("kind", C.POINTER(C.c_uint8)),
+        ("num_values", C.c_int32), ("values", DP),
+    ]
+
+
 class FactorStats(C.Structure):
     _fields_ = [("ok", C.c_int), ("columns_done", C.c_int), ("smallest_pivot", C.c_double),
                 ("largest_pivot", C.c_double), ("largest_multiplier", C.c_double), ("nnz", C.c_int64),
@@ -99,6 +111,12 @@ SIGNATURES = {
     "aphi_problem_get_double": (C.c_int, [C.c_void_p, C.c_char_p, DP, C.c_int64, I64P]),
     "aphi_frequency_sweep": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, DP]),
     "aphi_problem_describe": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc)]),
+    "aphi_problem_create_gauge_free": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "aphi_problem_describe_gauge_free": (C.c_int, [C.c_void_p, C.POINTER(GaugeFreeDesc)]),
+    "aphi_tet_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p]),
+    "aphi_gauge_free_attach": (C.c_int, [C.c_void_p, C.POINTER(GaugeFreeDesc)]),
+    "aphi_gauge_free_refill": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),
+    "aphi_gauge_free_potential": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, DP]),
     "aphi_context_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "aphi_context_destroy": (None, [C.c_void_p]),
     "aphi_context_synchronize": (C.c_int, [C.c_void_p]),

@@ -22,6 +22,7 @@ HOST_SOURCES = [
     "host/sparsity.cpp",
     "host/ordering.cpp",
     "host/symbolic.cpp",
+    "host/gauge_free.cpp",
     "capi/problem_capi.cpp",
 ]
 HIP_SOURCES = [

@@ -133,6 +133,16 @@ struct aphi_mesh {
     SolveGraph sgraph;
     bool assembled = false, equilibrated = false, factored = false;
     bool has_plan = true;  // false for a problem made from a bare pattern (no refill)
+    // gauge-free potential plan (aphi_gauge_free_attach), on a bare-pattern mesh
+    struct GaugeFree {
+        bool attached = false;
+        int num_nodes = 0, num_tets = 0, num_bodies = 0;
+        int64_t num_targets = 0;
+        std::vector<double> sigma, eps_r;
+        std::vector<int> port_column;
+        std::vector<Complex> port_value;
+        DevBuf coords, tets, tet_body, tet_on, column, prescribed, tptr, entry, kind, values, beta, terms;
+    } gf;
     int factor_precision = -1;
     double amax = 0.0, amax_eq = 1.0, eq_min = 1.0, eq_max = 1.0;
     aphi_factor_stats last_stats{};
@@ -1189,6 +1199,144 @@ extern "C" int aphi_terminal_current(aphi_mesh* m, int nrhs, const void* xv, int
             if (resistance) resistance[r] = z.real();
             if (inductance) inductance[r] = z.imag() / omega;
         }
+    });
+}
+
+extern "C" int aphi_tet_fields(aphi_mesh* m, const void* xv, double omega, int conditioning, void* e_tet) {
+    return aphi_capi::guarded([&] {
+        require(m && xv && e_tet, "aphi_tet_fields: null argument");
+        require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, "aphi_tet_fields: unknown conditioning");
+        require(m->tdofs.p != nullptr, "aphi_tet_fields: the mesh was created without per-tet unknowns");
+        HIPCHK(hipSetDevice(m->ctx->device));
+        const int nt = m->num_tets;
+        aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
+                                  m->tdofs.as<int>(), m->tsign.as<int8_t>(), m->tcoef.as<double>(), m->tfixed.as<cd>()};
+        const cd minus_jw{-0.0, -omega};
+        const cd scale = conditioning == APHI_SCALED_PHI ? cd{0.0, omega} : cd{1.0, 0.0};
+        hipLaunchKernelGGL(aphi_dev::k_tet_efield, dim3(blocks_for(nt)), dim3(256), 0, m->st(), nt, tp,
+                           static_cast<const cd*>(xv), minus_jw, scale, static_cast<cd*>(e_tet));
+        HIPCHK(hipGetLastError());
+    });
+}
+
+extern "C" int aphi_gauge_free_attach(aphi_mesh* m, const aphi_gauge_free_desc* d) {
+    return aphi_capi::guarded([&] {
+        require(m && d, "aphi_gauge_free_attach: null argument");
+        require(!m->has_plan, "aphi_gauge_free_attach: the mesh must come from the gauge-free problem");
+        require(d->unknowns == m->n && d->nnz == m->nnz, "aphi_gauge_free_attach: the plan is not this mesh's pattern");
+        require(d->num_nodes > 0 && d->num_tets > 0 && d->num_bodies > 0 && d->num_ports >= 0 && d->num_values >= 0 &&
+                    d->num_values <= 64,
+                "aphi_gauge_free_attach: bad sizes");
+        require(d->coords && d->tet_nodes && d->tet_body && d->body_sigma && d->body_eps_r && d->tet_on && d->column &&
+                    d->prescribed && d->target_ptr && d->entry && d->kind && (d->num_values == 0 || d->values) &&
+                    (d->num_ports == 0 || (d->port_column && d->port_value)),
+                "aphi_gauge_free_attach: null array");
+        HIPCHK(hipSetDevice(m->ctx->device));
+        auto& g = m->gf;
+        const int nv = d->num_nodes, nt = d->num_tets;
+        const int64_t targets = d->nnz + d->unknowns;
+        // the device indexes with these: check them here
+        for (int64_t i = 0; i < 4 * static_cast<int64_t>(nt); ++i)
+            require(d->tet_nodes[i] >= 0 && d->tet_nodes[i] < nv, "aphi_gauge_free_attach: a tet names a missing vertex");
+        for (int t = 0; t < nt; ++t)
+            require(d->tet_body[t] >= 0 && d->tet_body[t] < d->num_bodies, "aphi_gauge_free_attach: a tet's body is out of range");
+        for (int v = 0; v < nv; ++v)
+            require(d->column[v] >= -1 && d->column[v] < d->unknowns, "aphi_gauge_free_attach: a vertex column is out of range");
+        for (int p = 0; p < d->num_ports; ++p)
+            require(d->port_column[p] >= -1 && d->port_column[p] < d->unknowns, "aphi_gauge_free_attach: a port column is out of range");
+        require(d->target_ptr[0] == 0, "aphi_gauge_free_attach: target_ptr must start at 0");
+        for (int64_t t = 0; t < targets; ++t)
+            require(d->target_ptr[t + 1] >= d->target_ptr[t], "aphi_gauge_free_attach: target_ptr must not decrease");
+        const int64_t nc = d->target_ptr[targets];
+        for (int64_t c = 0; c < nc; ++c) {
+            require(d->entry[c] < static_cast<uint32_t>(nt) * 20u, "aphi_gauge_free_attach: a term names a tet outside the mesh");
+            const int k = d->kind[c];
+            require((k & 3) != 3 && ((k & 3) != 2 || (k >> 2) < d->num_values), "aphi_gauge_free_attach: bad term kind");
+            const uint32_t local = d->entry[c] % 20u;
+            require(c < d->target_ptr[d->nnz] ? (k == 0 && local < 16u) : ((k & 3) == 1 ? local >= 16u : ((k & 3) == 2 && local < 16u)),
+                    "aphi_gauge_free_attach: a term does not fit its target");
+            require(d->tet_on[d->entry[c] / 20u] != 0, "aphi_gauge_free_attach: a term names a tet that takes no part");
+        }
+        g.num_nodes = nv;
+        g.num_tets = nt;
+        g.num_bodies = d->num_bodies;
+        g.num_targets = targets;
+        g.sigma.assign(d->body_sigma, d->body_sigma + d->num_bodies);
+        g.eps_r.assign(d->body_eps_r, d->body_eps_r + d->num_bodies);
+        g.port_column.assign(d->port_column, d->port_column + d->num_ports);
+        g.port_value.clear();
+        for (int p = 0; p < d->num_ports; ++p) g.port_value.emplace_back(d->port_value[2 * p], d->port_value[2 * p + 1]);
+        upload(g.coords, d->coords, static_cast<size_t>(nv) * 3);
+        upload(g.tets, d->tet_nodes, static_cast<size_t>(nt) * 4);
+        upload(g.tet_body, d->tet_body, static_cast<size_t>(nt));
+        upload(g.tet_on, d->tet_on, static_cast<size_t>(nt));
+        upload(g.column, d->column, static_cast<size_t>(nv));
+        upload(g.prescribed, d->prescribed, static_cast<size_t>(nv) * 2);
+        upload(g.tptr, d->target_ptr, static_cast<size_t>(targets) + 1);
+        upload(g.entry, d->entry, static_cast<size_t>(nc));
+        upload(g.kind, d->kind, static_cast<size_t>(nc));
+        upload(g.values, d->values, static_cast<size_t>(d->num_values) * 2);
+        g.terms.reserve(static_cast<size_t>(nt) * 20 * sizeof(cd));
+        g.attached = true;
+    });
+}
+
+extern "C" int aphi_gauge_free_refill(aphi_mesh* m, const void* e_tet, double omega) {
+    return aphi_capi::guarded([&] {
+        require(m && e_tet, "aphi_gauge_free_refill: null argument");
+        require(m->gf.attached, "aphi_gauge_free_refill: no gauge-free plan attached");
+        if (omega == 0.0)
+            throw std::invalid_argument("aphi_gauge_free_refill: omega must be nonzero (the symmetric A-Phi solve that "
+                                        "produced E is undefined at DC)");
+        HIPCHK(hipSetDevice(m->ctx->device));
+        auto& g = m->gf;
+        // beta = sigma + j w eps0 eps_r per body, as src/gauge_free_potential.cpp:117 forms it
+        std::vector<cd> beta(static_cast<size_t>(g.num_bodies));
+        for (int b = 0; b < g.num_bodies; ++b)
+            beta[static_cast<size_t>(b)] = cd{g.sigma[static_cast<size_t>(b)], omega * kEps0 * g.eps_r[static_cast<size_t>(b)]};
+        hipStream_t s = m->st();
+        g.beta.reserve(beta.size() * sizeof(cd));
+        HIPCHK(hipMemcpyAsync(g.beta.p, beta.data(), beta.size() * sizeof(cd), hipMemcpyHostToDevice, s));
+        m->A.reserve(static_cast<size_t>(m->nnz) * sizeof(cd));
+        m->rhs.reserve(static_cast<size_t>(m->n) * sizeof(cd));
+        hipLaunchKernelGGL(aphi_dev::k_gf_tet, dim3(blocks_for(g.num_tets)), dim3(256), 0, s, g.num_tets,
+                           g.coords.as<double>(), g.tets.as<int>(), g.tet_body.as<int>(), g.tet_on.as<int>(),
+                           g.beta.as<cd>(), static_cast<const cd*>(e_tet), g.terms.as<cd>());
+        hipLaunchKernelGGL(aphi_dev::k_gf_reduce, dim3(blocks_for(g.num_targets)), dim3(256), 0, s, g.num_targets, m->nnz,
+                           g.tptr.as<int64_t>(), g.entry.as<uint32_t>(), g.kind.as<uint8_t>(), g.values.as<cd>(),
+                           g.terms.as<cd>(), m->A.as<cd>(), m->rhs.as<cd>());
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipGetLastError());
+        m->assembled = true;
+        m->equilibrated = false;
+        m->factored = false;
+        compute_amax(m);
+    });
+}
+
+extern "C" int aphi_gauge_free_potential(aphi_mesh* m, const void* xv, void* phi_vertex, double* port_potential) {
+    return aphi_capi::guarded([&] {
+        require(m && xv && phi_vertex, "aphi_gauge_free_potential: null argument");
+        require(m->gf.attached, "aphi_gauge_free_potential: no gauge-free plan attached");
+        HIPCHK(hipSetDevice(m->ctx->device));
+        auto& g = m->gf;
+        hipStream_t s = m->st();
+        hipLaunchKernelGGL(aphi_dev::k_gf_scatter, dim3(blocks_for(g.num_nodes)), dim3(256), 0, s, g.num_nodes,
+                           g.column.as<int>(), g.prescribed.as<cd>(), static_cast<const cd*>(xv), static_cast<cd*>(phi_vertex));
+        // port_potential (src/gauge_free_potential.cpp:172-176)
+        std::vector<cd> px(g.port_column.size(), cd{0.0, 0.0});
+        for (size_t p = 0; p < g.port_column.size(); ++p)
+            if (g.port_column[p] >= 0)
+                HIPCHK(hipMemcpyAsync(&px[p], static_cast<const cd*>(xv) + g.port_column[p], sizeof(cd),
+                                      hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        HIPCHK(hipGetLastError());
+        if (port_potential)
+            for (size_t p = 0; p < g.port_column.size(); ++p) {
+                const bool fixed = g.port_column[p] < 0;
+                port_potential[2 * p] = fixed ? g.port_value[p].real() : px[p].re;
+                port_potential[2 * p + 1] = fixed ? g.port_value[p].imag() : px[p].im;
+            }
     });
 }
 

@@ -15,6 +15,10 @@ using namespace aphi_amd;
 
 struct aphi_problem {
     bool from_csr = false;  // a bare pattern: no mesh, no assembly plan
+    bool gauge_free = false;  // the P1 system of the gauge-free potential (plan in gf)
+    GaugeFreePlan gf;
+    std::vector<double> gf_prescribed, gf_port_value, gf_values, gf_eps_r;
+    std::vector<int32_t> gf_column, gf_port_column, gf_tet_on;
     CubeSpec spec;
     Mesh mesh;
     BoundProblem bound;
@@ -191,6 +195,69 @@ extern "C" int aphi_problem_create_csr(int32_t n, const int32_t* row_ptr, const 
     });
 }
 
+extern "C" int aphi_problem_create_gauge_free(const aphi_problem* cube, int ordering, aphi_problem** out) {
+    return aphi_capi::guarded([&] {
+        if (!cube || !out) throw std::invalid_argument("aphi_problem_create_gauge_free: null argument");
+        if (cube->from_csr) throw std::invalid_argument("aphi_problem_create_gauge_free: the problem has no mesh");
+        if (ordering < 0 || ordering > 2) throw std::invalid_argument("aphi_problem_create_gauge_free: unknown ordering");
+        *out = nullptr;
+        auto p = std::make_unique<aphi_problem>();
+        p->from_csr = true;
+        p->gauge_free = true;
+        const auto t0 = std::chrono::steady_clock::now();
+        p->gf = build_gauge_free_plan(cube->mesh, cube->bound, cube->dofs);
+        if (p->gf.unknowns == 0) throw std::invalid_argument("aphi_problem_create_gauge_free: the potential has no unknowns");
+        p->pattern = p->gf.pattern;
+        p->dofs.num_total = p->gf.unknowns;
+        // geometry and materials for the device terms
+        p->coords = cube->coords;
+        p->tet_nodes = cube->tet_nodes;
+        p->tet_body = cube->tet_body;
+        p->body_sigma = cube->body_sigma;
+        p->gf_eps_r = cube->body_eps_r;
+        const GaugeFreePlan& g = p->gf;
+        p->gf_column.assign(g.column.begin(), g.column.end());
+        p->gf_port_column.assign(g.port_column.begin(), g.port_column.end());
+        p->gf_tet_on.assign(g.tet_on.begin(), g.tet_on.end());
+        for (const Complex& v : g.prescribed) { p->gf_prescribed.push_back(v.real()); p->gf_prescribed.push_back(v.imag()); }
+        for (const Complex& v : g.values) { p->gf_values.push_back(v.real()); p->gf_values.push_back(v.imag()); }
+        for (const Complex& v : cube->dofs.port_value) { p->gf_port_value.push_back(v.real()); p->gf_port_value.push_back(v.imag()); }
+        p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        finish_symbolic(p.get(), ordering, 0, 0, 0.0);
+        *out = p.release();
+    });
+}
+
+extern "C" int aphi_problem_describe_gauge_free(const aphi_problem* p, aphi_gauge_free_desc* d) {
+    return aphi_capi::guarded([&] {
+        if (!p || !d) throw std::invalid_argument("aphi_problem_describe_gauge_free: null argument");
+        if (!p->gauge_free) throw std::invalid_argument("aphi_problem_describe_gauge_free: not a gauge-free problem");
+        const GaugeFreePlan& g = p->gf;
+        *d = aphi_gauge_free_desc{};
+        d->num_nodes = static_cast<int32_t>(g.column.size());
+        d->num_tets = static_cast<int32_t>(g.tet_on.size());
+        d->num_bodies = static_cast<int32_t>(p->body_sigma.size());
+        d->unknowns = g.unknowns;
+        d->nnz = static_cast<int64_t>(g.pattern.nnz());
+        d->coords = p->coords.data();
+        d->tet_nodes = p->tet_nodes.data();
+        d->tet_body = p->tet_body.data();
+        d->body_sigma = p->body_sigma.data();
+        d->body_eps_r = p->gf_eps_r.data();
+        d->tet_on = p->gf_tet_on.data();
+        d->column = p->gf_column.data();
+        d->prescribed = p->gf_prescribed.data();
+        d->num_ports = static_cast<int32_t>(g.port_column.size());
+        d->port_column = p->gf_port_column.data();
+        d->port_value = p->gf_port_value.data();
+        d->target_ptr = g.target_ptr.data();
+        d->entry = g.entry.data();
+        d->kind = g.kind.data();
+        d->num_values = static_cast<int32_t>(g.values.size());
+        d->values = p->gf_values.data();
+    });
+}
+
 extern "C" void aphi_problem_destroy(aphi_problem* p) { delete p; }
 
 extern "C" int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out) {
@@ -313,6 +380,24 @@ extern "C" int aphi_problem_get_int(const aphi_problem* p, const char* name, int
             return copy_out(p->sp.rel, out, capacity, count);
         } else if (k == "slot_to_panel") {
             return copy_out(p->s2p, out, capacity, count);
+        } else if (k == "phi_state") {
+            for (PhiDof st : p->dofs.phi_state) tmp.push_back(static_cast<int64_t>(st));
+        } else if (k == "phi_port") {
+            return copy_out(p->dofs.phi_port, out, capacity, count);
+        } else if (k == "port_is_fixed") {
+            return copy_out(p->dofs.port_is_fixed, out, capacity, count);
+        } else if (k == "gf_column") {
+            return copy_out(p->gf.column, out, capacity, count);
+        } else if (k == "gf_port_column") {
+            return copy_out(p->gf.port_column, out, capacity, count);
+        } else if (k == "gf_tet_on") {
+            return copy_out(p->gf.tet_on, out, capacity, count);
+        } else if (k == "gf_target_ptr") {
+            return copy_out(p->gf.target_ptr, out, capacity, count);
+        } else if (k == "gf_entry") {
+            return copy_out(p->gf.entry, out, capacity, count);
+        } else if (k == "gf_kind") {
+            return copy_out(p->gf.kind, out, capacity, count);
         } else {
             throw std::invalid_argument("aphi_problem_get_int: unknown array '" + k + "'");
         }
@@ -340,6 +425,10 @@ extern "C" int aphi_problem_get_double(const aphi_problem* p, const char* name, 
             for (const Body& b : p->bound.bodies) { tmp.push_back(b.sigma); tmp.push_back(b.eps_r); tmp.push_back(b.mu_r); }
         } else if (k == "port_value") {
             for (const Complex& v : p->dofs.port_value) { tmp.push_back(v.real()); tmp.push_back(v.imag()); }
+        } else if (k == "gf_prescribed") {
+            src = &p->gf_prescribed;
+        } else if (k == "gf_values") {
+            src = &p->gf_values;
         } else {
             throw std::invalid_argument("aphi_problem_get_double: unknown array '" + k + "'");
         }

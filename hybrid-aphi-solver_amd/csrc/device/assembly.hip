@@ -533,6 +533,40 @@ __global__ void k_add_basis(int n, int nrhs, int nvec, const cd* __restrict__ Z,
 // terminal_current (src/postprocess.cpp:538-544): (J . axis) |V|. Written in the
 // reference's operation order (contraction off), so each term is the
 // reference's bit for bit. Non-conducting tets contribute nothing.
+namespace {
+// E = -j w A - grad Phi at the centroid of tet t (compute_fields' evaluate at
+// kCentroid, src/postprocess.cpp:357-391), in the reference's operation order.
+__device__ __forceinline__ void centroid_e(const TetFieldPlan& tp, const cd* __restrict__ xr, int t, const Geo& G,
+                                           cd minus_jw, cd scale, cd e[3]) {
+    const int* d = tp.dofs + 16 * static_cast<int64_t>(t);
+    const double L[4] = {0.25, 0.25, 0.25, 0.25};
+    cd av[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const cd ae = d[k] >= 0 ? xr[d[k]] : cd{0.0, 0.0};
+        double w[3];
+        whitney(G, k, L, w);
+        const double sg = static_cast<double>(tp.signs[6 * static_cast<int64_t>(t) + k]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) av[c] = cadd(av[c], cscale(ae, w[c] * sg));
+    }
+    cd gp[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+        const int ib = d[6 + b];
+        cd pn;
+        if (ib >= 0) pn = cmul(cscale(scale, tp.phi_coeff[10 * static_cast<int64_t>(t) + b]), xr[ib]);
+        else pn = tp.phi_fixed[10 * static_cast<int64_t>(t) + b];
+        double gr[3];
+        p2grad(G, b, L, gr);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gp[c] = cadd(gp[c], cscale(pn, gr[c]));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[c] = csub(cmul(minus_jw, av[c]), gp[c]);
+}
+}  // namespace
+
 __global__ void __launch_bounds__(256) k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd* __restrict__ x,
                                                      int64_t ldx, cd minus_jw, cd scale, double ax, double ay, double az,
                                                      cd* __restrict__ contrib) {
@@ -544,37 +578,85 @@ __global__ void __launch_bounds__(256) k_tet_current(int num_tets, TetFieldPlan 
     if (sigma > 0.0) {
         Geo G;
         tet_geometry(tp.coords, tp.tets, t, G);
-        const cd* xr = x + static_cast<int64_t>(r) * ldx;
-        const int* d = tp.dofs + 16 * static_cast<int64_t>(t);
-        const double L[4] = {0.25, 0.25, 0.25, 0.25};
-        cd av[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const cd ae = d[k] >= 0 ? xr[d[k]] : cd{0.0, 0.0};
-            double w[3];
-            whitney(G, k, L, w);
-            const double sg = static_cast<double>(tp.signs[6 * static_cast<int64_t>(t) + k]);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) av[c] = cadd(av[c], cscale(ae, w[c] * sg));
-        }
-        cd gp[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-#pragma unroll
-        for (int b = 0; b < 10; ++b) {
-            const int ib = d[6 + b];
-            cd pn;
-            if (ib >= 0) pn = cmul(cscale(scale, tp.phi_coeff[10 * static_cast<int64_t>(t) + b]), xr[ib]);
-            else pn = tp.phi_fixed[10 * static_cast<int64_t>(t) + b];
-            double gr[3];
-            p2grad(G, b, L, gr);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) gp[c] = cadd(gp[c], cscale(pn, gr[c]));
-        }
+        cd e[3];
+        centroid_e(tp, x + static_cast<int64_t>(r) * ldx, t, G, minus_jw, scale, e);
         cd j[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) j[c] = cscale(csub(cmul(minus_jw, av[c]), gp[c]), sigma);
+        for (int c = 0; c < 3; ++c) j[c] = cscale(e[c], sigma);
         out = cscale(cadd(cadd(cscale(j[0], ax), cscale(j[1], ay)), cscale(j[2], az)), G.vol);
     }
     contrib[static_cast<int64_t>(r) * num_tets + t] = out;
+}
+
+// e_tet of compute_fields (src/postprocess.cpp:389-391) for every tet, one per thread.
+__global__ void __launch_bounds__(256) k_tet_efield(int num_tets, TetFieldPlan tp, const cd* __restrict__ x, cd minus_jw,
+                                                    cd scale, cd* __restrict__ e_tet) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tets) return;
+    Geo G;
+    tet_geometry(tp.coords, tp.tets, t, G);
+    cd e[3];
+    centroid_e(tp, x, t, G, minus_jw, scale, e);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e_tet[3 * static_cast<int64_t>(t) + c] = e[c];
+}
+
+// ------------------------------------------------------------------ gauge-free potential
+//
+// Per tet of a participating body (src/gauge_free_potential.cpp:112-151), the
+// sixteen stiffness terms k_ij = (beta |V|) (grad L_i . grad L_j) and the four
+// source terms (beta |V|) (E . grad L_i), in the reference's operation order.
+__global__ void __launch_bounds__(256) k_gf_tet(int num_tets, const double* __restrict__ coords,
+                                                const int* __restrict__ tets, const int* __restrict__ tet_body,
+                                                const int* __restrict__ tet_on, const cd* __restrict__ beta,
+                                                const cd* __restrict__ e_tet, cd* __restrict__ terms) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tets || !tet_on[t]) return;
+    Geo G;
+    tet_geometry(coords, tets, t, G);
+    const cd bv = cscale(beta[tet_body[t]], G.vol);
+    cd* out = terms + 20 * static_cast<int64_t>(t);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) out[4 * i + j] = cscale(bv, dot3(G.g[i], G.g[j]));
+    const cd* E = e_tet + 3 * static_cast<int64_t>(t);
+    const cd e0 = E[0], e1 = E[1], e2 = E[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const cd edotg = cadd(cadd(cscale(e0, G.g[i][0]), cscale(e1, G.g[i][1])), cscale(e2, G.g[i][2]));
+        out[16 + i] = cmul(bv, edotg);
+    }
+}
+
+// One thread per slot / rhs row: its terms summed from zero in plan order (the
+// reference's tet, i, j order): slot += k_ij; rhs -= source_i; rhs -= k_ij * value.
+__global__ void __launch_bounds__(256) k_gf_reduce(int64_t num_targets, int64_t nnz, const int64_t* __restrict__ tptr,
+                                                   const uint32_t* __restrict__ entry, const uint8_t* __restrict__ kind,
+                                                   const cd* __restrict__ values, const cd* __restrict__ terms,
+                                                   cd* __restrict__ A, cd* __restrict__ rhs) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= num_targets) return;
+    cd acc{0.0, 0.0};
+    for (int64_t c = tptr[t]; c < tptr[t + 1]; ++c) {
+        const int k = kind[c];
+        const cd v = terms[entry[c]];
+        if ((k & 3) == 0) acc = cadd(acc, v);
+        else if ((k & 3) == 1) acc = csub(acc, v);
+        else acc = csub(acc, cmul(v, values[k >> 2]));
+    }
+    if (t < nnz) A[t] = acc;
+    else rhs[t - nnz] = acc;
+}
+
+// phi_vertex[v] = x[column[v]], or the prescribed value (src/gauge_free_potential.cpp:167-171)
+__global__ void __launch_bounds__(256) k_gf_scatter(int num_nodes, const int* __restrict__ column,
+                                                    const cd* __restrict__ prescribed, const cd* __restrict__ x,
+                                                    cd* __restrict__ phi) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= num_nodes) return;
+    const int c = column[v];
+    phi[v] = c >= 0 ? x[c] : prescribed[v];
 }
 
 // current[r] = -(sum over conducting tets, in tet order, of the flux terms) / length:

@@ -101,5 +101,13 @@ struct TetFieldPlan {
 __global__ void k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd* x, int64_t ldx, cd minus_jw, cd scale,
                               double ax, double ay, double az, cd* contrib);
 __global__ void k_current_sum(int num_tets, TetFieldPlan tp, const cd* contrib, double length, cd* current);
+__global__ void k_tet_efield(int num_tets, TetFieldPlan tp, const cd* x, cd minus_jw, cd scale, cd* e_tet);
+
+// gauge-free potential (src/gauge_free_potential.cpp:87-177)
+__global__ void k_gf_tet(int num_tets, const double* coords, const int* tets, const int* tet_body, const int* tet_on,
+                         const cd* beta, const cd* e_tet, cd* terms);
+__global__ void k_gf_reduce(int64_t num_targets, int64_t nnz, const int64_t* tptr, const uint32_t* entry,
+                            const uint8_t* kind, const cd* values, const cd* terms, cd* A, cd* rhs);
+__global__ void k_gf_scatter(int num_nodes, const int* column, const cd* prescribed, const cd* x, cd* phi);
 
 }  // namespace aphi_dev

@@ -334,6 +334,35 @@ std::vector<int64_t> slot_to_panel(const SparsityPattern& pattern, const SolverA
 // (8 * sum_j [m_j (m_j + 1) / 2 + 2 m_j], m_j = col_count[j] - 1).
 double ldlt_flops(const std::vector<int>& col_count);
 
+// ------------------------------------------------------------------ gauge-free potential
+//
+// The P1 system of recover_gauge_free_potential (src/gauge_free_potential.cpp:40-177):
+// <beta grad(Phi'), grad(lam)> = -<beta E, grad(lam)> over the mesh vertices,
+// beta = sigma + j w eps0 eps_r. Integer half only: which vertex is which unknown
+// (build_vertex_map, :47-85), the upper-triangle pattern the reference's per-row
+// std::map ends up with, and, for every stored slot and every rhs row, the
+// per-tet terms that land there in the reference's accumulation order (tets
+// ascending, then local vertex i, then local vertex j). The plan assumes
+// w != 0, where a tet takes part unless its body has sigma = 0 and eps_r = 0.
+struct GaugeFreePlan {
+    int unknowns = 0;
+    std::vector<int> column;             // per vertex: unknown, or -1 if eliminated
+    std::vector<Complex> prescribed;     // per vertex: value when eliminated, else 0
+    std::vector<int> port_column;        // per port: unknown, or -1 if the port is fixed
+    std::vector<uint8_t> tet_on;         // per tet: 1 if its body takes part
+    SparsityPattern pattern;             // upper CSR over the unknowns
+    // contributions to target t (slot t < nnz, rhs row t - nnz) are
+    // [target_ptr[t], target_ptr[t+1]); entry = tet * 20 + term, term 0..15 the
+    // stiffness k_ij (i * 4 + j), 16..19 the source term of vertex i;
+    // kind = 0 (slot += k_ij), 1 (rhs -= source_i), 2 | id << 2 (rhs -= k_ij * values[id])
+    std::vector<int64_t> target_ptr;
+    std::vector<uint32_t> entry;
+    std::vector<uint8_t> kind;
+    std::vector<Complex> values;         // distinct nonzero prescribed values
+};
+
+GaugeFreePlan build_gauge_free_plan(const Mesh& mesh, const BoundProblem& bound, const DofMap& dofs);
+
 // log-spaced sweep, src/input_file.cpp:19-50
 std::vector<double> expand_frequency_sweep(double start, double stop, int points, bool logarithmic);
 

@@ -104,10 +104,14 @@ int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out);
  * port_index, row_ptr, col_index, perm, iperm, parent, row_count, col_count,
  * scatter_live, scatter_pos, scatter_offset, scatter_slot, super_first,
  * super_parent, super_row_ptr, super_rows, super_rel, super_level, factor_perm, col_ref,
- * slot_to_panel, tet_dofs (6 edge + 10 phi per tet), body_ptr, body_tets. */
+ * slot_to_panel, tet_dofs (6 edge + 10 phi per tet), body_ptr, body_tets, phi_state
+ * (per P2 node: 0 free, 1 absent, 2 fixed, 3 port, 4 cut), phi_port, port_is_fixed;
+ * of a gauge-free problem also gf_column, gf_port_column, gf_tet_on,
+ * gf_target_ptr, gf_entry, gf_kind. */
 int aphi_problem_get_int(const aphi_problem* p, const char* name, int64_t* out, int64_t capacity, int64_t* count);
 /* Named double arrays: coords (3 per node), tet_phi_fixed (10 complex per tet),
- * body_param (sigma, eps_r, mu_r per body), port_value (complex per port). */
+ * body_param (sigma, eps_r, mu_r per body), port_value (complex per port); of a
+ * gauge-free problem also gf_prescribed (complex per vertex), gf_values. */
 int aphi_problem_get_double(const aphi_problem* p, const char* name, double* out, int64_t capacity, int64_t* count);
 /* The log sweep of src/input_file.cpp:19-50. */
 int aphi_frequency_sweep(double start, double stop, int points, int logarithmic, double* out);
@@ -162,6 +166,52 @@ typedef struct aphi_mesh_desc {
 /* Pointers into the problem's own arrays; valid while the problem lives. */
 int aphi_problem_describe(const aphi_problem* p, aphi_mesh_desc* out);
 
+/* ------------------------------------------------------------------ gauge-free potential
+ * recover_gauge_free_potential (src/gauge_free_potential.cpp:87-177): the P1
+ * system <beta grad Phi', grad lam> = -<beta E, grad lam> over the mesh
+ * vertices, beta = sigma + j w eps0 eps_r, solved through solve_symmetric.
+ *
+ * aphi_problem_create_gauge_free builds, from a cube problem, the vertex map
+ * (build_vertex_map, :47-85), the system's upper-triangle pattern (the
+ * reference's per-row std::map, :108-153) with its ordering / elimination tree /
+ * supernodal plan, and the ordered list of per-tet terms behind every slot and
+ * rhs entry. The result is a pattern-only problem (like aphi_problem_create_csr)
+ * whose describe() gives the device mesh; aphi_gauge_free_attach then hands that
+ * mesh the plan below, aphi_gauge_free_refill assembles values and rhs on the
+ * device from the field E of each tet (compute_fields' e_tet, see
+ * aphi_tet_fields), aphi_solve_symmetric solves, and aphi_gauge_free_potential
+ * maps the solution back to vertices and ports (:167-176). The plan assumes
+ * w != 0 (so every body with sigma or eps_r nonzero takes part); the refill
+ * refuses w == 0. */
+int aphi_problem_create_gauge_free(const aphi_problem* cube, int ordering, aphi_problem** out);
+
+typedef struct aphi_gauge_free_desc {
+    int32_t num_nodes, num_tets, num_bodies, unknowns;
+    int64_t nnz;
+    const double* coords;          /* 3 * num_nodes */
+    const int32_t* tet_nodes;      /* 4 * num_tets */
+    const int32_t* tet_body;       /* num_tets */
+    const double* body_sigma;      /* num_bodies */
+    const double* body_eps_r;      /* num_bodies */
+    const int32_t* tet_on;         /* num_tets: 1 if the tet's body takes part */
+    const int32_t* column;         /* num_nodes: unknown of each vertex, -1 if eliminated */
+    const double* prescribed;      /* 2 * num_nodes: value of an eliminated vertex (complex) */
+    int32_t num_ports;
+    const int32_t* port_column;    /* num_ports: unknown of a current-driven port, -1 if fixed */
+    const double* port_value;      /* 2 * num_ports: DofMap::port_value (complex) */
+    /* contributions to target t (slot t < nnz, rhs row t - nnz): [target_ptr[t],
+     * target_ptr[t+1]); entry = tet * 20 + term (0..15: k_ij at i * 4 + j,
+     * 16..19: source term of vertex i); kind = 0 (slot += k_ij), 1 (rhs -=
+     * source_i), 2 | id << 2 (rhs -= k_ij * values[id]) */
+    const int64_t* target_ptr;     /* nnz + unknowns + 1 */
+    const uint32_t* entry;
+    const uint8_t* kind;
+    int32_t num_values;            /* <= 64 */
+    const double* values;          /* 2 * num_values */
+} aphi_gauge_free_desc;
+
+int aphi_problem_describe_gauge_free(const aphi_problem* p, aphi_gauge_free_desc* out);
+
 typedef struct aphi_context aphi_context;
 typedef struct aphi_mesh aphi_mesh;
 
@@ -203,6 +253,23 @@ const void* aphi_rhs_device(const aphi_mesh* mesh);
 int aphi_terminal_current(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx, double omega, int conditioning,
                           const double* axis, double length, double voltage, double* current, double* resistance,
                           double* inductance);
+
+/* The field E = -j w A - grad Phi of every tet at its centroid, for one solution
+ * x (device, n complex): the e_tet of compute_fields (src/postprocess.cpp:311-391),
+ * in its operation order, so bit for bit the reference's for the same x. e_tet:
+ * device, 3 complex per tet. */
+int aphi_tet_fields(aphi_mesh* mesh, const void* x, double omega, int conditioning, void* e_tet);
+
+/* Hands a mesh made from a gauge-free problem its assembly plan (once). */
+int aphi_gauge_free_attach(aphi_mesh* mesh, const aphi_gauge_free_desc* desc);
+/* The assembly of recover_gauge_free_potential (src/gauge_free_potential.cpp:112-164)
+ * on the device: values and rhs of the P1 system from e_tet (device, 3 complex
+ * per tet of the source mesh), each sum in the reference's order. */
+int aphi_gauge_free_refill(aphi_mesh* mesh, const void* e_tet, double omega);
+/* phi_vertex (device, num_nodes complex) and port_potential (host, 2 * num_ports
+ * doubles) from the solution x (device, unknowns complex), as
+ * src/gauge_free_potential.cpp:167-176 does. */
+int aphi_gauge_free_potential(aphi_mesh* mesh, const void* x, void* phi_vertex, double* port_potential);
 
 /* compute_symmetric_equilibration(a, iterations) + apply (src/equilibration.cpp:94-162).
  * iterations 0 disables scaling. d_out (host, n) may be NULL. */

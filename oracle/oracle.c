@@ -707,65 +707,98 @@ void oracle_factor_sample(int n, const int32_t* row_ptr, const int32_t* col, con
  * them for finite values. out: I_re, I_im, R, L; terms (optional): the flux term
  * of each tet (2 doubles, zero for non-conducting tets). Returns 0 for a bad
  * length, like the reference's refusal. */
+/* E = -j w A - grad Phi at the centroid of tet t: compute_fields' evaluate at
+ * kCentroid (src/postprocess.cpp:311-391), in its operation order. Returns 0
+ * for a degenerate tet; g and vol receive the tet's geometry. */
+static int tet_centroid_e(int t, const double* coords, const int32_t* tet_nodes, const int32_t* tet_dofs,
+                          const double* tet_signs, const double* tet_phi_coeff, const double* tet_phi_fixed,
+                          const double* x, double omega, int conditioning, double e_re[3], double e_im[3],
+                          double g[4][3], double* vol) {
+    const double mjw_re = -0.0, mjw_im = -omega;                    /* -jw */
+    const double sc_re = conditioning == 2 ? 0.0 : 1.0, sc_im = conditioning == 2 ? omega : 0.0;
+    const double L[4] = {0.25, 0.25, 0.25, 0.25};
+    double p[4][3];
+    for (int j = 0; j < 4; ++j)
+        for (int c = 0; c < 3; ++c) p[j][c] = coords[3 * tet_nodes[4 * t + j] + c];
+    if (!tet_geometry(p, g, vol)) return 0;
+    const int32_t* d = tet_dofs + 16 * (int64_t)t;
+    double av_re[3] = {0.0, 0.0, 0.0}, av_im[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 6; ++k) {
+        double ae_re = 0.0, ae_im = 0.0;
+        if (d[k] >= 0) { ae_re = x[2 * d[k]]; ae_im = x[2 * d[k] + 1]; }
+        double w[3];
+        whitney(g, k, L, w);
+        const double sg = tet_signs[6 * (int64_t)t + k];
+        for (int c = 0; c < 3; ++c) {
+            const double wc = w[c] * sg;
+            av_re[c] = av_re[c] + ae_re * wc;
+            av_im[c] = av_im[c] + ae_im * wc;
+        }
+    }
+    double gp_re[3] = {0.0, 0.0, 0.0}, gp_im[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < 10; ++b) {
+        const int ib = d[6 + b];
+        double pn_re, pn_im;
+        if (ib >= 0) {
+            const double cf = tet_phi_coeff[10 * (int64_t)t + b];
+            const double s_re = sc_re * cf, s_im = sc_im * cf;          /* scale * coeff */
+            const double xr = x[2 * ib], xi = x[2 * ib + 1];
+            pn_re = s_re * xr - s_im * xi;                              /* (scale * coeff) * x */
+            pn_im = s_re * xi + s_im * xr;
+        } else {
+            pn_re = tet_phi_fixed[20 * (int64_t)t + 2 * b];
+            pn_im = tet_phi_fixed[20 * (int64_t)t + 2 * b + 1];
+        }
+        double gr[3];
+        p2grad(g, b, L, gr);
+        for (int c = 0; c < 3; ++c) {
+            gp_re[c] = gp_re[c] + pn_re * gr[c];
+            gp_im[c] = gp_im[c] + pn_im * gr[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        e_re[c] = (mjw_re * av_re[c] - mjw_im * av_im[c]) - gp_re[c];
+        e_im[c] = (mjw_re * av_im[c] + mjw_im * av_re[c]) - gp_im[c];
+    }
+    return 1;
+}
+
+/* compute_fields' e_tet (src/postprocess.cpp:311-391) for every tet: 3 complex per tet. */
+int oracle_tet_fields(int num_tets, const double* coords, const int32_t* tet_nodes, const int32_t* tet_dofs,
+                      const double* tet_signs, const double* tet_phi_coeff, const double* tet_phi_fixed,
+                      const double* x, double omega, int conditioning, double* e_out) {
+    for (int t = 0; t < num_tets; ++t) {
+        double e_re[3], e_im[3], g[4][3], vol;
+        if (!tet_centroid_e(t, coords, tet_nodes, tet_dofs, tet_signs, tet_phi_coeff, tet_phi_fixed, x, omega,
+                            conditioning, e_re, e_im, g, &vol))
+            return 0;
+        for (int c = 0; c < 3; ++c) {
+            e_out[6 * (int64_t)t + 2 * c] = e_re[c];
+            e_out[6 * (int64_t)t + 2 * c + 1] = e_im[c];
+        }
+    }
+    return 1;
+}
+
 int oracle_terminal_current(int num_tets, const double* coords, const int32_t* tet_nodes, const int32_t* tet_body,
                             const double* body_sigma, const int32_t* tet_dofs, const double* tet_signs,
                             const double* tet_phi_coeff, const double* tet_phi_fixed, const double* x, double omega,
                             int conditioning, const double* axis, double length, double voltage, double* out,
                             double* terms) {
     if (!(length > 0.0)) return 0;
-    const double mjw_re = -0.0, mjw_im = -omega;                    /* -jw */
-    const double sc_re = conditioning == 2 ? 0.0 : 1.0, sc_im = conditioning == 2 ? omega : 0.0;
-    const double L[4] = {0.25, 0.25, 0.25, 0.25};
     double f_re = 0.0, f_im = 0.0;
     for (int t = 0; t < num_tets; ++t) {
         const double sigma = body_sigma[tet_body[t]];
         if (terms) { terms[2 * t] = 0.0; terms[2 * t + 1] = 0.0; }
         if (!(sigma > 0.0)) continue;
-        double p[4][3], g[4][3], vol;
-        for (int j = 0; j < 4; ++j)
-            for (int c = 0; c < 3; ++c) p[j][c] = coords[3 * tet_nodes[4 * t + j] + c];
-        if (!tet_geometry(p, g, &vol)) return 0;
-        const int32_t* d = tet_dofs + 16 * (int64_t)t;
-        double av_re[3] = {0.0, 0.0, 0.0}, av_im[3] = {0.0, 0.0, 0.0};
-        for (int k = 0; k < 6; ++k) {
-            double ae_re = 0.0, ae_im = 0.0;
-            if (d[k] >= 0) { ae_re = x[2 * d[k]]; ae_im = x[2 * d[k] + 1]; }
-            double w[3];
-            whitney(g, k, L, w);
-            const double sg = tet_signs[6 * (int64_t)t + k];
-            for (int c = 0; c < 3; ++c) {
-                const double wc = w[c] * sg;
-                av_re[c] = av_re[c] + ae_re * wc;
-                av_im[c] = av_im[c] + ae_im * wc;
-            }
-        }
-        double gp_re[3] = {0.0, 0.0, 0.0}, gp_im[3] = {0.0, 0.0, 0.0};
-        for (int b = 0; b < 10; ++b) {
-            const int ib = d[6 + b];
-            double pn_re, pn_im;
-            if (ib >= 0) {
-                const double cf = tet_phi_coeff[10 * (int64_t)t + b];
-                const double s_re = sc_re * cf, s_im = sc_im * cf;          /* scale * coeff */
-                const double xr = x[2 * ib], xi = x[2 * ib + 1];
-                pn_re = s_re * xr - s_im * xi;                              /* (scale * coeff) * x */
-                pn_im = s_re * xi + s_im * xr;
-            } else {
-                pn_re = tet_phi_fixed[20 * (int64_t)t + 2 * b];
-                pn_im = tet_phi_fixed[20 * (int64_t)t + 2 * b + 1];
-            }
-            double gr[3];
-            p2grad(g, b, L, gr);
-            for (int c = 0; c < 3; ++c) {
-                gp_re[c] = gp_re[c] + pn_re * gr[c];
-                gp_im[c] = gp_im[c] + pn_im * gr[c];
-            }
-        }
+        double e_re[3], e_im[3], g[4][3], vol;
+        if (!tet_centroid_e(t, coords, tet_nodes, tet_dofs, tet_signs, tet_phi_coeff, tet_phi_fixed, x, omega,
+                            conditioning, e_re, e_im, g, &vol))
+            return 0;
         double j_re[3], j_im[3];
         for (int c = 0; c < 3; ++c) {
-            const double e_re = (mjw_re * av_re[c] - mjw_im * av_im[c]) - gp_re[c];
-            const double e_im = (mjw_re * av_im[c] + mjw_im * av_re[c]) - gp_im[c];
-            j_re[c] = sigma * e_re;
-            j_im[c] = sigma * e_im;
+            j_re[c] = sigma * e_re[c];
+            j_im[c] = sigma * e_im[c];
         }
         const double t_re = ((j_re[0] * axis[0] + j_re[1] * axis[1]) + j_re[2] * axis[2]) * vol;
         const double t_im = ((j_im[0] * axis[0] + j_im[1] * axis[1]) + j_im[2] * axis[2]) * vol;
@@ -780,4 +813,119 @@ int oracle_terminal_current(int num_tets, const double* coords, const int32_t* t
     out[2] = creal(Z);
     out[3] = cimag(Z) / omega;
     return 1;
+}
+
+/* ------------------------------------------------------------------ gauge-free potential */
+
+/* The vertex map and P1 system of recover_gauge_free_potential
+ * (src/gauge_free_potential.cpp:47-164): build_vertex_map, then per tet of a body
+ * with beta = sigma + j w eps0 eps_r != 0, k_ij = (beta |V|) (grad L_i . grad L_j)
+ * into the upper triangle and -(beta |V|) (E . grad L_i) into the rhs, eliminated
+ * neighbours' -k_ij value too. Rows are kept sorted by column as the reference's
+ * std::map keeps them, each entry starting from zero. phi_state per vertex:
+ * 0 Free, 1 Absent, 2 Fixed, 3 Port, 4 Cut. Returns the number of stored
+ * entries, or -1 when a row needs more than 64 entries or cap_nnz is too small. */
+#define GF_ROW_CAP 64
+int64_t oracle_gauge_free_system(int nv, int nt, const double* coords, const int32_t* tet_nodes, const int32_t* tet_body,
+                                 const double* body_sigma, const double* body_eps_r, const int32_t* phi_state,
+                                 const int32_t* phi_port, int num_ports, const int32_t* port_is_fixed,
+                                 const double* port_value, const double* e_tet, double omega, int64_t cap_nnz,
+                                 int32_t* unknowns_out, int32_t* column, double* prescribed, int32_t* port_column,
+                                 int32_t* row_ptr, int32_t* col, double* values, double* rhs) {
+    const double kEps0 = 8.8541878128e-12;
+    int unknowns = 0;
+    for (int v = 0; v < nv; ++v) { column[v] = -1; prescribed[2 * v] = 0.0; prescribed[2 * v + 1] = 0.0; }
+    for (int p = 0; p < num_ports; ++p) {
+        port_column[p] = -1;
+        if (port_is_fixed[p]) continue;
+        port_column[p] = unknowns++;
+    }
+    for (int v = 0; v < nv; ++v) {
+        if (phi_state[v] == 3) {
+            const int p = phi_port[v];
+            if (p < 0) continue;
+            if (port_is_fixed[p]) { prescribed[2 * v] = port_value[2 * p]; prescribed[2 * v + 1] = port_value[2 * p + 1]; }
+            else column[v] = port_column[p];
+        } else if (phi_state[v] == 0) {
+            column[v] = unknowns++;
+        }
+    }
+    *unknowns_out = unknowns;
+    int* rn = (int*)calloc((size_t)(unknowns > 0 ? unknowns : 1), sizeof(int));
+    int* rc = (int*)malloc(sizeof(int) * (size_t)(unknowns > 0 ? unknowns : 1) * GF_ROW_CAP);
+    double* rv = (double*)malloc(sizeof(double) * 2 * (size_t)(unknowns > 0 ? unknowns : 1) * GF_ROW_CAP);
+    for (int r = 0; r < unknowns; ++r) { rhs[2 * r] = 0.0; rhs[2 * r + 1] = 0.0; }
+    int64_t status = 0;
+    for (int t = 0; t < nt && status == 0; ++t) {
+        const int body = tet_body[t];
+        if (body < 0) continue;
+        const double b_re = body_sigma[body], b_im = omega * kEps0 * body_eps_r[body];
+        if (b_re == 0.0 && b_im == 0.0) continue;
+        double p[4][3], g[4][3], vol;
+        for (int j = 0; j < 4; ++j)
+            for (int c = 0; c < 3; ++c) p[j][c] = coords[3 * tet_nodes[4 * t + j] + c];
+        tet_geometry(p, g, &vol);
+        const double bv_re = b_re * vol, bv_im = b_im * vol;             /* beta * volume */
+        const double* E = e_tet + 6 * (int64_t)t;
+        for (int i = 0; i < 4 && status == 0; ++i) {
+            const int ri = column[tet_nodes[4 * t + i]];
+            if (ri < 0) continue;
+            /* E . grad L_i, complex times real, left to right */
+            const double ed_re = (E[0] * g[i][0] + E[2] * g[i][1]) + E[4] * g[i][2];
+            const double ed_im = (E[1] * g[i][0] + E[3] * g[i][1]) + E[5] * g[i][2];
+            rhs[2 * ri] = rhs[2 * ri] - (bv_re * ed_re - bv_im * ed_im);
+            rhs[2 * ri + 1] = rhs[2 * ri + 1] - (bv_re * ed_im + bv_im * ed_re);
+            for (int j = 0; j < 4; ++j) {
+                const double dg = (g[i][0] * g[j][0] + g[i][1] * g[j][1]) + g[i][2] * g[j][2];
+                const double k_re = bv_re * dg, k_im = bv_im * dg;
+                const int vj = tet_nodes[4 * t + j];
+                const int cj = column[vj];
+                if (cj < 0) {
+                    const double v_re = prescribed[2 * vj], v_im = prescribed[2 * vj + 1];
+                    if (v_re != 0.0 || v_im != 0.0) {
+                        rhs[2 * ri] = rhs[2 * ri] - (k_re * v_re - k_im * v_im);
+                        rhs[2 * ri + 1] = rhs[2 * ri + 1] - (k_re * v_im + k_im * v_re);
+                    }
+                    continue;
+                }
+                if (cj < ri) continue;
+                int* cols = rc + (int64_t)ri * GF_ROW_CAP;
+                double* vals = rv + 2 * (int64_t)ri * GF_ROW_CAP;
+                int q = 0;
+                while (q < rn[ri] && cols[q] < cj) ++q;
+                if (q == rn[ri] || cols[q] != cj) {
+                    if (rn[ri] == GF_ROW_CAP) { status = -1; break; }
+                    for (int z = rn[ri]; z > q; --z) {
+                        cols[z] = cols[z - 1];
+                        vals[2 * z] = vals[2 * z - 2];
+                        vals[2 * z + 1] = vals[2 * z - 1];
+                    }
+                    cols[q] = cj;
+                    vals[2 * q] = 0.0;
+                    vals[2 * q + 1] = 0.0;
+                    ++rn[ri];
+                }
+                vals[2 * q] = vals[2 * q] + k_re;
+                vals[2 * q + 1] = vals[2 * q + 1] + k_im;
+            }
+        }
+    }
+    int64_t nnz = 0;
+    if (status == 0) {
+        row_ptr[0] = 0;
+        for (int r = 0; r < unknowns; ++r) {
+            if (nnz + rn[r] > cap_nnz) { status = -1; break; }
+            for (int q = 0; q < rn[r]; ++q) {
+                col[nnz] = rc[(int64_t)r * GF_ROW_CAP + q];
+                values[2 * nnz] = rv[2 * ((int64_t)r * GF_ROW_CAP + q)];
+                values[2 * nnz + 1] = rv[2 * ((int64_t)r * GF_ROW_CAP + q) + 1];
+                ++nnz;
+            }
+            row_ptr[r + 1] = (int32_t)nnz;
+        }
+    }
+    free(rn);
+    free(rc);
+    free(rv);
+    return status == 0 ? nnz : -1;
 }

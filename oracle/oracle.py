@@ -196,3 +196,58 @@ def terminal_current(problem, x, omega, length, axis=(0.0, 0.0, 1.0), voltage=1.
     if terms:
         return I, out[2], out[3], tr.view(np.complex128)
     return I, out[2], out[3]
+
+
+def tet_fields(problem, x, omega, conditioning=1):
+    """compute_fields' e_tet for one solution x: (num_tets, 3) complex."""
+    nt = int(problem.counts["tets"])
+    arrays = [_f64(problem.doubles("coords")), _i32(problem.ints("tet_nodes")), _i32(problem.ints("tet_dofs")),
+              _f64(problem.ints("tet_edge_signs")), _f64(problem.doubles("tet_phi_coeff")),
+              _f64(problem.doubles("tet_phi_fixed"))]
+    types = [C.c_double, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double]
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros(6 * nt)
+    ok = lib().oracle_tet_fields(C.c_int(nt), *[_p(a, t) for a, t in zip(arrays, types)],
+                                 _p(xx.view(np.float64), C.c_double), C.c_double(omega), C.c_int(conditioning),
+                                 _p(out, C.c_double))
+    if not ok:
+        raise ValueError("tet_fields: a tet is degenerate")
+    return out.view(np.complex128).reshape(nt, 3)
+
+
+def gauge_free_system(problem, e_tet, omega):
+    """The vertex map and P1 system of recover_gauge_free_potential for the cube
+    problem's mesh and the field e_tet ((num_tets, 3) complex). Returns a dict
+    with unknowns, column, prescribed, port_column, row_ptr, col, values, rhs."""
+    nv, nt = int(problem.counts["nodes"]), int(problem.counts["tets"])
+    bp = _f64(problem.doubles("body_param")).reshape(-1, 3)
+    sig, eps = np.ascontiguousarray(bp[:, 0]), np.ascontiguousarray(bp[:, 1])
+    state = _i32(problem.ints("phi_state")[:nv])
+    port = _i32(problem.ints("phi_port")[:nv])
+    fixed = _i32(problem.ints("port_is_fixed"))
+    pval = _f64(problem.doubles("port_value"))
+    ee = np.ascontiguousarray(e_tet, dtype=np.complex128).reshape(-1)
+    cap = 64 * nv + 64
+    unk = np.zeros(1, dtype=np.int32)
+    column = np.zeros(nv, dtype=np.int32)
+    prescribed = np.zeros(2 * nv)
+    port_column = np.zeros(max(fixed.size, 1), dtype=np.int32)
+    row_ptr = np.zeros(nv + 1, dtype=np.int32)
+    col = np.zeros(cap, dtype=np.int32)
+    values = np.zeros(2 * cap)
+    rhs = np.zeros(2 * nv)
+    f = lib().oracle_gauge_free_system
+    f.restype = C.c_int64
+    nnz = f(C.c_int(nv), C.c_int(nt), _p(_f64(problem.doubles("coords")), C.c_double),
+            _p(_i32(problem.ints("tet_nodes")), C.c_int32), _p(_i32(problem.ints("tet_body")), C.c_int32),
+            _p(sig, C.c_double), _p(eps, C.c_double), _p(state, C.c_int32), _p(port, C.c_int32),
+            C.c_int(fixed.size), _p(fixed, C.c_int32), _p(pval, C.c_double), _p(ee.view(np.float64), C.c_double),
+            C.c_double(omega), C.c_int64(cap), _p(unk, C.c_int32), _p(column, C.c_int32), _p(prescribed, C.c_double),
+            _p(port_column, C.c_int32), _p(row_ptr, C.c_int32), _p(col, C.c_int32), _p(values, C.c_double),
+            _p(rhs, C.c_double))
+    if nnz < 0:
+        raise ValueError("gauge_free_system: a row exceeds the oracle's capacity")
+    n = int(unk[0])
+    return dict(unknowns=n, column=column, prescribed=prescribed.view(np.complex128),
+                port_column=port_column[:fixed.size], row_ptr=row_ptr[:n + 1], col=col[:nnz],
+                values=values[:2 * nnz].view(np.complex128).copy(), rhs=rhs[:2 * n].view(np.complex128).copy())

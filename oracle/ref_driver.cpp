@@ -9,7 +9,8 @@
 //
 // usage: ref_driver OUTDIR n side bar_i0 bar_i1 bar_j0 bar_j1 sigma freq_hz mode
 //   mode 0: symbolic only (topology, gauge, DOF map, pattern, scatter map, ordering)
-//   mode 1: + refill at freq_hz, equilibration, factorization, solve, R and L
+//   mode 1: + refill at freq_hz, equilibration, factorization, solve, R and L,
+//           the fields' e_tet and the gauge-free potential recovered from them
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -22,6 +23,7 @@
 #include "aphi_solver/dof_map.hpp"
 #include "aphi_solver/equilibration.hpp"
 #include "aphi_solver/factorization.hpp"
+#include "aphi_solver/gauge_free_potential.hpp"
 #include "aphi_solver/mesh.hpp"
 #include "aphi_solver/postprocess.hpp"
 #include "aphi_solver/problem.hpp"
@@ -193,17 +195,24 @@ int main(int argc, char** argv) {
         sol.omega = omega;
         const FieldOutput fields = compute_fields(mesh, bound, dofs, sol);
         const Z I = terminal_current(mesh, fields, Vec3(0.0, 0.0, 1.0), side);
+        std::vector<Z> e_flat;
+        for (const auto& e : fields.e_tet) e_flat.insert(e_flat.end(), e.begin(), e.end());
+        dump("e_tet", e_flat);
+        const GaugeFreePotential gfp = recover_gauge_free_potential(mesh, bound, dofs, fields, omega);
+        dump("gf_phi_vertex", gfp.phi_vertex);
+        dump("gf_port_potential", gfp.port_potential);
         const Z Zin = impedance(1.0, I);
         std::fprintf(meta,
                      ", \"freq\": %.17g, \"omega\": %.17g, \"scatter_map_bitwise_equal\": %s, \"factor_ok\": %s, "
                      "\"smallest_pivot\": %.17g, \"largest_pivot\": %.17g, \"largest_multiplier\": %.17g, "
                      "\"factor_nnz\": %zu, \"factorize_ms\": %.6g, \"solve_ok\": %s, \"backward_error\": %.17g, "
                      "\"residual\": %.17g, \"equilibration_min\": %.17g, \"equilibration_max\": %.17g, "
-                     "\"I_re\": %.17g, \"I_im\": %.17g, \"R\": %.17g, \"L\": %.17g",
+                     "\"I_re\": %.17g, \"I_im\": %.17g, \"R\": %.17g, \"L\": %.17g, \"gf_unknowns\": %d, "
+                     "\"gf_converged\": %s, \"gf_residual\": %.17g",
                      freq, omega, map_equal ? "true" : "false", fok ? "true" : "false", fs.smallest_pivot,
                      fs.largest_pivot, fs.largest_multiplier, fs.nnz, fs.milliseconds, ok ? "true" : "false",
                      rep.backward_error, rep.residual, rep.equilibration_min, rep.equilibration_max, I.real(), I.imag(),
-                     Zin.real(), Zin.imag() / omega);
+                     Zin.real(), Zin.imag() / omega, gfp.unknowns, gfp.converged ? "true" : "false", gfp.residual);
     }
     std::fprintf(meta, "}\n");
     std::fclose(meta);

@@ -55,10 +55,11 @@ def run(case, outdir):
             arrays[a] = v.astype(np.int32)
     if mode >= 1:
         for a, dt in (("values", np.complex128), ("rhs", np.complex128), ("equil_d", np.float64),
-                      ("diag", np.complex128), ("x", np.complex128)):
+                      ("diag", np.complex128), ("x", np.complex128), ("e_tet", np.complex128),
+                      ("gf_phi_vertex", np.complex128), ("gf_port_potential", np.complex128)):
             v = np.fromfile(os.path.join(outdir, a + ".bin"), dtype=dt)
             digests[a] = digest(v)
-            if full or a in ("x", "diag", "equil_d", "rhs"):
+            if full or a in ("x", "diag", "equil_d", "rhs", "gf_phi_vertex", "gf_port_potential"):
                 arrays[a] = v
     meta["sha256"] = digests
     return meta, arrays
