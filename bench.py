@@ -32,6 +32,9 @@ MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_FP64_PEAK_TFLOPS = 78.6
 
 
+RHS_SEED = 20261203
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -99,9 +102,11 @@ def main():
                           "milliseconds": st["milliseconds"], "ok": bool(st["ok"])}))
         return
     n = prob.n
-    gen = torch.Generator(device="cuda").manual_seed(20261203)
-    B = torch.complex(torch.randn(len(freqs), a.nrhs, n, generator=gen, device="cuda", dtype=torch.float64),
-                      torch.randn(len(freqs), a.nrhs, n, generator=gen, device="cuda", dtype=torch.float64))
+    # right-hand sides of frequency f: the library's counter-based generator
+    # (Philox-4x32-10 + Box-Muller, the same bits on host and device), stream
+    # RHS_SEED, normals from offset 2 f nrhs n on
+    B = torch.stack([A.random_complex_rhs(ctx, RHS_SEED, a.nrhs, n, offset=2 * fi * a.nrhs * n)
+                     for fi in range(len(freqs))])
     X = torch.empty(a.nrhs, n, dtype=torch.complex128, device="cuda")
     # the sweep in ascending frequency, as the reference's solve_mesh loop runs it
     # (tools/solve_mesh.cpp:207); --steps below 64 takes every (64/steps)-th point
@@ -227,7 +232,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 (FP32 MFMA) factor + c128 GMRES refinement; c128 factor where FP32 is predicted slower",
-        "data": "synthetic: seeded N(0,1) complex right-hand sides; generated Kuhn-cube mesh",
+        "data": "synthetic: seeded N(0,1) complex right-hand sides (Philox-4x32-10, seed %d); generated Kuhn-cube mesh" % RHS_SEED,
+        "seed": RHS_SEED,
         "config": {"workload": f"{a.n}^3-hex Kuhn cube, 6x3 off-centre bar, 64 log frequencies 1 Hz-100 MHz swept in "
                                f"ascending order ({len(order)} timed), {a.nrhs} RHS per point, FP32-MFMA LDL^T + "
                                f"FP64 GMRES-IR to 1e-12",

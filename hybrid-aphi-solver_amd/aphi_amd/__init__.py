@@ -25,7 +25,7 @@ __all__ = [
     "LogicError", "DeviceError", "CubeConfig", "Problem", "Context", "Mesh", "frequency_sweep",
     "APHI_FP32", "APHI_FP64", "APHI_NATURAL", "APHI_ROW_SCALED", "APHI_SCALED_PHI",
     "APHI_ORDER_AMD", "APHI_ORDER_NATURAL", "APHI_ORDER_RCM", "lib", "PrecisionPolicy",
-    "recover_gauge_free_potential",
+    "recover_gauge_free_potential", "philox4x32_10", "random_normal_host", "random_normal", "random_complex_rhs",
 ]
 
 
@@ -72,6 +72,39 @@ def _check(status):
     if status in (_capi.APHI_DEVICE_ERROR, _capi.APHI_OUT_OF_MEMORY):
         raise DeviceError(msg)
     raise RuntimeError(msg)
+
+
+def philox4x32_10(counter, key):
+    """One Philox-4x32-10 block (four 32-bit words) of counter[4] and key[2]."""
+    c = (C.c_uint32 * 4)(*[int(v) & 0xffffffff for v in counter])
+    k = (C.c_uint32 * 2)(*[int(v) & 0xffffffff for v in key])
+    o = (C.c_uint32 * 4)()
+    _check(lib().aphi_philox4x32_10(c, k, o))
+    return [int(v) for v in o]
+
+
+def random_normal_host(seed, count, offset=0):
+    """Normal numbers offset .. offset + count - 1 of stream ``seed`` (numpy float64)."""
+    out = np.zeros(int(count), dtype=np.float64)
+    _check(lib().aphi_random_normal_host(int(seed), int(offset), int(count), out.ctypes.data_as(_capi.DP)))
+    return out
+
+
+def random_normal(ctx, seed, shape, offset=0):
+    """The same numbers as random_normal_host, drawn on the context's device into a
+    float64 tensor of ``shape``."""
+    import torch
+    t = torch.empty(shape, dtype=torch.float64, device=f"cuda:{ctx.device}")
+    _check(lib().aphi_random_normal(ctx._h, int(seed), int(offset), t.numel(), C.c_void_p(t.data_ptr())))
+    ctx.synchronize()
+    return t
+
+
+def random_complex_rhs(ctx, seed, nrhs, n, offset=0):
+    """nrhs x n complex right-hand sides with iid N(0,1) real and imaginary parts:
+    entry (r, i) takes normals offset + 2(r n + i) (real) and the next (imaginary)."""
+    import torch
+    return torch.view_as_complex(random_normal(ctx, seed, (nrhs, n, 2), offset))
 
 
 def frequency_sweep(start, stop, points, logarithmic=True):

@@ -27,6 +27,7 @@ HOST_SOURCES = [
 ]
 HIP_SOURCES = [
     "device/assembly.hip",
+    "device/random.hip",
     "device/factor.hip",
     "device/solve.hip",
     "capi/device_capi.hip",

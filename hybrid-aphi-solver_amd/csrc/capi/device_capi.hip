@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../device/factor.hpp"
+#include "../device/philox.hpp"
 #include "status.hpp"
 
 using aphi_capi::DeviceError;
@@ -1199,6 +1200,37 @@ extern "C" int aphi_terminal_current(aphi_mesh* m, int nrhs, const void* xv, int
             if (resistance) resistance[r] = z.real();
             if (inductance) inductance[r] = z.imag() / omega;
         }
+    });
+}
+
+extern "C" int aphi_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
+    return aphi_capi::guarded([&] {
+        require(ctr && key && out, "aphi_philox4x32_10: null argument");
+        const aphi_rng::Block b = aphi_rng::philox4x32_10(ctr, key);
+        for (int i = 0; i < 4; ++i) out[i] = b.w[i];
+    });
+}
+
+extern "C" int aphi_random_normal_host(uint64_t seed, uint64_t offset, int64_t count, double* out) {
+    return aphi_capi::guarded([&] {
+        require(count >= 0 && (count == 0 || out), "aphi_random_normal_host: bad arguments");
+        for (int64_t i = 0; i < count; ++i) {
+            const uint64_t k = offset + static_cast<uint64_t>(i);
+            double z0, z1;
+            aphi_rng::normal_pair(seed, k >> 1, z0, z1);
+            out[i] = (k & 1) ? z1 : z0;
+        }
+    });
+}
+
+extern "C" int aphi_random_normal(aphi_context* ctx, uint64_t seed, uint64_t offset, int64_t count, void* out) {
+    return aphi_capi::guarded([&] {
+        require(ctx && count >= 0 && (count == 0 || out), "aphi_random_normal: bad arguments");
+        if (count == 0) return;
+        HIPCHK(hipSetDevice(ctx->device));
+        hipLaunchKernelGGL(aphi_dev::k_random_normal, dim3(blocks_for(count)), dim3(256), 0, ctx->stream, seed, offset,
+                           count, static_cast<double*>(out));
+        HIPCHK(hipGetLastError());
     });
 }
 

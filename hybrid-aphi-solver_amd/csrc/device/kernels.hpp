@@ -103,6 +103,9 @@ __global__ void k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd*
 __global__ void k_current_sum(int num_tets, TetFieldPlan tp, const cd* contrib, double length, cd* current);
 __global__ void k_tet_efield(int num_tets, TetFieldPlan tp, const cd* x, cd minus_jw, cd scale, cd* e_tet);
 
+// random.hip: normal variates of the counter-based generator (philox.hpp)
+__global__ void k_random_normal(uint64_t seed, uint64_t offset, int64_t count, double* out);
+
 // gauge-free potential (src/gauge_free_potential.cpp:87-177)
 __global__ void k_gf_tet(int num_tets, const double* coords, const int* tets, const int* tet_body, const int* tet_on,
                          const cd* beta, const cd* e_tet, cd* terms);

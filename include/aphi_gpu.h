@@ -254,6 +254,21 @@ int aphi_terminal_current(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx,
                           const double* axis, double length, double voltage, double* current, double* resistance,
                           double* inductance);
 
+/* ------------------------------------------------------------------ seeded draws
+ * One counter-based generator for every seeded input (right-hand sides, problem
+ * draws), the same bits on the host and on the device: Philox-4x32-10 (Salmon et
+ * al., SC 2011) and a Box-Muller transform whose log, sqrt, sin and cos are
+ * written in IEEE additions, multiplications and divisions. Normal number k of
+ * stream `seed` uses counter block k / 2. No counterpart in the reference, whose
+ * inputs come from files; this replaces its callers' ad-hoc draws for the
+ * synthetic workloads. */
+/* One Philox-4x32-10 block: out = philox(ctr[4], key[2]). */
+int aphi_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+/* out[i] = normal number offset + i of stream seed (host memory). */
+int aphi_random_normal_host(uint64_t seed, uint64_t offset, int64_t count, double* out);
+/* The same into device memory, on the context's stream. */
+int aphi_random_normal(aphi_context* ctx, uint64_t seed, uint64_t offset, int64_t count, void* out);
+
 /* The field E = -j w A - grad Phi of every tet at its centroid, for one solution
  * x (device, n complex): the e_tet of compute_fields (src/postprocess.cpp:311-391),
  * in its operation order, so bit for bit the reference's for the same x. e_tet:

@@ -125,3 +125,4 @@ def test_grid_matrix_fp32_refined(gpu_context):
             assert np.linalg.norm(Xg[r] - ox[r]) / np.linalg.norm(ox[r]) <= bound
     finally:
         m.close()
+
