@@ -26,6 +26,7 @@ __all__ = [
     "APHI_FP32", "APHI_FP64", "APHI_NATURAL", "APHI_ROW_SCALED", "APHI_SCALED_PHI",
     "APHI_ORDER_AMD", "APHI_ORDER_NATURAL", "APHI_ORDER_RCM", "lib", "PrecisionPolicy",
     "recover_gauge_free_potential", "philox4x32_10", "random_normal_host", "random_normal", "random_complex_rhs",
+    "random_uniform_host", "batched_cube_configs",
 ]
 
 
@@ -88,6 +89,34 @@ def random_normal_host(seed, count, offset=0):
     out = np.zeros(int(count), dtype=np.float64)
     _check(lib().aphi_random_normal_host(int(seed), int(offset), int(count), out.ctypes.data_as(_capi.DP)))
     return out
+
+
+def random_uniform_host(seed, count, offset=0):
+    """Uniform numbers in [0, 1), offset .. offset + count - 1 of stream ``seed``."""
+    out = np.zeros(int(count), dtype=np.float64)
+    _check(lib().aphi_random_uniform_host(int(seed), int(offset), int(count), out.ctypes.data_as(_capi.DP)))
+    return out
+
+
+def batched_cube_configs(seed, count):
+    """The seeded problems of the batched workload: for problem i, uniforms
+    8i .. 8i + 7 of stream ``seed`` give the cube side U[0.05, 0.2] m, hexes per
+    edge U{4..12}, a conductor block of 1-4 hexes per side (at most n - 1, so air
+    remains) spanning z at a uniform position, sigma log-uniform in [1e6, 6e7] S/m and a frequency log-uniform in
+    [1 Hz, 1 MHz]. Returns (configs, frequencies)."""
+    import math
+    u = random_uniform_host(seed, 8 * int(count))
+    configs, freqs = [], []
+    for i in range(int(count)):
+        v = u[8 * i: 8 * i + 8]
+        n = 4 + min(int(v[1] * 9), 8)
+        # at most n - 1 hexes per side, so that air remains (a body needs tets)
+        bx, by = min(1 + min(int(v[2] * 4), 3), n - 1), min(1 + min(int(v[3] * 4), 3), n - 1)
+        i0, j0 = min(int(v[4] * (n - bx + 1)), n - bx), min(int(v[5] * (n - by + 1)), n - by)
+        sigma = math.exp(math.log(1e6) + (math.log(6e7) - math.log(1e6)) * v[6])
+        configs.append(CubeConfig(n=n, side=0.05 + 0.15 * v[0], bar=(i0, i0 + bx, j0, j0 + by), sigma=sigma))
+        freqs.append(math.exp(math.log(1e6) * v[7]))
+    return configs, np.array(freqs)
 
 
 def random_normal(ctx, seed, shape, offset=0):
@@ -153,6 +182,23 @@ class Problem:
         _check(lib().aphi_problem_create_cube(C.byref(spec), C.byref(h)))
         self._h = h
         self._read_counts()
+
+    @classmethod
+    def cubes(cls, configs, threads=0):
+        """Many cube problems built on up to ``threads`` host threads (0: one per
+        hardware thread), each the same as Problem(config)."""
+        configs = list(configs)
+        specs = (_capi.CubeSpec * max(len(configs), 1))(*[c.spec() for c in configs])
+        hs = (C.c_void_p * max(len(configs), 1))()
+        _check(lib().aphi_problem_create_cubes(specs, len(configs), int(threads), hs))
+        out = []
+        for c, h in zip(configs, hs):
+            p = cls.__new__(cls)
+            p.config = c
+            p._h = C.c_void_p(h)
+            p._read_counts()
+            out.append(p)
+        return out
 
     @classmethod
     def from_csr(cls, n, row_ptr, col_index, ordering=APHI_ORDER_AMD, max_factor_nnz=0, relax_small=0,

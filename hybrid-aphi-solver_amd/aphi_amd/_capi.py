@@ -116,6 +116,8 @@ SIGNATURES = {
     "aphi_tet_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p]),
     "aphi_philox4x32_10": (C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "aphi_random_normal_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, DP]),
+    "aphi_random_uniform_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, DP]),
+    "aphi_problem_create_cubes": (C.c_int, [C.POINTER(CubeSpec), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "aphi_random_normal": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, C.c_void_p]),
     "aphi_gauge_free_attach": (C.c_int, [C.c_void_p, C.POINTER(GaugeFreeDesc)]),
     "aphi_gauge_free_refill": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double]),

@@ -1223,6 +1223,18 @@ extern "C" int aphi_random_normal_host(uint64_t seed, uint64_t offset, int64_t c
     });
 }
 
+extern "C" int aphi_random_uniform_host(uint64_t seed, uint64_t offset, int64_t count, double* out) {
+    return aphi_capi::guarded([&] {
+        require(count >= 0 && (count == 0 || out), "aphi_random_uniform_host: bad arguments");
+        for (int64_t i = 0; i < count; ++i) {
+            const uint64_t k = offset + static_cast<uint64_t>(i);
+            double u0, u1;
+            aphi_rng::uniform_pair(seed, k >> 1, u0, u1);
+            out[i] = (k & 1) ? u1 : u0;
+        }
+    });
+}
+
 extern "C" int aphi_random_normal(aphi_context* ctx, uint64_t seed, uint64_t offset, int64_t count, void* out) {
     return aphi_capi::guarded([&] {
         require(ctx && count >= 0 && (count == 0 || out), "aphi_random_normal: bad arguments");

@@ -2,7 +2,9 @@
 // (mesh -> binding -> DOF map -> pattern -> scatter map -> assembly plan ->
 // ordering -> elimination tree -> supernodes) and expose its integer arrays.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <memory>
 #include <cstring>
 #include <string>
@@ -152,6 +154,45 @@ extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem
             p->col_ref[static_cast<size_t>(j)] = p->analysis.permutation.iperm[static_cast<size_t>(p->sp.perm[static_cast<size_t>(j)])];
         p->super_level.assign(p->sp.level.begin(), p->sp.level.end());
         *out = p.release();
+    });
+}
+
+extern "C" int aphi_problem_create_cubes(const aphi_cube_spec* specs, int count, int threads, aphi_problem** out) {
+    return aphi_capi::guarded([&] {
+        if (count < 0 || (count > 0 && (!specs || !out))) throw std::invalid_argument("aphi_problem_create_cubes: bad arguments");
+        for (int i = 0; i < count; ++i) out[i] = nullptr;
+        if (count == 0) return;
+        int nt = threads > 0 ? threads : static_cast<int>(std::thread::hardware_concurrency());
+        nt = std::max(1, std::min(nt, count));
+        // problems are independent and the build keeps no shared state: the
+        // workers take the next index until none is left; the first failure (by
+        // index) is reported and every problem built is released
+        std::atomic<int> next{0};
+        std::vector<int> status(static_cast<size_t>(count), APHI_OK);
+        std::vector<std::string> message(static_cast<size_t>(count));
+        auto work = [&] {
+            for (int i = next++; i < count; i = next++) {
+                aphi_problem* p = nullptr;
+                status[static_cast<size_t>(i)] = aphi_problem_create_cube(&specs[i], &p);
+                if (status[static_cast<size_t>(i)] == APHI_OK) out[i] = p;
+                else message[static_cast<size_t>(i)] = aphi_last_error();
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+        for (int i = 0; i < count; ++i)
+            if (status[static_cast<size_t>(i)] != APHI_OK) {
+                for (int j = 0; j < count; ++j) {
+                    delete out[j];
+                    out[j] = nullptr;
+                }
+                const std::string msg = "problem " + std::to_string(i) + ": " + message[static_cast<size_t>(i)];
+                if (status[static_cast<size_t>(i)] == APHI_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+                if (status[static_cast<size_t>(i)] == APHI_LOGIC_ERROR) throw std::logic_error(msg);
+                throw std::runtime_error(msg);
+            }
     });
 }
 

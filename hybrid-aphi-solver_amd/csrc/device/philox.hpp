@@ -146,6 +146,17 @@ __host__ __device__ inline void det_sincos_2pi(double u, double& sn, double& cs)
     }
 }
 
+// uniforms 2b and 2b + 1 of stream `seed`, in [0, 1): the 53-bit words (w1:w0 >> 11)
+// and (w3:w2 >> 11) of counter block b, times 2^-53 (exact)
+__host__ __device__ inline void uniform_pair(uint64_t seed, uint64_t b, double& u0, double& u1) {
+    const uint32_t ctr[4] = {static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), 0u, 0u};
+    const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+    const Block r = philox4x32_10(ctr, key);
+    const double inv53 = 1.0 / 9007199254740992.0;
+    u0 = static_cast<double>(((static_cast<uint64_t>(r.w[1]) << 32) | r.w[0]) >> 11) * inv53;
+    u1 = static_cast<double>(((static_cast<uint64_t>(r.w[3]) << 32) | r.w[2]) >> 11) * inv53;
+}
+
 // normals 2b and 2b + 1 of stream `seed`
 __host__ __device__ inline void normal_pair(uint64_t seed, uint64_t b, double& z0, double& z1) {
 #pragma clang fp contract(off)

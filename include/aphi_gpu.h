@@ -84,6 +84,11 @@ typedef struct aphi_problem_counts {
 } aphi_problem_counts;
 
 int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem** out);
+/* count cube problems at once, built on up to `threads` host threads (0: one per
+ * hardware thread): the batched and serving workloads' host symbolic phase, each
+ * problem exactly what aphi_problem_create_cube gives for its spec. On a refusal
+ * every out[i] is NULL and the message names the first failing index. */
+int aphi_problem_create_cubes(const aphi_cube_spec* specs, int count, int threads, aphi_problem** out);
 /* A problem from a caller's matrix pattern alone: the upper-triangle CSR of an
  * n x n complex-symmetric matrix (row_ptr n+1 entries from 0, columns strictly
  * ascending per row, none below the diagonal), refused with the messages of
@@ -266,7 +271,10 @@ int aphi_terminal_current(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx,
 int aphi_philox4x32_10(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 /* out[i] = normal number offset + i of stream seed (host memory). */
 int aphi_random_normal_host(uint64_t seed, uint64_t offset, int64_t count, double* out);
-/* The same into device memory, on the context's stream. */
+/* Uniform number offset + i of stream seed in [0, 1), host memory: the 53-bit
+ * words (w1:w0 >> 11) and (w3:w2 >> 11) of counter block k / 2, times 2^-53. */
+int aphi_random_uniform_host(uint64_t seed, uint64_t offset, int64_t count, double* out);
+/* The normals into device memory, on the context's stream. */
 int aphi_random_normal(aphi_context* ctx, uint64_t seed, uint64_t offset, int64_t count, void* out);
 
 /* The field E = -j w A - grad Phi of every tet at its centroid, for one solution
