@@ -60,6 +60,10 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling (0 skips)")
+    ap.add_argument("--serving-requests", type=int, default=32,
+                    help="requests of the serving run (mixed sizes 8..32, Poisson arrivals; 0 skips)")
+    ap.add_argument("--serving-rate", type=float, default=2.0, help="arrival rate of the serving run (requests/s)")
+    ap.add_argument("--serving-slots", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (HBM counter calibration) and exit")
     ap.add_argument("--factor-only", choices=("fp32", "fp64"), default=None,
@@ -220,6 +224,28 @@ def main():
                          f"extrapolated at that rate: {est:.0f} s per frequency point (solve excluded)"}
         log("cpu baseline:", cpu["sample"])
 
+    serving = None
+    if a.serving_requests > 0:
+        # serving mode: a seeded Poisson stream of solve requests (meshes of 8..32
+        # hexes per edge repeating with p = 0.7, 1 Hz..100 MHz, 1..8 RHS, FP32 +
+        # refinement to 1e-12) with continuous batching over slots and a 200 GB HBM
+        # budget; the sweep's mesh is released first
+        from aphi_amd.serving import Server, serving_requests
+        mesh.close()
+        del B, X
+        torch.cuda.empty_cache()
+        reqs = serving_requests(RHS_SEED, a.serving_requests, a.serving_rate)
+        srv = Server(slots=a.serving_slots, hbm_budget=200e9)
+        try:
+            _, serving = srv.run(reqs)
+        finally:
+            srv.close()
+        serving.update(rate_per_s=a.serving_rate, seed=RHS_SEED,
+                       workload="Poisson stream, meshes n in {8,16,24,32} (p = 0.7 repeat), 1 Hz-100 MHz, 1-8 RHS, "
+                                "FP32-MFMA + FP64 refinement to 1e-12")
+        log(f"serving: {serving['requests']} requests, {serving['solves_per_s']:.1f} solves/s, "
+            f"p50 {serving['p50_ms']:.0f} ms, p99 {serving['p99_ms']:.0f} ms, evictions {serving['evictions']}")
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -265,6 +291,7 @@ def main():
                      "flops_per_factor": st32["update_flops"]},
         "roofline_fp64": {"achieved": ach64, "peak": MFMA_FP64_PEAK_TFLOPS, "frac": ach64 / MFMA_FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
     mesh.close()

@@ -332,6 +332,11 @@ class Mesh:
     def __del__(self):
         self.close()
 
+    def set_solve_graphs(self, on=True):
+        """Graph capture and replay of the solve's level loops (off for meshes used
+        beside other threads' GPU work)."""
+        _check(lib().aphi_mesh_set_solve_graphs(self._h, int(on)))
+
     def set_profiling(self, on=True):
         _check(lib().aphi_mesh_set_profiling(self._h, int(on)))
 

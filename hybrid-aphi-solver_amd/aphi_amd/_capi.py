@@ -128,6 +128,7 @@ SIGNATURES = {
     "aphi_mesh_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(C.c_void_p)]),
     "aphi_mesh_destroy": (None, [C.c_void_p]),
     "aphi_mesh_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "aphi_mesh_set_solve_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "aphi_mesh_device_bytes": (C.c_int64, [C.c_void_p]),
     "aphi_refill": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "aphi_system_download": (C.c_int, [C.c_void_p, DP, DP]),
@@ -154,6 +155,9 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"libaphi_amd.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+    # PyTorch (the device allocator) ships its own HIP runtime: load it first, so
+    # the library binds to that same runtime instead of starting a second one
+    import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -72,10 +72,14 @@ struct DevBuf {
     template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// host -> device on the library's own stream (never the legacy default stream,
+// whose implicit ordering would tie it to other threads' streams), complete on return
 template <typename T>
-void upload(DevBuf& b, const T* src, size_t n) {
+void upload(hipStream_t s, DevBuf& b, const T* src, size_t n) {
     b.reserve(n * sizeof(T));
-    if (n) HIPCHK(hipMemcpy(b.p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    if (!n) return;
+    HIPCHK(hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
 }
 
 size_t csize(int precision) { return precision == APHI_FP64 ? sizeof(cd) : sizeof(aphi_dev::cf); }
@@ -96,6 +100,7 @@ struct aphi_context {
     hipStream_t side = nullptr;  // second stream of the factorization (default priority)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t main_done = nullptr, side_done = nullptr;  // ordering only (no timing)
+    hipEvent_t caller = nullptr;  // the caller's legacy-stream work, joined at entry
 };
 
 // The solve's per-level launches (about 1,200 at 465k unknowns) captured once
@@ -134,6 +139,7 @@ struct aphi_mesh {
     SolveGraph sgraph;
     bool assembled = false, equilibrated = false, factored = false;
     bool has_plan = true;  // false for a problem made from a bare pattern (no refill)
+    bool solve_graphs_on = true;  // aphi_mesh_set_solve_graphs
     // gauge-free potential plan (aphi_gauge_free_attach), on a bare-pattern mesh
     struct GaugeFree {
         bool attached = false;
@@ -381,6 +387,16 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
 
 // APHI_SOLVE_GRAPHS=0 launches the solve's level loops one by one instead of
 // replaying their captured graph
+// The library's streams do not wait for the legacy default stream on their own
+// (they are non-blocking). An entry point that reads or writes caller device
+// memory first orders its stream after the work the caller has queued there (a
+// framework filling b, reading an earlier x), and synchronizes before it returns.
+void after_caller(aphi_context* c) {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventRecord(c->caller, nullptr));
+    HIPCHK(hipStreamWaitEvent(c->stream, c->caller, 0));
+}
+
 bool solve_graphs() {
     const char* e = std::getenv("APHI_SOLVE_GRAPHS");
     return !(e && e[0] == '0');
@@ -447,7 +463,7 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
                                          m->P.p, m->dinv.p, m->XT256.p, m->Yv.p, m->X.p, n, m->bpart.p, m->tsum.p);
     }
     };
-    if (solve_graphs()) {
+    if (solve_graphs() && m->solve_graphs_on) {
         const int prec = std::is_same<T, float>::value ? APHI_FP32 : APHI_FP64;
         const int groups = aphi_dev::rhs_groups(nrhs);
         const std::vector<const void*> key = {m->P.p,  m->X256.p, m->XT256.p, m->dinv.p, m->bt.p, m->W.p,
@@ -786,6 +802,7 @@ extern "C" int aphi_context_create(int device, aphi_context** out) {
         HIPCHK(hipEventCreate(&c->ev1));
         HIPCHK(hipEventCreateWithFlags(&c->main_done, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c->side_done, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->caller, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -799,6 +816,7 @@ extern "C" void aphi_context_destroy(aphi_context* c) {
     (void)hipEventDestroy(c->ev1);
     (void)hipEventDestroy(c->main_done);
     (void)hipEventDestroy(c->side_done);
+    (void)hipEventDestroy(c->caller);
     (void)hipStreamDestroy(c->side);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -848,12 +866,12 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             for (int t = 0; t < d->num_tets; ++t)
                 if (d->tet_body[t] < 0 || d->tet_body[t] >= d->num_bodies)
                     throw std::invalid_argument("aphi_mesh_create: a tet's body index is out of range");
-            upload(m->coords, d->coords, static_cast<size_t>(d->num_nodes) * 3);
-            upload(m->tets, d->tet_nodes, static_cast<size_t>(d->num_tets) * 4);
-            upload(m->tet_body, d->tet_body, static_cast<size_t>(d->num_tets));
-            upload(m->tptr, d->target_ptr, static_cast<size_t>(m->num_targets) + 1);
-            upload(m->entry, d->entry, static_cast<size_t>(nc));
-            upload(m->kind, d->kind, static_cast<size_t>(nc));
+            upload(m->st(), m->coords, d->coords, static_cast<size_t>(d->num_nodes) * 3);
+            upload(m->st(), m->tets, d->tet_nodes, static_cast<size_t>(d->num_tets) * 4);
+            upload(m->st(), m->tet_body, d->tet_body, static_cast<size_t>(d->num_tets));
+            upload(m->st(), m->tptr, d->target_ptr, static_cast<size_t>(m->num_targets) + 1);
+            upload(m->st(), m->entry, d->entry, static_cast<size_t>(nc));
+            upload(m->st(), m->kind, d->kind, static_cast<size_t>(nc));
             for (int64_t c = 0; c < nc; ++c) {
                 const int kk = d->kind[c] & 0x0f;
                 if (kk <= 5 && d->entry[c] >= static_cast<uint32_t>(d->num_tets) * 196u)
@@ -862,8 +880,8 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                     throw std::invalid_argument("aphi_mesh_create: a contribution names an unknown prescribed value");
             }
         }
-        upload(m->row_ptr, d->row_ptr, static_cast<size_t>(n) + 1);
-        upload(m->col, d->col_index, static_cast<size_t>(d->nnz));
+        upload(m->st(), m->row_ptr, d->row_ptr, static_cast<size_t>(n) + 1);
+        upload(m->st(), m->col, d->col_index, static_cast<size_t>(d->nnz));
         {
             std::vector<int> row_of(static_cast<size_t>(d->nnz));
             std::vector<int> tp(static_cast<size_t>(n) + 1, 0);
@@ -884,22 +902,22 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                     ts[static_cast<size_t>(cur[static_cast<size_t>(c)])] = k;
                     tr[static_cast<size_t>(cur[static_cast<size_t>(c)]++)] = r;
                 }
-            upload(m->row_of, row_of.data(), row_of.size());
-            upload(m->ttptr, tp.data(), tp.size());
-            upload(m->tslot, ts.data(), ts.size());
-            upload(m->trow, tr.data(), tr.size());
+            upload(m->st(), m->row_of, row_of.data(), row_of.size());
+            upload(m->st(), m->ttptr, tp.data(), tp.size());
+            upload(m->st(), m->tslot, ts.data(), ts.size());
+            upload(m->st(), m->trow, tr.data(), tr.size());
         }
-        upload(m->perm, d->factor_perm, static_cast<size_t>(n));
-        upload(m->col_ref, d->col_ref, static_cast<size_t>(n));
-        upload(m->first, d->super_first, static_cast<size_t>(ns) + 1);
-        upload(m->srow_ptr, d->super_row_ptr, static_cast<size_t>(ns) + 1);
+        upload(m->st(), m->perm, d->factor_perm, static_cast<size_t>(n));
+        upload(m->st(), m->col_ref, d->col_ref, static_cast<size_t>(n));
+        upload(m->st(), m->first, d->super_first, static_cast<size_t>(ns) + 1);
+        upload(m->st(), m->srow_ptr, d->super_row_ptr, static_cast<size_t>(ns) + 1);
         m->rows_total = d->super_row_ptr[ns];
-        upload(m->srows, d->super_rows, static_cast<size_t>(m->rows_total));
-        upload(m->rel, d->super_rel, static_cast<size_t>(m->rows_total));
-        upload(m->panel_off, d->panel_offset, static_cast<size_t>(ns) + 1);
+        upload(m->st(), m->srows, d->super_rows, static_cast<size_t>(m->rows_total));
+        upload(m->st(), m->rel, d->super_rel, static_cast<size_t>(m->rows_total));
+        upload(m->st(), m->panel_off, d->panel_offset, static_cast<size_t>(ns) + 1);
         m->factor_entries = d->panel_offset[ns];
-        upload(m->cb_off, d->cb_offset, static_cast<size_t>(ns));
-        upload(m->s2p, d->slot_to_panel, static_cast<size_t>(d->nnz));
+        upload(m->st(), m->cb_off, d->cb_offset, static_cast<size_t>(ns));
+        upload(m->st(), m->s2p, d->slot_to_panel, static_cast<size_t>(d->nnz));
         for (int64_t k = 0; k < d->nnz; ++k)
             if (d->slot_to_panel[k] < 0 || d->slot_to_panel[k] >= m->factor_entries)
                 throw std::invalid_argument("aphi_mesh_create: slot_to_panel points outside the factor");
@@ -920,8 +938,8 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             for (int s = 0; s < ns; ++s)
                 if (d->super_parent[s] >= 0) ch[static_cast<size_t>(cur[static_cast<size_t>(d->super_parent[s])]++)] = s;
         }
-        upload(m->child_ptr, cptr.data(), cptr.size());
-        upload(m->child, ch.data(), ch.size());
+        upload(m->st(), m->child_ptr, cptr.data(), cptr.size());
+        upload(m->st(), m->child, ch.data(), ch.size());
         m->levels.assign(static_cast<size_t>(d->num_levels), Level{});
         std::vector<std::vector<int>> small(static_cast<size_t>(d->num_levels)), large(static_cast<size_t>(d->num_levels));
         const int small_max = small_front_max();
@@ -959,7 +977,7 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             m->max_level_count = std::max(m->max_level_count, lv.count());
             m->max_bwd_partials = std::max<int64_t>(m->max_bwd_partials, static_cast<int64_t>(lv.count()) * std::max(maxrb, 1));
         }
-        upload(m->nodes, node_list.data(), node_list.size());
+        upload(m->st(), m->nodes, node_list.data(), node_list.size());
         {
             // diagonal-block inverses: full kNB x kNB blocks, then one nb x nb
             std::vector<int64_t> xo(static_cast<size_t>(ns));
@@ -971,7 +989,7 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                 off += (kk / aphi_dev::kNB) * aphi_dev::kNB * aphi_dev::kNB + rest * rest;
             }
             m->xinv_entries = off;
-            upload(m->xinv_off, xo.data(), xo.size());
+            upload(m->st(), m->xinv_off, xo.data(), xo.size());
             // kSB-block inverses for the solves, and the (supernode, block) list that builds them
             std::vector<int64_t> xb(static_cast<size_t>(ns));
             std::vector<int2> pairs;
@@ -985,10 +1003,10 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             }
             m->x256_entries = off;
             m->num_inv_pairs = static_cast<int>(pairs.size());
-            upload(m->x256_off, xb.data(), xb.size());
-            upload(m->inv_pairs, pairs.data(), pairs.size());
+            upload(m->st(), m->x256_off, xb.data(), xb.size());
+            upload(m->st(), m->inv_pairs, pairs.data(), pairs.size());
         }
-        upload(m->ea_start, ea.data(), ea.size());
+        upload(m->st(), m->ea_start, ea.data(), ea.size());
         {
             // extend-add bounds for the children of blocked fronts
             std::vector<int64_t> bo(static_cast<size_t>(ns), -1);
@@ -1010,16 +1028,16 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
                 }
                 if (pos != cu) throw std::invalid_argument("aphi_mesh_create: relative indices exceed the parent front");
             }
-            upload(m->ea_bound_off, bo.data(), bo.size());
-            upload(m->ea_bound, bv.data(), bv.size());
+            upload(m->st(), m->ea_bound_off, bo.data(), bo.size());
+            upload(m->st(), m->ea_bound, bv.data(), bv.size());
         }
         if (plan && d->tet_dofs && d->tet_edge_signs && d->tet_phi_coeff && d->tet_phi_fixed) {
             const size_t nt = static_cast<size_t>(d->num_tets);
-            upload(m->tdofs, d->tet_dofs, nt * 16);
-            upload(m->tsign, d->tet_edge_signs, nt * 6);
-            upload(m->tcoef, d->tet_phi_coeff, nt * 10);
-            upload(m->tfixed, d->tet_phi_fixed, nt * 20);
-            upload(m->bsigma, d->body_sigma, static_cast<size_t>(d->num_bodies));
+            upload(m->st(), m->tdofs, d->tet_dofs, nt * 16);
+            upload(m->st(), m->tsign, d->tet_edge_signs, nt * 6);
+            upload(m->st(), m->tcoef, d->tet_phi_coeff, nt * 10);
+            upload(m->st(), m->tfixed, d->tet_phi_fixed, nt * 20);
+            upload(m->st(), m->bsigma, d->body_sigma, static_cast<size_t>(d->num_bodies));
         }
         m->flag.reserve(sizeof(int));
         *out = m.release();
@@ -1033,6 +1051,13 @@ extern "C" void aphi_mesh_destroy(aphi_mesh* m) {
     for (hipEvent_t e : m->prof_events) (void)hipEventDestroy(e);
     if (m->sgraph.exec) (void)hipGraphExecDestroy(m->sgraph.exec);
     delete m;
+}
+
+extern "C" int aphi_mesh_set_solve_graphs(aphi_mesh* m, int on) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_mesh_set_solve_graphs: null mesh");
+        m->solve_graphs_on = on != 0;
+    });
 }
 
 extern "C" int aphi_mesh_set_profiling(aphi_mesh* m, int on) {
@@ -1049,7 +1074,9 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
                            &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
-                           &m->bpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv};
+                           &m->bpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv, &m->gf.coords,
+                           &m->gf.tets, &m->gf.tet_body, &m->gf.tet_on, &m->gf.column, &m->gf.prescribed, &m->gf.tptr,
+                           &m->gf.entry, &m->gf.kind, &m->gf.values, &m->gf.beta, &m->gf.terms};
     int64_t total = 0;
     for (const DevBuf* b : all) total += static_cast<int64_t>(b->bytes);
     return total;
@@ -1176,6 +1203,7 @@ extern "C" int aphi_terminal_current(aphi_mesh* m, int nrhs, const void* xv, int
         require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, "aphi_terminal_current: unknown conditioning");
         require(m->tdofs.p != nullptr, "aphi_terminal_current: the mesh was created without per-tet unknowns");
         if (!(length > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
+        after_caller(m->ctx);
         hipStream_t s = m->st();
         const int nt = m->num_tets;
         m->tcontrib.reserve(static_cast<size_t>(nt) * nrhs * sizeof(cd));
@@ -1240,8 +1268,10 @@ extern "C" int aphi_random_normal(aphi_context* ctx, uint64_t seed, uint64_t off
         require(ctx && count >= 0 && (count == 0 || out), "aphi_random_normal: bad arguments");
         if (count == 0) return;
         HIPCHK(hipSetDevice(ctx->device));
+        after_caller(ctx);
         hipLaunchKernelGGL(aphi_dev::k_random_normal, dim3(blocks_for(count)), dim3(256), 0, ctx->stream, seed, offset,
                            count, static_cast<double*>(out));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
         HIPCHK(hipGetLastError());
     });
 }
@@ -1251,6 +1281,7 @@ extern "C" int aphi_tet_fields(aphi_mesh* m, const void* xv, double omega, int c
         require(m && xv && e_tet, "aphi_tet_fields: null argument");
         require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, "aphi_tet_fields: unknown conditioning");
         require(m->tdofs.p != nullptr, "aphi_tet_fields: the mesh was created without per-tet unknowns");
+        after_caller(m->ctx);
         HIPCHK(hipSetDevice(m->ctx->device));
         const int nt = m->num_tets;
         aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
@@ -1259,6 +1290,7 @@ extern "C" int aphi_tet_fields(aphi_mesh* m, const void* xv, double omega, int c
         const cd scale = conditioning == APHI_SCALED_PHI ? cd{0.0, omega} : cd{1.0, 0.0};
         hipLaunchKernelGGL(aphi_dev::k_tet_efield, dim3(blocks_for(nt)), dim3(256), 0, m->st(), nt, tp,
                            static_cast<const cd*>(xv), minus_jw, scale, static_cast<cd*>(e_tet));
+        HIPCHK(hipStreamSynchronize(m->st()));
         HIPCHK(hipGetLastError());
     });
 }
@@ -1310,16 +1342,16 @@ extern "C" int aphi_gauge_free_attach(aphi_mesh* m, const aphi_gauge_free_desc* 
         g.port_column.assign(d->port_column, d->port_column + d->num_ports);
         g.port_value.clear();
         for (int p = 0; p < d->num_ports; ++p) g.port_value.emplace_back(d->port_value[2 * p], d->port_value[2 * p + 1]);
-        upload(g.coords, d->coords, static_cast<size_t>(nv) * 3);
-        upload(g.tets, d->tet_nodes, static_cast<size_t>(nt) * 4);
-        upload(g.tet_body, d->tet_body, static_cast<size_t>(nt));
-        upload(g.tet_on, d->tet_on, static_cast<size_t>(nt));
-        upload(g.column, d->column, static_cast<size_t>(nv));
-        upload(g.prescribed, d->prescribed, static_cast<size_t>(nv) * 2);
-        upload(g.tptr, d->target_ptr, static_cast<size_t>(targets) + 1);
-        upload(g.entry, d->entry, static_cast<size_t>(nc));
-        upload(g.kind, d->kind, static_cast<size_t>(nc));
-        upload(g.values, d->values, static_cast<size_t>(d->num_values) * 2);
+        upload(m->st(), g.coords, d->coords, static_cast<size_t>(nv) * 3);
+        upload(m->st(), g.tets, d->tet_nodes, static_cast<size_t>(nt) * 4);
+        upload(m->st(), g.tet_body, d->tet_body, static_cast<size_t>(nt));
+        upload(m->st(), g.tet_on, d->tet_on, static_cast<size_t>(nt));
+        upload(m->st(), g.column, d->column, static_cast<size_t>(nv));
+        upload(m->st(), g.prescribed, d->prescribed, static_cast<size_t>(nv) * 2);
+        upload(m->st(), g.tptr, d->target_ptr, static_cast<size_t>(targets) + 1);
+        upload(m->st(), g.entry, d->entry, static_cast<size_t>(nc));
+        upload(m->st(), g.kind, d->kind, static_cast<size_t>(nc));
+        upload(m->st(), g.values, d->values, static_cast<size_t>(d->num_values) * 2);
         g.terms.reserve(static_cast<size_t>(nt) * 20 * sizeof(cd));
         g.attached = true;
     });
@@ -1329,6 +1361,7 @@ extern "C" int aphi_gauge_free_refill(aphi_mesh* m, const void* e_tet, double om
     return aphi_capi::guarded([&] {
         require(m && e_tet, "aphi_gauge_free_refill: null argument");
         require(m->gf.attached, "aphi_gauge_free_refill: no gauge-free plan attached");
+        after_caller(m->ctx);
         if (omega == 0.0)
             throw std::invalid_argument("aphi_gauge_free_refill: omega must be nonzero (the symmetric A-Phi solve that "
                                         "produced E is undefined at DC)");
@@ -1362,6 +1395,7 @@ extern "C" int aphi_gauge_free_potential(aphi_mesh* m, const void* xv, void* phi
     return aphi_capi::guarded([&] {
         require(m && xv && phi_vertex, "aphi_gauge_free_potential: null argument");
         require(m->gf.attached, "aphi_gauge_free_potential: no gauge-free plan attached");
+        after_caller(m->ctx);
         HIPCHK(hipSetDevice(m->ctx->device));
         auto& g = m->gf;
         hipStream_t s = m->st();
@@ -1408,13 +1442,14 @@ extern "C" int aphi_factor_diagonal(aphi_mesh* m, double* out) {
         // D sits on each panel's diagonal: column c of supernode s at panel_off[s] + c*m + c
         std::vector<int> first(static_cast<size_t>(m->num_super) + 1);
         std::vector<int64_t> rp(static_cast<size_t>(m->num_super) + 1), po(static_cast<size_t>(m->num_super) + 1);
-        HIPCHK(hipMemcpy(first.data(), m->first.p, first.size() * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(rp.data(), m->srow_ptr.p, rp.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(po.data(), m->panel_off.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
         const size_t cs = m->factor_precision == APHI_FP64 ? sizeof(cd) : sizeof(aphi_dev::cf);
         std::vector<unsigned char> host(static_cast<size_t>(m->factor_entries) * cs);
-        HIPCHK(hipStreamSynchronize(m->st()));
-        HIPCHK(hipMemcpy(host.data(), m->P.p, host.size(), hipMemcpyDeviceToHost));
+        hipStream_t s = m->st();
+        HIPCHK(hipMemcpyAsync(first.data(), m->first.p, first.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(rp.data(), m->srow_ptr.p, rp.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(po.data(), m->panel_off.p, po.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(host.data(), m->P.p, host.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
         for (int s = 0; s < m->num_super; ++s) {
             const int64_t mm = rp[static_cast<size_t>(s) + 1] - rp[static_cast<size_t>(s)];
             for (int c = 0; c < first[static_cast<size_t>(s) + 1] - first[static_cast<size_t>(s)]; ++c) {
@@ -1438,6 +1473,7 @@ extern "C" int aphi_solve(aphi_mesh* m, int nrhs, const void* b, int64_t ldb, vo
     return aphi_capi::guarded([&] {
         require(m && b && x, "aphi_solve: null argument");
         require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_solve: nrhs must be 1..64 and ld >= n");
+        after_caller(m->ctx);
         solve_any(m, nrhs, static_cast<const cd*>(b), ldb, static_cast<cd*>(x), ldx, 0);
         HIPCHK(hipStreamSynchronize(m->st()));
     });
@@ -1449,6 +1485,7 @@ extern "C" int aphi_backward_error(aphi_mesh* m, int nrhs, const void* x, int64_
         require(m && x && b, "aphi_backward_error: null argument");
         require(m->assembled, "aphi_backward_error: nothing assembled");
         require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_backward_error: nrhs must be 1..64 and ld >= n");
+        after_caller(m->ctx);
         measure(m, nrhs, static_cast<const cd*>(x), ldx, static_cast<const cd*>(b), ldb, be, rr);
     });
 }
@@ -1459,6 +1496,7 @@ extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int6
         require(m && bv && xv && opt && rep, "aphi_solve_symmetric: null argument");
         require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_solve_symmetric: nrhs must be 1..64 and ld >= n");
         require(opt->max_refinements >= 0, "aphi_solve_symmetric: max_refinements must be >= 0");
+        after_caller(m->ctx);
         const auto t0 = std::chrono::steady_clock::now();
         const cd* b = static_cast<const cd*>(bv);
         cd* x = static_cast<cd*>(xv);

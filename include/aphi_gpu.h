@@ -231,6 +231,13 @@ void aphi_mesh_destroy(aphi_mesh* mesh);
 /* Per-launch event timing of the factorization's trailing-update kernel (adds
  * synchronisation-free event records around each launch; off by default). */
 int aphi_mesh_set_profiling(aphi_mesh* mesh, int on);
+/* The solve's per-level launches are captured into a HIP graph and replayed
+ * (on by default; APHI_SOLVE_GRAPHS=0 in the environment turns it off for all
+ * meshes). A capture fails while another host thread issues work on the legacy
+ * default stream (a synchronous copy, a library running there), so a mesh used
+ * beside other threads' GPU work -- a serving slot -- turns it off here and
+ * launches the same kernels one by one (bitwise the same results). */
+int aphi_mesh_set_solve_graphs(aphi_mesh* mesh, int on);
 /* Device bytes currently held by this mesh (plan + numeric buffers). */
 int64_t aphi_mesh_device_bytes(const aphi_mesh* mesh);
 
