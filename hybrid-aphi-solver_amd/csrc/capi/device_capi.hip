@@ -212,8 +212,8 @@ void equilibrate(aphi_mesh* m, int iterations, double* d_out) {
         hipLaunchKernelGGL(aphi_dev::k_copy_cd, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->A.as<cd>(), m->work.as<cd>());
         hipLaunchKernelGGL(aphi_dev::k_fill_d, dim3(blocks_for(n)), dim3(256), 0, s, static_cast<int64_t>(n), m->rowmax.as<double>(), 0.0);
         for (int it = 0; it < iterations; ++it) {
-            hipLaunchKernelGGL(aphi_dev::k_eq_rowmax, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->row_of.as<int>(),
-                               m->col.as<int>(), m->work.as<cd>(), m->rowmax.as<double>());
+            hipLaunchKernelGGL(aphi_dev::k_eq_rowmax16, dim3(blocks_for(n, 16)), dim3(256), 0, s, n, m->row_ptr.as<int>(),
+                               m->ttptr.as<int>(), m->tslot.as<int>(), m->work.as<cd>(), m->rowmax.as<double>());
             hipLaunchKernelGGL(aphi_dev::k_eq_factor, dim3(blocks_for(n)), dim3(256), 0, s, n, m->rowmax.as<double>(),
                                m->fac.as<double>(), m->d.as<double>());
             hipLaunchKernelGGL(aphi_dev::k_eq_scale, dim3(blocks_for(m->nnz)), dim3(256), 0, s, m->nnz, m->row_of.as<int>(),

@@ -298,13 +298,27 @@ __global__ void k_fill_d(int64_t n, double* __restrict__ p, double v) {
     if (i < n) p[i] = v;
 }
 
-__global__ void k_eq_rowmax(int64_t nnz, const int* __restrict__ row_of, const int* __restrict__ col,
-                            const cd* __restrict__ w, double* __restrict__ rowmax) {
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= nnz) return;
-    const double mag = hypot(w[k].re, w[k].im);
-    atomic_max_nonneg(&rowmax[row_of[k]], mag);
-    atomic_max_nonneg(&rowmax[col[k]], mag);
+// compute_symmetric_equilibration's row maxima (src/equilibration.cpp:94-139),
+// gathered: 16 lanes per row i take
+// the stored entries of row i (CSR) and of column i (entries (r, i), r < i,
+// through the transpose index), then a max over the 16 lanes: no atomics, and
+// the maximum does not depend on the order of the comparisons.
+__global__ void __launch_bounds__(256) k_eq_rowmax16(int n, const int* __restrict__ row_ptr, const int* __restrict__ tptr,
+                                                     const int* __restrict__ tslot, const cd* __restrict__ w,
+                                                     double* __restrict__ rowmax) {
+    const int i = static_cast<int>(blockIdx.x) * 16 + static_cast<int>(threadIdx.x) / 16;
+    const int q = static_cast<int>(threadIdx.x) % 16;
+    double mx = 0.0;
+    if (i < n) {
+        for (int k = row_ptr[i] + q; k < row_ptr[i + 1]; k += 16) mx = fmax(mx, hypot(w[k].re, w[k].im));
+        for (int t = tptr[i] + q; t < tptr[i + 1]; t += 16) {
+            const cd v = w[tslot[t]];
+            mx = fmax(mx, hypot(v.re, v.im));
+        }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 16));
+    if (i < n && q == 0) rowmax[i] = mx;
 }
 
 __global__ void k_eq_factor(int n, double* __restrict__ rowmax, double* __restrict__ factor, double* __restrict__ d) {
@@ -363,8 +377,46 @@ __global__ void __launch_bounds__(256) k_symv16(int n, int nrhs, const int* __re
     if (i >= n) return;
     const cd* xg = xi + static_cast<int64_t>(g) * n * 16 + q;
     cd acc = {0.0, 0.0};
-    for (int t = tptr[i]; t < tptr[i + 1]; ++t) acc = cadd(acc, cmul(a[tslot[t]], xg[static_cast<int64_t>(trow[t]) * 16]));
-    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) acc = cadd(acc, cmul(a[k], xg[static_cast<int64_t>(col[k]) * 16]));
+    // eight terms at a time: their indices, then their operands, are all loaded
+    // before the first product (one memory latency per eight terms); the sum
+    // still runs term by term in the reference's order
+    constexpr int U = 8;
+    const int t0 = tptr[i], t1 = tptr[i + 1];
+    for (int t = t0; t < t1; t += U) {
+        int sl[U], rw[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int tt = min(t + u, t1 - 1);
+            sl[u] = tslot[tt];
+            rw[u] = trow[tt];
+        }
+        cd av[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            av[u] = a[sl[u]];
+            xv[u] = xg[static_cast<int64_t>(rw[u]) * 16];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (t + u < t1) acc = cadd(acc, cmul(av[u], xv[u]));
+    }
+    const int k0 = row_ptr[i], k1 = row_ptr[i + 1];
+    for (int k = k0; k < k1; k += U) {
+        int cl[U];
+        cd av[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k + u, k1 - 1);
+            cl[u] = col[kk];
+            av[u] = a[kk];
+        }
+        cd xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = xg[static_cast<int64_t>(cl[u]) * 16];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k + u < k1) acc = cadd(acc, cmul(av[u], xv[u]));
+    }
     if (r < nrhs) y[r * ldy + i] = acc;
 }
 

@@ -363,7 +363,9 @@ __global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, in
         lo[u] = __builtin_amdgcn_readfirstlane(l0);
         hi[u] = __builtin_amdgcn_readfirstlane(h0);
     }
-    for (int j = 0; j < kNB; ++j) {
+    // steps j >= nb only meet the identity padding: every entry read afterwards
+    // (L, D for r, c < nb; X for r < nb) is final after step nb - 1
+    for (int j = 0; j < nb; ++j) {
         const C<T>* cj = colA + (j & 1) * kNB;
         C<T>* cn = colA + ((j + 1) & 1) * kNB;
         const C<T>* rj = rowX + (j & 1) * kNB;

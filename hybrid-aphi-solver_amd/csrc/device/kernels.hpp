@@ -64,7 +64,7 @@ __global__ void k_assemble_reduce(int64_t num_targets, int num_slots, const int6
                                   const uint8_t* kind, const cd* E, FreqScales sc, cd* values, cd* rhs);
 __global__ void k_copy_cd(int64_t n, const cd* src, cd* dst);
 __global__ void k_fill_d(int64_t n, double* p, double v);
-__global__ void k_eq_rowmax(int64_t nnz, const int* row_of, const int* col, const cd* w, double* rowmax);
+__global__ void k_eq_rowmax16(int n, const int* row_ptr, const int* tptr, const int* tslot, const cd* w, double* rowmax);
 __global__ void k_eq_factor(int n, double* rowmax, double* factor, double* d);
 __global__ void k_eq_scale(int64_t nnz, const int* row_of, const int* col, const double* factor, cd* w);
 __global__ void k_eq_apply(int64_t nnz, const int* row_of, const int* col, const double* d, const cd* a, cd* out);
