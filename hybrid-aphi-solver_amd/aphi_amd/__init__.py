@@ -271,6 +271,19 @@ class Problem:
                                              C.byref(cnt)))
         return out
 
+    def write_potential(self, path, x, omega, conditioning=APHI_ROW_SCALED, vtk=False):
+        """potential_at_nodes + write_potential (or write_vtk without field data) of
+        the solution x (numpy or device tensor, n complex); returns the byte count."""
+        if hasattr(x, "cpu"):
+            x = x.reshape(-1).cpu().numpy()
+        xx = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        if xx.size != self.n:
+            raise ValueError("write_potential: x must hold n values")
+        nb = C.c_int64()
+        _check(lib().aphi_write_potential(self._h, str(path).encode(), xx.view(np.float64).ctypes.data_as(_capi.DP),
+                                          float(omega), int(conditioning), 1 if vtk else 0, C.byref(nb)))
+        return nb.value
+
     def describe(self):
         d = _capi.MeshDesc()
         _check(lib().aphi_problem_describe(self._h, C.byref(d)))

@@ -110,6 +110,7 @@ SIGNATURES = {
     "aphi_problem_get_int": (C.c_int, [C.c_void_p, C.c_char_p, I64P, C.c_int64, I64P]),
     "aphi_problem_get_double": (C.c_int, [C.c_void_p, C.c_char_p, DP, C.c_int64, I64P]),
     "aphi_frequency_sweep": (C.c_int, [C.c_double, C.c_double, C.c_int, C.c_int, DP]),
+    "aphi_write_potential": (C.c_int, [C.c_void_p, C.c_char_p, DP, C.c_double, C.c_int, C.c_int, I64P]),
     "aphi_problem_describe": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc)]),
     "aphi_problem_create_gauge_free": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "aphi_problem_describe_gauge_free": (C.c_int, [C.c_void_p, C.POINTER(GaugeFreeDesc)]),

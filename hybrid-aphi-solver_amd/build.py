@@ -23,6 +23,7 @@ HOST_SOURCES = [
     "host/ordering.cpp",
     "host/symbolic.cpp",
     "host/gauge_free.cpp",
+    "host/writers.cpp",
     "capi/problem_capi.cpp",
 ]
 HIP_SOURCES = [

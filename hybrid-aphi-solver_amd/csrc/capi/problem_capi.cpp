@@ -299,6 +299,21 @@ extern "C" int aphi_problem_describe_gauge_free(const aphi_problem* p, aphi_gaug
     });
 }
 
+extern "C" int aphi_write_potential(const aphi_problem* p, const char* path, const double* x, double omega,
+                                    int conditioning, int format, int64_t* bytes) {
+    return aphi_capi::guarded([&] {
+        if (!p || !path || !x) throw std::invalid_argument("aphi_write_potential: null argument");
+        if (p->from_csr) throw std::invalid_argument("aphi_write_potential: the problem has no mesh");
+        if (conditioning < 0 || conditioning > 2) throw std::invalid_argument("aphi_write_potential: unknown conditioning");
+        if (format != 0 && format != 1) throw std::invalid_argument("aphi_write_potential: format must be 0 (text) or 1 (VTK)");
+        const NodalPotential np = potential_at_nodes(p->mesh, p->dofs, reinterpret_cast<const Complex*>(x),
+                                                     p->dofs.num_total, omega, conditioning);
+        const size_t n = format == 0 ? write_potential(path, np, omega, conditioning)
+                                     : write_vtk_potential(path, p->mesh, np, omega, conditioning);
+        if (bytes) *bytes = static_cast<int64_t>(n);
+    });
+}
+
 extern "C" void aphi_problem_destroy(aphi_problem* p) { delete p; }
 
 extern "C" int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out) {

@@ -363,6 +363,27 @@ struct GaugeFreePlan {
 
 GaugeFreePlan build_gauge_free_plan(const Mesh& mesh, const BoundProblem& bound, const DofMap& dofs);
 
+// ------------------------------------------------------------------ result files
+//
+// The reference's version string, as its writers print it (include/aphi_solver/version.hpp).
+constexpr const char* kReferenceVersion = "0.0.1-dev";
+
+// potential_at_nodes (src/postprocess.cpp:42-123): Phi at every P2 node (vertices,
+// then edge midpoints), with a status letter F / T / C / P / A.
+struct NodalPotential {
+    int num_vertices = 0, num_edges = 0;
+    std::vector<Vec3> position;
+    std::vector<Complex> value;
+    std::vector<char> status;
+};
+NodalPotential potential_at_nodes(const Mesh& mesh, const DofMap& dofs, const Complex* x, int n, double omega,
+                                  int conditioning);
+// write_potential (src/postprocess.cpp:549-606) without run information; returns bytes written
+size_t write_potential(const std::string& path, const NodalPotential& p, double omega, int conditioning);
+// write_vtk (src/postprocess.cpp:1055-1276) without field data or run information
+size_t write_vtk_potential(const std::string& path, const Mesh& mesh, const NodalPotential& p, double omega,
+                           int conditioning);
+
 // log-spaced sweep, src/input_file.cpp:19-50
 std::vector<double> expand_frequency_sweep(double start, double stop, int points, bool logarithmic);
 

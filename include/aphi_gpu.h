@@ -118,6 +118,13 @@ int aphi_problem_get_int(const aphi_problem* p, const char* name, int64_t* out, 
  * body_param (sigma, eps_r, mu_r per body), port_value (complex per port); of a
  * gauge-free problem also gf_prescribed (complex per vertex), gf_values. */
 int aphi_problem_get_double(const aphi_problem* p, const char* name, double* out, int64_t capacity, int64_t* count);
+/* Result files of a solution x (host, num_total complex, e.g. downloaded from
+ * the device), in the reference's formats and number formatting: format 0 is
+ * potential_at_nodes + write_potential (src/postprocess.cpp:42-123, 549-606),
+ * format 1 is write_vtk without field data (src/postprocess.cpp:1055-1276), both
+ * without run information. *bytes (may be NULL) receives the file size. */
+int aphi_write_potential(const aphi_problem* p, const char* path, const double* x, double omega, int conditioning,
+                         int format, int64_t* bytes);
 /* The log sweep of src/input_file.cpp:19-50. */
 int aphi_frequency_sweep(double start, double stop, int points, int logarithmic, double* out);
 

@@ -10,7 +10,8 @@
 // usage: ref_driver OUTDIR n side bar_i0 bar_i1 bar_j0 bar_j1 sigma freq_hz mode
 //   mode 0: symbolic only (topology, gauge, DOF map, pattern, scatter map, ordering)
 //   mode 1: + refill at freq_hz, equilibration, factorization, solve, R and L,
-//           the fields' e_tet and the gauge-free potential recovered from them
+//           the fields' e_tet, the gauge-free potential recovered from them, and
+//           the potential.out / VTK files of the solution (no run information)
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -201,6 +202,9 @@ int main(int argc, char** argv) {
         const GaugeFreePotential gfp = recover_gauge_free_potential(mesh, bound, dofs, fields, omega);
         dump("gf_phi_vertex", gfp.phi_vertex);
         dump("gf_port_potential", gfp.port_potential);
+        const NodalPotential pot = potential_at_nodes(mesh, bound, dofs, sol);
+        write_potential(g_out + "/potential.out", pot, sol, nullptr);
+        write_vtk(g_out + "/solution.vtk", mesh, pot, sol);
         const Z Zin = impedance(1.0, I);
         std::fprintf(meta,
                      ", \"freq\": %.17g, \"omega\": %.17g, \"scatter_map_bitwise_equal\": %s, \"factor_ok\": %s, "

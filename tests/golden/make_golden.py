@@ -61,6 +61,10 @@ def run(case, outdir):
             digests[a] = digest(v)
             if full or a in ("x", "diag", "equil_d", "rhs", "gf_phi_vertex", "gf_port_potential"):
                 arrays[a] = v
+        for fname in ("potential.out", "solution.vtk"):
+            data = open(os.path.join(outdir, fname), "rb").read()
+            digests[fname] = hashlib.sha256(data).hexdigest()
+            meta["bytes_" + fname] = len(data)
     meta["sha256"] = digests
     return meta, arrays
 

@@ -278,3 +278,21 @@ def test_solve_graph_replays_match_eager(case, precision, monkeypatch):
             torch.cuda.synchronize()
             if other is b3:
                 assert np.array_equal(xo.cpu().numpy().view(np.uint64), eager3.view(np.uint64))
+
+
+def test_potential_file_from_device_solution(case, tmp_path):
+    """write_potential fed from the device solution (a tensor): the values it writes
+    are within 1e-10 of those written from the reference's own solution."""
+    m, p, meta = case["m"], case["p"], case["meta"]
+    m.refill(meta["omega"])
+    x, rep = m.solve_symmetric(dev(case["rhs"]), precision=A.APHI_FP64)
+    assert rep["ok"]
+    p.write_potential(tmp_path / "dev.out", x, meta["omega"])
+    p.write_potential(tmp_path / "ref.out", case["x"], meta["omega"])
+
+    def values(path):
+        rows = [ln.split() for ln in open(path) if not ln.startswith("#")]
+        return np.array([complex(float(r[4]), float(r[5])) for r in rows])
+
+    a, b = values(tmp_path / "dev.out"), values(tmp_path / "ref.out")
+    assert a.shape == b.shape and np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-10
