@@ -349,6 +349,77 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
     });
 }
 
+// The same update for complex64 with 32 rows per workgroup: lane (p, kq) loads
+// rows i0 + 2p and i0 + 2p + 1 of a column with one 16-byte load (each load
+// instruction covers 256 contiguous bytes of four columns, twice the run of the
+// one-row-per-lane mapping), and the two rows go to two 16-row MFMA tiles (even
+// rows, odd rows). Each output sums the same products, grouped and ordered as
+// in k_fwd_update.
+__global__ void __launch_bounds__(256) k_fwd_update2(const int* __restrict__ nodes, int B, FrontPlan fp,
+                                                     const cf* __restrict__ P, cf* W, const cf* __restrict__ Y, int n,
+                                                     int64_t R) {
+    using VF = V4<float>;
+    __shared__ cf part[2][4][256];
+    const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z);
+    const Sn x = sn(fp, s);
+    if (x.k <= B) return;
+    const int nb = min(kSB, x.k - B);
+    const int i0 = B + nb + static_cast<int>(blockIdx.x) * 32;
+    if (i0 >= x.m) return;
+    const int tid = static_cast<int>(threadIdx.x), wave = tid / 64, lane = tid % 64;
+    const int p = lane % 16, kq = lane / 16, k0 = 64 * wave;
+    const cf* Ps = P + fp.panel_off[s];
+    const cf* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
+    VF r0 = VF{0, 0, 0, 0}, i0a = VF{0, 0, 0, 0}, r1 = VF{0, 0, 0, 0}, i1a = VF{0, 0, 0, 0};
+    if (k0 < nb) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const int row = min(i0 + 2 * p, x.m - 1);  // rows row and row + 1 (the second may lie past m: never stored)
+        f4 a[16];
+        cf b[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            a[u] = *reinterpret_cast<const f4*>(Ps + static_cast<int64_t>(B + min(k0 + 4 * u + kq, nb - 1)) * x.m + row);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = k0 + 4 * u + kq;
+            const cf v = Yg[min(k, nb - 1) * kQ + p];  // clamped read, then select
+            b[u] = k < nb ? v : cf{0, 0};
+        }
+        __builtin_amdgcn_sched_barrier(0);  // every operand load in flight before the first MFMA
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            r0 = Mfma<float>::mac(a[u].x, b[u].re, r0);
+            r0 = Mfma<float>::mac(-a[u].y, b[u].im, r0);
+            i0a = Mfma<float>::mac(a[u].x, b[u].im, i0a);
+            i0a = Mfma<float>::mac(a[u].y, b[u].re, i0a);
+            r1 = Mfma<float>::mac(a[u].z, b[u].re, r1);
+            r1 = Mfma<float>::mac(-a[u].w, b[u].im, r1);
+            i1a = Mfma<float>::mac(a[u].z, b[u].im, i1a);
+            i1a = Mfma<float>::mac(a[u].w, b[u].re, i1a);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        part[0][wave][Mfma<float>::row(lane, v) * 16 + p] = cf{r0[v], i0a[v]};
+        part[1][wave][Mfma<float>::row(lane, v) * 16 + p] = cf{r1[v], i1a[v]};
+    }
+    __syncthreads();
+    cf* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        cf t = part[h][0][tid];
+        t = gadd(t, part[h][1][tid]);
+        t = gadd(t, part[h][2][tid]);
+        t = gadd(t, part[h][3][tid]);
+        const int i = i0 + 2 * (tid / 16) + h;  // tile row tid / 16 of tile h is front row i0 + 2 (tid / 16) + h
+        if (i < x.m) {
+            cf& dst = Wg[static_cast<int64_t>(i) * kQ + tid % 16];
+            dst = gsub(dst, t);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ backward steps
 //
 // Step B (blocks in decreasing order): t_B = L[rows below, B:B+nb]^T x[rows below]
@@ -399,6 +470,69 @@ __global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nod
     }
     C<T>* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
     reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
+        if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;
+    });
+}
+
+// The same partial sums for complex64 with one 16-byte load per lane for two
+// adjacent rows of a column (eight contiguous rows of each of 16 columns per load
+// instruction instead of four): within each group of eight rows, the even rows
+// form one MFMA k-group and the odd rows the next.
+__global__ void __launch_bounds__(256) k_bwd_partial2(const int* __restrict__ nodes, int B, int stride, int ctiles,
+                                                      FrontPlan fp, const cf* __restrict__ P, const cf* __restrict__ X,
+                                                      int n, cf* __restrict__ partial) {
+    using VF = V4<float>;
+    __shared__ cf part[4][256];
+    const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
+    const Sn x = sn(fp, s);
+    if (x.k <= B) return;
+    const int nb = min(kSB, x.k - B);
+    const int chunk = static_cast<int>(blockIdx.x) / ctiles, c0 = (static_cast<int>(blockIdx.x) % ctiles) * 16;
+    const int i0 = B + nb + chunk * kBwdRows;
+    if (i0 >= x.m || c0 >= nb) return;
+    const int iw = i0 + 64 * (static_cast<int>(threadIdx.x) / 64);
+    const cf* Ps = P + fp.panel_off[s];
+    const int* rows = fp.rows + x.rp;
+    const cf* Xg = X + static_cast<int64_t>(g) * n * kQ;
+    VF accr = VF{0, 0, 0, 0}, acci = VF{0, 0, 0, 0};
+    if (iw < x.m) {
+        typedef float f4 __attribute__((ext_vector_type(4)));
+        const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
+        const cf* col = Ps + static_cast<int64_t>(B + min(c0 + p, nb - 1)) * x.m;
+        // rows iw + 8u + 2kq and the next one; the row indices first (one wait)
+        int gr[16];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int r = iw + 8 * u + 2 * kq;
+            gr[2 * u] = rows[min(r, x.m - 1)];
+            gr[2 * u + 1] = rows[min(r + 1, x.m - 1)];
+        }
+        f4 a[8];
+        cf b[16];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)  // the second row of the pair may lie past m: its product is masked
+            a[u] = *reinterpret_cast<const f4*>(col + min(iw + 8 * u + 2 * kq, x.m - 1));
+#pragma unroll
+        for (int u = 0; u < 16; ++u) b[u] = Xg[static_cast<int64_t>(gr[u]) * kQ + p];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int r = iw + 8 * u + 2 * kq;
+            const cf b0 = r < x.m ? b[2 * u] : cf{0, 0};
+            const cf b1 = r + 1 < x.m ? b[2 * u + 1] : cf{0, 0};
+            accr = Mfma<float>::mac(a[u].x, b0.re, accr);
+            accr = Mfma<float>::mac(-a[u].y, b0.im, accr);
+            acci = Mfma<float>::mac(a[u].x, b0.im, acci);
+            acci = Mfma<float>::mac(a[u].y, b0.re, acci);
+            accr = Mfma<float>::mac(a[u].z, b1.re, accr);
+            accr = Mfma<float>::mac(-a[u].w, b1.im, accr);
+            acci = Mfma<float>::mac(a[u].z, b1.im, acci);
+            acci = Mfma<float>::mac(a[u].w, b1.re, acci);
+        }
+    }
+    cf* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
+    reduce4<float>(part, accr, acci, [&](int p, int q, cf t) {
         if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;
     });
 }
@@ -544,9 +678,14 @@ void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max
     const int G = rhs_groups(nrhs);
     hipLaunchKernelGGL(k_fwd_diag<T>, dim3((max_nb + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
                        static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
-    if (max_rows_below > 0)
-        hipLaunchKernelGGL(k_fwd_update<T>, dim3((max_rows_below + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
-                           static_cast<const C<T>*>(P), static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n, R);
+    if (max_rows_below > 0) {
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL(k_fwd_update2, dim3((max_rows_below + 31) / 32, count, G), dim3(256), 0, st, nodes, B, fp,
+                               static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y), n, R);
+        else
+            hipLaunchKernelGGL(k_fwd_update<T>, dim3((max_rows_below + 15) / 16, count, G), dim3(256), 0, st, nodes, B,
+                               fp, static_cast<const C<T>*>(P), static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n, R);
+    }
 }
 
 template <typename T>
@@ -558,8 +697,14 @@ void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max
     const int ctiles = (max_nb + 15) / 16;
     const int stride = bwd_chunks(max_rows_below);
     if (max_rows_below > 0) {
-        hipLaunchKernelGGL(k_bwd_partial<T>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride, ctiles,
-                           fp, static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(partial));
+        if constexpr (std::is_same<T, float>::value)
+            hipLaunchKernelGGL(k_bwd_partial2, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride,
+                               ctiles, fp, static_cast<const cf*>(P), static_cast<const cf*>(X), n,
+                               static_cast<cf*>(partial));
+        else
+            hipLaunchKernelGGL(k_bwd_partial<T>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride,
+                               ctiles, fp, static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n,
+                               static_cast<C<T>*>(partial));
         hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(max_nb, count, G), dim3(256), 0, st, nodes, B, stride, fp,
                            static_cast<const C<T>*>(partial), static_cast<C<T>*>(tsum));
     }
