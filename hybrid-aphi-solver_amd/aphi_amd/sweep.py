@@ -27,9 +27,17 @@ class PrecisionPolicy:
     on when it rises towards the new frequency. A point where FP32 refinement did
     not converge counts as twice the refinement budget, and a frequency whose
     nearest neighbours all failed is left to FP64.
+
+    An FP32 attempt whose expected refinement runs towards the budget may fail:
+    refinement stalls (the stall rule stops it after a few solves) and the point
+    is refactored in FP64. The FP32 cost is therefore an expectation: with the
+    failure likelihood taken as 0 up to 60 % of the budget and rising
+    quadratically to 1 at the budget (refinement counts climb steeply just before
+    refinement breaks down), a failure costing the FP32 factor, ``stall_solves``
+    solves and the FP64 point on top.
     """
 
-    def __init__(self, fp64_factor_ratio=2.0, budget=30):
+    def __init__(self, fp64_factor_ratio=2.0, budget=30, stall_solves=4):
         self.points = {}          # log10(f) -> refinement solves (inf: FP32 not viable)
         self.factor32 = None
         self.factor64 = None
@@ -37,6 +45,7 @@ class PrecisionPolicy:
         self.solve64 = None
         self.ratio = fp64_factor_ratio
         self.budget = budget
+        self.stall_solves = stall_solves
 
     def expected_solves(self, freq):
         if not self.points:
@@ -78,7 +87,15 @@ class PrecisionPolicy:
             return APHI_FP64
         f64 = self.factor64 if self.factor64 is not None else self.ratio * self.factor32
         s64 = self.solve64 if self.solve64 is not None else self.solve32
-        return APHI_FP32 if self.factor32 + (1.0 + k) * self.solve32 <= f64 + s64 else APHI_FP64
+        return APHI_FP32 if self.fp32_cost(k, f64 + s64) <= f64 + s64 else APHI_FP64
+
+    def fp32_cost(self, k, fp64_point):
+        """Expected cost of an FP32 attempt expecting k refinement solves."""
+        ok = self.factor32 + (1.0 + k) * self.solve32
+        start = 0.6 * self.budget
+        risk = min(1.0, max(0.0, (k - start) / (self.budget - start))) ** 2
+        failed = self.factor32 + self.stall_solves * self.solve32 + fp64_point
+        return (1.0 - risk) * ok + risk * failed
 
     def record(self, freq, precision, report):
         """Feeds back one solve_symmetric report (precision = the one requested)."""

@@ -34,12 +34,35 @@ def test_cost_model_threshold():
     p = A.PrecisionPolicy()
     p.record(1e3, A.APHI_FP32, rep(3, A.APHI_FP32, 800.0, 100.0))        # 25 ms per solve
     p.record(1e5, A.APHI_FP64, rep(0, A.APHI_FP64, 1650.0, 35.0))
+    p.record(1e4, A.APHI_FP32, rep(12, A.APHI_FP32, 800.0, 13 * 25.0))
+    # between them: about 5.7 solves, far from the budget: the plain cost decides,
+    # 800 + 6.7*25 = 968 <= 1650 + 35
+    assert p.choose(3e3) == A.APHI_FP32
     p.record(1e6, A.APHI_FP32, rep(30, A.APHI_FP32, 800.0, 31 * 25.0))   # slow but converged
-    # just above it the rising trend (3 solves at 1e3, 30 at 1e6) predicts about
-    # 31.5: 800 + 32.5*25 = 1612 <= 1650 + 35, still FP32
-    assert p.choose(1.2e6) == A.APHI_FP32
-    p.record(2e6, A.APHI_FP32, rep(36, A.APHI_FP32, 800.0, 37 * 25.0))  # 800 + 37*25 = 1725 > 1685
+    # just above it the rising trend predicts more than the budget: 800 + 32.5*25 =
+    # 1612 would beat 1685 if it converged, but failure is then certain and costs
+    # 800 + 4*25 + 1685, so FP64
+    assert p.choose(1.2e6) == A.APHI_FP64
+    p.record(2e6, A.APHI_FP32, rep(36, A.APHI_FP32, 800.0, 37 * 25.0))
     assert p.choose(3e6) == A.APHI_FP64
+
+
+def test_failure_risk_near_the_budget():
+    """The FP32 cost is weighted by the likelihood of a stalled attempt: none up to
+    60 % of the budget, certain at the budget."""
+    p = A.PrecisionPolicy(budget=30)
+    p.record(1e2, A.APHI_FP32, rep(2, A.APHI_FP32, 615.0, 3 * 19.0))
+    p.record(1e5, A.APHI_FP64, rep(0, A.APHI_FP64, 1364.0, 29.0))
+    fp64 = 1364.0 + 29.0
+    assert p.fp32_cost(18.0, fp64) == 615.0 + 19.0 * 19.0                 # no risk up to 18
+    r = ((24.0 - 18.0) / 12.0) ** 2
+    assert abs(p.fp32_cost(24.0, fp64) - ((1 - r) * (615.0 + 25.0 * 19.0) + r * (615.0 + 4 * 19.0 + fp64))) < 1e-9
+    assert p.fp32_cost(30.0, fp64) == 615.0 + 4 * 19.0 + fp64             # certain failure
+    k = 25.9                                                                # a rising trend near the budget
+    plain = 615.0 + (1.0 + k) * 19.0
+    assert plain < fp64 < p.fp32_cost(k, fp64)                              # cheaper if it converged, not worth the risk
+    k = 20.6
+    assert p.fp32_cost(k, fp64) < fp64                                      # small risk: FP32 still pays
 
 
 def test_interpolates_between_an_easy_and_a_failed_neighbour():
