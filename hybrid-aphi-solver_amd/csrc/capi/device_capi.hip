@@ -317,8 +317,9 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
         // columns. After a block, its rank-kOuter update is split: the next
         // block's columns on the main stream, the rest of the trailing front on
         // the side stream, so that the next block's factorization (main) runs
-        // while the bulk of the update (side) does. The next split waits for the
-        // side stream's previous update, whose targets overlap.
+        // while the bulk of the update (side) does. The side stream's update starts
+        // as soon as the block's panel is final; the next block's part waits for
+        // the side stream's previous update, whose targets overlap it.
         const int nouter = (maxk + aphi_dev::kOuter - 1) / aphi_dev::kOuter;
         bool side_busy = false;
         for (int ob = 0; ob < nouter; ++ob) {
@@ -336,13 +337,17 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                                                pivot_floor, st);
                 update(s, b, aphi_dev::kNB, 0, bo + aphi_dev::kOuter);
             }
+            // the previous block's rest update must be done before this block's
+            // next-block part (they share the columns [bo + kOuter, bo + 2 kOuter))
             if (side_busy) HIPCHK(hipStreamWaitEvent(s, m->ctx->side_done, 0));
             if (ob + 1 < nouter) {
-                update(s, bo, aphi_dev::kOuter, 0, bo + 2 * aphi_dev::kOuter);
+                // the rest only needs this block's panel (final here): it starts now,
+                // beside the next-block part, whose columns it does not touch
                 HIPCHK(hipEventRecord(m->ctx->main_done, s));
                 HIPCHK(hipStreamWaitEvent(m->ctx->side, m->ctx->main_done, 0));
                 update(m->ctx->side, bo, aphi_dev::kOuter, bo + 2 * aphi_dev::kOuter, -1);
                 HIPCHK(hipEventRecord(m->ctx->side_done, m->ctx->side));
+                update(s, bo, aphi_dev::kOuter, 0, bo + 2 * aphi_dev::kOuter);
                 side_busy = true;
             } else {
                 update(s, bo, aphi_dev::kOuter, 0, -1);
