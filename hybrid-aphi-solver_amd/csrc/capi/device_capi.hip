@@ -257,9 +257,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     m->X256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
     m->XT256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
     m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
-    aphi_dev::FactorStatsDev init{std::numeric_limits<double>::infinity(), 0.0, 0.0, INT_MAX, 0};
-    HIPCHK(hipMemcpyAsync(m->stats.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
     HIPCHK(hipEventRecord(m->ctx->ev0, s));
+    aphi_dev::launch_stats_init(s, m->stats.as<aphi_dev::FactorStatsDev>());
     aphi_dev::launch_zero<T>(s, m->factor_entries + aphi_dev::kPanelTail, m->P.p);
     aphi_dev::launch_scatter_panel<T>(s, m->nnz, m->s2p.as<int64_t>(), m->Aeq.as<cd>(), m->P.p);
     const int* nodes = m->nodes.as<int>();
@@ -360,9 +359,16 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     aphi_dev::launch_dinv<T>(s, m->num_super, fp, m->P.p, m->dinv.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(m->ctx->ev1, s));
-    aphi_dev::FactorStatsDev h{};
-    HIPCHK(hipMemcpyAsync(&h, m->stats.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    std::vector<aphi_dev::StatSlot> slots(aphi_dev::kStatSlots);
+    HIPCHK(hipMemcpyAsync(slots.data(), m->stats.p, sizeof(aphi_dev::FactorStatsDev), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    aphi_dev::StatSlot h = slots[0];
+    for (const aphi_dev::StatSlot& q : slots) {
+        h.smallest_pivot = std::min(h.smallest_pivot, q.smallest_pivot);
+        h.largest_pivot = std::max(h.largest_pivot, q.largest_pivot);
+        h.largest_multiplier = std::max(h.largest_multiplier, q.largest_multiplier);
+        h.fail_col = std::min(h.fail_col, q.fail_col);
+    }
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, m->ctx->ev0, m->ctx->ev1));
     aphi_factor_stats r{};

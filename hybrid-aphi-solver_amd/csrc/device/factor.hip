@@ -60,43 +60,53 @@ __device__ __forceinline__ void stat_min(double* a, double v) {
     atomicMin(reinterpret_cast<unsigned long long*>(a), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
-// Records |d| and checks it against the floor (src/factorization.cpp:191-210).
-template <typename T>
-__device__ __forceinline__ void record_pivot(FactorStatsDev* st, const FrontPlan& fp, int col, C<T> d, double floor_) {
-    const double mag = hypot(static_cast<double>(d.re), static_cast<double>(d.im));
-    stat_min(&st->smallest_pivot, mag);
-    atomic_max_nonneg(&st->largest_pivot, mag);
-    if (!(mag > floor_)) atomicMin(&st->fail_col, fp.col_ref[col]);
+// The statistics slot of the calling wave: neighbouring workgroups and the waves
+// of one workgroup land on different slots.
+__device__ __forceinline__ StatSlot* stat_slot(FactorStatsDev* st) {
+    const unsigned k = blockIdx.x * 37u + blockIdx.y * 101u + threadIdx.x / 64u;
+    return &st->slot[k % kStatSlots];
 }
 
-// Pivot statistics of up to 64 pivots held one per lane by one wavefront (lanes
-// with active = false hold none): reduced across the wave first, then one
-// atomic per statistic, since atomics from every lane on the same address
-// serialise.
+// One lane's running pivot statistics: |D| extremes and the smallest reference
+// column whose pivot is not above the floor (src/factorization.cpp:191-210).
+struct PivotAcc {
+    double mn = __builtin_inf();
+    double mx = 0.0;
+    int fail = INT_MAX;
+    // col: factor-order column; its reference column is looked up only on failure
+    __device__ __forceinline__ void add(double mag, const int* col_ref, int col, double floor_) {
+        mn = fmin(mn, mag);
+        mx = fmax(mx, mag);
+        if (!(mag > floor_)) fail = min(fail, col_ref[col]);
+    }
+};
+
 template <typename T>
-__device__ __forceinline__ void wave_pivot_stats(FactorStatsDev* st, const FrontPlan& fp, int col, C<T> d, bool active,
-                                                 double floor_) {
-    const double mag = active ? hypot(static_cast<double>(d.re), static_cast<double>(d.im)) : 0.0;
-    double mn = active ? mag : __builtin_inf(), mx = mag;
-    int fail = (active && !(mag > floor_)) ? fp.col_ref[col] : INT_MAX;
+__device__ __forceinline__ double cmag(C<T> d) {
+    return hypot(static_cast<double>(d.re), static_cast<double>(d.im));
+}
+
+// Reduces the lanes' pivot statistics across the (whole, converged) wave, then
+// one atomic per statistic into the wave's slot.
+__device__ __forceinline__ void flush_pivots(FactorStatsDev* st, PivotAcc a) {
     for (int off = 32; off > 0; off >>= 1) {
-        mn = fmin(mn, __shfl_xor(mn, off));
-        mx = fmax(mx, __shfl_xor(mx, off));
-        fail = min(fail, __shfl_xor(fail, off));
+        a.mn = fmin(a.mn, __shfl_xor(a.mn, off));
+        a.mx = fmax(a.mx, __shfl_xor(a.mx, off));
+        a.fail = min(a.fail, __shfl_xor(a.fail, off));
     }
     if ((threadIdx.x & 63) == 0) {
-        if (mn < __builtin_inf()) stat_min(&st->smallest_pivot, mn);
-        atomic_max_nonneg(&st->largest_pivot, mx);
-        if (fail != INT_MAX) atomicMin(&st->fail_col, fail);
+        StatSlot* sl = stat_slot(st);
+        if (a.mn < __builtin_inf()) stat_min(&sl->smallest_pivot, a.mn);
+        if (a.mx > 0.0) atomic_max_nonneg(&sl->largest_pivot, a.mx);
+        if (a.fail != INT_MAX) atomicMin(&sl->fail_col, a.fail);
     }
 }
 
-// Block-wide max of a non-negative value into the stats.
-__device__ __forceinline__ void block_max_multiplier(FactorStatsDev* st, double v, double* red) {
-    // wave64 reduction then one atomic per wave
+// Block-wide max of a non-negative value into the stats: a wave64 reduction, then
+// one atomic per wave into its slot.
+__device__ __forceinline__ void block_max_multiplier(FactorStatsDev* st, double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&st->largest_multiplier, v);
-    (void)red;
+    if ((threadIdx.x & 63) == 0 && v > 0.0) atomic_max_nonneg(&stat_slot(st)->largest_multiplier, v);
 }
 
 }  // namespace
@@ -107,6 +117,11 @@ template <typename T>
 __global__ void k_zero(int64_t n, C<T>* __restrict__ p) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = C<T>{0, 0};
+}
+
+__global__ void k_stats_init(FactorStatsDev* st) {
+    const int i = threadIdx.x;
+    if (i < kStatSlots) st->slot[i] = StatSlot{__builtin_inf(), 0.0, 0.0, INT_MAX, 0};
 }
 
 // The equilibrated A, scattered into the panels (each entry lands in the front
@@ -156,9 +171,10 @@ __global__ void __launch_bounds__(256) k_front_small(const int* __restrict__ nod
     __syncthreads();
 
     double lmax = 0.0;
+    PivotAcc piv;
     for (int j = 0; j < k; ++j) {
         const C<T> d = F[j * m + j];
-        if (tid == 0) record_pivot<T>(st, fp, fp.first[s] + j, d, pivot_floor);
+        if (tid == 0) piv.add(cmag<T>(d), fp.col_ref, fp.first[s] + j, pivot_floor);
         const C<T> rd = grecip(d);
         for (int i = j + 1 + tid; i < m; i += nt) {
             const C<T> l = gmul(F[j * m + i], rd);
@@ -177,7 +193,8 @@ __global__ void __launch_bounds__(256) k_front_small(const int* __restrict__ nod
         for (int i = j + 1 + tid; i < m; i += nt) F[j * m + i] = lcol[i];
         __syncthreads();
     }
-    block_max_multiplier(st, lmax, nullptr);
+    if (tid < 64) flush_pivots(st, piv);
+    block_max_multiplier(st, lmax);
 
     // Inverses of the unit-lower diagonal blocks (kNB columns each), built in the
     // unused strict upper triangle of F: Linv[r][c] (r > c) lives at F[r*m + c].
@@ -411,8 +428,12 @@ __global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, in
         }
     }
     __syncthreads();
-    if (tid < 64) wave_pivot_stats<T>(st, fp, fp.first[s] + b + min(tid, nb - 1), Dl[tid], tid < nb, pivot_floor);
-    block_max_multiplier(st, lmax, nullptr);
+    if (tid < 64) {
+        PivotAcc piv;
+        if (tid < nb) piv.add(cmag<T>(Dl[tid]), fp.col_ref, fp.first[s] + b + tid, pivot_floor);
+        flush_pivots(st, piv);
+    }
+    block_max_multiplier(st, lmax);
     // M[q][c] = Linv[c][q] / D[c] (c >= q) at mws[q*kNB + c]; the inverse, column q
     // and row c at q*nb + c, is kept for the solves
     C<T>* M = mws + static_cast<int64_t>(blockIdx.x) * kNB * kNB;
@@ -480,7 +501,7 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
             }
         }
     }
-    block_max_multiplier(st, lmax, nullptr);
+    block_max_multiplier(st, lmax);
 }
 
 // ------------------------------------------------------------------ trailing update (MFMA)
@@ -812,6 +833,10 @@ void launch_scatter_panel(hipStream_t st, int64_t nnz, const int64_t* s2p, const
     if (nnz <= 0) return;
     hipLaunchKernelGGL(k_scatter_panel<T>, dim3(static_cast<unsigned>((nnz + 255) / 256)), dim3(256), 0, st, nnz, s2p, a,
                        static_cast<C<T>*>(P));
+}
+
+void launch_stats_init(hipStream_t st, FactorStatsDev* stats) {
+    hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(kStatSlots), 0, st, stats);
 }
 
 template <typename T>

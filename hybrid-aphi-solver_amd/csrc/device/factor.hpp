@@ -17,6 +17,7 @@ constexpr int kQ = 16;          // right-hand sides per interleaved solve group
 constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a multiple of 64)
 constexpr int kBwdRows = 256;     // rows per backward partial sum
 
+void launch_stats_init(hipStream_t st, FactorStatsDev* stats);  // every slot to its identity
 template <typename T> void launch_zero(hipStream_t st, int64_t n, void* p);
 template <typename T> void launch_scatter_panel(hipStream_t st, int64_t nnz, const int64_t* s2p, const cd* a, void* P);
 template <typename T> size_t small_lds_bytes(int max_m);

@@ -49,12 +49,19 @@ struct FrontPlan {
     const int64_t* x256_off;   // ns
 };
 
-struct FactorStatsDev {
+// Factorization statistics, spread over kStatSlots cache-line slots so that the
+// atomics of many workgroups do not all meet on one address (same-address
+// device-scope atomics serialise); the host reduces the slots afterwards.
+constexpr int kStatSlots = 256;
+struct alignas(128) StatSlot {
     double smallest_pivot;   // atomic min of |D|
     double largest_pivot;    // atomic max of |D|
     double largest_multiplier;
     int fail_col;            // smallest reference column whose pivot failed, or INT_MAX
     int pad;
+};
+struct FactorStatsDev {
+    StatSlot slot[kStatSlots];
 };
 
 // assembly.hip
