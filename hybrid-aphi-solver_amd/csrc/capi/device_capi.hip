@@ -284,29 +284,41 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                                        (L.large_max_m + aphi_dev::kEaRows - 1) / aphi_dev::kEaRows, fp, m->P.p, m->CB.p);
         int maxk = 0;
         for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
-        const int ts = aphi_dev::update_tile<T>();
-        // one update launch: the largest tile count over the level's fronts, and its flops
         // one update launch on stream `on`: the largest tile count over the level's
-        // fronts, and its flops (region semantics of launch_update)
+        // fronts, and its flops (region semantics of launch_update); complex float
+        // takes the small tiles when the tallest region is short
         const auto update = [&](hipStream_t on, int c0, int width, int tlo, int thi) {
+            const auto region = [&](int i, int& t0, int& tend) {
+                const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
+                if (kk <= c0) return false;
+                const int c1 = std::min(c0 + width, kk);
+                t0 = std::max(c1, std::min(tlo, kk));
+                tend = thi < 0 ? mm : std::min(thi, kk);
+                return tend > t0;
+            };
+            int hmax = 0;
+            for (int i = 0; i < L.large_count; ++i) {
+                int t0, tend;
+                if (region(i, t0, tend)) hmax = std::max(hmax, L.m[static_cast<size_t>(L.small_count + i)] - t0);
+            }
+            if (hmax == 0) return;
+            const bool small = std::is_same<T, float>::value && hmax <= aphi_dev::kSmallTileRows;
+            const int ts = aphi_dev::update_tile<T>(small);
             int tiles = 0;
             double fl = 0.0;
             for (int i = 0; i < L.large_count; ++i) {
+                int t0, tend;
+                if (!region(i, t0, tend)) continue;
                 const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
-                if (kk <= c0) continue;
                 const int c1 = std::min(c0 + width, kk);
-                const int t0 = std::max(c1, std::min(tlo, kk));
-                const int tend = thi < 0 ? mm : std::min(thi, kk);
-                if (tend <= t0) continue;
                 const int nrt = (mm - t0 + ts - 1) / ts, nct = (tend - t0 + ts - 1) / ts;
                 tiles = std::max(tiles, nct * nrt - nct * (nct - 1) / 2);
                 const double kw = c1 - c0, w = tend - t0, h = mm - t0;
                 fl += 8.0 * kw * (w * h - w * (w - 1.0) / 2.0);
             }
-            if (tiles == 0) return;
             const bool timed = m->profiling;
             if (timed) HIPCHK(hipEventRecord(next_event(), on));
-            aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, 0, fp, m->P.p, m->CB.p);
+            aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, 0, fp, m->P.p, m->CB.p, small);
             if (timed) HIPCHK(hipEventRecord(next_event(), on));
             ++launches;
             update_flops += fl;

@@ -519,11 +519,20 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
 // accumulator sets, see k_update), 64 x 64 for complex double with four waves of
 // 32 x 32 (two sets). The per-wave accumulators set the register budget: 96 or 64
 // registers of accumulators per lane, two waves per SIMD.
-template <typename T> struct UpdTile;
-template <> struct UpdTile<float> {
+//
+// Small fronts waste most of a 128 x 128 tile, so complex float also has a 64 x 64
+// variant (Small) with four waves of 32 x 32 and 16-column stages (36 KB of LDS,
+// four workgroups per compute unit); its wave DMA covers 64 rows of two columns.
+// Either tiling accumulates each entry's products in the same order (k ascending,
+// two per MFMA), so the two give identical results.
+template <typename T, bool Small = false> struct UpdTile;
+template <> struct UpdTile<float, false> {
     static constexpr int TS = 128, WX = 2, WY = 4, GX = 4, GY = 2, KC = 32, NW = GX * GY, MINB = 1, WPC = 1;
 };
-template <> struct UpdTile<double> {
+template <> struct UpdTile<float, true> {
+    static constexpr int TS = 64, WX = 2, WY = 2, GX = 2, GY = 2, KC = 16, NW = GX * GY, MINB = 4, WPC = 4;
+};
+template <> struct UpdTile<double, false> {
     static constexpr int TS = 64, WX = 2, WY = 2, GX = 2, GY = 2, KC = 8, NW = GX * GY, MINB = 2, WPC = 3;
 };
 
@@ -554,17 +563,19 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
 // row i, so stores are contiguous. Reads past a column's end (rows >= m) land in
 // rows of the tile that are never stored, so the factor storage carries a small
 // zeroed tail for them.
-template <typename T>
-__global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
+template <typename T, bool Small = false>
+__global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Small>::MINB))
     k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi, FrontPlan fp, C<T>* P, C<T>* CB) {
     using V4 = typename Mfma<T>::v4;
-    using UT = UpdTile<T>;
+    using UT = UpdTile<T, Small>;
     constexpr int TS = UT::TS, WX = UT::WX, WY = UT::WY, KC = UT::KC, NW = UT::NW;
     static_assert(UT::GX * WX * 16 == TS && UT::GY * WY * 16 == TS, "waves tile the update tile");
-    static_assert((2 * KC) % NW == 0, "stage DMAs divide among the waves");
     constexpr int RPL = 16 / static_cast<int>(sizeof(C<T>));  // rows per lane per DMA
-    static_assert(TS == 64 * RPL, "one wave DMA covers one tile column segment");
-    constexpr int LDR = TS + 16;                         // LDS row stride (complex)
+    constexpr int CPD = 64 * RPL / TS;                       // tile column segments per wave DMA
+    static_assert(TS * CPD == 64 * RPL && (CPD == 1 || CPD == 2), "a wave DMA covers whole column segments");
+    static_assert((2 * KC / CPD) % NW == 0 && KC % CPD == 0, "stage DMAs divide among the waves");
+    // LDS row stride (complex): padded, except where one DMA fills two adjacent rows
+    constexpr int LDR = CPD == 1 ? TS + 16 : TS;
     __shared__ C<T> Lc[2][KC][LDR], Wc[2][KC][LDR];
     __shared__ C<T> Dsh[kOuter];
     const int s = nodes[blockIdx.y];
@@ -610,13 +621,15 @@ __global__ void __launch_bounds__(64 * UpdTile<T>::NW, UpdTile<T>::MINB)
     const int nst = (K + KC - 1) / KC;
 
     // stage st of the tile at (i0, j0): 2*KC column segments (L rows j0.. then rows i0..), 2*KC/NW per wave
+    // (CPD = 2: lanes 0-31 fill row q of the LDS buffer, lanes 32-63 row q + 1)
     const auto issue_stage = [&](int buf, int st, int i0, int j0) {
+        const int sub = lane / (64 / CPD), lr = lane % (64 / CPD);
 #pragma unroll
-        for (int u = 0; u < 2 * KC / NW; ++u) {
-            const int jdx = wave + NW * u;
-            const int mat = jdx / KC, q = jdx % KC;
-            const int c = min(st * KC + q, K - 1);
-            const int row = min((mat ? i0 : j0) + RPL * lane, fr.m - 1);
+        for (int u = 0; u < 2 * KC / CPD / NW; ++u) {
+            const int seg = (wave + NW * u) * CPD;
+            const int mat = seg / KC, q = seg % KC;
+            const int c = min(st * KC + q + sub, K - 1);
+            const int row = min((mat ? i0 : j0) + RPL * lr, fr.m - 1);
             lds_dma16(L0 + static_cast<int64_t>(c) * fr.m + row, mat ? &Wc[buf][q][0] : &Lc[buf][q][0]);
         }
     };
@@ -893,8 +906,15 @@ int device_cus() {
 
 template <typename T>
 void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
-                   int max_wgs, const FrontPlan& fp, void* P, void* CB) {
+                   int max_wgs, const FrontPlan& fp, void* P, void* CB, bool small) {
     if (count <= 0 || max_tiles <= 0) return;
+    if constexpr (std::is_same<T, float>::value) {
+        if (small) {
+            hipLaunchKernelGGL((k_update<float, true>), dim3(max_tiles, count), dim3(64 * UpdTile<float, true>::NW), 0,
+                               st, nodes, c0, width, tlo, thi, fp, static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
+            return;
+        }
+    }
     // workgroups per front: one per tile, or (max_wgs > 0) a persistent grid of
     // about max_wgs workgroups over all fronts, a multiple of 8 (one share per
     // XCD). A persistent grid holds its compute units until it ends, so the
@@ -907,7 +927,8 @@ void launch_update(hipStream_t st, const int* nodes, int count, int c0, int widt
 }
 
 template <typename T>
-int update_tile() {
+int update_tile(bool small) {
+    if constexpr (std::is_same<T, float>::value) return small ? UpdTile<float, true>::TS : UpdTile<float>::TS;
     return UpdTile<T>::TS;
 }
 
@@ -923,13 +944,15 @@ template void launch_front_small<float>(hipStream_t, const int*, int, int, const
                                         double, FactorStatsDev*);
 template void launch_extend_add<double>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
 template void launch_extend_add<float>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
-template void launch_update<double>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*);
-template void launch_update<float>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*);
+template void launch_update<double>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*,
+                                   bool);
+template void launch_update<float>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*,
+                                  bool);
 template <typename T> int update_resident_wgs() { return device_cus() * UpdTile<T>::WPC; }
 template int update_resident_wgs<double>();
 template int update_resident_wgs<float>();
-template int update_tile<double>();
-template int update_tile<float>();
+template int update_tile<double>(bool);
+template int update_tile<float>(bool);
 template void launch_block_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,
                                         void*, void*, double, FactorStatsDev*);
 template void launch_block_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,

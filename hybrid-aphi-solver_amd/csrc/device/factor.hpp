@@ -12,6 +12,10 @@ constexpr int kNB = 64;       // block-column width of the blocked front factori
 constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64)
 constexpr int kOuter = 512;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
+// complex float updates whose tallest region has at most this many rows use the
+// 64 x 64 tiles (measured on the 32^3 cube: 591 ms with none, 586 ms at 8192,
+// 623 ms with all)
+constexpr int kSmallTileRows = 8192;
 constexpr int kSB = 256;        // solve block: columns per forward/backward step (multiple of kNB)
 constexpr int kQ = 16;          // right-hand sides per interleaved solve group
 constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a multiple of 64)
@@ -31,11 +35,13 @@ void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, i
 // front, of the columns [max(c1, min(tlo, k)), thi < 0 ? m : min(thi, k)) and
 // every row below them.
 // max_wgs = 0: one workgroup per tile; > 0: a persistent grid of about that many
-// workgroups (for an update that has the device to itself)
+// workgroups (for an update that has the device to itself). small: the 64 x 64
+// tiles of complex float (ignored for complex double); max_tiles counts tiles of
+// update_tile<T>(small).
 template <typename T>
 void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
-                   int max_wgs, const FrontPlan& fp, void* P, void* CB);
-template <typename T> int update_tile();          // tile side of k_update
+                   int max_wgs, const FrontPlan& fp, void* P, void* CB, bool small = false);
+template <typename T> int update_tile(bool small = false);  // tile side of k_update
 template <typename T> int update_resident_wgs();  // k_update workgroups the device holds at once
 template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
