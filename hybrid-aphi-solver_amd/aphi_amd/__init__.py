@@ -350,6 +350,14 @@ class Mesh:
         beside other threads' GPU work)."""
         _check(lib().aphi_mesh_set_solve_graphs(self._h, int(on)))
 
+    def set_update_tiles(self, mode="auto"):
+        """Complex-float trailing-update tiling: "auto" (64 x 64 tiles for regions of
+        at most 8192 rows), "small" or "large"; the factor is bitwise the same."""
+        modes = {"auto": 0, "small": 1, "large": 2}
+        if mode not in modes:
+            raise ValueError(f"update tiles must be one of {sorted(modes)}")
+        _check(lib().aphi_mesh_set_update_tiles(self._h, modes[mode]))
+
     def set_profiling(self, on=True):
         _check(lib().aphi_mesh_set_profiling(self._h, int(on)))
 

@@ -130,6 +130,7 @@ SIGNATURES = {
     "aphi_mesh_destroy": (None, [C.c_void_p]),
     "aphi_mesh_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "aphi_mesh_set_solve_graphs": (C.c_int, [C.c_void_p, C.c_int]),
+    "aphi_mesh_set_update_tiles": (C.c_int, [C.c_void_p, C.c_int]),
     "aphi_mesh_device_bytes": (C.c_int64, [C.c_void_p]),
     "aphi_refill": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "aphi_system_download": (C.c_int, [C.c_void_p, DP, DP]),

@@ -140,6 +140,7 @@ struct aphi_mesh {
     bool assembled = false, equilibrated = false, factored = false;
     bool has_plan = true;  // false for a problem made from a bare pattern (no refill)
     bool solve_graphs_on = true;  // aphi_mesh_set_solve_graphs
+    int update_tiles = 0;         // aphi_mesh_set_update_tiles: 0 by region height, 1 small, 2 large
     // gauge-free potential plan (aphi_gauge_free_attach), on a bare-pattern mesh
     struct GaugeFree {
         bool attached = false;
@@ -302,7 +303,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                 if (region(i, t0, tend)) hmax = std::max(hmax, L.m[static_cast<size_t>(L.small_count + i)] - t0);
             }
             if (hmax == 0) return;
-            const bool small = std::is_same<T, float>::value && hmax <= aphi_dev::kSmallTileRows;
+            const bool small = std::is_same<T, float>::value &&
+                               (m->update_tiles == 0 ? hmax <= aphi_dev::kSmallTileRows : m->update_tiles == 1);
             const int ts = aphi_dev::update_tile<T>(small);
             int tiles = 0;
             double fl = 0.0;
@@ -1080,6 +1082,14 @@ extern "C" int aphi_mesh_set_solve_graphs(aphi_mesh* m, int on) {
     return aphi_capi::guarded([&] {
         require(m != nullptr, "aphi_mesh_set_solve_graphs: null mesh");
         m->solve_graphs_on = on != 0;
+    });
+}
+
+extern "C" int aphi_mesh_set_update_tiles(aphi_mesh* m, int mode) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_mesh_set_update_tiles: null mesh");
+        require(mode >= 0 && mode <= 2, "aphi_mesh_set_update_tiles: mode must be 0, 1 or 2");
+        m->update_tiles = mode;
     });
 }
 

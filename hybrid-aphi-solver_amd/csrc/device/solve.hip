@@ -86,6 +86,21 @@ __device__ __forceinline__ void reduce4(C<T> (*part)[256], const V4<T>& accr, co
     out(tid / 16, tid % 16, t);
 }
 
+// reduce4 for NW = 4 waves; a one-wave workgroup (the levels of many small
+// fronts, where one wave covers the whole inner dimension) emits each lane's own
+// four entries instead.
+template <typename T, int NW, typename FO>
+__device__ __forceinline__ void reduce_waves(C<T> (*part)[256], const V4<T>& accr, const V4<T>& acci, FO out) {
+    static_assert(NW == 1 || NW == 4, "one or four waves");
+    if constexpr (NW == 1) {
+        const int lane = static_cast<int>(threadIdx.x % 64);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out(Mfma<T>::row(lane, v), lane % 16, C<T>{accr[v], acci[v]});
+    } else {
+        reduce4<T>(part, accr, acci, out);
+    }
+}
+
 }  // namespace
 
 // b in the caller's layout (column r at b + r * ldb) -> permuted, equilibrated,
@@ -283,11 +298,12 @@ __global__ void __launch_bounds__(64) k_dinv(FrontPlan fp, const C<T>* __restric
 
 // y rows [16 bx, 16 bx + 16) of the block: y = X w (X lower triangular, stored
 // with zeros above the diagonal); wave w takes the columns [64 w, 64 w + 64)
-template <typename T>
-__global__ void __launch_bounds__(256) k_fwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
-                                                  const C<T>* __restrict__ X256, const C<T>* __restrict__ W,
-                                                  C<T>* __restrict__ Y, int n, int64_t R) {
-    __shared__ C<T> part[4][256];
+// (NW = 1: blocks of at most 64 columns)
+template <typename T, int NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_fwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
+                                                      const C<T>* __restrict__ X256, const C<T>* __restrict__ W,
+                                                      C<T>* __restrict__ Y, int n, int64_t R) {
+    __shared__ C<T> part[NW][256];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z);
     const Sn x = sn(fp, s);
@@ -308,7 +324,7 @@ __global__ void __launch_bounds__(256) k_fwd_diag(const int* __restrict__ nodes,
                       return k0 + k < kmax ? v : C<T>{0, 0};
                   });
     C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
-    reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
+    reduce_waves<T, NW>(part, accr, acci, [&](int p, int q, C<T> t) {
         if (c0 + p < nb) Yg[(c0 + p) * kQ + q] = t;
     });
 }
@@ -354,12 +370,14 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
 // instruction covers 256 contiguous bytes of four columns, twice the run of the
 // one-row-per-lane mapping), and the two rows go to two 16-row MFMA tiles (even
 // rows, odd rows). Each output sums the same products, grouped and ordered as
-// in k_fwd_update.
-__global__ void __launch_bounds__(256) k_fwd_update2(const int* __restrict__ nodes, int B, FrontPlan fp,
-                                                     const cf* __restrict__ P, cf* W, const cf* __restrict__ Y, int n,
-                                                     int64_t R) {
+// in k_fwd_update. NW = 1: blocks of at most 64 columns, each lane updating its
+// own eight entries.
+template <int NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__ nodes, int B, FrontPlan fp,
+                                                         const cf* __restrict__ P, cf* W, const cf* __restrict__ Y, int n,
+                                                         int64_t R) {
     using VF = V4<float>;
-    __shared__ cf part[2][4][256];
+    __shared__ cf part[2][NW][256];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z);
     const Sn x = sn(fp, s);
@@ -399,23 +417,37 @@ __global__ void __launch_bounds__(256) k_fwd_update2(const int* __restrict__ nod
             i1a = Mfma<float>::mac(a[u].w, b[u].re, i1a);
         }
     }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        part[0][wave][Mfma<float>::row(lane, v) * 16 + p] = cf{r0[v], i0a[v]};
-        part[1][wave][Mfma<float>::row(lane, v) * 16 + p] = cf{r1[v], i1a[v]};
-    }
-    __syncthreads();
     cf* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
+    if constexpr (NW == 1) {
+        // tile row r of tile h is front row i0 + 2 r + h
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        cf t = part[h][0][tid];
-        t = gadd(t, part[h][1][tid]);
-        t = gadd(t, part[h][2][tid]);
-        t = gadd(t, part[h][3][tid]);
-        const int i = i0 + 2 * (tid / 16) + h;  // tile row tid / 16 of tile h is front row i0 + 2 (tid / 16) + h
-        if (i < x.m) {
-            cf& dst = Wg[static_cast<int64_t>(i) * kQ + tid % 16];
-            dst = gsub(dst, t);
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = i0 + 2 * Mfma<float>::row(lane, v) + h;
+                if (i < x.m) {
+                    cf& dst = Wg[static_cast<int64_t>(i) * kQ + p];
+                    dst = gsub(dst, h ? cf{r1[v], i1a[v]} : cf{r0[v], i0a[v]});
+                }
+            }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            part[0][wave][Mfma<float>::row(lane, v) * 16 + p] = cf{r0[v], i0a[v]};
+            part[1][wave][Mfma<float>::row(lane, v) * 16 + p] = cf{r1[v], i1a[v]};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            cf t = part[h][0][tid];
+            t = gadd(t, part[h][1][tid]);
+            t = gadd(t, part[h][2][tid]);
+            t = gadd(t, part[h][3][tid]);
+            const int i = i0 + 2 * (tid / 16) + h;  // tile row tid / 16 of tile h is front row i0 + 2 (tid / 16) + h
+            if (i < x.m) {
+                cf& dst = Wg[static_cast<int64_t>(i) * kQ + tid % 16];
+                dst = gsub(dst, t);
+            }
         }
     }
 }
@@ -477,12 +509,14 @@ __global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nod
 // The same partial sums for complex64 with one 16-byte load per lane for two
 // adjacent rows of a column (eight contiguous rows of each of 16 columns per load
 // instruction instead of four): within each group of eight rows, the even rows
-// form one MFMA k-group and the odd rows the next.
-__global__ void __launch_bounds__(256) k_bwd_partial2(const int* __restrict__ nodes, int B, int stride, int ctiles,
-                                                      FrontPlan fp, const cf* __restrict__ P, const cf* __restrict__ X,
-                                                      int n, cf* __restrict__ partial) {
+// form one MFMA k-group and the odd rows the next. NW = 1: at most 64 rows below
+// the block (one chunk, one wave).
+template <int NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_bwd_partial2(const int* __restrict__ nodes, int B, int stride, int ctiles,
+                                                          FrontPlan fp, const cf* __restrict__ P, const cf* __restrict__ X,
+                                                          int n, cf* __restrict__ partial) {
     using VF = V4<float>;
-    __shared__ cf part[4][256];
+    __shared__ cf part[NW][256];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
@@ -532,7 +566,7 @@ __global__ void __launch_bounds__(256) k_bwd_partial2(const int* __restrict__ no
         }
     }
     cf* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
-    reduce4<float>(part, accr, acci, [&](int p, int q, cf t) {
+    reduce_waves<float, NW>(part, accr, acci, [&](int p, int q, cf t) {
         if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;
     });
 }
@@ -568,12 +602,13 @@ __global__ void __launch_bounds__(256) k_bwd_reduce(const int* __restrict__ node
 
 // x rows [16 bx, 16 bx + 16) of the block: x[c] = sum_{k >= c} X[k][c] z[k],
 // z = y D^-1 - t (D^-1 from k_dinv); wave w takes k in [c0 + 64 w, c0 + 64 w + 64)
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
-                                                  const C<T>* __restrict__ dinv, const C<T>* __restrict__ XT256,
-                                                  const C<T>* __restrict__ Y, const C<T>* __restrict__ tsum,
-                                                  C<T>* __restrict__ X, int n) {
-    __shared__ C<T> part[4][256];
+// (NW = 1: blocks of at most 64 columns)
+template <typename T, int NW = 4>
+__global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
+                                                      const C<T>* __restrict__ dinv, const C<T>* __restrict__ XT256,
+                                                      const C<T>* __restrict__ Y, const C<T>* __restrict__ tsum,
+                                                      C<T>* __restrict__ X, int n) {
+    __shared__ C<T> part[NW][256];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
@@ -612,7 +647,7 @@ __global__ void __launch_bounds__(256) k_bwd_diag(const int* __restrict__ nodes,
         }
     }
     C<T>* Xg = X + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
-    reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
+    reduce_waves<T, NW>(part, accr, acci, [&](int p, int q, C<T> t) {
         if (c0 + p < nb) Xg[(c0 + p) * kQ + q] = t;
     });
 }
@@ -676,13 +711,23 @@ void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max
                      const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R) {
     if (count <= 0 || max_nb <= 0) return;
     const int G = rhs_groups(nrhs);
-    hipLaunchKernelGGL(k_fwd_diag<T>, dim3((max_nb + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
-                       static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
+    // blocks of at most 64 columns (the levels of many small fronts): one-wave workgroups
+    const bool one = max_nb <= 64;
+    if (one)
+        hipLaunchKernelGGL((k_fwd_diag<T, 1>), dim3((max_nb + 15) / 16, count, G), dim3(64), 0, st, nodes, B, fp,
+                           static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
+    else
+        hipLaunchKernelGGL((k_fwd_diag<T, 4>), dim3((max_nb + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
+                           static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
     if (max_rows_below > 0) {
-        if constexpr (std::is_same<T, float>::value)
-            hipLaunchKernelGGL(k_fwd_update2, dim3((max_rows_below + 31) / 32, count, G), dim3(256), 0, st, nodes, B, fp,
-                               static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y), n, R);
-        else
+        if constexpr (std::is_same<T, float>::value) {
+            if (one)
+                hipLaunchKernelGGL(k_fwd_update2<1>, dim3((max_rows_below + 31) / 32, count, G), dim3(64), 0, st, nodes,
+                                   B, fp, static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y), n, R);
+            else
+                hipLaunchKernelGGL(k_fwd_update2<4>, dim3((max_rows_below + 31) / 32, count, G), dim3(256), 0, st, nodes,
+                                   B, fp, static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y), n, R);
+        } else
             hipLaunchKernelGGL(k_fwd_update<T>, dim3((max_rows_below + 15) / 16, count, G), dim3(256), 0, st, nodes, B,
                                fp, static_cast<const C<T>*>(P), static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n, R);
     }
@@ -696,21 +741,38 @@ void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max
     const int G = rhs_groups(nrhs);
     const int ctiles = (max_nb + 15) / 16;
     const int stride = bwd_chunks(max_rows_below);
+    // the sums t the diagonal step reads: with one chunk of rows the partial sums
+    // are already t (same layout), and the reduction is skipped
+    const void* t = tsum;
     if (max_rows_below > 0) {
-        if constexpr (std::is_same<T, float>::value)
-            hipLaunchKernelGGL(k_bwd_partial2, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride,
-                               ctiles, fp, static_cast<const cf*>(P), static_cast<const cf*>(X), n,
-                               static_cast<cf*>(partial));
-        else
+        if constexpr (std::is_same<T, float>::value) {
+            if (max_rows_below <= 64)
+                hipLaunchKernelGGL(k_bwd_partial2<1>, dim3(stride * ctiles, count, G), dim3(64), 0, st, nodes, B, stride,
+                                   ctiles, fp, static_cast<const cf*>(P), static_cast<const cf*>(X), n,
+                                   static_cast<cf*>(partial));
+            else
+                hipLaunchKernelGGL(k_bwd_partial2<4>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride,
+                                   ctiles, fp, static_cast<const cf*>(P), static_cast<const cf*>(X), n,
+                                   static_cast<cf*>(partial));
+        } else {
             hipLaunchKernelGGL(k_bwd_partial<T>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride,
                                ctiles, fp, static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n,
                                static_cast<C<T>*>(partial));
-        hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(max_nb, count, G), dim3(256), 0, st, nodes, B, stride, fp,
-                           static_cast<const C<T>*>(partial), static_cast<C<T>*>(tsum));
+        }
+        if (stride == 1)
+            t = partial;
+        else
+            hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(max_nb, count, G), dim3(256), 0, st, nodes, B, stride, fp,
+                               static_cast<const C<T>*>(partial), static_cast<C<T>*>(tsum));
     }
-    hipLaunchKernelGGL(k_bwd_diag<T>, dim3(ctiles, count, G), dim3(256), 0, st, nodes, B, fp,
-                       static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
-                       static_cast<const C<T>*>(tsum), static_cast<C<T>*>(X), n);
+    if (max_nb <= 64)
+        hipLaunchKernelGGL((k_bwd_diag<T, 1>), dim3(ctiles, count, G), dim3(64), 0, st, nodes, B, fp,
+                           static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
+                           static_cast<const C<T>*>(t), static_cast<C<T>*>(X), n);
+    else
+        hipLaunchKernelGGL((k_bwd_diag<T, 4>), dim3(ctiles, count, G), dim3(256), 0, st, nodes, B, fp,
+                           static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
+                           static_cast<const C<T>*>(t), static_cast<C<T>*>(X), n);
 }
 
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
