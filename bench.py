@@ -285,7 +285,7 @@ def main():
         "solve_ms_fp64": solve64_ms,
         "roofline": {"bound": "mfma", "achieved": ach32, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": ach32 / MFMA_FP32_PEAK_TFLOPS, "traffic": update_traffic(a.n),
-                     "kernel": "k_update<float> (MFMA trailing update)",
+                     "kernel": "k_update<float, *> (MFMA trailing update, 128x128 and 64x64 tiles together)",
                      "launches": int(st32["update_launches"]),
                      "avg_launch_ms": st32["update_ms"] / max(1, st32["update_launches"]),
                      "flops_per_factor": st32["update_flops"]},

@@ -322,7 +322,9 @@ __global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, in
     constexpr int NA = kNB * (kNB + 1) / 2;      // A items (lower triangle with the diagonal)
     constexpr int NX = kNB * (kNB - 1) / 2;      // X items (strict lower triangle)
     constexpr int NT = 1024;
-    constexpr int NE = (NA + NX + NT - 1) / NT;  // items per thread
+    // items per thread: A items in the first NEA slots, X items in the next NEX, so
+    // that every slot's kind is known at compile time (slots padded with empty items)
+    constexpr int NEA = (NA + NT - 1) / NT, NEX = (NX + NT - 1) / NT, NE = NEA + NEX;
     C<T>* Xs = reinterpret_cast<C<T>*>(smem_raw); // Xs[c*LD + r] = X[r][c]
     C<T>* colA = Xs + kNB * LD;                   // two pivot-column buffers
     C<T>* rowX = colA + 2 * kNB;                  // two pivot-row buffers
@@ -340,9 +342,8 @@ __global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, in
     T vre[NE], vim[NE];
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
-        const int item = tid + NT * u;
-        const bool x = item >= NA;
-        const int e = x ? item - NA : item;
+        const bool x = u >= NEA;
+        const int e = tid + NT * (x ? u - NEA : u);
         const int len = x ? kNB - 1 : kNB;       // entries of column 0
         const int total = x ? NX : NA;
         const float q = 2.0f * len + 1.0f;
@@ -373,6 +374,10 @@ __global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, in
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
         int l0 = isx[u] ? ic[u] : 0, h0 = isx[u] ? ir[u] - 1 : ic[u];
+        if (ic[u] > kNB) {  // an empty slot: no step
+            l0 = kNB;
+            h0 = -1;
+        }
         for (int off = 32; off > 0; off >>= 1) {
             l0 = min(l0, __shfl_xor(l0, off));
             h0 = max(h0, __shfl_xor(h0, off));
