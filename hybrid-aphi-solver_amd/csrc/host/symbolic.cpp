@@ -65,6 +65,91 @@ void reach_counts(const SymmetricAdjacency& adj, const std::vector<int>& parent,
     }
 }
 
+// The same row and column counts in time nearly linear in nnz(A) (Gilbert, Ng and
+// Peyton, SIAM J. Matrix Anal. Appl. 15(4), 1994, as set out in Davis, "Direct
+// Methods for Sparse Linear Systems", SIAM 2006, section 4.5): the row subtree of
+// row i is the union of the tree paths from its leaves up to i. Visiting the
+// nodes in postorder, node j is a leaf of row subtree i (A(i, j) != 0, i > j)
+// exactly when its first descendant comes after the previous leaf's; the new path
+// from j stops below q, the least common ancestor of j and the previous leaf,
+// found by a path-compressed union-find over the nodes already finished. Row i
+// gains depth(j) - depth(q) nodes (q = i for the first leaf); column counts come
+// from the same leaves as differences summed up the tree.
+void skeleton_counts(const SymmetricAdjacency& adj, const std::vector<int>& parent, std::vector<int>& rows,
+                     std::vector<int>& cols) {
+    const int n = adj.rows();
+    const auto at = [](std::vector<int>& v, int i) -> int& { return v[static_cast<size_t>(i)]; };
+    // postorder, children ascending
+    std::vector<int> head(static_cast<size_t>(n), -1), next(static_cast<size_t>(n), -1), post, stack;
+    post.reserve(static_cast<size_t>(n));
+    for (int j = n - 1; j >= 0; --j) {
+        const int p = parent[static_cast<size_t>(j)];
+        if (p < 0) continue;
+        at(next, j) = at(head, p);
+        at(head, p) = j;
+    }
+    for (int r = 0; r < n; ++r) {
+        if (parent[static_cast<size_t>(r)] >= 0) continue;
+        stack.push_back(r);
+        while (!stack.empty()) {
+            const int v = stack.back();
+            const int c = at(head, v);
+            if (c < 0) {
+                stack.pop_back();
+                post.push_back(v);
+            } else {
+                at(head, v) = at(next, c);
+                stack.push_back(c);
+            }
+        }
+    }
+    // depth (roots 0; parents come after their children in postorder) and first
+    // descendant (postorder position); a leaf of the tree starts its column at 1
+    std::vector<int> depth(static_cast<size_t>(n), 0), first(static_cast<size_t>(n), -1);
+    cols.assign(static_cast<size_t>(n), 0);
+    for (int k = n - 1; k >= 0; --k) {
+        const int j = post[static_cast<size_t>(k)], p = parent[static_cast<size_t>(j)];
+        at(depth, j) = p < 0 ? 0 : at(depth, p) + 1;
+    }
+    for (int k = 0; k < n; ++k) {
+        int j = post[static_cast<size_t>(k)];
+        at(cols, j) = at(first, j) == -1 ? 1 : 0;
+        for (; j != -1 && at(first, j) == -1; j = parent[static_cast<size_t>(j)]) at(first, j) = k;
+    }
+    std::vector<int> ancestor(static_cast<size_t>(n)), maxfirst(static_cast<size_t>(n), -1),
+        prevleaf(static_cast<size_t>(n), -1);
+    std::iota(ancestor.begin(), ancestor.end(), 0);
+    rows.assign(static_cast<size_t>(n), 1);
+    for (int k = 0; k < n; ++k) {
+        const int j = post[static_cast<size_t>(k)], p = parent[static_cast<size_t>(j)];
+        if (p != -1) --at(cols, p);
+        for (int t = adj.offset[static_cast<size_t>(j)]; t < adj.offset[static_cast<size_t>(j) + 1]; ++t) {
+            const int i = adj.neighbor[static_cast<size_t>(t)];
+            if (i <= j || at(first, j) <= at(maxfirst, i)) continue;  // j is no leaf of row subtree i
+            at(maxfirst, i) = at(first, j);
+            const int jprev = at(prevleaf, i);
+            at(prevleaf, i) = j;
+            int q = i;
+            if (jprev != -1) {
+                for (q = jprev; q != at(ancestor, q);) q = at(ancestor, q);
+                for (int u = jprev; u != q;) {
+                    const int up = at(ancestor, u);
+                    at(ancestor, u) = q;
+                    u = up;
+                }
+                --at(cols, q);
+            }
+            ++at(cols, j);
+            at(rows, i) += at(depth, j) - at(depth, q);
+        }
+        if (p != -1) at(ancestor, j) = p;
+    }
+    for (int j = 0; j < n; ++j) {  // parents have larger indices than their children
+        const int p = parent[static_cast<size_t>(j)];
+        if (p != -1) at(cols, p) += at(cols, j);
+    }
+}
+
 }  // namespace
 
 std::vector<int> factor_row_counts(const SparsityPattern& permuted, const std::vector<int>& parent) {
@@ -83,7 +168,7 @@ SolverAnalysis analyze(const SparsityPattern& pattern, Ordering ordering, size_t
     a.permuted = permute_pattern(pattern, a.permutation);
     a.parent = elimination_tree(a.permuted);
     const SymmetricAdjacency adj = SymmetricAdjacency::build(a.permuted);
-    reach_counts(adj, a.parent, a.row_count, &a.col_count);
+    skeleton_counts(adj, a.parent, a.row_count, a.col_count);
     for (int c : a.row_count) a.predicted_nnz += static_cast<size_t>(c);
     for (int p : a.parent)
         if (p < 0) ++a.num_roots;
