@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <future>
 #include <thread>
 #include <memory>
 #include <cstring>
@@ -104,9 +105,20 @@ extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem
         p->dofs = build_dof_map(p->bound, p->mesh);
         p->pattern = build_sparsity(p->dofs, p->bound, p->mesh, true);
         p->map = build_scatter_map(p->pattern, p->dofs, p->mesh, p->bound);
-        p->plan = build_assembly_plan(p->pattern, p->map, p->dofs, p->mesh, p->bound);
-        p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        finish_symbolic(p.get(), spec->ordering, 0, spec->relax_small, spec->relax_fill);
+        // the assembly plan needs only the pattern and the scatter map: it is built
+        // on a second thread while the ordering and symbolic analysis run
+        auto plan = std::async(std::launch::async, [&] {
+            return build_assembly_plan(p->pattern, p->map, p->dofs, p->mesh, p->bound);
+        });
+        try {
+            finish_symbolic(p.get(), spec->ordering, 0, spec->relax_small, spec->relax_fill);
+        } catch (...) {
+            plan.wait();
+            throw;
+        }
+        p->plan = plan.get();
+        p->build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
+                       p->analyze_ms - p->build_ms;
 
         const Mesh& m = p->mesh;
         p->coords.resize(static_cast<size_t>(m.num_nodes()) * 3);
