@@ -21,10 +21,10 @@ inline void for_earlier(const SymmetricAdjacency& adj, int k, F&& f) {
 
 }  // namespace
 
-std::vector<int> elimination_tree(const SparsityPattern& permuted) {
-    if (permuted.rows != permuted.cols) throw std::invalid_argument("elimination_tree: the pattern must be square");
-    const int n = permuted.rows;
-    const SymmetricAdjacency adj = SymmetricAdjacency::build(permuted);
+namespace {
+
+std::vector<int> etree_of(const SymmetricAdjacency& adj) {
+    const int n = adj.rows();
     std::vector<int> parent(static_cast<size_t>(n), -1), anc(static_cast<size_t>(n), -1);
     for (int k = 0; k < n; ++k) {
         for_earlier(adj, k, [&](int i) {
@@ -37,6 +37,13 @@ std::vector<int> elimination_tree(const SparsityPattern& permuted) {
         });
     }
     return parent;
+}
+
+}  // namespace
+
+std::vector<int> elimination_tree(const SparsityPattern& permuted) {
+    if (permuted.rows != permuted.cols) throw std::invalid_argument("elimination_tree: the pattern must be square");
+    return etree_of(SymmetricAdjacency::build(permuted));
 }
 
 namespace {
@@ -166,8 +173,9 @@ SolverAnalysis analyze(const SparsityPattern& pattern, Ordering ordering, size_t
     a.ordering = ordering;
     a.permutation = compute_ordering(pattern, ordering);
     a.permuted = permute_pattern(pattern, a.permutation);
-    a.parent = elimination_tree(a.permuted);
+    if (a.permuted.rows != a.permuted.cols) throw std::invalid_argument("analyze: the pattern must be square");
     const SymmetricAdjacency adj = SymmetricAdjacency::build(a.permuted);
+    a.parent = etree_of(adj);
     skeleton_counts(adj, a.parent, a.row_count, a.col_count);
     for (int c : a.row_count) a.predicted_nnz += static_cast<size_t>(c);
     for (int p : a.parent)
