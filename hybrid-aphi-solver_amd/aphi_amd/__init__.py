@@ -352,7 +352,8 @@ class Mesh:
 
     def set_update_tiles(self, mode="auto"):
         """Complex-float trailing-update tiling: "auto" (64 x 64 tiles for regions of
-        at most 8192 rows), "small" or "large"; the factor is bitwise the same."""
+        at most 8192 rows or 512 columns), "small" or "large"; the factor is bitwise
+        the same."""
         modes = {"auto": 0, "small": 1, "large": 2}
         if mode not in modes:
             raise ValueError(f"update tiles must be one of {sorted(modes)}")

@@ -287,7 +287,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
         for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
         // one update launch on stream `on`: the largest tile count over the level's
         // fronts, and its flops (region semantics of launch_update); complex float
-        // takes the small tiles when the tallest region is short
+        // takes the small tiles when the tallest region is short or the widest is
+        // narrow (the main stream's updates, on the block chain)
         const auto update = [&](hipStream_t on, int c0, int width, int tlo, int thi) {
             const auto region = [&](int i, int& t0, int& tend) {
                 const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
@@ -297,14 +298,19 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                 tend = thi < 0 ? mm : std::min(thi, kk);
                 return tend > t0;
             };
-            int hmax = 0;
+            int hmax = 0, wmax = 0;
             for (int i = 0; i < L.large_count; ++i) {
                 int t0, tend;
-                if (region(i, t0, tend)) hmax = std::max(hmax, L.m[static_cast<size_t>(L.small_count + i)] - t0);
+                if (region(i, t0, tend)) {
+                    hmax = std::max(hmax, L.m[static_cast<size_t>(L.small_count + i)] - t0);
+                    wmax = std::max(wmax, tend - t0);
+                }
             }
             if (hmax == 0) return;
-            const bool small = std::is_same<T, float>::value &&
-                               (m->update_tiles == 0 ? hmax <= aphi_dev::kSmallTileRows : m->update_tiles == 1);
+            const bool small =
+                std::is_same<T, float>::value &&
+                (m->update_tiles == 0 ? (hmax <= aphi_dev::kSmallTileRows || wmax <= aphi_dev::kSmallTileCols)
+                                      : m->update_tiles == 1);
             const int ts = aphi_dev::update_tile<T>(small);
             int tiles = 0;
             double fl = 0.0;

@@ -12,10 +12,13 @@ constexpr int kNB = 64;       // block-column width of the blocked front factori
 constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64)
 constexpr int kOuter = 512;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
-// complex float updates whose tallest region has at most this many rows use the
-// 64 x 64 tiles (measured on the 32^3 cube: 591 ms with none, 586 ms at 8192,
-// 623 ms with all)
+// complex float updates whose tallest region has at most kSmallTileRows rows, or
+// whose widest region has at most kSmallTileCols columns (the block chain's own
+// updates: four times the workgroups of the large tiles on a narrow region), use
+// the 64 x 64 tiles (measured on the 32^3 cube: 591 ms with none, 586 ms by
+// height at 8192, 577 ms adding width at 512, 623 ms with all)
 constexpr int kSmallTileRows = 8192;
+constexpr int kSmallTileCols = 512;
 constexpr int kSB = 256;        // solve block: columns per forward/backward step (multiple of kNB)
 constexpr int kQ = 16;          // right-hand sides per interleaved solve group
 constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a multiple of 64)

@@ -246,8 +246,9 @@ int aphi_mesh_set_profiling(aphi_mesh* mesh, int on);
  * launches the same kernels one by one (bitwise the same results). */
 int aphi_mesh_set_solve_graphs(aphi_mesh* mesh, int on);
 /* Tiling of the complex-float factorization's trailing update: 0 (default) picks
- * 64 x 64 tiles for launches whose tallest region has at most 8192 rows and
- * 128 x 128 tiles otherwise; 1 forces the small tiles, 2 the large ones. Both
+ * 64 x 64 tiles for launches whose tallest region has at most 8192 rows or whose
+ * widest region has at most 512 columns, and 128 x 128 tiles otherwise; 1 forces
+ * the small tiles, 2 the large ones. Both
  * give bitwise the same factor. No counterpart in the reference (tuning and
  * testing switch); other modes are refused (-1). */
 int aphi_mesh_set_update_tiles(aphi_mesh* mesh, int mode);
