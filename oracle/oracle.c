@@ -571,6 +571,17 @@ void oracle_backward_error(int n, const int32_t* row_ptr, const int32_t* col, co
     free(ax);
 }
 
+/* The residual r = b - A x itself, with A x accumulated as backward_error's
+ * matvec does (include/aphi_solver/sparse_symmetric.hpp:116-131). */
+void oracle_residual(int n, const int32_t* row_ptr, const int32_t* col, const double* values, const double* xv,
+                     const double* bv, double* rv) {
+    const cplx* a = (const cplx*)values;
+    const cplx* b = (const cplx*)bv;
+    cplx* r = (cplx*)rv;
+    symv(n, row_ptr, col, a, (const cplx*)xv, r);
+    for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
+}
+
 /* solve_symmetric, src/factorization.cpp:332-398, for nrhs right-hand sides
  * (column-major, ld = n): equilibrate (iterations, 0 disables), factor P A~ P^T
  * with the given permutation, solve, x = D y. stats[0..4] as ldlt();

@@ -134,6 +134,20 @@ def backward_error(problem, values, x, b):
     return out[0], out[1]
 
 
+def residual(problem, values, x, b):
+    """r = b - A x on the original system (the matvec of backward_error)."""
+    n = problem.n
+    rp, ci = _i32(problem.ints("row_ptr")), _i32(problem.ints("col_index"))
+    v = np.ascontiguousarray(values, dtype=np.complex128)
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    bb = np.ascontiguousarray(b, dtype=np.complex128)
+    r = np.zeros(n, dtype=np.complex128)
+    lib().oracle_residual(C.c_int(n), _p(rp, C.c_int32), _p(ci, C.c_int32), _p(v.view(np.float64), C.c_double),
+                          _p(xx.view(np.float64), C.c_double), _p(bb.view(np.float64), C.c_double),
+                          _p(r.view(np.float64), C.c_double))
+    return r
+
+
 def csr_solve_symmetric_raw(n, row_ptr, col, values, b, perm, iterations=0, pivot_floor=0.0):
     """solve_symmetric on a raw upper-triangle CSR (for hand-built test matrices)."""
     rp, ci = _i32(row_ptr), _i32(col)

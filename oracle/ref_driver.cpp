@@ -9,6 +9,7 @@
 //
 // usage: ref_driver OUTDIR n side bar_i0 bar_i1 bar_j0 bar_j1 sigma freq_hz mode
 //   mode 0: symbolic only (topology, gauge, DOF map, pattern, scatter map, ordering)
+//   mode 2: + refill at freq_hz and one solve_symmetric, with I, R and L (large meshes)
 //   mode 1: + refill at freq_hz, equilibration, factorization, solve, R and L,
 //           the fields' e_tet, the gauge-free potential recovered from them, and
 //           the potential.out / VTK files of the solution (no run information)
@@ -165,7 +166,38 @@ int main(int argc, char** argv) {
                        "\"tree_edges\": %d",
                  n, mesh.num_nodes(), mesh.num_edges(), mesh.num_tets(), dofs.num_a, dofs.num_phi, dofs.num_total,
                  pattern.nnz(), an.predicted_nnz, an.num_roots, bound.gauge.tree_edge_count);
-    if (mode >= 1) {
+    if (mode == 2) {
+        // one solve_symmetric (a single factorization), then I, R and L
+        const double omega = 2.0 * 3.14159265358979323846 * freq;
+        SymmetricSystem sys = make_symmetric_system(pattern, dofs.num_total);
+        refill(sys, bound, mesh, dofs, pattern, omega, Conditioning::RowScaled, &map);
+        dump("values", sys.matrix.values());
+        dump("rhs", sys.rhs);
+        std::vector<Z> x;
+        SolveReport rep;
+        SolveOptions opt;
+        opt.equilibration_iterations = 10;
+        const bool ok = solve_symmetric(sys.matrix, sys.rhs, x, rep, opt, &an);
+        dump("x", x);
+        Solution sol;
+        sol.x = x;
+        sol.conditioning = Conditioning::RowScaled;
+        sol.omega = omega;
+        const FieldOutput fields = compute_fields(mesh, bound, dofs, sol);
+        const Z I = terminal_current(mesh, fields, Vec3(0.0, 0.0, 1.0), side);
+        const Z Zin = impedance(1.0, I);
+        std::fprintf(meta,
+                     ", \"freq\": %.17g, \"omega\": %.17g, \"solve_ok\": %s, \"smallest_pivot\": %.17g, "
+                     "\"largest_pivot\": %.17g, \"largest_multiplier\": %.17g, \"factor_nnz\": %zu, "
+                     "\"factorize_ms\": %.6g, \"backward_error\": %.17g, \"residual\": %.17g, "
+                     "\"equilibration_min\": %.17g, \"equilibration_max\": %.17g, \"I_re\": %.17g, \"I_im\": %.17g, "
+                     "\"R\": %.17g, \"L\": %.17g",
+                     freq, omega, ok ? "true" : "false", rep.smallest_pivot, rep.largest_pivot,
+                     rep.largest_multiplier, rep.factor_nnz, rep.factorize_ms, rep.backward_error, rep.residual,
+                     rep.equilibration_min, rep.equilibration_max, I.real(), I.imag(), Zin.real(),
+                     Zin.imag() / omega);
+    }
+    if (mode == 1) {
         const double omega = 2.0 * 3.14159265358979323846 * freq;
         SymmetricSystem sys = make_symmetric_system(pattern, dofs.num_total);
         refill(sys, bound, mesh, dofs, pattern, omega, Conditioning::RowScaled);

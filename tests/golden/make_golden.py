@@ -26,7 +26,8 @@ INT_ARRAYS = ["edges", "tet_edges", "tet_edge_signs", "tree_edge", "dirichlet_ed
               "scatter_offset", "scatter_slot"]
 FULL_INTS = {"perm", "parent", "col_count", "tree_edge", "edge_index"}
 
-# name, n, side, bar (i0, i1, j0, j1), sigma, frequency, mode (0 symbolic, 1 numeric), keep everything in full
+# name, n, side, bar (i0, i1, j0, j1), sigma, frequency, mode (0 symbolic, 1 numeric, 2 one solve with I, R, L),
+# keep everything in full
 CASES = [
     ("cube2", 2, 0.1, (0, 1, 0, 1), 5.8e7, 1e3, 1, True),
     ("cube3_offcentre", 3, 0.05, (1, 2, 0, 2), 1e6, 1e6, 1, True),
@@ -34,6 +35,9 @@ CASES = [
     ("cube8", 8, 0.2, (3, 5, 3, 5), 5.8e7, 1e5, 1, False),
     ("config2", 24, 0.1, (10, 14, 10, 14), 5.8e7, 10.0, 0, False),
     ("config3", 32, 0.1, (6, 12, 20, 23), 5.8e7, 1.0, 0, False),
+    # the 24^3 sweep's two end points (10 Hz and 10 MHz are exact in the log sweep)
+    ("config2_10hz", 24, 0.1, (10, 14, 10, 14), 5.8e7, 10.0, 2, False),
+    ("config2_10mhz", 24, 0.1, (10, 14, 10, 14), 5.8e7, 1e7, 2, False),
 ]
 
 
@@ -51,9 +55,15 @@ def run(case, outdir):
     for a in INT_ARRAYS:
         v = np.fromfile(os.path.join(outdir, a + ".bin"), dtype=np.int64)
         digests[a] = digest(v)
-        if full or (a in FULL_INTS and mode >= 1):
+        if full or (a in FULL_INTS and mode == 1):
             arrays[a] = v.astype(np.int32)
-    if mode >= 1:
+    if mode == 2:
+        for a, dt in (("values", np.complex128), ("rhs", np.complex128), ("x", np.complex128)):
+            v = np.fromfile(os.path.join(outdir, a + ".bin"), dtype=dt)
+            digests[a] = digest(v)
+            if a == "x":
+                arrays[a] = v
+    if mode == 1:
         for a, dt in (("values", np.complex128), ("rhs", np.complex128), ("equil_d", np.float64),
                       ("diag", np.complex128), ("x", np.complex128), ("e_tet", np.complex128),
                       ("gf_phi_vertex", np.complex128), ("gf_port_potential", np.complex128)):
