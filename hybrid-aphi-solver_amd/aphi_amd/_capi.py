@@ -145,6 +145,8 @@ SIGNATURES = {
     "aphi_backward_error": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, DP, DP]),
     "aphi_terminal_current": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, DP,
                                         C.c_double, C.c_double, DP, DP, DP]),
+    "aphi_terminal_current_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int,
+                                                DP, C.c_double, C.c_double, DP, DP, DP]),
 }
 
 _lib = None

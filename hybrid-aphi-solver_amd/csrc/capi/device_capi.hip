@@ -13,6 +13,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <type_traits>
 #include <string>
 #include <vector>
@@ -94,8 +95,18 @@ struct Level {
 
 }  // namespace
 
+// Scratch of aphi_terminal_current_noalloc: the per-tet, per-right-hand-side
+// current terms (num_tets * nrhs entries) and, after them, the nrhs sums. One copy
+// per device, held until the process ends; every call on a device shares it.
+namespace {
+constexpr int64_t kNoAllocTerms = int64_t(1) << 21;  // 32 MiB of complex128 terms
+constexpr int kNoAllocSums = 64;                      // nrhs <= 64
+}  // namespace
+static __device__ __align__(256) aphi_dev::cd scratch[kNoAllocTerms + kNoAllocSums];
+
 struct aphi_context {
     int device = 0;
+    aphi_dev::cd* scratch = nullptr;  // this device's address of `scratch` (hipGetSymbolAddress)
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;  // second stream of the factorization (default priority)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -816,6 +827,24 @@ void factorize(aphi_mesh* m, int precision, double pivot_floor, aphi_factor_stat
 
 // ------------------------------------------------------------------ C-ABI
 
+namespace {
+// The device address of `scratch` on `device` (current), looked up once per
+// device: the array's name in host code is only a stand-in for the symbol.
+aphi_dev::cd* scratch_address(int device) {
+    static std::mutex mu;
+    static std::vector<aphi_dev::cd*> table;
+    std::lock_guard<std::mutex> lock(mu);
+    if (table.size() <= static_cast<size_t>(device)) table.resize(static_cast<size_t>(device) + 1, nullptr);
+    if (!table[static_cast<size_t>(device)]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw DeviceError("hipGetSymbolAddress: no address for the scratch array", 1);
+        table[static_cast<size_t>(device)] = static_cast<aphi_dev::cd*>(p);
+    }
+    return table[static_cast<size_t>(device)];
+}
+}  // namespace
+
 extern "C" int aphi_context_create(int device, aphi_context** out) {
     return aphi_capi::guarded([&] {
         require(out != nullptr, "aphi_context_create: null output");
@@ -825,8 +854,10 @@ extern "C" int aphi_context_create(int device, aphi_context** out) {
         if (device < 0 || device >= count)
             throw std::invalid_argument("aphi_context_create: no HIP device " + std::to_string(device));
         HIPCHK(hipSetDevice(device));
+        aphi_dev::cd* scr = scratch_address(device);
         auto* c = new aphi_context;
         c->device = device;
+        c->scratch = scr;
         HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&c->ev0));
@@ -1233,40 +1264,75 @@ Complex divide_like_libgcc(Complex x, Complex y) {
 }
 }  // namespace
 
+namespace {
+// terminal_current + impedance of nrhs solutions, with the per-tet terms in
+// `terms` (num_tets * nrhs entries) and the sums in `sums` (nrhs entries)
+void terminal_current_into(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, double omega, int conditioning,
+                           const double* axis, double length, double voltage, double* current, double* resistance,
+                           double* inductance, cd* terms, cd* sums) {
+    hipStream_t s = m->st();
+    const int nt = m->num_tets;
+    aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
+                              m->tdofs.as<int>(), m->tsign.as<int8_t>(), m->tcoef.as<double>(), m->tfixed.as<cd>()};
+    // -jw and the ScaledPhi substitution factor (Solution::phi_scale, src/postprocess.cpp:35-40)
+    const cd minus_jw{-0.0, -omega};
+    const cd scale = conditioning == APHI_SCALED_PHI ? cd{0.0, omega} : cd{1.0, 0.0};
+    // every term the sum reads is written here first: nothing to clear
+    hipLaunchKernelGGL(aphi_dev::k_tet_current, dim3(blocks_for(nt), nrhs), dim3(256), 0, s, nt, tp, nrhs,
+                       static_cast<const cd*>(xv), ldx, minus_jw, scale, axis[0], axis[1], axis[2], terms);
+    hipLaunchKernelGGL(aphi_dev::k_current_sum, dim3(nrhs), dim3(64), 0, s, nt, tp, terms, length, sums);
+    cd h[64];
+    HIPCHK(hipMemcpyAsync(h, sums, static_cast<size_t>(nrhs) * sizeof(cd), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    for (int r = 0; r < nrhs; ++r) {
+        current[2 * r] = h[r].re;
+        current[2 * r + 1] = h[r].im;
+        const Complex z = divide_like_libgcc(Complex(voltage, 0.0), Complex(h[r].re, h[r].im));
+        if (resistance) resistance[r] = z.real();
+        if (inductance) inductance[r] = z.imag() / omega;
+    }
+}
+
+void check_terminal_current(const aphi_mesh* m, const void* xv, int nrhs, int64_t ldx, int conditioning,
+                            const double* axis, const double* current, double length, const char* who) {
+    const std::string w(who);
+    require(m && xv && axis && current, (w + ": null argument").c_str());
+    require(nrhs >= 1 && nrhs <= 64 && ldx >= m->n, (w + ": nrhs must be 1..64 and ldx >= n").c_str());
+    require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, (w + ": unknown conditioning").c_str());
+    require(m->tdofs.p != nullptr, (w + ": the mesh was created without per-tet unknowns").c_str());
+    if (!(length > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
+}
+}  // namespace
+
 extern "C" int aphi_terminal_current(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, double omega, int conditioning,
                                      const double* axis, double length, double voltage, double* current,
                                      double* resistance, double* inductance) {
     return aphi_capi::guarded([&] {
-        require(m && xv && axis && current, "aphi_terminal_current: null argument");
-        require(nrhs >= 1 && nrhs <= 64 && ldx >= m->n, "aphi_terminal_current: nrhs must be 1..64 and ldx >= n");
-        require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, "aphi_terminal_current: unknown conditioning");
-        require(m->tdofs.p != nullptr, "aphi_terminal_current: the mesh was created without per-tet unknowns");
-        if (!(length > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
+        check_terminal_current(m, xv, nrhs, ldx, conditioning, axis, current, length, "aphi_terminal_current");
         after_caller(m->ctx);
-        hipStream_t s = m->st();
-        const int nt = m->num_tets;
-        m->tcontrib.reserve(static_cast<size_t>(nt) * nrhs * sizeof(cd));
+        m->tcontrib.reserve(static_cast<size_t>(m->num_tets) * nrhs * sizeof(cd));
         m->tcurrent.reserve(static_cast<size_t>(nrhs) * sizeof(cd));
-        aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
-                                  m->tdofs.as<int>(), m->tsign.as<int8_t>(), m->tcoef.as<double>(), m->tfixed.as<cd>()};
-        // -jw and the ScaledPhi substitution factor (Solution::phi_scale, src/postprocess.cpp:35-40)
-        const cd minus_jw{-0.0, -omega};
-        const cd scale = conditioning == APHI_SCALED_PHI ? cd{0.0, omega} : cd{1.0, 0.0};
-        hipLaunchKernelGGL(aphi_dev::k_tet_current, dim3(blocks_for(nt), nrhs), dim3(256), 0, s, nt, tp, nrhs,
-                           static_cast<const cd*>(xv), ldx, minus_jw, scale, axis[0], axis[1], axis[2], m->tcontrib.as<cd>());
-        hipLaunchKernelGGL(aphi_dev::k_current_sum, dim3(nrhs), dim3(64), 0, s, nt, tp, m->tcontrib.as<cd>(), length,
-                           m->tcurrent.as<cd>());
-        std::vector<cd> h(static_cast<size_t>(nrhs));
-        HIPCHK(hipMemcpyAsync(h.data(), m->tcurrent.p, h.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        HIPCHK(hipGetLastError());
-        for (int r = 0; r < nrhs; ++r) {
-            current[2 * r] = h[static_cast<size_t>(r)].re;
-            current[2 * r + 1] = h[static_cast<size_t>(r)].im;
-            const Complex z = divide_like_libgcc(Complex(voltage, 0.0), Complex(h[static_cast<size_t>(r)].re, h[static_cast<size_t>(r)].im));
-            if (resistance) resistance[r] = z.real();
-            if (inductance) inductance[r] = z.imag() / omega;
-        }
+        terminal_current_into(m, nrhs, xv, ldx, omega, conditioning, axis, length, voltage, current, resistance,
+                              inductance, m->tcontrib.as<cd>(), m->tcurrent.as<cd>());
+    });
+}
+
+extern "C" int aphi_terminal_current_noalloc(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, double omega,
+                                             int conditioning, const double* axis, double length, double voltage,
+                                             double* current, double* resistance, double* inductance) {
+    return aphi_capi::guarded([&] {
+        check_terminal_current(m, xv, nrhs, ldx, conditioning, axis, current, length,
+                               "aphi_terminal_current_noalloc");
+        if (static_cast<int64_t>(m->num_tets) * nrhs > kNoAllocTerms)
+            throw std::invalid_argument("aphi_terminal_current_noalloc: num_tets * nrhs = " +
+                                        std::to_string(static_cast<int64_t>(m->num_tets) * nrhs) +
+                                        " exceeds the scratch of " + std::to_string(kNoAllocTerms) +
+                                        " terms; pass fewer right-hand sides per call");
+        require(m->ctx->scratch != nullptr, "aphi_terminal_current_noalloc: the context has no scratch address");
+        after_caller(m->ctx);
+        terminal_current_into(m, nrhs, xv, ldx, omega, conditioning, axis, length, voltage, current, resistance,
+                              inductance, m->ctx->scratch, m->ctx->scratch + kNoAllocTerms);
     });
 }
 

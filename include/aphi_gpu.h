@@ -280,6 +280,19 @@ int aphi_terminal_current(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx,
                           const double* axis, double length, double voltage, double* current, double* resistance,
                           double* inductance);
 
+/* aphi_terminal_current without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * per-tet terms and the sums go to a scratch array compiled into the library,
+ * one per device (32 MiB, held until the process ends), so num_tets * nrhs must
+ * be at most 2,097,152 (e.g. up to 10 right-hand sides on 196,608 tets); larger
+ * calls are refused (APHI_INVALID_ARGUMENT). Results are bitwise those of
+ * aphi_terminal_current. Calls that overlap on one device (from several
+ * threads or contexts) share the array and race: serialise them. */
+int aphi_terminal_current_noalloc(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx, double omega,
+                                  int conditioning, const double* axis, double length, double voltage,
+                                  double* current, double* resistance, double* inductance);
+
 /* ------------------------------------------------------------------ seeded draws
  * One counter-based generator for every seeded input (right-hand sides, problem
  * draws), the same bits on the host and on the device: Philox-4x32-10 (Salmon et
