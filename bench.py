@@ -51,6 +51,70 @@ def update_traffic(n):
         return json.load(f)["traffic_bytes_per_launch"]
 
 
+BATCH_SEED = 20261204
+
+
+def batched_leg(A, ctx, count, steps):
+    """The batched workload: `count` seeded problems (cube side U[0.05, 0.2] m, 4-12
+    hexes per edge, a conductor block of 1-4 hexes per side spanning z, sigma
+    log-uniform in [1e6, 6e7] S/m, frequency log-uniform in [1 Hz, 1 MHz]) in one
+    device plan: one refill launch set at every problem's own frequency, one FP32
+    factorization (a replayed graph), the port right-hand sides solved and refined
+    per problem to backward error 1e-12. Timed against the same problems solved one
+    mesh at a time."""
+    import torch
+    configs, freqs = A.batched_cube_configs(BATCH_SEED, count)
+    omegas = 2 * np.pi * freqs
+    t0 = time.perf_counter()
+    problems = A.Problem.cubes(configs)
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch = A.Batch(ctx, problems)
+    create_s = time.perf_counter() - t0
+    batch.refill(omegas)
+    table, t_def, t_tuned = batch.tune_update_tiles(repeats=2)
+    b = batch.rhs_device()[None, :].contiguous()
+    x = torch.empty_like(b)
+    batch.solve_symmetric(b, x, precision=A.APHI_FP32, max_refinements=30, tolerance=1e-12)  # warm-up (capture)
+    ctx.synchronize()
+    lat = []
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        batch.refill(omegas)
+        b = batch.rhs_device()[None, :].contiguous()
+        _, reps = batch.solve_symmetric(b, x, precision=A.APHI_FP32, max_refinements=30, tolerance=1e-12)
+        ctx.synchronize()
+        lat.append(time.perf_counter() - s0)
+    t_batch = min(lat)
+    st = batch.factorize_ldlt(A.APHI_FP32)
+    worst = max(r["backward_error"] for r in reps)
+    worst_s = max(r["scaled_backward_error"] for r in reps)
+    unknowns = int(batch.n)
+    batch.close()
+    # the same problems one mesh at a time (meshes created untimed)
+    meshes = [A.Mesh(ctx, p) for p in problems]
+    ctx.synchronize()
+    s0 = time.perf_counter()
+    for m, w in zip(meshes, omegas):
+        m.refill(w)
+        m.solve_symmetric(m.rhs_device()[None, :], precision=A.APHI_FP32, max_refinements=30, tolerance=1e-12)
+    ctx.synchronize()
+    t_single = time.perf_counter() - s0
+    for m in meshes:
+        m.close()
+    out = {"problems": count, "seed": BATCH_SEED, "unknowns": unknowns, "value": count / t_batch,
+           "unit": "problems/s (refill + FP32 factor + port solve refined to backward error <= 1e-12)",
+           "ms_per_batch": 1e3 * t_batch, "factor_ms": st["milliseconds"],
+           "one_at_a_time_problems_per_s": count / t_single, "speedup_vs_one_at_a_time": t_single / t_batch,
+           "backward_error_max": worst, "scaled_backward_error_max": worst_s,
+           "host_build_s": build_s, "device_create_s": create_s,
+           "update_tile_table": table.tolist(), "tune_ms_default": t_def, "tune_ms_tuned": t_tuned}
+    log(f"batched: {count} problems ({unknowns} unknowns) {out['value']:.1f} problems/s "
+        f"({1e3 * t_batch:.1f} ms per batch, factor {st['milliseconds']:.1f} ms); one at a time "
+        f"{out['one_at_a_time_problems_per_s']:.1f} problems/s; max be {worst:.1e} scaled {worst_s:.1e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,6 +128,9 @@ def main():
                     help="requests of the serving run (mixed sizes 8..32, Poisson arrivals; 0 skips)")
     ap.add_argument("--serving-rate", type=float, default=2.0, help="arrival rate of the serving run (requests/s)")
     ap.add_argument("--serving-slots", type=int, default=2)
+    ap.add_argument("--batch-problems", type=int, default=256,
+                    help="problems of the batched run (seeded draws of 4..12 hexes per edge; 0 skips)")
+    ap.add_argument("--batch-steps", type=int, default=3, help="timed passes over the batch")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device copy of known size (HBM counter calibration) and exit")
     ap.add_argument("--factor-only", choices=("fp32", "fp64"), default=None,
@@ -224,6 +291,10 @@ def main():
                          f"extrapolated at that rate: {est:.0f} s per frequency point (solve excluded)"}
         log("cpu baseline:", cpu["sample"])
 
+    batched = None
+    if a.batch_problems > 0:
+        batched = batched_leg(A, ctx, a.batch_problems, a.batch_steps)
+
     serving = None
     if a.serving_requests > 0:
         # serving mode: a seeded Poisson stream of solve requests (meshes of 8..32
@@ -291,6 +362,7 @@ def main():
                      "flops_per_factor": st32["update_flops"]},
         "roofline_fp64": {"achieved": ach64, "peak": MFMA_FP64_PEAK_TFLOPS, "frac": ach64 / MFMA_FP64_PEAK_TFLOPS},
         "cpu_baseline": cpu,
+        "batched": batched,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

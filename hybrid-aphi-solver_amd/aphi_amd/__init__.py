@@ -26,7 +26,7 @@ __all__ = [
     "APHI_FP32", "APHI_FP64", "APHI_NATURAL", "APHI_ROW_SCALED", "APHI_SCALED_PHI",
     "APHI_ORDER_AMD", "APHI_ORDER_NATURAL", "APHI_ORDER_RCM", "lib", "PrecisionPolicy",
     "recover_gauge_free_potential", "philox4x32_10", "random_normal_host", "random_normal", "random_complex_rhs",
-    "random_uniform_host", "batched_cube_configs",
+    "random_uniform_host", "batched_cube_configs", "Batch",
 ]
 
 
@@ -359,6 +359,26 @@ class Mesh:
             raise ValueError(f"update tiles must be one of {sorted(modes)}")
         _check(lib().aphi_mesh_set_update_tiles(self._h, modes[mode]))
 
+    def tune_update_tiles(self, repeats=2):
+        """Autotunes the complex-float update tiles per launch class on this mesh's
+        assembled system (aphi_mesh_tune_update_tiles); returns (table, ms with the
+        default rule, ms with the tuned table). The table (8 x 8, 0 default, 1
+        small, 2 large) is kept only if it is faster."""
+        tab = np.zeros(64, dtype=np.int8)
+        a, b = C.c_double(), C.c_double()
+        _check(lib().aphi_mesh_tune_update_tiles(self._h, int(repeats), tab.ctypes.data_as(C.POINTER(C.c_int8)),
+                                                 C.byref(a), C.byref(b)))
+        return tab.reshape(8, 8), a.value, b.value
+
+    def update_tile_table(self):
+        tab = np.zeros(64, dtype=np.int8)
+        _check(lib().aphi_mesh_get_update_tile_table(self._h, tab.ctypes.data_as(C.POINTER(C.c_int8))))
+        return tab.reshape(8, 8)
+
+    def set_update_tile_table(self, table):
+        tab = np.ascontiguousarray(np.asarray(table, dtype=np.int8).reshape(64))
+        _check(lib().aphi_mesh_set_update_tile_table(self._h, tab.ctypes.data_as(C.POINTER(C.c_int8))))
+
     def set_profiling(self, on=True):
         _check(lib().aphi_mesh_set_profiling(self._h, int(on)))
 
@@ -383,10 +403,11 @@ class Mesh:
         _check(lib().aphi_system_upload(self._h, v.ctypes.data_as(_capi.DP), r.ctypes.data_as(_capi.DP)))
 
     def rhs_device(self):
-        """The assembled right-hand side as a device tensor (a copy)."""
+        """The assembled right-hand side as a device tensor (a device-side copy)."""
         import torch
-        vals, rhs = self.system()
-        return torch.from_numpy(rhs).to(f"cuda:{self.ctx.device}")
+        out = torch.empty(self.n, dtype=torch.complex128, device=f"cuda:{self.ctx.device}")
+        _check(lib().aphi_rhs_copy(self._h, C.c_void_p(out.data_ptr())))
+        return out
 
     # compute_fields' e_tet (src/postprocess.cpp:311-391)
     def tet_fields(self, x, omega, conditioning=APHI_ROW_SCALED):
@@ -493,6 +514,94 @@ class Mesh:
         _check(lib().aphi_backward_error(self._h, b2.shape[0], _dev_ptr(x2), self.n, _dev_ptr(b2), self.n,
                                          be.ctypes.data_as(_capi.DP), rr.ctypes.data_as(_capi.DP)))
         return be, rr
+
+
+class Batch(Mesh):
+    """Many independent problems in one device plan (aphi_batch_create): the
+    block-diagonal union of their meshes, assembled, factored and solved with
+    shared launches, each problem measured on its own. Vectors span every
+    problem's unknowns: problem p's are rows[p]:rows[p + 1]."""
+
+    def __init__(self, ctx: Context, problems):
+        self.ctx = ctx
+        self.problems = list(problems)
+        descs = (_capi.MeshDesc * len(self.problems))(*[p.describe() for p in self.problems])
+        h = C.c_void_p()
+        if not ctx._h:
+            raise ValueError("Batch: the context is closed")
+        _check(lib().aphi_batch_create(ctx._h, descs, len(self.problems), C.byref(h)))
+        self._h = h
+        ctx._meshes.add(self)
+        self.count = len(self.problems)
+        self.rows = np.zeros(self.count + 1, dtype=np.int64)
+        self.tets = np.zeros(self.count + 1, dtype=np.int64)
+        _check(lib().aphi_batch_info(self._h, None, self.rows.ctypes.data_as(_capi.I64P),
+                                     self.tets.ctypes.data_as(_capi.I64P)))
+        self.n = int(self.rows[-1])
+        self.nnz = int(sum(int(p.counts["nnz"]) for p in self.problems))
+        self.problem = None
+
+    def part(self, v, p):
+        """Problem p's unknowns of a vector (last axis) spanning the batch."""
+        return v[..., int(self.rows[p]):int(self.rows[p + 1])]
+
+    # refill per problem -- assembly.hpp:197 at each problem's own frequency
+    def refill(self, omegas, conditioning=APHI_ROW_SCALED):
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(omegas, dtype=np.float64), (self.count,)))
+        _check(lib().aphi_batch_refill(self._h, w.ctypes.data_as(_capi.DP), int(conditioning)))
+
+    def solve_symmetric(self, b, x=None, precision=APHI_FP64, equilibration_iterations=10, pivot_floor=0.0,
+                        max_refinements=0, tolerance=1e-12):
+        """solve_symmetric of every problem; returns (x, reports): one report per
+        problem with its own backward errors (shape (nrhs,))."""
+        import torch
+        b2 = b.reshape(-1, self.n)
+        if x is None:
+            x = torch.empty_like(b2)
+        k = b2.shape[0]
+        opt = _capi.SolveOptions(int(precision), int(equilibration_iterations), float(pivot_floor),
+                                 int(max_refinements), float(tolerance))
+        reps = (_capi.SolveReport * self.count)()
+        be = np.zeros(self.count * k, dtype=np.float64)
+        status = lib().aphi_batch_solve_symmetric(self._h, k, _dev_ptr(b2), self.n, _dev_ptr(x), self.n, C.byref(opt),
+                                                  reps, be.ctypes.data_as(_capi.DP))
+        if status != _capi.APHI_PIVOT_FAILED:
+            _check(status)
+        out = []
+        for p in range(self.count):
+            d = {f: getattr(reps[p], f) for f, _ in _capi.SolveReport._fields_}
+            d["backward_errors"] = be[p * k:(p + 1) * k].copy()
+            out.append(d)
+        return x.reshape(b.shape), out
+
+    def backward_error(self, x, b):
+        """(backward errors, relative residuals), each of shape (count, nrhs)."""
+        x2 = x.reshape(-1, self.n)
+        b2 = b.reshape(-1, self.n)
+        k = b2.shape[0]
+        be = np.zeros(self.count * k)
+        rr = np.zeros(self.count * k)
+        _check(lib().aphi_batch_backward_error(self._h, k, _dev_ptr(x2), self.n, _dev_ptr(b2), self.n,
+                                               be.ctypes.data_as(_capi.DP), rr.ctypes.data_as(_capi.DP)))
+        return be.reshape(self.count, k), rr.reshape(self.count, k)
+
+    def terminal_current(self, x, omegas, lengths, axis=(0.0, 0.0, 1.0), voltages=1.0, conditioning=APHI_ROW_SCALED):
+        """(I, R, L) per problem and right-hand side, each of shape (count, nrhs)."""
+        x2 = x.reshape(-1, self.n)
+        k = x2.shape[0]
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(omegas, dtype=np.float64), (self.count,)))
+        ln = np.ascontiguousarray(np.broadcast_to(np.asarray(lengths, dtype=np.float64), (self.count,)))
+        vv = np.ascontiguousarray(np.broadcast_to(np.asarray(voltages, dtype=np.float64), (self.count,)))
+        ax = np.ascontiguousarray(axis, dtype=np.float64)
+        cur = np.zeros(2 * self.count * k)
+        res = np.zeros(self.count * k)
+        ind = np.zeros(self.count * k)
+        _check(lib().aphi_batch_terminal_current(self._h, k, _dev_ptr(x2), self.n, w.ctypes.data_as(_capi.DP),
+                                                 int(conditioning), ax.ctypes.data_as(_capi.DP),
+                                                 ln.ctypes.data_as(_capi.DP), vv.ctypes.data_as(_capi.DP),
+                                                 cur.ctypes.data_as(_capi.DP), res.ctypes.data_as(_capi.DP),
+                                                 ind.ctypes.data_as(_capi.DP)))
+        return cur.view(np.complex128).reshape(self.count, k), res.reshape(self.count, k), ind.reshape(self.count, k)
 
 
 def recover_gauge_free_potential(source, gf, x, omega, conditioning=APHI_ROW_SCALED, precision=APHI_FP64,

@@ -132,10 +132,14 @@ SIGNATURES = {
     "aphi_mesh_set_solve_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "aphi_mesh_set_update_tiles": (C.c_int, [C.c_void_p, C.c_int]),
     "aphi_mesh_device_bytes": (C.c_int64, [C.c_void_p]),
+    "aphi_mesh_tune_update_tiles": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int8), DP, DP]),
+    "aphi_mesh_get_update_tile_table": (C.c_int, [C.c_void_p, C.POINTER(C.c_int8)]),
+    "aphi_mesh_set_update_tile_table": (C.c_int, [C.c_void_p, C.POINTER(C.c_int8)]),
     "aphi_refill": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
     "aphi_system_download": (C.c_int, [C.c_void_p, DP, DP]),
     "aphi_system_upload": (C.c_int, [C.c_void_p, DP, DP]),
     "aphi_rhs_device": (C.c_void_p, [C.c_void_p]),
+    "aphi_rhs_copy": (C.c_int, [C.c_void_p, C.c_void_p]),
     "aphi_equilibrate": (C.c_int, [C.c_void_p, C.c_int, DP]),
     "aphi_factorize": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.POINTER(FactorStats)]),
     "aphi_factor_diagonal": (C.c_int, [C.c_void_p, DP]),
@@ -145,6 +149,15 @@ SIGNATURES = {
     "aphi_backward_error": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, DP, DP]),
     "aphi_terminal_current": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int, DP,
                                         C.c_double, C.c_double, DP, DP, DP]),
+    "aphi_batch_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "aphi_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), I64P, I64P]),
+    "aphi_batch_refill": (C.c_int, [C.c_void_p, DP, C.c_int]),
+    "aphi_batch_solve_symmetric": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                             C.POINTER(SolveOptions), C.POINTER(SolveReport), DP]),
+    "aphi_batch_backward_error": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                            DP, DP]),
+    "aphi_batch_terminal_current": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, DP, C.c_int, DP, DP, DP,
+                                              DP, DP, DP]),
     "aphi_terminal_current_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_int64, C.c_double, C.c_int,
                                                 DP, C.c_double, C.c_double, DP, DP, DP]),
 }

@@ -1,15 +1,22 @@
 """Serving loop for the A-Phi solve: a stream of solve requests of mixed mesh
 sizes and frequencies, run with continuous batching on one GPU.
 
-Each worker slot owns a context (its own HIP streams) and is driven by its own
-host thread; the C-ABI releases the interpreter lock, so the slots' refills,
-factorizations and solves overlap on the device. A request is taken by the
-first free slot once it has arrived. Symbolic problems (host) are built once
-per distinct mesh, on a host thread pool as soon as a request for a new mesh
-arrives, and reused; device meshes are cached per slot and admitted against an
-HBM budget, least recently used idle meshes evicted first. Every computation
-of a request depends only on its own inputs (fixed-order reductions, no float
-atomics), so its results are the same alone and in the mix.
+Worker slots are host threads; a request is taken by the first free slot once it
+has arrived, and the C-ABI releases the interpreter lock, so the slots' refills,
+factorizations and solves overlap on the device. Symbolic problems (host) are
+built once per distinct mesh, on a host thread pool as soon as a request for a
+new mesh arrives, and reused. Device meshes are kept in one cache shared by all
+slots, each mesh with a context (its own HIP streams) of its own, so a mesh is
+uploaded and held once whichever slot serves it; a mesh serves one request at a
+time (its numeric buffers are per mesh), and a free slot takes the earliest
+arrived request whose mesh is not busy. New meshes are admitted against an HBM
+budget, least recently used idle meshes evicted first. Each distinct mesh keeps
+a precision policy (aphi_amd.PrecisionPolicy) over the frequencies it has
+served, so a request at a frequency where the FP32 factor would need more
+refinement than an FP64 factor costs is factored in FP64 from the start. Every
+computation of a request depends only on its own inputs (fixed-order
+reductions, no float atomics), so its results are the same alone and in the mix
+for the same precision.
 """
 import hashlib
 import math
@@ -20,8 +27,9 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (APHI_FP32, APHI_ROW_SCALED, Context, CubeConfig, Mesh, Problem, random_complex_rhs,
+from . import (APHI_FP32, APHI_FP64, APHI_ROW_SCALED, Context, CubeConfig, Mesh, Problem, random_complex_rhs,
                random_uniform_host)
+from .sweep import PrecisionPolicy
 
 
 @dataclass
@@ -74,58 +82,65 @@ def _estimate_bytes(p):
 
 
 class Server:
-    """Runs request streams with ``slots`` concurrent workers on one device."""
+    """Runs request streams with ``slots`` concurrent workers on one device.
 
-    def __init__(self, slots=2, hbm_budget=200e9, precision=APHI_FP32, tolerance=1e-12, max_refinements=40,
+    precision: APHI_FP32 (FP32 factor + refinement, FP64 where refinement fails),
+    APHI_FP64, or "auto" (per mesh, the precision policy's choice by frequency)."""
+
+    def __init__(self, slots=2, hbm_budget=200e9, precision="auto", tolerance=1e-12, max_refinements=30,
                  host_threads=8, device=0, keep_solutions=False):
         self.slots = int(slots)
         self.budget = float(hbm_budget)
         self.precision, self.tolerance, self.max_refinements = precision, tolerance, max_refinements
         self.keep_solutions = keep_solutions
+        self.device = device
         import torch  # the device allocator: initialised here, not first in a worker thread
         torch.empty(1, device=f"cuda:{device}")
-        self.ctxs = [Context(device) for _ in range(self.slots)]
         self.pool = ThreadPoolExecutor(max_workers=int(host_threads))
         self.problems = {}           # key -> Future[Problem]
-        self.meshes = {}             # (slot, key) -> dict(mesh, bytes, last, busy)
+        self.meshes = {}             # key -> dict(mesh, ctx, bytes, last, busy)
+        self.policies = {}           # key -> PrecisionPolicy
         self.lock = threading.Condition()
         self.reserved = 0
         self.evictions = 0
         self.host_builds = 0
+        self.uploads = 0
 
     def close(self):
         with self.lock:
             for e in self.meshes.values():
-                if e["mesh"] is not None:
-                    e["mesh"].close()
+                self._drop(e)
             self.meshes.clear()
         self.pool.shutdown(wait=True)
-        for c in self.ctxs:
-            c.close()
+
+    @staticmethod
+    def _drop(e):
+        if e.get("mesh") is not None:
+            e["mesh"].close()
+        if e.get("ctx") is not None:
+            e["ctx"].close()
+        e["mesh"] = e["ctx"] = None
 
     # ---------------------------------------------------------------- memory
-    def _admit(self, slot, key, need):
-        """Reserves ``need`` bytes for a new mesh of ``slot``, evicting idle meshes
-        (least recently used first); waits while other slots hold the memory. The
-        reservation is entered (busy, no mesh yet) under the lock, so every
-        reserved byte belongs to an entry."""
+    def _admit(self, e, need):
+        """Reserves ``need`` bytes for the new mesh of entry ``e`` (busy, in the
+        cache), evicting idle meshes (least recently used first); waits while busy
+        meshes hold the memory. Every reserved byte belongs to an entry."""
         with self.lock:
             while self.reserved + need > self.budget:
-                idle = sorted((e["last"], k) for k, e in self.meshes.items() if not e["busy"])
+                idle = sorted((x["last"], k) for k, x in self.meshes.items() if not x["busy"])
                 if not idle:
-                    if not any(e["busy"] for e in self.meshes.values()):
+                    if not any(x["busy"] and x is not e for x in self.meshes.values()):
                         raise MemoryError(f"a mesh needs {need / 1e9:.1f} GB, over the HBM budget")
                     self.lock.wait()
                     continue
                 k = idle[0][1]
-                e = self.meshes.pop(k)
-                e["mesh"].close()
-                self.reserved -= e["bytes"]
+                x = self.meshes.pop(k)
+                self._drop(x)
+                self.reserved -= x["bytes"]
                 self.evictions += 1
             self.reserved += need
-            e = dict(mesh=None, bytes=need, last=time.perf_counter(), busy=True)
-            self.meshes[key] = e
-            return e
+            e["bytes"] = need
 
     def _problem(self, req):
         key = req.key()
@@ -137,98 +152,120 @@ class Server:
                 self.host_builds += 1
         return fut
 
-    def _mesh(self, slot, req):
-        key = (slot, req.key())
-        with self.lock:
-            e = self.meshes.get(key)
-            if e is not None:
-                e["busy"] = True
-                return e
-        p = self._problem(req).result()
-        e = self._admit(slot, key, _estimate_bytes(p))
+    def _build(self, req, e):
+        """Uploads the request's mesh into the cache entry ``e`` (a busy
+        placeholder put there when the request was taken)."""
         try:
-            e["mesh"] = Mesh(self.ctxs[slot], p)
+            p = self._problem(req).result()
+            self._admit(e, _estimate_bytes(p))
+            e["ctx"] = Context(self.device)
+            e["mesh"] = Mesh(e["ctx"], p)
             # other slots' threads issue GPU work at any time: no graph capture
             e["mesh"].set_solve_graphs(False)
+            with self.lock:
+                self.uploads += 1
         except Exception:
             with self.lock:
-                self.meshes.pop(key, None)
+                if self.meshes.get(req.key()) is e:
+                    self.meshes.pop(req.key())
                 self.reserved -= e["bytes"]
+                e["bytes"] = 0
+                self._drop(e)
                 self.lock.notify_all()
             raise
-        return e
 
-    def _release(self, e):
+    def _release(self, key, e):
         with self.lock:
-            actual = e["mesh"].device_bytes()
-            self.reserved += actual - e["bytes"]
-            e["bytes"] = actual
-            e["busy"] = False
-            e["last"] = time.perf_counter()
+            if self.meshes.get(key) is e:
+                actual = e["mesh"].device_bytes()
+                self.reserved += actual - e["bytes"]
+                e["bytes"] = actual
+                e["busy"] = False
+                e["last"] = time.perf_counter()
             self.lock.notify_all()
 
+    def _choose(self, req):
+        if self.precision != "auto":
+            return self.precision
+        with self.lock:
+            pol = self.policies.setdefault(req.key(), PrecisionPolicy(budget=self.max_refinements))
+            return pol.choose(req.frequency)
+
+    def _record(self, req, prec, rep):
+        if self.precision != "auto":
+            return
+        with self.lock:
+            self.policies[req.key()].record(req.frequency, prec, rep)
+
     # ---------------------------------------------------------------- requests
-    def _serve(self, slot, req):
-        e = self._mesh(slot, req)
-        try:
-            m = e["mesh"]
-            omega = 2.0 * math.pi * req.frequency
-            m.refill(omega, APHI_ROW_SCALED)
-            B = random_complex_rhs(self.ctxs[slot], req.seed, req.nrhs, m.n)
-            X, rep = m.solve_symmetric(B, precision=self.precision, equilibration_iterations=10,
-                                       max_refinements=self.max_refinements, tolerance=self.tolerance)
-            xh = X.cpu().numpy()
-            out = dict(id=req.id, n=m.n, nrhs=req.nrhs, ok=bool(rep["ok"]), backward_error=rep["backward_error"],
-                       refinement_steps=rep["refinement_steps"], precision_used=rep["precision_used"],
-                       digest=hashlib.sha256(xh.tobytes()).hexdigest())
-            if self.keep_solutions:
-                out["x"] = xh
-            return out
-        finally:
-            self._release(e)
+    def _serve(self, req, e):
+        m = e["mesh"]
+        omega = 2.0 * math.pi * req.frequency
+        m.refill(omega, APHI_ROW_SCALED)
+        B = random_complex_rhs(e["ctx"], req.seed, req.nrhs, m.n)
+        prec = self._choose(req)
+        X, rep = m.solve_symmetric(B, precision=prec, equilibration_iterations=10,
+                                   max_refinements=self.max_refinements, tolerance=self.tolerance)
+        self._record(req, prec, rep)
+        xh = X.cpu().numpy()
+        be, sbe = rep["backward_error"], rep["scaled_backward_error"]
+        out = dict(id=req.id, n=m.n, nrhs=req.nrhs, backward_error=be, scaled_backward_error=sbe,
+                   ok=bool(be <= self.tolerance and sbe <= self.tolerance),
+                   refinement_steps=rep["refinement_steps"], precision_requested=prec,
+                   precision_used=rep["precision_used"], factorize_ms=rep["factorize_ms"],
+                   digest=hashlib.sha256(xh.tobytes()).hexdigest())
+        if self.keep_solutions:
+            out["x"] = xh
+        return out
 
     def run(self, requests):
         """Serves ``requests`` in real time (each becomes available at its arrival
         time after the call starts). Returns (results by id, summary)."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         queue, results = [], {}
-        done = threading.Event()
-        qlock = threading.Condition()
+        state = dict(done=False)
         t0 = time.perf_counter()
 
-        def pick(slot):
-            """The earliest request whose mesh this slot holds; else the earliest one
-            no other slot holds; else the earliest."""
-            with self.lock:
-                mine = {k for (s_, k) in self.meshes if s_ == slot}
-                others = {k for (s_, k) in self.meshes if s_ != slot}
+        def take():
+            """Under the lock: the earliest queued request whose mesh is not busy,
+            with its cache entry marked busy (a placeholder if it is not resident)."""
             for r in queue:
-                if r.key() in mine:
-                    return r
-            for r in queue:
-                if r.key() not in others:
-                    return r
-            return queue[0]
+                e = self.meshes.get(r.key())
+                if e is None or not e["busy"]:
+                    queue.remove(r)
+                    if e is None:
+                        e = dict(mesh=None, ctx=None, bytes=0, last=time.perf_counter(), busy=True)
+                        self.meshes[r.key()] = e
+                    e["busy"] = True
+                    return r, e
+            return None, None
 
         def worker(slot):
             while True:
-                with qlock:
-                    while not queue and not done.is_set():
-                        qlock.wait()
-                    if not queue:
-                        return
-                    req = pick(slot)
-                    queue.remove(req)
+                with self.lock:
+                    while True:
+                        req, e = take()
+                        if req is not None:
+                            break
+                        if state["done"] and not queue:
+                            return
+                        self.lock.wait()
                 start = time.perf_counter()
                 try:
-                    res = self._serve(slot, req)
+                    if e["mesh"] is None:
+                        self._build(req, e)
+                    res = self._serve(req, e)
                 except Exception as ex:  # reported per request; the loop goes on
                     res = dict(id=req.id, n=0, nrhs=req.nrhs, ok=False, error=f"{type(ex).__name__}: {ex}",
-                               backward_error=float("inf"), refinement_steps=0, precision_used=-1, digest="")
+                               backward_error=float("inf"), scaled_backward_error=float("inf"), refinement_steps=0,
+                               precision_requested=-1, precision_used=-1, factorize_ms=0.0, digest="")
+                finally:
+                    self._release(req.key(), e)
                 end = time.perf_counter()
                 res.update(slot=slot, arrival=req.arrival, start=start - t0, end=end - t0,
                            latency_ms=(end - t0 - req.arrival) * 1e3)
-                results[req.id] = res
+                with self.lock:
+                    results[req.id] = res
 
         threads = [threading.Thread(target=worker, args=(s,)) for s in range(self.slots)]
         for th in threads:
@@ -239,22 +276,27 @@ class Server:
                 if delay > 0:
                     time.sleep(delay)
                 self._problem(req)  # start its host build at arrival
-                with qlock:
+                with self.lock:
                     queue.append(req)
-                    qlock.notify()
+                    self.lock.notify_all()
         finally:
-            with qlock:
-                done.set()
-                qlock.notify_all()
+            with self.lock:
+                state["done"] = True
+                self.lock.notify_all()
             for th in threads:
                 th.join()
         makespan = max(r["end"] for r in results.values()) if results else 0.0
         lat = np.array([results[r.id]["latency_ms"] for r in reqs])
+        fp64 = sum(1 for r in reqs if results[r.id]["precision_used"] == APHI_FP64)
+        stalled = sum(1 for r in reqs if results[r.id]["precision_requested"] == APHI_FP32
+                      and results[r.id]["precision_used"] == APHI_FP64)
         summary = dict(requests=len(reqs), slots=self.slots, makespan_s=makespan,
                        p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                        requests_per_s=len(reqs) / makespan if makespan else 0.0,
                        solves_per_s=sum(r.nrhs for r in reqs) / makespan if makespan else 0.0,
                        max_backward_error=max(results[r.id]["backward_error"] for r in reqs),
+                       max_scaled_backward_error=max(results[r.id]["scaled_backward_error"] for r in reqs),
                        all_ok=all(results[r.id]["ok"] for r in reqs), evictions=self.evictions,
-                       host_builds=self.host_builds, hbm_budget_gb=self.budget / 1e9)
+                       host_builds=self.host_builds, device_uploads=self.uploads, fp64_requests=fp64,
+                       fp32_attempts_refactored=stalled, hbm_budget_gb=self.budget / 1e9)
         return results, summary

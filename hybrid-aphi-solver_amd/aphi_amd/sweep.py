@@ -19,8 +19,10 @@ class PrecisionPolicy:
 
     Cost model per point: FP32 = factor32 + (1 + solves) * solve32, FP64 =
     factor64 + solve64, with the times taken from the solve reports as they come
-    in (factor64 starts as twice factor32, the FLOP-rate ratio of the two MFMA
-    paths). The refinement solves expected at a new frequency are interpolated
+    in, each the smallest seen so far (a report taken while other work shared the
+    device, or one that includes a first call's graph capture, overstates it;
+    factor64 starts as `fp64_factor_ratio` times factor32, the measured ratio of
+    the two MFMA paths on the 32^3 cube). The refinement solves expected at a new frequency are interpolated
     between its nearest solved neighbours below and above, linearly in
     log-frequency on log(1 + solves); with solved points on one side only (a
     sweep in frequency order) the line through the two nearest ones is carried
@@ -37,7 +39,7 @@ class PrecisionPolicy:
     solves and the FP64 point on top.
     """
 
-    def __init__(self, fp64_factor_ratio=2.0, budget=30, stall_solves=4):
+    def __init__(self, fp64_factor_ratio=2.3, budget=30, stall_solves=4):
         self.points = {}          # log10(f) -> refinement solves (inf: FP32 not viable)
         self.factor32 = None
         self.factor64 = None
@@ -102,14 +104,15 @@ class PrecisionPolicy:
         x = math.log10(freq)
         steps = report["refinement_steps"]
         solve_each = report["solve_ms"] / max(1, steps + 1)
+        low = lambda old, new: new if old is None else min(old, new)  # noqa: E731
         if precision == APHI_FP32:
             if report["precision_used"] == APHI_FP32:
                 self.points[x] = float(steps)
-                self.factor32 = report["factorize_ms"]
-                self.solve32 = solve_each
+                self.factor32 = low(self.factor32, report["factorize_ms"])
+                self.solve32 = low(self.solve32, solve_each)
             else:
                 self.points[x] = math.inf
         else:
             # the FP64 timings only: what FP32 would have needed here stays unknown
-            self.factor64 = report["factorize_ms"]
-            self.solve64 = solve_each
+            self.factor64 = low(self.factor64, report["factorize_ms"])
+            self.solve64 = low(self.solve64, solve_each)

@@ -129,7 +129,6 @@ struct aphi_mesh {
     int64_t nnz = 0, num_targets = 0, rows_total = 0, factor_entries = 0, cb_entries = 0, predicted_nnz = 0;
     double flops = 0.0;
     std::vector<double> sigma, eps_r, mu_r;
-    std::vector<Complex> value_tab;
     std::vector<Level> levels;
     int max_level_count = 0;
     int64_t max_bwd_partials = 0;  // count * backward row chunks (aphi_dev::bwd_chunks) over levels, per kSB * padded nrhs
@@ -148,9 +147,28 @@ struct aphi_mesh {
     DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
     int solve_nrhs_cap = 0;
     SolveGraph sgraph;
+    // The factorization's launches (both streams, from the panel scatter to the
+    // inverse diagonal) captured once and replayed at every frequency: they
+    // depend on the values only through buffers whose addresses form the key.
+    struct FactorGraph {
+        hipGraphExec_t exec = nullptr;
+        int precision = -1;
+        double pivot_floor = 0.0;
+        int64_t tiles_version = -1;
+        std::vector<const void*> key;
+        int64_t launches = 0;
+        double update_flops = 0.0;
+    } fgraph;
+    // Tiling of the complex-float trailing update per launch class (tallest
+    // region by bucket x widest region by bucket): 0 = the default rule, 1 = 64 x 64
+    // tiles, 2 = 128 x 128 tiles (both give the same bits); set by the autotuner.
+    static constexpr int kHeightClasses = 8, kWidthClasses = 8;
+    int8_t tile_table[kHeightClasses][kWidthClasses] = {};
+    int64_t tiles_version = 0;
     bool assembled = false, equilibrated = false, factored = false;
     bool has_plan = true;  // false for a problem made from a bare pattern (no refill)
-    bool solve_graphs_on = true;  // aphi_mesh_set_solve_graphs
+    bool solve_graphs_on = true;  // aphi_mesh_set_solve_graphs: graph replay of the solve
+    bool graphs_on = true;        // and of the factorization
     int update_tiles = 0;         // aphi_mesh_set_update_tiles: 0 by region height, 1 small, 2 large
     // gauge-free potential plan (aphi_gauge_free_attach), on a bare-pattern mesh
     struct GaugeFree {
@@ -162,8 +180,19 @@ struct aphi_mesh {
         std::vector<Complex> port_value;
         DevBuf coords, tets, tet_body, tet_on, column, prescribed, tptr, entry, kind, values, beta, terms;
     } gf;
+    // The problems a mesh holds side by side: one for a mesh of one problem, more
+    // for a batch (aphi_batch_create). Problem s owns unknowns [rows[s], rows[s+1]),
+    // stored entries [slots[s], slots[s+1]), tets [tets[s], tets[s+1]) and bodies
+    // [bodies[s], bodies[s+1]); its matrix is its own diagonal block.
+    struct Segments {
+        int count = 1;
+        std::vector<int64_t> rows, slots, tets, bodies;
+        std::vector<std::vector<Complex>> values;  // prescribed values of each problem
+        DevBuf d_rows, d_slots, d_tets, row_seg, scales;
+    } seg;
     int factor_precision = -1;
-    double amax = 0.0, amax_eq = 1.0, eq_min = 1.0, eq_max = 1.0;
+    std::vector<double> amax, amax_eq;  // max |A_ij| and max |(D A D)_ij| per problem
+    double eq_min = 1.0, eq_max = 1.0;
     aphi_factor_stats last_stats{};
     bool profiling = false;
     std::vector<hipEvent_t> prof_events;
@@ -198,16 +227,22 @@ void require(bool ok, const char* msg) {
 
 // ------------------------------------------------------------------ numerics helpers
 
-void compute_amax(aphi_mesh* m) {
-    const int parts = 256;
-    m->partials.reserve(std::max<size_t>(m->partials.bytes, parts * sizeof(double)));
-    hipLaunchKernelGGL(aphi_dev::k_absmax_partial, dim3(parts), dim3(256), 0, m->st(), m->nnz, m->A.as<cd>(),
+// max |v_k| over each problem's stored entries of v (nnz entries)
+std::vector<double> segment_absmax(aphi_mesh* m, const cd* v) {
+    const int parts = m->seg.count == 1 ? 256 : 8;
+    const int ns = m->seg.count;
+    m->partials.reserve(std::max<size_t>(m->partials.bytes, static_cast<size_t>(parts) * ns * sizeof(double)));
+    hipLaunchKernelGGL(aphi_dev::k_seg_absmax, dim3(parts, ns), dim3(256), 0, m->st(), m->seg.d_slots.as<int64_t>(), v,
                        m->partials.as<double>());
-    std::vector<double> h(parts);
-    HIPCHK(hipMemcpyAsync(h.data(), m->partials.p, parts * sizeof(double), hipMemcpyDeviceToHost, m->st()));
+    std::vector<double> h(static_cast<size_t>(parts) * ns), out(static_cast<size_t>(ns), 0.0);
+    HIPCHK(hipMemcpyAsync(h.data(), m->partials.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, m->st()));
     HIPCHK(hipStreamSynchronize(m->st()));
-    m->amax = *std::max_element(h.begin(), h.end());
+    for (int sg = 0; sg < ns; ++sg)
+        for (int p = 0; p < parts; ++p) out[static_cast<size_t>(sg)] = std::max(out[static_cast<size_t>(sg)], h[static_cast<size_t>(sg) * parts + p]);
+    return out;
 }
+
+void compute_amax(aphi_mesh* m) { m->amax = segment_absmax(m, m->A.as<cd>()); }
 
 void equilibrate(aphi_mesh* m, int iterations, double* d_out) {
     require(m->assembled, "equilibrate: the system has not been assembled (call aphi_refill or aphi_system_upload)");
@@ -243,41 +278,55 @@ void equilibrate(aphi_mesh* m, int iterations, double* d_out) {
     m->eq_min = n ? *std::min_element(h.begin(), h.end()) : 1.0;
     m->eq_max = n ? *std::max_element(h.begin(), h.end()) : 1.0;
     if (d_out) std::memcpy(d_out, h.data(), h.size() * sizeof(double));
-    {
-        const int parts = 256;
-        hipLaunchKernelGGL(aphi_dev::k_absmax_partial, dim3(parts), dim3(256), 0, s, m->nnz, m->Aeq.as<cd>(),
-                           m->partials.as<double>());
-        std::vector<double> hp(parts);
-        HIPCHK(hipMemcpyAsync(hp.data(), m->partials.p, parts * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        m->amax_eq = *std::max_element(hp.begin(), hp.end());
-    }
+    m->amax_eq = segment_absmax(m, m->Aeq.as<cd>());
     m->equilibrated = true;
     m->factored = false;
 }
 
+// Launch class of a trailing update for the tile table: its tallest region's
+// rows and its widest region's columns, each by power-of-two bucket.
+int height_class(int rows) {
+    int c = 0;
+    for (int v = 1024; c < aphi_mesh::kHeightClasses - 1 && rows > v; v *= 2) ++c;  // <=1024, <=2048, ..
+    return c;
+}
+int width_class(int cols) {
+    int c = 0;
+    for (int v = 64; c < aphi_mesh::kWidthClasses - 1 && cols > v; v *= 2) ++c;  // <=64, <=128, ..
+    return c;
+}
+
+// APHI_FACTOR_GRAPHS=0 launches the factorization one kernel at a time instead of
+// replaying its captured graph
+bool factor_graphs() {
+    const char* e = std::getenv("APHI_FACTOR_GRAPHS");
+    return !(e && e[0] == '0');
+}
+
+// Per-launch record of a profiled factorization (aphi_mesh_tune_update_tiles).
+struct UpdateLaunch {
+    int hclass, wclass;
+    bool small;
+    double flops;
+    size_t ev;        // index of its start event; the stop event follows
+    double ms = 0.0;  // measured duration
+};
+
+// The factorization's launches on the mesh's two streams, from the statistics
+// reset to the inverse diagonal. Returns the number of trailing-update launches
+// and their flops; records them in `log` when events are taken.
 template <typename T>
-void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
+int64_t factor_sequence(aphi_mesh* m, double pivot_floor, double& update_flops, std::vector<UpdateLaunch>* log,
+                        size_t& ev_used) {
     hipStream_t s = m->st();
     const aphi_dev::FrontPlan fp = m->plan();
-    const size_t cs = sizeof(typename aphi_dev::cplx<T>::type);
-    m->P.reserve(static_cast<size_t>(m->factor_entries + aphi_dev::kPanelTail) * cs);
-    m->CB.reserve(static_cast<size_t>(std::max<int64_t>(m->cb_entries, 1)) * cs);
-    m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
-    m->Xinv.reserve(static_cast<size_t>(std::max<int64_t>(m->xinv_entries, 1)) * cs);
-    m->dinv.reserve(static_cast<size_t>(m->n) * cs);
-    m->X256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
-    m->XT256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
-    m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
-    HIPCHK(hipEventRecord(m->ctx->ev0, s));
     aphi_dev::launch_stats_init(s, m->stats.as<aphi_dev::FactorStatsDev>());
     aphi_dev::launch_zero<T>(s, m->factor_entries + aphi_dev::kPanelTail, m->P.p);
     aphi_dev::launch_scatter_panel<T>(s, m->nnz, m->s2p.as<int64_t>(), m->Aeq.as<cd>(), m->P.p);
     const int* nodes = m->nodes.as<int>();
     auto* st = m->stats.as<aphi_dev::FactorStatsDev>();
     int64_t launches = 0;
-    double update_flops = 0.0;
-    size_t ev_used = 0;
+    update_flops = 0.0;
     const auto next_event = [&]() {
         if (ev_used == m->prof_events.size()) {
             hipEvent_t e;
@@ -298,8 +347,9 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
         for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
         // one update launch on stream `on`: the largest tile count over the level's
         // fronts, and its flops (region semantics of launch_update); complex float
-        // takes the small tiles when the tallest region is short or the widest is
-        // narrow (the main stream's updates, on the block chain)
+        // picks its tiles by the launch's class in the tile table, or by default
+        // the small tiles when the tallest region is short or the widest is narrow
+        // (the main stream's updates, on the block chain)
         const auto update = [&](hipStream_t on, int c0, int width, int tlo, int thi) {
             const auto region = [&](int i, int& t0, int& tend) {
                 const int mm = L.m[static_cast<size_t>(L.small_count + i)], kk = L.k[static_cast<size_t>(L.small_count + i)];
@@ -318,10 +368,13 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                 }
             }
             if (hmax == 0) return;
-            const bool small =
-                std::is_same<T, float>::value &&
-                (m->update_tiles == 0 ? (hmax <= aphi_dev::kSmallTileRows || wmax <= aphi_dev::kSmallTileCols)
-                                      : m->update_tiles == 1);
+            const int hc = height_class(hmax), wc = width_class(wmax);
+            bool small = false;
+            if (std::is_same<T, float>::value) {
+                const int table = m->tile_table[hc][wc];
+                const int mode = m->update_tiles != 0 ? m->update_tiles : table;
+                small = mode == 0 ? (hmax <= aphi_dev::kSmallTileRows || wmax <= aphi_dev::kSmallTileCols) : mode == 1;
+            }
             const int ts = aphi_dev::update_tile<T>(small);
             int tiles = 0;
             double fl = 0.0;
@@ -335,10 +388,12 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
                 const double kw = c1 - c0, w = tend - t0, h = mm - t0;
                 fl += 8.0 * kw * (w * h - w * (w - 1.0) / 2.0);
             }
-            const bool timed = m->profiling;
-            if (timed) HIPCHK(hipEventRecord(next_event(), on));
+            if (log) {
+                log->push_back(UpdateLaunch{hc, wc, small, fl, ev_used, 0.0});
+                HIPCHK(hipEventRecord(next_event(), on));
+            }
             aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, 0, fp, m->P.p, m->CB.p, small);
-            if (timed) HIPCHK(hipEventRecord(next_event(), on));
+            if (log) HIPCHK(hipEventRecord(next_event(), on));
             ++launches;
             update_flops += fl;
         };
@@ -388,6 +443,61 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     aphi_dev::launch_inv_blocks<T>(s, m->inv_pairs.as<int2>(), m->num_inv_pairs, fp, m->P.p, m->Xinv.p, m->X256.p,
                                    m->XT256.p);
     aphi_dev::launch_dinv<T>(s, m->num_super, fp, m->P.p, m->dinv.p);
+    return launches;
+}
+
+template <typename T>
+void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out, std::vector<UpdateLaunch>* log = nullptr) {
+    hipStream_t s = m->st();
+    const size_t cs = sizeof(typename aphi_dev::cplx<T>::type);
+    m->P.reserve(static_cast<size_t>(m->factor_entries + aphi_dev::kPanelTail) * cs);
+    m->CB.reserve(static_cast<size_t>(std::max<int64_t>(m->cb_entries, 1)) * cs);
+    m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
+    m->Xinv.reserve(static_cast<size_t>(std::max<int64_t>(m->xinv_entries, 1)) * cs);
+    m->dinv.reserve(static_cast<size_t>(m->n) * cs);
+    m->X256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
+    m->XT256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
+    m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
+    int64_t launches = 0;
+    double update_flops = 0.0;
+    size_t ev_used = 0;
+    const bool timed = m->profiling || log != nullptr;
+    std::vector<UpdateLaunch> own;
+    std::vector<UpdateLaunch>* rec = log ? log : (m->profiling ? &own : nullptr);
+    HIPCHK(hipEventRecord(m->ctx->ev0, s));
+    if (!timed && factor_graphs() && m->graphs_on) {
+        const int prec = std::is_same<T, float>::value ? APHI_FP32 : APHI_FP64;
+        const std::vector<const void*> key = {m->P.p,    m->CB.p,    m->mws.p,   m->Xinv.p, m->dinv.p, m->X256.p,
+                                              m->XT256.p, m->stats.p, m->Aeq.p,  m->s2p.p,  m->nodes.p};
+        auto& g = m->fgraph;
+        if (!(g.exec && g.precision == prec && g.pivot_floor == pivot_floor && g.key == key &&
+              g.tiles_version == m->tiles_version)) {
+            if (g.exec) HIPCHK(hipGraphExecDestroy(g.exec));
+            g.exec = nullptr;
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+            try {
+                g.launches = factor_sequence<T>(m, pivot_floor, g.update_flops, nullptr, ev_used);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(s, &graph));
+            const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(ie);
+            g.precision = prec;
+            g.pivot_floor = pivot_floor;
+            g.key = key;
+            g.tiles_version = m->tiles_version;
+        }
+        HIPCHK(hipGraphLaunch(g.exec, s));
+        launches = g.launches;
+        update_flops = g.update_flops;
+    } else {
+        launches = factor_sequence<T>(m, pivot_floor, update_flops, rec, ev_used);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(m->ctx->ev1, s));
     std::vector<aphi_dev::StatSlot> slots(aphi_dev::kStatSlots);
@@ -413,11 +523,12 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out) {
     r.flops = m->flops;
     r.update_launches = launches;
     r.update_flops = update_flops;
-    if (m->profiling) {
+    if (rec) {
         double total = 0.0;
-        for (size_t e = 0; e + 1 < ev_used; e += 2) {
+        for (UpdateLaunch& u : *rec) {
             float t = 0.0f;
-            HIPCHK(hipEventElapsedTime(&t, m->prof_events[e], m->prof_events[e + 1]));
+            HIPCHK(hipEventElapsedTime(&t, m->prof_events[u.ev], m->prof_events[u.ev + 1]));
+            u.ms = t;
             total += t;
         }
         r.update_ms = total;
@@ -457,8 +568,9 @@ void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     m->tsum.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kSB * q * cs);
     m->colscale.reserve(nrhs * sizeof(double));
     m->res.reserve(static_cast<size_t>(m->n) * nrhs * sizeof(cd));
-    m->norms.reserve(static_cast<size_t>(64) * nrhs * 6 * sizeof(double));
-    m->partials.reserve(std::max<size_t>(m->partials.bytes, static_cast<size_t>(64) * nrhs * sizeof(double)));
+    const size_t cols = static_cast<size_t>(nrhs) * m->seg.count;  // (problem, right-hand side) pairs
+    m->norms.reserve(static_cast<size_t>(64) * cols * 6 * sizeof(double));
+    m->partials.reserve(std::max<size_t>(m->partials.bytes, static_cast<size_t>(64) * cols * sizeof(double)));
     m->solve_nrhs_cap = nrhs;
 }
 
@@ -557,58 +669,57 @@ void symv(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, cd* y, int64_t ldy) 
                        m->ttptr.as<int>(), m->tslot.as<int>(), m->trow.as<int>(), m->A.as<cd>(), m->xil.as<cd>(), y, ldy);
 }
 
-// backward error and relative residual per RHS, and (when bes is given) the
-// backward error of the equilibrated system; leaves r = b - A x in m->res
+// backward error and relative residual per column c = r * count + s (problem s
+// of the mesh, right-hand side r; c = r for a mesh of one problem), each on its
+// own problem's rows and max |A_ij|, and (when bes is given) the backward error
+// of the equilibrated system; leaves r = b - A x in m->res
 void measure(aphi_mesh* m, int nrhs, const cd* x, int64_t ldx, const cd* b, int64_t ldb, double* be, double* rr,
              double* bes = nullptr, double* scaled_den = nullptr) {
     hipStream_t s = m->st();
-    const int n = m->n;
+    const int n = m->n, ns = m->seg.count, ncol = nrhs * ns;
     ensure_solve_buffers(m, nrhs);
     cd* res = m->res.as<cd>();
     symv(m, nrhs, x, ldx, res, static_cast<int64_t>(n));
     hipLaunchKernelGGL(aphi_dev::k_residual, dim3(blocks_for(n), nrhs), dim3(256), 0, s, n, nrhs, b, ldb, res,
                        static_cast<int64_t>(n));
-    const int parts = 64;
+    const int parts = ns == 1 ? 64 : 4;
+    const int64_t* off = m->seg.d_rows.as<int64_t>();
     double* nrm = m->norms.as<double>();
-    hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, res, static_cast<int64_t>(n), nrm);
-    hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, x, ldx, nrm + parts * nrhs);
-    hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, b, ldb, nrm + 2 * parts * nrhs);
+    const size_t block = static_cast<size_t>(parts) * ncol;
+    const dim3 grid(parts, ncol);
+    hipLaunchKernelGGL(aphi_dev::k_seg_sumsq, grid, dim3(256), 0, s, ns, off, res, static_cast<int64_t>(n), nullptr, 0, nrm);
+    hipLaunchKernelGGL(aphi_dev::k_seg_sumsq, grid, dim3(256), 0, s, ns, off, x, ldx, nullptr, 0, nrm + block);
+    hipLaunchKernelGGL(aphi_dev::k_seg_sumsq, grid, dim3(256), 0, s, ns, off, b, ldb, nullptr, 0, nrm + 2 * block);
     const bool scaled = bes != nullptr && m->equilibrated;
     if (scaled) {
         const double* dd = m->d.as<double>();
-        hipLaunchKernelGGL(aphi_dev::k_sumsq_weighted_partial, dim3(parts, nrhs), dim3(256), 0, s, n, res,
-                           static_cast<int64_t>(n), dd, 1, nrm + 3 * parts * nrhs);
-        hipLaunchKernelGGL(aphi_dev::k_sumsq_weighted_partial, dim3(parts, nrhs), dim3(256), 0, s, n, x, ldx, dd, -1,
-                           nrm + 4 * parts * nrhs);
-        hipLaunchKernelGGL(aphi_dev::k_sumsq_weighted_partial, dim3(parts, nrhs), dim3(256), 0, s, n, b, ldb, dd, 1,
-                           nrm + 5 * parts * nrhs);
+        hipLaunchKernelGGL(aphi_dev::k_seg_sumsq, grid, dim3(256), 0, s, ns, off, res, static_cast<int64_t>(n), dd, 1,
+                           nrm + 3 * block);
+        hipLaunchKernelGGL(aphi_dev::k_seg_sumsq, grid, dim3(256), 0, s, ns, off, x, ldx, dd, -1, nrm + 4 * block);
+        hipLaunchKernelGGL(aphi_dev::k_seg_sumsq, grid, dim3(256), 0, s, ns, off, b, ldb, dd, 1, nrm + 5 * block);
     }
-    std::vector<double> h(static_cast<size_t>((scaled ? 6 : 3) * parts * nrhs));
+    std::vector<double> h((scaled ? 6 : 3) * block);
     HIPCHK(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
-    for (int r = 0; r < nrhs; ++r) {
-        double sr = 0, sx = 0, sb = 0;
-        for (int p = 0; p < parts; ++p) {
-            sr += h[static_cast<size_t>(r * parts + p)];
-            sx += h[static_cast<size_t>(parts * nrhs + r * parts + p)];
-            sb += h[static_cast<size_t>(2 * parts * nrhs + r * parts + p)];
-        }
-        const double num = std::sqrt(sr), den = m->amax * std::sqrt(sx) + std::sqrt(sb);
-        if (be) be[r] = den > 0.0 ? num / den : num;
-        if (rr) rr[r] = sb > 0.0 ? num / std::sqrt(sb) : num;
+    const auto sum = [&](int which, int c) {
+        double t = 0.0;
+        for (int p = 0; p < parts; ++p) t += h[which * block + static_cast<size_t>(c) * parts + p];
+        return t;
+    };
+    for (int c = 0; c < ncol; ++c) {
+        const int sg = c % ns;
+        const double sr = sum(0, c), sx = sum(1, c), sb = sum(2, c);
+        const double num = std::sqrt(sr), den = m->amax[static_cast<size_t>(sg)] * std::sqrt(sx) + std::sqrt(sb);
+        if (be) be[c] = den > 0.0 ? num / den : num;
+        if (rr) rr[c] = sb > 0.0 ? num / std::sqrt(sb) : num;
         if (scaled) {
-            double tr = 0, tx = 0, tb = 0;
-            for (int p = 0; p < parts; ++p) {
-                tr += h[static_cast<size_t>(3 * parts * nrhs + r * parts + p)];
-                tx += h[static_cast<size_t>(4 * parts * nrhs + r * parts + p)];
-                tb += h[static_cast<size_t>(5 * parts * nrhs + r * parts + p)];
-            }
-            const double dn = m->amax_eq * std::sqrt(tx) + std::sqrt(tb);
-            bes[r] = dn > 0.0 ? std::sqrt(tr) / dn : std::sqrt(tr);
-            if (scaled_den) scaled_den[r] = dn;
+            const double tr = sum(3, c), tx = sum(4, c), tb = sum(5, c);
+            const double dn = m->amax_eq[static_cast<size_t>(sg)] * std::sqrt(tx) + std::sqrt(tb);
+            bes[c] = dn > 0.0 ? std::sqrt(tr) / dn : std::sqrt(tr);
+            if (scaled_den) scaled_den[c] = dn;
         } else if (bes) {
-            bes[r] = be ? be[r] : 0.0;
+            bes[c] = be ? be[c] : 0.0;
         }
     }
 }
@@ -639,14 +750,19 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
     const int n = m->n;
     hipStream_t s = m->st();
     const int RS = kGmresRestart;
+    // one Krylov process per column c = r * nseg + sg: right-hand side r of problem
+    // sg (a batch's problems share the launches, never an inner product)
+    const int nseg = m->seg.count, ncol = nrhs * nseg;
+    const int64_t* off = m->seg.d_rows.as<int64_t>();
+    const int* row_seg = nseg > 1 ? m->seg.row_seg.as<int>() : nullptr;
     const int64_t vs = static_cast<int64_t>(nrhs) * n;
-    const int parts = 64;
+    const int parts = nseg == 1 ? 64 : 4;
     m->gV.reserve(static_cast<size_t>(RS + 1) * vs * sizeof(cd));
     m->gZ.reserve(static_cast<size_t>(RS) * vs * sizeof(cd));
     m->gW.reserve(static_cast<size_t>(vs) * sizeof(cd));
     m->gU.reserve(static_cast<size_t>(vs) * sizeof(cd));
-    m->gDots.reserve(static_cast<size_t>(RS + 1) * nrhs * parts * sizeof(cd));
-    m->gSmall.reserve(static_cast<size_t>(RS + 1) * nrhs * sizeof(cd));
+    m->gDots.reserve(static_cast<size_t>(RS + 1) * ncol * parts * sizeof(cd));
+    m->gSmall.reserve(static_cast<size_t>(RS + 1) * ncol * sizeof(cd));
     cd* V = m->gV.as<cd>();
     cd* Z = m->gZ.as<cd>();
     cd* w = m->gW.as<cd>();
@@ -654,36 +770,38 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
     cd* dots = m->gDots.as<cd>();
     const double* d = m->d.as<double>();
     const dim3 grid(blocks_for(n), nrhs);
-    std::vector<double> den(static_cast<size_t>(nrhs));
-    std::vector<Complex> H(static_cast<size_t>(nrhs) * (RS + 1) * RS), cs(static_cast<size_t>(nrhs) * RS),
-        g(static_cast<size_t>(nrhs) * (RS + 1));
-    std::vector<double> cr(static_cast<size_t>(nrhs) * RS);
-    std::vector<double> hn(static_cast<size_t>(nrhs)), inv(static_cast<size_t>(nrhs));
-    std::vector<cd> hup(static_cast<size_t>(RS + 1) * nrhs), hdot(static_cast<size_t>(RS + 1) * nrhs * parts);
+    const size_t nc = static_cast<size_t>(ncol);
+    std::vector<double> den(nc);
+    std::vector<Complex> H(nc * (RS + 1) * RS), cs(nc * RS), g(nc * (RS + 1));
+    std::vector<double> cr(nc * RS);
+    std::vector<double> hn(nc), inv(nc);
+    std::vector<cd> hup(static_cast<size_t>(RS + 1) * nc), hdot(static_cast<size_t>(RS + 1) * nc * parts);
     const auto worst = [&]() {
         double wv = 0.0;
-        for (int r = 0; r < nrhs; ++r) {
-            const double e = std::max(be[static_cast<size_t>(r)], bs[static_cast<size_t>(r)]);
+        for (size_t c = 0; c < nc; ++c) {
+            const double e = std::max(be[c], bs[c]);
             wv = std::isnan(e) ? std::numeric_limits<double>::infinity() : std::max(wv, e);
         }
         return wv;
     };
-    // per-RHS 2-norms of w (host sums of fixed partials)
+    // per-column 2-norms of v (host sums of fixed partials)
     const auto norms = [&](const cd* v, std::vector<double>& out) {
         double* nrm = m->norms.as<double>();
-        hipLaunchKernelGGL(aphi_dev::k_sumsq_partial, dim3(parts, nrhs), dim3(256), 0, s, n, v, static_cast<int64_t>(n), nrm);
-        std::vector<double> h(static_cast<size_t>(parts) * nrhs);
+        hipLaunchKernelGGL(aphi_dev::k_seg_sumsq, dim3(parts, ncol), dim3(256), 0, s, nseg, off, v,
+                           static_cast<int64_t>(n), nullptr, 0, nrm);
+        std::vector<double> h(static_cast<size_t>(parts) * nc);
         HIPCHK(hipMemcpyAsync(h.data(), nrm, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        for (int r = 0; r < nrhs; ++r) {
+        for (size_t c = 0; c < nc; ++c) {
             double t = 0.0;
-            for (int p = 0; p < parts; ++p) t += h[static_cast<size_t>(r * parts + p)];
-            out[static_cast<size_t>(r)] = std::sqrt(t);
+            for (int p = 0; p < parts; ++p) t += h[c * parts + p];
+            out[c] = std::sqrt(t);
         }
     };
     const auto scale_cols = [&](const cd* src, const std::vector<double>& f, cd* dst) {
         HIPCHK(hipMemcpyAsync(m->gSmall.p, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(aphi_dev::k_scale_cols, grid, dim3(256), 0, s, n, nrhs, src, m->gSmall.as<double>(), dst);
+        hipLaunchKernelGGL(aphi_dev::k_scale_cols, grid, dim3(256), 0, s, n, nrhs, nseg, row_seg, src,
+                           m->gSmall.as<double>(), dst);
     };
     int solves = 0;
     for (;;) {
@@ -694,18 +812,15 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
         // V_0 = D r / ||D r||
         hipLaunchKernelGGL(aphi_dev::k_scale_d, grid, dim3(256), 0, s, n, nrhs, d, 1, m->res.as<cd>(),
                            static_cast<int64_t>(n), V, static_cast<int64_t>(n));
-        std::vector<double> beta(static_cast<size_t>(nrhs));
+        std::vector<double> beta(nc);
         norms(V, beta);
         double start = 0.0;  // estimated worst scaled backward error at this restart
-        for (int r = 0; r < nrhs; ++r) {
-            const double dr = den[static_cast<size_t>(r)];
-            start = std::max(start, dr > 0.0 ? beta[static_cast<size_t>(r)] / dr : beta[static_cast<size_t>(r)]);
-        }
-        for (int r = 0; r < nrhs; ++r) inv[static_cast<size_t>(r)] = beta[static_cast<size_t>(r)] > 0.0 ? 1.0 / beta[static_cast<size_t>(r)] : 0.0;
+        for (size_t c = 0; c < nc; ++c) start = std::max(start, den[c] > 0.0 ? beta[c] / den[c] : beta[c]);
+        for (size_t c = 0; c < nc; ++c) inv[c] = beta[c] > 0.0 ? 1.0 / beta[c] : 0.0;
         scale_cols(V, inv, V);
         std::fill(H.begin(), H.end(), Complex(0.0, 0.0));
         std::fill(g.begin(), g.end(), Complex(0.0, 0.0));
-        for (int r = 0; r < nrhs; ++r) g[static_cast<size_t>(r) * (RS + 1)] = Complex(beta[static_cast<size_t>(r)], 0.0);
+        for (size_t c = 0; c < nc; ++c) g[c * (RS + 1)] = Complex(beta[c], 0.0);
         int k = 0;
         for (int j = 0; j < RS && solves < max_solves; ++j) {
             cd* Vj = V + static_cast<int64_t>(j) * vs;
@@ -718,63 +833,64 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
             hipLaunchKernelGGL(aphi_dev::k_scale_d, grid, dim3(256), 0, s, n, nrhs, d, 1, w, static_cast<int64_t>(n), w,
                                static_cast<int64_t>(n));
             for (int pass = 0; pass < 2; ++pass) {
-                hipLaunchKernelGGL(aphi_dev::k_dot_partial, dim3(parts, j + 1, nrhs), dim3(256), 0, s, n, nrhs, V, vs, w, dots);
-                HIPCHK(hipMemcpyAsync(hdot.data(), dots, static_cast<size_t>(j + 1) * nrhs * parts * sizeof(cd),
+                hipLaunchKernelGGL(aphi_dev::k_dot_partial, dim3(parts, j + 1, ncol), dim3(256), 0, s, n, nrhs, nseg, off,
+                                   V, vs, w, dots);
+                HIPCHK(hipMemcpyAsync(hdot.data(), dots, static_cast<size_t>(j + 1) * nc * parts * sizeof(cd),
                                       hipMemcpyDeviceToHost, s));
                 HIPCHK(hipStreamSynchronize(s));
                 for (int v = 0; v <= j; ++v)
-                    for (int r = 0; r < nrhs; ++r) {
+                    for (size_t c = 0; c < nc; ++c) {
                         double tr = 0.0, ti = 0.0;
                         for (int p = 0; p < parts; ++p) {
-                            const cd& c = hdot[(static_cast<size_t>(v) * nrhs + r) * parts + p];
-                            tr += c.re;
-                            ti += c.im;
+                            const cd& q = hdot[(static_cast<size_t>(v) * nc + c) * parts + p];
+                            tr += q.re;
+                            ti += q.im;
                         }
-                        hup[static_cast<size_t>(v) * nrhs + r] = cd{tr, ti};
-                        H[(static_cast<size_t>(r) * (RS + 1) + v) * RS + j] += Complex(tr, ti);
+                        hup[static_cast<size_t>(v) * nc + c] = cd{tr, ti};
+                        H[(c * (RS + 1) + v) * RS + j] += Complex(tr, ti);
                     }
-                HIPCHK(hipMemcpyAsync(m->gSmall.p, hup.data(), static_cast<size_t>(j + 1) * nrhs * sizeof(cd),
+                HIPCHK(hipMemcpyAsync(m->gSmall.p, hup.data(), static_cast<size_t>(j + 1) * nc * sizeof(cd),
                                       hipMemcpyHostToDevice, s));
-                hipLaunchKernelGGL(aphi_dev::k_sub_basis, grid, dim3(256), 0, s, n, nrhs, j + 1, V, vs, m->gSmall.as<cd>(), w);
+                hipLaunchKernelGGL(aphi_dev::k_sub_basis, grid, dim3(256), 0, s, n, nrhs, nseg, row_seg, j + 1, V, vs,
+                                   m->gSmall.as<cd>(), w);
             }
             norms(w, hn);
-            for (int r = 0; r < nrhs; ++r) inv[static_cast<size_t>(r)] = hn[static_cast<size_t>(r)] > 0.0 ? 1.0 / hn[static_cast<size_t>(r)] : 0.0;
+            for (size_t c = 0; c < nc; ++c) inv[c] = hn[c] > 0.0 ? 1.0 / hn[c] : 0.0;
             scale_cols(w, inv, V + static_cast<int64_t>(j + 1) * vs);
-            // Givens rotations per right-hand side; |g_{j+1}| = ||D r|| of the updated x
+            // Givens rotations per column; |g_{j+1}| = ||D r|| of the updated x
             double ratio = 0.0;
-            for (int r = 0; r < nrhs; ++r) {
-                Complex* Hr = &H[static_cast<size_t>(r) * (RS + 1) * RS];
-                Complex* gr = &g[static_cast<size_t>(r) * (RS + 1)];
-                Hr[static_cast<size_t>(j + 1) * RS + j] = Complex(hn[static_cast<size_t>(r)], 0.0);
+            for (size_t c = 0; c < nc; ++c) {
+                Complex* Hr = &H[c * (RS + 1) * RS];
+                Complex* gr = &g[c * (RS + 1)];
+                Hr[static_cast<size_t>(j + 1) * RS + j] = Complex(hn[c], 0.0);
                 for (int i = 0; i < j; ++i) {
-                    const double c = cr[static_cast<size_t>(r) * RS + i];
-                    const Complex sn = cs[static_cast<size_t>(r) * RS + i];
+                    const double cc = cr[c * RS + i];
+                    const Complex sn = cs[c * RS + i];
                     const Complex a = Hr[static_cast<size_t>(i) * RS + j], bb = Hr[static_cast<size_t>(i + 1) * RS + j];
-                    Hr[static_cast<size_t>(i) * RS + j] = c * a + sn * bb;
-                    Hr[static_cast<size_t>(i + 1) * RS + j] = -std::conj(sn) * a + c * bb;
+                    Hr[static_cast<size_t>(i) * RS + j] = cc * a + sn * bb;
+                    Hr[static_cast<size_t>(i + 1) * RS + j] = -std::conj(sn) * a + cc * bb;
                 }
                 const Complex a = Hr[static_cast<size_t>(j) * RS + j], bb = Hr[static_cast<size_t>(j + 1) * RS + j];
                 const double rho = std::sqrt(std::norm(a) + std::norm(bb));
-                double c = 1.0;
+                double cc = 1.0;
                 Complex sn(0.0, 0.0);
                 if (rho > 0.0) {
                     if (std::abs(a) == 0.0) {
-                        c = 0.0;
+                        cc = 0.0;
                         sn = std::conj(bb) / rho;
                     } else {
-                        c = std::abs(a) / rho;
+                        cc = std::abs(a) / rho;
                         sn = (a / std::abs(a)) * std::conj(bb) / rho;
                     }
                 }
-                cr[static_cast<size_t>(r) * RS + j] = c;
-                cs[static_cast<size_t>(r) * RS + j] = sn;
-                Hr[static_cast<size_t>(j) * RS + j] = c * a + sn * bb;
+                cr[c * RS + j] = cc;
+                cs[c * RS + j] = sn;
+                Hr[static_cast<size_t>(j) * RS + j] = cc * a + sn * bb;
                 Hr[static_cast<size_t>(j + 1) * RS + j] = Complex(0.0, 0.0);
                 const Complex gj = gr[j];
-                gr[j] = c * gj;
+                gr[j] = cc * gj;
                 gr[j + 1] = -std::conj(sn) * gj;
-                const double dr = den[static_cast<size_t>(r)];
-                ratio = std::max(ratio, dr > 0.0 ? std::abs(gr[j + 1]) / dr : std::abs(gr[j + 1]));
+                ratio = std::max(ratio, den[c] > 0.0 ? std::abs(gr[j + 1]) / den[c] : std::abs(gr[j + 1]));
             }
             k = j + 1;
             if (trace_refinement()) std::fprintf(stderr, "  gmres %2d: estimated scaled backward error %.3e\n", solves, ratio);
@@ -790,10 +906,10 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
                 }
             }
         }
-        // y = R^-1 g per right-hand side, then x += Z y
-        for (int r = 0; r < nrhs; ++r) {
-            const Complex* Hr = &H[static_cast<size_t>(r) * (RS + 1) * RS];
-            const Complex* gr = &g[static_cast<size_t>(r) * (RS + 1)];
+        // y = R^-1 g per column, then x += Z y
+        for (size_t c = 0; c < nc; ++c) {
+            const Complex* Hr = &H[c * (RS + 1) * RS];
+            const Complex* gr = &g[c * (RS + 1)];
             std::vector<Complex> y(static_cast<size_t>(k));
             for (int i = k - 1; i >= 0; --i) {
                 Complex t = gr[i];
@@ -801,11 +917,11 @@ int gmres_refine(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_
                 const Complex dg = Hr[static_cast<size_t>(i) * RS + i];
                 y[static_cast<size_t>(i)] = std::abs(dg) > 0.0 ? t / dg : Complex(0.0, 0.0);
             }
-            for (int i = 0; i < k; ++i)
-                hup[static_cast<size_t>(i) * nrhs + r] = cd{y[static_cast<size_t>(i)].real(), y[static_cast<size_t>(i)].imag()};
+            for (int i = 0; i < k; ++i) hup[static_cast<size_t>(i) * nc + c] = cd{y[static_cast<size_t>(i)].real(), y[static_cast<size_t>(i)].imag()};
         }
-        HIPCHK(hipMemcpyAsync(m->gSmall.p, hup.data(), static_cast<size_t>(k) * nrhs * sizeof(cd), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(aphi_dev::k_add_basis, grid, dim3(256), 0, s, n, nrhs, k, Z, vs, m->gSmall.as<cd>(), x, ldx);
+        HIPCHK(hipMemcpyAsync(m->gSmall.p, hup.data(), static_cast<size_t>(k) * nc * sizeof(cd), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(aphi_dev::k_add_basis, grid, dim3(256), 0, s, n, nrhs, nseg, row_seg, k, Z, vs,
+                           m->gSmall.as<cd>(), x, ldx);
     }
 }
 
@@ -891,10 +1007,10 @@ extern "C" int aphi_context_synchronize(aphi_context* c) {
     });
 }
 
-extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi_mesh** out) {
-    return aphi_capi::guarded([&] {
-        require(ctx && d && out, "aphi_mesh_create: null argument");
-        *out = nullptr;
+namespace {
+// The device plan of one mesh description (aphi_mesh_create), or of the
+// block-diagonal union of a batch (aphi_batch_create); segments set by the caller.
+std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d) {
         require(d->n > 0 && d->nnz > 0 && d->num_super > 0, "aphi_mesh_create: empty mesh");
         // without an assembly plan (a problem made from a bare pattern) the system
         // arrives through aphi_system_upload instead of aphi_refill
@@ -921,7 +1037,6 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
         m->sigma.assign(d->body_sigma, d->body_sigma + d->num_bodies);
         m->eps_r.assign(d->body_eps_r, d->body_eps_r + d->num_bodies);
         m->mu_r.assign(d->body_mu_r, d->body_mu_r + d->num_bodies);
-        for (int i = 0; i < d->num_values; ++i) m->value_tab.emplace_back(d->values[2 * i], d->values[2 * i + 1]);
         const int n = d->n, ns = d->num_super;
         if (plan) {
             const int64_t nc = d->target_ptr[m->num_targets];
@@ -1102,6 +1217,43 @@ extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi
             upload(m->st(), m->bsigma, d->body_sigma, static_cast<size_t>(d->num_bodies));
         }
         m->flag.reserve(sizeof(int));
+        return m;
+}
+
+// Uploads the problem offsets of the mesh's segments (and, for a batch, the
+// problem of every unknown) and sizes the per-problem scale table.
+void upload_segments(aphi_mesh* m) {
+    auto& g = m->seg;
+    upload(m->st(), g.d_rows, g.rows.data(), g.rows.size());
+    upload(m->st(), g.d_slots, g.slots.data(), g.slots.size());
+    upload(m->st(), g.d_tets, g.tets.data(), g.tets.size());
+    g.scales.reserve(static_cast<size_t>(g.count) * sizeof(aphi_dev::FreqScales));
+    if (g.count > 1) {
+        std::vector<int> rs(static_cast<size_t>(m->n));
+        for (int sg = 0; sg < g.count; ++sg)
+            for (int64_t i = g.rows[static_cast<size_t>(sg)]; i < g.rows[static_cast<size_t>(sg) + 1]; ++i)
+                rs[static_cast<size_t>(i)] = sg;
+        upload(m->st(), g.row_seg, rs.data(), rs.size());
+    }
+    m->amax.assign(static_cast<size_t>(g.count), 0.0);
+    m->amax_eq.assign(static_cast<size_t>(g.count), 1.0);
+}
+}  // namespace
+
+extern "C" int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* d, aphi_mesh** out) {
+    return aphi_capi::guarded([&] {
+        require(ctx && d && out, "aphi_mesh_create: null argument");
+        *out = nullptr;
+        auto m = build_mesh(ctx, d);
+        auto& g = m->seg;
+        g.count = 1;
+        g.rows = {0, d->n};
+        g.slots = {0, d->nnz};
+        g.tets = {0, std::max(d->num_tets, 0)};
+        g.bodies = {0, std::max(d->num_bodies, 0)};
+        g.values.assign(1, {});
+        for (int i = 0; i < d->num_values; ++i) g.values[0].emplace_back(d->values[2 * i], d->values[2 * i + 1]);
+        upload_segments(m.get());
         *out = m.release();
     });
 }
@@ -1112,6 +1264,7 @@ extern "C" void aphi_mesh_destroy(aphi_mesh* m) {
     (void)hipStreamSynchronize(m->ctx->stream);
     for (hipEvent_t e : m->prof_events) (void)hipEventDestroy(e);
     if (m->sgraph.exec) (void)hipGraphExecDestroy(m->sgraph.exec);
+    if (m->fgraph.exec) (void)hipGraphExecDestroy(m->fgraph.exec);
     delete m;
 }
 
@@ -1119,6 +1272,7 @@ extern "C" int aphi_mesh_set_solve_graphs(aphi_mesh* m, int on) {
     return aphi_capi::guarded([&] {
         require(m != nullptr, "aphi_mesh_set_solve_graphs: null mesh");
         m->solve_graphs_on = on != 0;
+        m->graphs_on = on != 0;
     });
 }
 
@@ -1127,6 +1281,89 @@ extern "C" int aphi_mesh_set_update_tiles(aphi_mesh* m, int mode) {
         require(m != nullptr, "aphi_mesh_set_update_tiles: null mesh");
         require(mode >= 0 && mode <= 2, "aphi_mesh_set_update_tiles: mode must be 0, 1 or 2");
         m->update_tiles = mode;
+    });
+}
+
+extern "C" int aphi_mesh_get_update_tile_table(const aphi_mesh* m, int8_t* table) {
+    return aphi_capi::guarded([&] {
+        require(m && table, "aphi_mesh_get_update_tile_table: null argument");
+        std::memcpy(table, m->tile_table, sizeof(m->tile_table));
+    });
+}
+
+extern "C" int aphi_mesh_set_update_tile_table(aphi_mesh* m, const int8_t* table) {
+    return aphi_capi::guarded([&] {
+        require(m && table, "aphi_mesh_set_update_tile_table: null argument");
+        for (size_t i = 0; i < sizeof(m->tile_table); ++i)
+            require(table[i] >= 0 && table[i] <= 2, "aphi_mesh_set_update_tile_table: entries must be 0, 1 or 2");
+        std::memcpy(m->tile_table, table, sizeof(m->tile_table));
+        ++m->tiles_version;
+    });
+}
+
+extern "C" int aphi_mesh_tune_update_tiles(aphi_mesh* m, int repeats, int8_t* table_out, double* ms_default,
+                                           double* ms_tuned) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_mesh_tune_update_tiles: null mesh");
+        require(repeats >= 1 && repeats <= 16, "aphi_mesh_tune_update_tiles: repeats must be 1..16");
+        require(m->assembled, "aphi_mesh_tune_update_tiles: the system has not been assembled");
+        HIPCHK(hipSetDevice(m->ctx->device));
+        if (!m->equilibrated) equilibrate(m, 0, nullptr);
+        if (m->factor_precision != APHI_FP32) {
+            m->P.release();
+            m->CB.release();
+            m->mws.release();
+        }
+        m->factor_precision = APHI_FP32;
+        const int saved_mode = m->update_tiles;
+        const bool saved_prof = m->profiling;
+        m->profiling = false;
+        constexpr int H = aphi_mesh::kHeightClasses, W = aphi_mesh::kWidthClasses;
+        // 1. every launch timed with the small tiles, then with the large ones:
+        //    the faster tiling per launch class (its summed launch time)
+        double t[2][H][W] = {};
+        bool seen[H][W] = {};
+        for (int mode = 1; mode <= 2; ++mode) {
+            m->update_tiles = mode;
+            for (int rep = 0; rep < repeats; ++rep) {
+                std::vector<UpdateLaunch> log;
+                aphi_factor_stats st{};
+                factorize_t<float>(m, 0.0, &st, &log);
+                for (const UpdateLaunch& u : log) {
+                    t[mode - 1][u.hclass][u.wclass] += u.ms;
+                    seen[u.hclass][u.wclass] = true;
+                }
+            }
+        }
+        m->update_tiles = saved_mode;
+        int8_t tuned[H][W] = {};
+        for (int h = 0; h < H; ++h)
+            for (int w = 0; w < W; ++w)
+                if (seen[h][w]) tuned[h][w] = t[0][h][w] <= t[1][h][w] ? 1 : 2;
+        // 2. whole factorizations (graph replay, as used) with the default rule and
+        //    with the tuned table: the table is kept only if it is faster
+        const auto best_of = [&](const int8_t (*tab)[W]) {
+            std::memcpy(m->tile_table, tab, sizeof(m->tile_table));
+            ++m->tiles_version;
+            double best = std::numeric_limits<double>::infinity();
+            for (int rep = 0; rep < repeats + 1; ++rep) {  // the first run captures the graph
+                aphi_factor_stats st{};
+                factorize_t<float>(m, 0.0, &st);
+                if (rep > 0 || repeats == 0) best = std::min(best, st.milliseconds);
+            }
+            return best;
+        };
+        const int8_t zero[H][W] = {};
+        const double t_default = best_of(zero);
+        const double t_tuned = best_of(tuned);
+        if (!(t_tuned < t_default)) {
+            std::memcpy(m->tile_table, zero, sizeof(m->tile_table));
+            ++m->tiles_version;
+        }
+        m->profiling = saved_prof;
+        if (table_out) std::memcpy(table_out, m->tile_table, sizeof(m->tile_table));
+        if (ms_default) *ms_default = t_default;
+        if (ms_tuned) *ms_tuned = t_tuned;
     });
 }
 
@@ -1152,74 +1389,117 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     return total;
 }
 
-extern "C" int aphi_refill(aphi_mesh* m, double omega, int conditioning) {
-    return aphi_capi::guarded([&] {
-        require(m != nullptr, "aphi_refill: null mesh");
-        if (!m->has_plan)
-            throw std::logic_error("aphi_refill: the mesh was created from a bare pattern and has no assembly plan; "
-                                   "hand its values over with aphi_system_upload");
-        HIPCHK(hipSetDevice(m->ctx->device));
-        // formulation_scales + the symmetric guard of src/assembly.cpp:12-45, 484-491
-        const Complex jw(0.0, omega);
-        Complex row(1.0, 0.0), column(1.0, 0.0);
-        if (conditioning == APHI_NATURAL) {
-            throw std::invalid_argument(
-                "refill: conditioning = natural does not give a symmetric matrix, so only half of it cannot be "
-                "stored. Use row_scaled or scaled_phi for symmetric storage.");
-        } else if (conditioning == APHI_ROW_SCALED) {
-            if (omega == 0.0)
-                throw std::invalid_argument("formulation_scales: the row-scaled formulation divides the Phi rows by "
-                                            "j*omega, which is undefined at DC.");
-            row = Complex(1.0, 0.0) / jw;
-        } else if (conditioning == APHI_SCALED_PHI) {
-            if (omega == 0.0)
-                throw std::invalid_argument("formulation_scales: the scaled-potential formulation substitutes "
-                                            "Phi = j*omega*Phi', which is undefined at DC.");
-            column = jw;
-        } else {
-            throw std::invalid_argument("formulation_scales: unknown formulation");
-        }
-        aphi_dev::FreqScales sc{};
-        const auto put = [](cd& dst, const Complex& v) { dst = cd{v.real(), v.imag()}; };
-        put(sc.column, column);
-        put(sc.row_jw, row * jw);
-        put(sc.row_column, row * column);
-        for (int i = 0; i < m->num_values; ++i) {
-            const Complex v = m->value_tab[static_cast<size_t>(i)];
-            put(sc.over_column[i], v / column);
-            put(sc.times_row[i], row * v);
-        }
+namespace {
+// formulation_scales + the symmetric guard of src/assembly.cpp:12-45, 484-491, and
+// the per-frequency scalars of the scatter formed from them with std::complex
+// std::complex<double> product and quotient as g++ evaluates them for finite
+// operands on x86-64 (the reference's build): (ac - bd, ad + bc) with no fused
+// multiply-add, and libgcc's __divdc3 (Smith's method, no scaling). This file's
+// host code is compiled by clang, whose own complex division rounds differently
+// (1 ulp apart at some frequencies), so the scatter's scalars are formed here.
+Complex mul_like_gcc(Complex x, Complex y) {
+#pragma clang fp contract(off)
+    const double a = x.real(), b = x.imag(), c = y.real(), d = y.imag();
+    return Complex(a * c - b * d, a * d + b * c);
+}
+Complex div_like_gcc(Complex x, Complex y) {
+#pragma clang fp contract(off)
+    const double a = x.real(), b = x.imag(), c = y.real(), d = y.imag();
+    if (std::fabs(c) < std::fabs(d)) {
+        const double ratio = c / d, denom = (c * ratio) + d;
+        return Complex(((a * ratio) + b) / denom, ((b * ratio) - a) / denom);
+    }
+    const double ratio = d / c, denom = (d * ratio) + c;
+    return Complex(((b * ratio) + a) / denom, (b - (a * ratio)) / denom);
+}
+
+aphi_dev::FreqScales freq_scales(double omega, int conditioning, const std::vector<Complex>& values) {
+    const Complex jw(0.0, omega);
+    Complex row(1.0, 0.0), column(1.0, 0.0);
+    if (conditioning == APHI_NATURAL) {
+        throw std::invalid_argument(
+            "refill: conditioning = natural does not give a symmetric matrix, so only half of it cannot be "
+            "stored. Use row_scaled or scaled_phi for symmetric storage.");
+    } else if (conditioning == APHI_ROW_SCALED) {
+        if (omega == 0.0)
+            throw std::invalid_argument("formulation_scales: the row-scaled formulation divides the Phi rows by "
+                                        "j*omega, which is undefined at DC.");
+        row = div_like_gcc(Complex(1.0, 0.0), jw);
+    } else if (conditioning == APHI_SCALED_PHI) {
+        if (omega == 0.0)
+            throw std::invalid_argument("formulation_scales: the scaled-potential formulation substitutes "
+                                        "Phi = j*omega*Phi', which is undefined at DC.");
+        column = jw;
+    } else {
+        throw std::invalid_argument("formulation_scales: unknown formulation");
+    }
+    aphi_dev::FreqScales sc{};
+    const auto put = [](cd& dst, const Complex& v) { dst = cd{v.real(), v.imag()}; };
+    put(sc.column, column);
+    put(sc.row_jw, mul_like_gcc(row, jw));
+    put(sc.row_column, mul_like_gcc(row, column));
+    for (size_t i = 0; i < values.size(); ++i) {
+        // prescribed(): a zero stays zero (src/assembly.cpp:188-190)
+        put(sc.over_column[i], values[i] == Complex(0.0, 0.0) ? values[i] : div_like_gcc(values[i], column));
+        put(sc.times_row[i], mul_like_gcc(row, values[i]));
+    }
+    return sc;
+}
+
+// refill of every problem of the mesh, problem s at angular frequency omegas[s]
+void refill_all(aphi_mesh* m, const double* omegas, int conditioning) {
+    if (!m->has_plan)
+        throw std::logic_error("aphi_refill: the mesh was created from a bare pattern and has no assembly plan; "
+                               "hand its values over with aphi_system_upload");
+    HIPCHK(hipSetDevice(m->ctx->device));
+    const auto& g = m->seg;
+    std::vector<aphi_dev::FreqScales> sc(static_cast<size_t>(g.count));
+    std::vector<aphi_dev::BodyCoef> coef(static_cast<size_t>(m->num_bodies));
+    for (int sg = 0; sg < g.count; ++sg) {
+        const double omega = omegas[sg];
+        sc[static_cast<size_t>(sg)] = freq_scales(omega, conditioning, g.values[static_cast<size_t>(sg)]);
         // element_coefficients, src/element_matrix.cpp:10-25
-        std::vector<aphi_dev::BodyCoef> coef(static_cast<size_t>(m->num_bodies));
-        for (int b = 0; b < m->num_bodies; ++b) {
+        for (int64_t b = g.bodies[static_cast<size_t>(sg)]; b < g.bodies[static_cast<size_t>(sg) + 1]; ++b) {
             const double mu = m->mu_r[static_cast<size_t>(b)] * kMu0;
             const double eps = m->eps_r[static_cast<size_t>(b)] * kEps0;
             coef[static_cast<size_t>(b)].nu = 1.0 / mu;
             coef[static_cast<size_t>(b)].alpha = cd{-omega * omega * eps, omega * m->sigma[static_cast<size_t>(b)]};
             coef[static_cast<size_t>(b)].beta = cd{m->sigma[static_cast<size_t>(b)], omega * eps};
         }
-        hipStream_t s = m->st();
-        m->coef.reserve(coef.size() * sizeof(aphi_dev::BodyCoef));
-        HIPCHK(hipMemcpyAsync(m->coef.p, coef.data(), coef.size() * sizeof(aphi_dev::BodyCoef), hipMemcpyHostToDevice, s));
-        m->E.reserve(static_cast<size_t>(m->num_tets) * 196 * sizeof(cd));
-        m->A.reserve(static_cast<size_t>(m->nnz) * sizeof(cd));
-        m->rhs.reserve(static_cast<size_t>(m->n) * sizeof(cd));
-        HIPCHK(hipMemsetAsync(m->flag.p, 0, sizeof(int), s));
-        hipLaunchKernelGGL(aphi_dev::k_element, dim3(blocks_for(m->num_tets, 16)), dim3(256), 0, s, m->num_tets,
-                           m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(),
-                           m->coef.as<aphi_dev::BodyCoef>(), m->E.as<cd>(), m->flag.as<int>());
-        hipLaunchKernelGGL(aphi_dev::k_assemble_reduce, dim3(blocks_for(m->num_targets)), dim3(256), 0, s, m->num_targets,
-                           static_cast<int>(m->nnz), m->tptr.as<int64_t>(), m->entry.as<uint32_t>(), m->kind.as<uint8_t>(),
-                           m->E.as<cd>(), sc, m->A.as<cd>(), m->rhs.as<cd>());
-        int flag = 0;
-        HIPCHK(hipMemcpyAsync(&flag, m->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        HIPCHK(hipGetLastError());
-        if (flag) throw std::runtime_error("compute_tet_geometry: degenerate (zero-volume) tet");
-        m->assembled = true;
-        m->equilibrated = false;
-        m->factored = false;
-        compute_amax(m);
+    }
+    hipStream_t s = m->st();
+    m->coef.reserve(coef.size() * sizeof(aphi_dev::BodyCoef));
+    HIPCHK(hipMemcpyAsync(m->coef.p, coef.data(), coef.size() * sizeof(aphi_dev::BodyCoef), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(g.scales.p, sc.data(), sc.size() * sizeof(aphi_dev::FreqScales), hipMemcpyHostToDevice, s));
+    m->E.reserve(static_cast<size_t>(m->num_tets) * 196 * sizeof(cd));
+    m->A.reserve(static_cast<size_t>(m->nnz) * sizeof(cd));
+    m->rhs.reserve(static_cast<size_t>(m->n) * sizeof(cd));
+    hipLaunchKernelGGL(aphi_dev::k_fill_flag, dim3(1), dim3(64), 0, s, m->flag.as<int>());
+    hipLaunchKernelGGL(aphi_dev::k_element, dim3(blocks_for(m->num_tets, 16)), dim3(256), 0, s, m->num_tets,
+                       m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(),
+                       m->coef.as<aphi_dev::BodyCoef>(), m->E.as<cd>(), m->flag.as<int>());
+    hipLaunchKernelGGL(aphi_dev::k_assemble_reduce, dim3(blocks_for(m->num_targets)), dim3(256), 0, s, m->num_targets,
+                       m->nnz, m->tptr.as<int64_t>(), m->entry.as<uint32_t>(), m->kind.as<uint8_t>(), m->E.as<cd>(),
+                       g.scales.as<aphi_dev::FreqScales>(), g.count, g.d_slots.as<int64_t>(), g.d_rows.as<int64_t>(),
+                       m->A.as<cd>(), m->rhs.as<cd>());
+    int flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, m->flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    if (flag) throw std::runtime_error("compute_tet_geometry: degenerate (zero-volume) tet");
+    m->assembled = true;
+    m->equilibrated = false;
+    m->factored = false;
+    compute_amax(m);
+}
+}  // namespace
+
+extern "C" int aphi_refill(aphi_mesh* m, double omega, int conditioning) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_refill: null mesh");
+        // one frequency for every problem the mesh holds
+        const std::vector<double> omegas(static_cast<size_t>(m->seg.count), omega);
+        refill_all(m, omegas.data(), conditioning);
     });
 }
 
@@ -1250,37 +1530,79 @@ extern "C" int aphi_system_upload(aphi_mesh* m, const double* values, const doub
 
 extern "C" const void* aphi_rhs_device(const aphi_mesh* m) { return m ? m->rhs.p : nullptr; }
 
-namespace {
-// std::complex<double> division as g++'s libgcc __divdc3 evaluates it for finite
-// operands (Smith's method, no scaling), so V / I is the reference's impedance.
-Complex divide_like_libgcc(Complex x, Complex y) {
-    const double a = x.real(), b = x.imag(), c = y.real(), d = y.imag();
-    if (std::fabs(c) < std::fabs(d)) {
-        const double ratio = c / d, denom = (c * ratio) + d;
-        return Complex(((a * ratio) + b) / denom, ((b * ratio) - a) / denom);
-    }
-    const double ratio = d / c, denom = (d * ratio) + c;
-    return Complex(((b * ratio) + a) / denom, (b - (a * ratio)) / denom);
+extern "C" int aphi_rhs_copy(aphi_mesh* m, void* dst) {
+    return aphi_capi::guarded([&] {
+        require(m && dst, "aphi_rhs_copy: null argument");
+        require(m->assembled, "aphi_rhs_copy: nothing assembled");
+        after_caller(m->ctx);
+        HIPCHK(hipMemcpyAsync(dst, m->rhs.p, static_cast<size_t>(m->n) * sizeof(cd), hipMemcpyDeviceToDevice, m->st()));
+        HIPCHK(hipStreamSynchronize(m->st()));
+    });
 }
-}  // namespace
+
 
 namespace {
-// terminal_current + impedance of nrhs solutions, with the per-tet terms in
-// `terms` (num_tets * nrhs entries) and the sums in `sums` (nrhs entries)
-void terminal_current_into(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, double omega, int conditioning,
-                           const double* axis, double length, double voltage, double* current, double* resistance,
-                           double* inductance, cd* terms, cd* sums) {
+// terminal_current + impedance of nrhs solutions of every problem of the mesh:
+// problem s at omegas[s], length lengths[s], drive voltage voltages[s]; outputs
+// per column c = r * count + s. The per-tet terms go to `terms` (num_tets * nrhs
+// entries), the sums to `sums` (nrhs * count entries); `factors` (2 * count,
+// device) carries a batch's per-problem -j w and Phi scale.
+void terminal_current_into(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, const double* omegas, int conditioning,
+                           const double* axis, const double* lengths, const double* voltages, double* current,
+                           double* resistance, double* inductance, cd* terms, cd* sums, cd* factors,
+                           double* d_lengths) {
+    hipStream_t s = m->st();
+    const int nt = m->num_tets, ns = m->seg.count;
+    aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
+                              m->tdofs.as<int>(), m->tsign.as<int8_t>(), m->tcoef.as<double>(), m->tfixed.as<cd>()};
+    // -jw and the ScaledPhi substitution factor (Solution::phi_scale, src/postprocess.cpp:35-40)
+    const auto minus_jw = [](double w) { return cd{-0.0, -w}; };
+    const auto phi_scale = [&](double w) { return conditioning == APHI_SCALED_PHI ? cd{0.0, w} : cd{1.0, 0.0}; };
+    if (ns > 1) {
+        std::vector<cd> f(2 * static_cast<size_t>(ns));
+        for (int sg = 0; sg < ns; ++sg) {
+            f[2 * static_cast<size_t>(sg)] = minus_jw(omegas[sg]);
+            f[2 * static_cast<size_t>(sg) + 1] = phi_scale(omegas[sg]);
+        }
+        HIPCHK(hipMemcpyAsync(factors, f.data(), f.size() * sizeof(cd), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_lengths, lengths, static_cast<size_t>(ns) * sizeof(double), hipMemcpyHostToDevice, s));
+    }
+    // every term the sums read is written here first: nothing to clear
+    hipLaunchKernelGGL(aphi_dev::k_tet_current, dim3(blocks_for(nt), nrhs), dim3(256), 0, s, nt, tp, nrhs,
+                       static_cast<const cd*>(xv), ldx, minus_jw(omegas[0]), phi_scale(omegas[0]), axis[0], axis[1],
+                       axis[2], ns, m->seg.d_tets.as<int64_t>(), factors, terms);
+    hipLaunchKernelGGL(aphi_dev::k_current_sum, dim3(nrhs * ns), dim3(64), 0, s, nt, ns, m->seg.d_tets.as<int64_t>(), tp,
+                       terms, ns > 1 ? d_lengths : nullptr, lengths[0], sums);
+    std::vector<cd> h(static_cast<size_t>(nrhs) * ns);
+    HIPCHK(hipMemcpyAsync(h.data(), sums, h.size() * sizeof(cd), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    for (size_t c = 0; c < h.size(); ++c) {
+        const int sg = static_cast<int>(c % static_cast<size_t>(ns));
+        current[2 * c] = h[c].re;
+        current[2 * c + 1] = h[c].im;
+        const Complex z = div_like_gcc(Complex(voltages[sg], 0.0), Complex(h[c].re, h[c].im));
+        if (resistance) resistance[c] = z.real();
+        if (inductance) inductance[c] = z.imag() / omegas[sg];
+    }
+}
+
+// the same for a mesh of one problem, sums in a fixed array (no allocation)
+void terminal_current_one(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, double omega, int conditioning,
+                          const double* axis, double length, double voltage, double* current, double* resistance,
+                          double* inductance, cd* terms, cd* sums) {
     hipStream_t s = m->st();
     const int nt = m->num_tets;
     aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
                               m->tdofs.as<int>(), m->tsign.as<int8_t>(), m->tcoef.as<double>(), m->tfixed.as<cd>()};
-    // -jw and the ScaledPhi substitution factor (Solution::phi_scale, src/postprocess.cpp:35-40)
     const cd minus_jw{-0.0, -omega};
     const cd scale = conditioning == APHI_SCALED_PHI ? cd{0.0, omega} : cd{1.0, 0.0};
     // every term the sum reads is written here first: nothing to clear
     hipLaunchKernelGGL(aphi_dev::k_tet_current, dim3(blocks_for(nt), nrhs), dim3(256), 0, s, nt, tp, nrhs,
-                       static_cast<const cd*>(xv), ldx, minus_jw, scale, axis[0], axis[1], axis[2], terms);
-    hipLaunchKernelGGL(aphi_dev::k_current_sum, dim3(nrhs), dim3(64), 0, s, nt, tp, terms, length, sums);
+                       static_cast<const cd*>(xv), ldx, minus_jw, scale, axis[0], axis[1], axis[2], 1,
+                       m->seg.d_tets.as<int64_t>(), nullptr, terms);
+    hipLaunchKernelGGL(aphi_dev::k_current_sum, dim3(nrhs), dim3(64), 0, s, nt, 1, m->seg.d_tets.as<int64_t>(), tp, terms,
+                       nullptr, length, sums);
     cd h[64];
     HIPCHK(hipMemcpyAsync(h, sums, static_cast<size_t>(nrhs) * sizeof(cd), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1288,20 +1610,25 @@ void terminal_current_into(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, 
     for (int r = 0; r < nrhs; ++r) {
         current[2 * r] = h[r].re;
         current[2 * r + 1] = h[r].im;
-        const Complex z = divide_like_libgcc(Complex(voltage, 0.0), Complex(h[r].re, h[r].im));
+        const Complex z = div_like_gcc(Complex(voltage, 0.0), Complex(h[r].re, h[r].im));
         if (resistance) resistance[r] = z.real();
         if (inductance) inductance[r] = z.imag() / omega;
     }
 }
 
 void check_terminal_current(const aphi_mesh* m, const void* xv, int nrhs, int64_t ldx, int conditioning,
-                            const double* axis, const double* current, double length, const char* who) {
+                            const double* axis, const double* current, const char* who) {
     const std::string w(who);
     require(m && xv && axis && current, (w + ": null argument").c_str());
     require(nrhs >= 1 && nrhs <= 64 && ldx >= m->n, (w + ": nrhs must be 1..64 and ldx >= n").c_str());
     require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, (w + ": unknown conditioning").c_str());
     require(m->tdofs.p != nullptr, (w + ": the mesh was created without per-tet unknowns").c_str());
-    if (!(length > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
+}
+
+void require_single(const aphi_mesh* m, const char* who) {
+    if (m && m->seg.count != 1)
+        throw std::invalid_argument(std::string(who) + ": the mesh is a batch of " + std::to_string(m->seg.count) +
+                                    " problems; use the aphi_batch_ entry points");
 }
 }  // namespace
 
@@ -1309,12 +1636,14 @@ extern "C" int aphi_terminal_current(aphi_mesh* m, int nrhs, const void* xv, int
                                      const double* axis, double length, double voltage, double* current,
                                      double* resistance, double* inductance) {
     return aphi_capi::guarded([&] {
-        check_terminal_current(m, xv, nrhs, ldx, conditioning, axis, current, length, "aphi_terminal_current");
+        check_terminal_current(m, xv, nrhs, ldx, conditioning, axis, current, "aphi_terminal_current");
+        require_single(m, "aphi_terminal_current");
+        if (!(length > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
         after_caller(m->ctx);
         m->tcontrib.reserve(static_cast<size_t>(m->num_tets) * nrhs * sizeof(cd));
         m->tcurrent.reserve(static_cast<size_t>(nrhs) * sizeof(cd));
-        terminal_current_into(m, nrhs, xv, ldx, omega, conditioning, axis, length, voltage, current, resistance,
-                              inductance, m->tcontrib.as<cd>(), m->tcurrent.as<cd>());
+        terminal_current_one(m, nrhs, xv, ldx, omega, conditioning, axis, length, voltage, current, resistance,
+                             inductance, m->tcontrib.as<cd>(), m->tcurrent.as<cd>());
     });
 }
 
@@ -1322,8 +1651,9 @@ extern "C" int aphi_terminal_current_noalloc(aphi_mesh* m, int nrhs, const void*
                                              int conditioning, const double* axis, double length, double voltage,
                                              double* current, double* resistance, double* inductance) {
     return aphi_capi::guarded([&] {
-        check_terminal_current(m, xv, nrhs, ldx, conditioning, axis, current, length,
-                               "aphi_terminal_current_noalloc");
+        check_terminal_current(m, xv, nrhs, ldx, conditioning, axis, current, "aphi_terminal_current_noalloc");
+        require_single(m, "aphi_terminal_current_noalloc");
+        if (!(length > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
         if (static_cast<int64_t>(m->num_tets) * nrhs > kNoAllocTerms)
             throw std::invalid_argument("aphi_terminal_current_noalloc: num_tets * nrhs = " +
                                         std::to_string(static_cast<int64_t>(m->num_tets) * nrhs) +
@@ -1331,8 +1661,8 @@ extern "C" int aphi_terminal_current_noalloc(aphi_mesh* m, int nrhs, const void*
                                         " terms; pass fewer right-hand sides per call");
         require(m->ctx->scratch != nullptr, "aphi_terminal_current_noalloc: the context has no scratch address");
         after_caller(m->ctx);
-        terminal_current_into(m, nrhs, xv, ldx, omega, conditioning, axis, length, voltage, current, resistance,
-                              inductance, m->ctx->scratch, m->ctx->scratch + kNoAllocTerms);
+        terminal_current_one(m, nrhs, xv, ldx, omega, conditioning, axis, length, voltage, current, resistance,
+                             inductance, m->ctx->scratch, m->ctx->scratch + kNoAllocTerms);
     });
 }
 
@@ -1384,6 +1714,7 @@ extern "C" int aphi_random_normal(aphi_context* ctx, uint64_t seed, uint64_t off
 extern "C" int aphi_tet_fields(aphi_mesh* m, const void* xv, double omega, int conditioning, void* e_tet) {
     return aphi_capi::guarded([&] {
         require(m && xv && e_tet, "aphi_tet_fields: null argument");
+        require_single(m, "aphi_tet_fields");
         require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, "aphi_tet_fields: unknown conditioning");
         require(m->tdofs.p != nullptr, "aphi_tet_fields: the mesh was created without per-tet unknowns");
         after_caller(m->ctx);
@@ -1588,6 +1919,7 @@ extern "C" int aphi_backward_error(aphi_mesh* m, int nrhs, const void* x, int64_
                                    double* be, double* rr) {
     return aphi_capi::guarded([&] {
         require(m && x && b, "aphi_backward_error: null argument");
+        require_single(m, "aphi_backward_error");
         require(m->assembled, "aphi_backward_error: nothing assembled");
         require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_backward_error: nrhs must be 1..64 and ld >= n");
         after_caller(m->ctx);
@@ -1595,84 +1927,411 @@ extern "C" int aphi_backward_error(aphi_mesh* m, int nrhs, const void* x, int64_
     });
 }
 
+namespace {
+// solve_symmetric with refinement over every (problem, right-hand side) column;
+// fills the report's shared fields (equilibration, factor, timings) and leaves
+// the per-column measures in be / rr / bs. Throws PivotFailure after filling r.
+void solve_symmetric_all(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ldx,
+                         const aphi_solve_options* opt, aphi_solve_report& r, std::vector<double>& be,
+                         std::vector<double>& rr, std::vector<double>& bs) {
+    require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_solve_symmetric: nrhs must be 1..64 and ld >= n");
+    require(opt->max_refinements >= 0, "aphi_solve_symmetric: max_refinements must be >= 0");
+    after_caller(m->ctx);
+    const auto t0 = std::chrono::steady_clock::now();
+    r = aphi_solve_report{};
+    equilibrate(m, opt->equilibration_iterations, nullptr);
+    r.equilibration_min = m->eq_min;
+    r.equilibration_max = m->eq_max;
+    aphi_factor_stats fs{};
+    factorize(m, opt->precision, opt->pivot_floor, &fs);
+    r.smallest_pivot = fs.smallest_pivot;
+    r.largest_pivot = fs.largest_pivot;
+    r.largest_multiplier = fs.largest_multiplier;
+    r.factor_nnz = fs.nnz;
+    r.columns_done = fs.columns_done;
+    r.factorize_ms = fs.milliseconds;
+    if (!fs.ok) throw PivotFailure("solve_symmetric: the factorization hit a pivot at or below the floor");
+    const auto s0 = std::chrono::steady_clock::now();
+    solve_any(m, nrhs, b, ldb, x, ldx, 0);
+    const size_t ncol = static_cast<size_t>(nrhs) * m->seg.count;
+    be.assign(ncol, 0.0);
+    rr.assign(ncol, 0.0);
+    bs.assign(ncol, 0.0);
+    int steps = 0, since_factor = 0;
+    r.precision_used = opt->precision;
+    if (opt->precision == APHI_FP32 && opt->max_refinements > 0) {
+        // FP32 factor: GMRES refinement; if it does not converge within the
+        // budget, the loop below refactors in FP64
+        steps = gmres_refine(m, nrhs, b, ldb, x, ldx, opt->tolerance, opt->max_refinements, be, rr, bs);
+        since_factor = opt->max_refinements;
+    }
+    // largest entry, +inf when any is NaN (so a non-finite solution never passes)
+    const auto worst_of = [](const std::vector<double>& v) {
+        double w = 0.0;
+        for (double e : v) w = std::isnan(e) ? std::numeric_limits<double>::infinity() : std::max(w, e);
+        return w;
+    };
+    for (;;) {
+        measure(m, nrhs, x, ldx, b, ldb, be.data(), rr.data(), bs.data());
+        const double worst = std::max(worst_of(be), worst_of(bs));
+        const bool finite = std::isfinite(worst);
+        if (opt->max_refinements == 0 && finite) break;
+        if (worst <= opt->tolerance) break;
+        if (since_factor >= opt->max_refinements || !finite) {
+            if (m->factor_precision != APHI_FP32) break;
+            // FP32 refinement did not converge (or overflowed): factor in FP64
+            // and keep refining; a non-finite iterate restarts from b.
+            aphi_factor_stats f64{};
+            factorize(m, APHI_FP64, opt->pivot_floor, &f64);
+            r.factorize_ms += f64.milliseconds;
+            r.precision_used = APHI_FP64;
+            since_factor = 0;
+            if (!f64.ok) break;
+            if (!finite) {
+                solve_any(m, nrhs, b, ldb, x, ldx, 0);
+                continue;
+            }
+            if (opt->max_refinements == 0) continue;
+        }
+        solve_any(m, nrhs, m->res.as<cd>(), m->n, x, ldx, 1);
+        ++steps;
+        ++since_factor;
+    }
+    const auto s1 = std::chrono::steady_clock::now();
+    r.refinement_steps = steps;
+    r.backward_error = worst_of(be);
+    r.scaled_backward_error = worst_of(bs);
+    r.residual = worst_of(rr);
+    r.solve_ms = std::chrono::duration<double, std::milli>(s1 - s0).count();
+    r.total_ms = std::chrono::duration<double, std::milli>(s1 - t0).count();
+    r.ok = std::isfinite(r.backward_error) ? 1 : 0;
+}
+}  // namespace
+
 extern "C" int aphi_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int64_t ldb, void* xv, int64_t ldx,
                                     const aphi_solve_options* opt, aphi_solve_report* rep, double* backward_errors) {
     return aphi_capi::guarded([&] {
         require(m && bv && xv && opt && rep, "aphi_solve_symmetric: null argument");
-        require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n, "aphi_solve_symmetric: nrhs must be 1..64 and ld >= n");
-        require(opt->max_refinements >= 0, "aphi_solve_symmetric: max_refinements must be >= 0");
-        after_caller(m->ctx);
-        const auto t0 = std::chrono::steady_clock::now();
-        const cd* b = static_cast<const cd*>(bv);
-        cd* x = static_cast<cd*>(xv);
+        require_single(m, "aphi_solve_symmetric");
         aphi_solve_report r{};
-        equilibrate(m, opt->equilibration_iterations, nullptr);
-        r.equilibration_min = m->eq_min;
-        r.equilibration_max = m->eq_max;
-        aphi_factor_stats fs{};
-        factorize(m, opt->precision, opt->pivot_floor, &fs);
-        r.smallest_pivot = fs.smallest_pivot;
-        r.largest_pivot = fs.largest_pivot;
-        r.largest_multiplier = fs.largest_multiplier;
-        r.factor_nnz = fs.nnz;
-        r.columns_done = fs.columns_done;
-        r.factorize_ms = fs.milliseconds;
-        if (!fs.ok) {
+        std::vector<double> be, rr, bs;
+        try {
+            solve_symmetric_all(m, nrhs, static_cast<const cd*>(bv), ldb, static_cast<cd*>(xv), ldx, opt, r, be, rr, bs);
+        } catch (const PivotFailure&) {
             *rep = r;
-            throw PivotFailure("solve_symmetric: the factorization hit a pivot at or below the floor");
+            throw;
         }
-        const auto s0 = std::chrono::steady_clock::now();
-        solve_any(m, nrhs, b, ldb, x, ldx, 0);
-        std::vector<double> be(static_cast<size_t>(nrhs)), rr(static_cast<size_t>(nrhs)), bs(static_cast<size_t>(nrhs));
-        int steps = 0, since_factor = 0;
-        r.precision_used = opt->precision;
-        if (opt->precision == APHI_FP32 && opt->max_refinements > 0) {
-            // FP32 factor: GMRES refinement; if it does not converge within the
-            // budget, the loop below refactors in FP64
-            steps = gmres_refine(m, nrhs, b, ldb, x, ldx, opt->tolerance, opt->max_refinements, be, rr, bs);
-            since_factor = opt->max_refinements;
-        }
-        // largest entry, +inf when any is NaN (so a non-finite solution never passes)
-        const auto worst_of = [](const std::vector<double>& v) {
-            double w = 0.0;
-            for (double e : v) w = std::isnan(e) ? std::numeric_limits<double>::infinity() : std::max(w, e);
-            return w;
-        };
-        for (;;) {
-            measure(m, nrhs, x, ldx, b, ldb, be.data(), rr.data(), bs.data());
-            const double worst = std::max(worst_of(be), worst_of(bs));
-            const bool finite = std::isfinite(worst);
-            if (opt->max_refinements == 0 && finite) break;
-            if (worst <= opt->tolerance) break;
-            if (since_factor >= opt->max_refinements || !finite) {
-                if (m->factor_precision != APHI_FP32) break;
-                // FP32 refinement did not converge (or overflowed): factor in FP64
-                // and keep refining; a non-finite iterate restarts from b.
-                aphi_factor_stats f64{};
-                factorize(m, APHI_FP64, opt->pivot_floor, &f64);
-                r.factorize_ms += f64.milliseconds;
-                r.precision_used = APHI_FP64;
-                since_factor = 0;
-                if (!f64.ok) break;
-                if (!finite) {
-                    solve_any(m, nrhs, b, ldb, x, ldx, 0);
-                    continue;
-                }
-                if (opt->max_refinements == 0) continue;
-            }
-            solve_any(m, nrhs, m->res.as<cd>(), m->n, x, ldx, 1);
-            ++steps;
-            ++since_factor;
-        }
-        const auto s1 = std::chrono::steady_clock::now();
-        r.refinement_steps = steps;
-        r.backward_error = worst_of(be);
-        r.scaled_backward_error = worst_of(bs);
-        r.residual = worst_of(rr);
-        r.solve_ms = std::chrono::duration<double, std::milli>(s1 - s0).count();
-        r.total_ms = std::chrono::duration<double, std::milli>(s1 - t0).count();
-        r.ok = std::isfinite(r.backward_error) ? 1 : 0;
         if (backward_errors) std::memcpy(backward_errors, be.data(), be.size() * sizeof(double));
         *rep = r;
+    });
+}
+
+// ------------------------------------------------------------------ batches
+
+namespace {
+// The block-diagonal union of several meshes' descriptions: problem p's unknowns,
+// stored entries, tets, bodies, supernodes, panels and contribution blocks follow
+// those of problems 0 .. p-1; its integer references are shifted to match, its
+// supernode levels are kept (so fronts of equal height of every problem share a
+// level, children before parents), and its assembly targets keep their order
+// (all problems' slots, then all problems' rhs rows).
+struct UnionDesc {
+    aphi_mesh_desc d{};
+    std::vector<double> coords, sigma, eps_r, mu_r, values, phi_coeff, phi_fixed;
+    std::vector<int32_t> tet_nodes, tet_body, row_ptr, col, fperm, col_ref, first, parent, rows, rel, level, dofs;
+    std::vector<int64_t> tptr, srow_ptr, panel_off, cb_off, s2p;
+    std::vector<uint32_t> entry;
+    std::vector<uint8_t> kind;
+    std::vector<int8_t> signs;
+};
+
+void build_union(const aphi_mesh_desc* ds, int count, UnionDesc& u, aphi_mesh::Segments& seg) {
+    int64_t nodes = 0, tets = 0, bodies = 0, n = 0, nnz = 0, ns = 0, rows = 0, panel = 0, cb = 0, nc = 0;
+    int levels = 0, nvals = 0;
+    double flops = 0.0;
+    int64_t pred = 0;
+    seg.count = count;
+    seg.rows.assign(1, 0);
+    seg.slots.assign(1, 0);
+    seg.tets.assign(1, 0);
+    seg.bodies.assign(1, 0);
+    seg.values.clear();
+    for (int p = 0; p < count; ++p) {
+        const aphi_mesh_desc& d = ds[p];
+        require(d.n > 0 && d.nnz > 0 && d.num_super > 0 && d.num_tets > 0 && d.target_ptr && d.tet_dofs &&
+                    d.tet_edge_signs && d.tet_phi_coeff && d.tet_phi_fixed,
+                "aphi_batch_create: every problem needs a mesh with an assembly plan and per-tet unknowns");
+        require(d.num_values >= 0 && d.num_values <= 8, "aphi_batch_create: at most 8 prescribed values per problem");
+        nodes += d.num_nodes;
+        tets += d.num_tets;
+        bodies += d.num_bodies;
+        n += d.n;
+        nnz += d.nnz;
+        ns += d.num_super;
+        rows += d.super_row_ptr[d.num_super];
+        panel += d.panel_offset[d.num_super];
+        cb += d.cb_pool_entries;
+        nc += d.target_ptr[d.nnz + d.n];
+        levels = std::max(levels, d.num_levels);
+        nvals = std::max(nvals, d.num_values);
+        flops += d.flops;
+        pred += d.predicted_nnz;
+        seg.rows.push_back(n);
+        seg.slots.push_back(nnz);
+        seg.tets.push_back(tets);
+        seg.bodies.push_back(bodies);
+        std::vector<Complex> v;
+        for (int i = 0; i < d.num_values; ++i) v.emplace_back(d.values[2 * i], d.values[2 * i + 1]);
+        seg.values.push_back(v);
+    }
+    require(n < INT_MAX && nnz < INT_MAX && ns < INT_MAX && nodes < INT_MAX, "aphi_batch_create: the batch exceeds 32-bit indexing");
+    require(tets * 196 < static_cast<int64_t>(UINT32_MAX), "aphi_batch_create: too many tets for the assembly plan");
+    u.coords.reserve(static_cast<size_t>(3 * nodes));
+    int64_t o_nodes = 0, o_tets = 0, o_bodies = 0, o_n = 0, o_nnz = 0, o_ns = 0, o_rows = 0, o_panel = 0, o_cb = 0;
+    u.tptr.assign(1, 0);
+    // slot targets of every problem first, then rhs targets of every problem
+    for (int pass = 0; pass < 2; ++pass) {
+        int64_t ot = 0;
+        for (int p = 0; p < count; ++p) {
+            const aphi_mesh_desc& d = ds[p];
+            const int64_t t0 = pass == 0 ? 0 : d.nnz, t1 = pass == 0 ? d.nnz : d.nnz + d.n;
+            for (int64_t t = t0; t < t1; ++t) {
+                for (int64_t c = d.target_ptr[t]; c < d.target_ptr[t + 1]; ++c) {
+                    const int kk = d.kind[c] & 0x0f;
+                    u.entry.push_back(kk <= 5 ? d.entry[c] + static_cast<uint32_t>(ot) * 196u : d.entry[c]);
+                    u.kind.push_back(d.kind[c]);
+                }
+                u.tptr.push_back(static_cast<int64_t>(u.entry.size()));
+            }
+            ot += d.num_tets;
+        }
+    }
+    for (int p = 0; p < count; ++p) {
+        const aphi_mesh_desc& d = ds[p];
+        u.coords.insert(u.coords.end(), d.coords, d.coords + 3 * static_cast<int64_t>(d.num_nodes));
+        for (int64_t i = 0; i < 4 * static_cast<int64_t>(d.num_tets); ++i)
+            u.tet_nodes.push_back(d.tet_nodes[i] + static_cast<int32_t>(o_nodes));
+        for (int t = 0; t < d.num_tets; ++t) u.tet_body.push_back(d.tet_body[t] + static_cast<int32_t>(o_bodies));
+        u.sigma.insert(u.sigma.end(), d.body_sigma, d.body_sigma + d.num_bodies);
+        u.eps_r.insert(u.eps_r.end(), d.body_eps_r, d.body_eps_r + d.num_bodies);
+        u.mu_r.insert(u.mu_r.end(), d.body_mu_r, d.body_mu_r + d.num_bodies);
+        for (int i = 0; i < d.n; ++i) u.row_ptr.push_back(d.row_ptr[i] + static_cast<int32_t>(o_nnz));
+        for (int64_t k = 0; k < d.nnz; ++k) u.col.push_back(d.col_index[k] + static_cast<int32_t>(o_n));
+        for (int i = 0; i < d.n; ++i) {
+            u.fperm.push_back(d.factor_perm[i] + static_cast<int32_t>(o_n));
+            u.col_ref.push_back(d.col_ref[i] + static_cast<int32_t>(o_n));
+        }
+        for (int sn = 0; sn < d.num_super; ++sn) {
+            u.first.push_back(d.super_first[sn] + static_cast<int32_t>(o_n));
+            u.parent.push_back(d.super_parent[sn] >= 0 ? d.super_parent[sn] + static_cast<int32_t>(o_ns) : -1);
+            u.srow_ptr.push_back(d.super_row_ptr[sn] + o_rows);
+            u.level.push_back(d.super_level[sn]);
+            u.panel_off.push_back(d.panel_offset[sn] + o_panel);
+            u.cb_off.push_back(d.cb_offset[sn] >= 0 ? d.cb_offset[sn] + o_cb : -1);
+        }
+        for (int64_t i = 0; i < d.super_row_ptr[d.num_super]; ++i) {
+            u.rows.push_back(d.super_rows[i] + static_cast<int32_t>(o_n));
+            u.rel.push_back(d.super_rel[i]);
+        }
+        for (int64_t k = 0; k < d.nnz; ++k) u.s2p.push_back(d.slot_to_panel[k] + o_panel);
+        for (int64_t i = 0; i < 16 * static_cast<int64_t>(d.num_tets); ++i)
+            u.dofs.push_back(d.tet_dofs[i] >= 0 ? d.tet_dofs[i] + static_cast<int32_t>(o_n) : -1);
+        u.signs.insert(u.signs.end(), d.tet_edge_signs, d.tet_edge_signs + 6 * static_cast<int64_t>(d.num_tets));
+        u.phi_coeff.insert(u.phi_coeff.end(), d.tet_phi_coeff, d.tet_phi_coeff + 10 * static_cast<int64_t>(d.num_tets));
+        u.phi_fixed.insert(u.phi_fixed.end(), d.tet_phi_fixed, d.tet_phi_fixed + 20 * static_cast<int64_t>(d.num_tets));
+        o_nodes += d.num_nodes;
+        o_tets += d.num_tets;
+        o_bodies += d.num_bodies;
+        o_n += d.n;
+        o_nnz += d.nnz;
+        o_ns += d.num_super;
+        o_rows += d.super_row_ptr[d.num_super];
+        o_panel += d.panel_offset[d.num_super];
+        o_cb += d.cb_pool_entries;
+    }
+    u.row_ptr.push_back(static_cast<int32_t>(nnz));
+    u.first.push_back(static_cast<int32_t>(n));
+    u.srow_ptr.push_back(rows);
+    u.panel_off.push_back(panel);
+    // the kind bytes keep each problem's own value ids; the refill takes the values
+    // from the problem's scale table, so the union's table only sizes the check
+    u.values.assign(2 * static_cast<size_t>(std::max(nvals, 1)), 0.0);
+    aphi_mesh_desc& d = u.d;
+    d.num_nodes = static_cast<int32_t>(nodes);
+    d.num_tets = static_cast<int32_t>(tets);
+    d.num_bodies = static_cast<int32_t>(bodies);
+    d.n = static_cast<int32_t>(n);
+    d.nnz = nnz;
+    d.coords = u.coords.data();
+    d.tet_nodes = u.tet_nodes.data();
+    d.tet_body = u.tet_body.data();
+    d.body_sigma = u.sigma.data();
+    d.body_eps_r = u.eps_r.data();
+    d.body_mu_r = u.mu_r.data();
+    d.target_ptr = u.tptr.data();
+    d.entry = u.entry.data();
+    d.kind = u.kind.data();
+    d.num_values = nvals;
+    d.values = u.values.data();
+    d.row_ptr = u.row_ptr.data();
+    d.col_index = u.col.data();
+    d.factor_perm = u.fperm.data();
+    d.col_ref = u.col_ref.data();
+    d.num_super = static_cast<int32_t>(ns);
+    d.num_levels = levels;
+    d.super_first = u.first.data();
+    d.super_parent = u.parent.data();
+    d.super_row_ptr = u.srow_ptr.data();
+    d.super_rows = u.rows.data();
+    d.super_rel = u.rel.data();
+    d.super_level = u.level.data();
+    d.panel_offset = u.panel_off.data();
+    d.cb_offset = u.cb_off.data();
+    d.cb_pool_entries = cb;
+    d.slot_to_panel = u.s2p.data();
+    d.flops = flops;
+    d.predicted_nnz = pred;
+    d.tet_dofs = u.dofs.data();
+    d.tet_edge_signs = u.signs.data();
+    d.tet_phi_coeff = u.phi_coeff.data();
+    d.tet_phi_fixed = u.phi_fixed.data();
+    (void)nc;
+}
+
+void require_batch_args(const aphi_mesh* m, const char* who) {
+    require(m != nullptr, (std::string(who) + ": null mesh").c_str());
+}
+}  // namespace
+
+extern "C" int aphi_batch_create(aphi_context* ctx, const aphi_mesh_desc* descs, int count, aphi_mesh** out) {
+    return aphi_capi::guarded([&] {
+        require(ctx && descs && out, "aphi_batch_create: null argument");
+        require(count >= 1, "aphi_batch_create: count must be >= 1");
+        *out = nullptr;
+        UnionDesc u;
+        aphi_mesh::Segments seg;
+        build_union(descs, count, u, seg);
+        auto m = build_mesh(ctx, &u.d);
+        m->seg.count = seg.count;
+        m->seg.rows = seg.rows;
+        m->seg.slots = seg.slots;
+        m->seg.tets = seg.tets;
+        m->seg.bodies = seg.bodies;
+        m->seg.values = seg.values;
+        upload_segments(m.get());
+        *out = m.release();
+    });
+}
+
+extern "C" int aphi_batch_info(const aphi_mesh* m, int* count, int64_t* row_offsets, int64_t* tet_offsets) {
+    return aphi_capi::guarded([&] {
+        require_batch_args(m, "aphi_batch_info");
+        if (count) *count = m->seg.count;
+        if (row_offsets) std::memcpy(row_offsets, m->seg.rows.data(), m->seg.rows.size() * sizeof(int64_t));
+        if (tet_offsets) std::memcpy(tet_offsets, m->seg.tets.data(), m->seg.tets.size() * sizeof(int64_t));
+    });
+}
+
+extern "C" int aphi_batch_refill(aphi_mesh* m, const double* omegas, int conditioning) {
+    return aphi_capi::guarded([&] {
+        require_batch_args(m, "aphi_batch_refill");
+        require(omegas != nullptr, "aphi_batch_refill: null frequencies");
+        refill_all(m, omegas, conditioning);
+    });
+}
+
+extern "C" int aphi_batch_solve_symmetric(aphi_mesh* m, int nrhs, const void* bv, int64_t ldb, void* xv, int64_t ldx,
+                                          const aphi_solve_options* opt, aphi_solve_report* reports,
+                                          double* backward_errors) {
+    return aphi_capi::guarded([&] {
+        require_batch_args(m, "aphi_batch_solve_symmetric");
+        require(bv && xv && opt && reports, "aphi_batch_solve_symmetric: null argument");
+        const int ns = m->seg.count;
+        aphi_solve_report r{};
+        std::vector<double> be, rr, bs;
+        const auto fill = [&](bool have) {
+            for (int sg = 0; sg < ns; ++sg) {
+                aphi_solve_report q = r;
+                if (have) {
+                    double wb = 0.0, ws = 0.0, wr = 0.0;
+                    const auto mx = [](double a, double e) { return std::isnan(e) ? std::numeric_limits<double>::infinity() : std::max(a, e); };
+                    for (int c = sg; c < nrhs * ns; c += ns) {
+                        wb = mx(wb, be[static_cast<size_t>(c)]);
+                        ws = mx(ws, bs[static_cast<size_t>(c)]);
+                        wr = mx(wr, rr[static_cast<size_t>(c)]);
+                    }
+                    q.backward_error = wb;
+                    q.scaled_backward_error = ws;
+                    q.residual = wr;
+                    q.ok = std::isfinite(wb) ? 1 : 0;
+                }
+                reports[sg] = q;
+            }
+        };
+        try {
+            solve_symmetric_all(m, nrhs, static_cast<const cd*>(bv), ldb, static_cast<cd*>(xv), ldx, opt, r, be, rr, bs);
+        } catch (const PivotFailure&) {
+            fill(false);
+            throw;
+        }
+        fill(true);
+        if (backward_errors)
+            for (int rhs = 0; rhs < nrhs; ++rhs)
+                for (int sg = 0; sg < ns; ++sg)
+                    backward_errors[static_cast<size_t>(sg) * nrhs + rhs] = be[static_cast<size_t>(rhs) * ns + sg];
+    });
+}
+
+extern "C" int aphi_batch_backward_error(aphi_mesh* m, int nrhs, const void* x, int64_t ldx, const void* b, int64_t ldb,
+                                         double* be, double* rr) {
+    return aphi_capi::guarded([&] {
+        require_batch_args(m, "aphi_batch_backward_error");
+        require(x && b, "aphi_batch_backward_error: null argument");
+        require(m->assembled, "aphi_batch_backward_error: nothing assembled");
+        require(nrhs >= 1 && nrhs <= 64 && ldb >= m->n && ldx >= m->n,
+                "aphi_batch_backward_error: nrhs must be 1..64 and ld >= n");
+        after_caller(m->ctx);
+        const int ns = m->seg.count;
+        std::vector<double> e(static_cast<size_t>(nrhs) * ns), q(e.size());
+        measure(m, nrhs, static_cast<const cd*>(x), ldx, static_cast<const cd*>(b), ldb, e.data(), q.data());
+        for (int rhs = 0; rhs < nrhs; ++rhs)
+            for (int sg = 0; sg < ns; ++sg) {
+                const size_t c = static_cast<size_t>(rhs) * ns + sg, o = static_cast<size_t>(sg) * nrhs + rhs;
+                if (be) be[o] = e[c];
+                if (rr) rr[o] = q[c];
+            }
+    });
+}
+
+extern "C" int aphi_batch_terminal_current(aphi_mesh* m, int nrhs, const void* xv, int64_t ldx, const double* omegas,
+                                           int conditioning, const double* axis, const double* lengths,
+                                           const double* voltages, double* current, double* resistance,
+                                           double* inductance) {
+    return aphi_capi::guarded([&] {
+        check_terminal_current(m, xv, nrhs, ldx, conditioning, axis, current, "aphi_batch_terminal_current");
+        require(omegas && lengths && voltages, "aphi_batch_terminal_current: null argument");
+        const int ns = m->seg.count;
+        for (int sg = 0; sg < ns; ++sg)
+            if (!(lengths[sg] > 0.0)) throw std::invalid_argument("terminal_current: length must be positive");
+        after_caller(m->ctx);
+        m->tcontrib.reserve(static_cast<size_t>(m->num_tets) * nrhs * sizeof(cd));
+        m->tcurrent.reserve(static_cast<size_t>(nrhs) * ns * sizeof(cd) + static_cast<size_t>(2 * ns) * sizeof(cd) +
+                            static_cast<size_t>(ns) * sizeof(double));
+        cd* sums = m->tcurrent.as<cd>();
+        cd* factors = sums + static_cast<size_t>(nrhs) * ns;
+        double* dl = reinterpret_cast<double*>(factors + 2 * static_cast<size_t>(ns));
+        std::vector<double> cur(2 * static_cast<size_t>(nrhs) * ns), res(static_cast<size_t>(nrhs) * ns),
+            ind(static_cast<size_t>(nrhs) * ns);
+        terminal_current_into(m, nrhs, xv, ldx, omegas, conditioning, axis, lengths, voltages, cur.data(), res.data(),
+                              ind.data(), m->tcontrib.as<cd>(), sums, factors, dl);
+        // per problem, then per right-hand side
+        for (int rhs = 0; rhs < nrhs; ++rhs)
+            for (int sg = 0; sg < ns; ++sg) {
+                const size_t c = static_cast<size_t>(rhs) * ns + sg, o = static_cast<size_t>(sg) * nrhs + rhs;
+                current[2 * o] = cur[2 * c];
+                current[2 * o + 1] = cur[2 * c + 1];
+                if (resistance) resistance[o] = res[c];
+                if (inductance) inductance[o] = ind[c];
+            }
     });
 }

@@ -235,14 +235,31 @@ __global__ void __launch_bounds__(256) k_element(int num_tets, const double* __r
 // One thread per target (a matrix slot, or an rhs entry at num_slots + i). The
 // contributions are summed in the recorded order, each formed exactly as the
 // reference's scatter forms it.
-__global__ void __launch_bounds__(256) k_assemble_reduce(int64_t num_targets, int num_slots,
+__global__ void __launch_bounds__(256) k_assemble_reduce(int64_t num_targets, int64_t num_slots,
                                                          const int64_t* __restrict__ tptr,
                                                          const uint32_t* __restrict__ entry,
                                                          const uint8_t* __restrict__ kind,
-                                                         const cd* __restrict__ E, FreqScales sc,
+                                                         const cd* __restrict__ E, const FreqScales* __restrict__ scales,
+                                                         int nseg, const int64_t* __restrict__ seg_slots,
+                                                         const int64_t* __restrict__ seg_rows,
                                                          cd* __restrict__ values, cd* __restrict__ rhs) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= num_targets) return;
+    // the problem this target belongs to (a batch is several problems side by
+    // side: their slots, then their rhs rows), by bisection of the offsets
+    int seg = 0;
+    if (nseg > 1) {
+        const int64_t* off = t < num_slots ? seg_slots : seg_rows;
+        const int64_t key = t < num_slots ? t : t - num_slots;
+        int lo = 0, hi = nseg;  // off[lo] <= key < off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (off[mid] <= key) lo = mid;
+            else hi = mid;
+        }
+        seg = lo;
+    }
+    const FreqScales& sc = scales[seg];
     cd acc = {0.0, 0.0};
     for (int64_t c = tptr[t]; c < tptr[t + 1]; ++c) {
         const int k = kind[c];
@@ -291,6 +308,10 @@ __global__ void __launch_bounds__(256) k_assemble_reduce(int64_t num_targets, in
 __global__ void k_copy_cd(int64_t n, const cd* __restrict__ src, cd* __restrict__ dst) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_fill_flag(int* __restrict__ flag) {
+    if (threadIdx.x == 0) *flag = 0;
 }
 
 __global__ void k_fill_d(int64_t n, double* __restrict__ p, double v) {
@@ -446,43 +467,6 @@ __global__ void __launch_bounds__(256) k_sumsq_partial(int n, const cd* __restri
     if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = red[0];
 }
 
-// Per-RHS sums of (d_i |v_i|)^2 (power = 1) or (|v_i| / d_i)^2 (power = -1): the
-// norms of the equilibrated system's residual, solution and right-hand side.
-__global__ void __launch_bounds__(256) k_sumsq_weighted_partial(int n, const cd* __restrict__ v, int64_t ldv,
-                                                                const double* __restrict__ d, int power,
-                                                                double* __restrict__ partials) {
-    __shared__ double red[256];
-    const int c = blockIdx.y;
-    const cd* col = v + c * ldv;
-    double s = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double w = power > 0 ? d[i] : 1.0 / d[i];
-        const double re = col[i].re * w, im = col[i].im * w;
-        s += re * re + im * im;
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = red[0];
-}
-
-__global__ void __launch_bounds__(256) k_absmax_partial(int64_t n, const cd* __restrict__ v, double* __restrict__ partials) {
-    __shared__ double red[256];
-    double s = 0.0;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256)
-        s = fmax(s, hypot(v[i].re, v[i].im));
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
-}
-
 // b~[i] = d[perm[i]] * b[perm[i]]  (scale_rhs then P b; the order of the product
 // (b * (d, 0)) follows src/equilibration.cpp:73-82)
 __global__ void k_scale_permute_in(int n, int nrhs, const int* __restrict__ perm, const double* __restrict__ d,
@@ -520,15 +504,25 @@ __global__ void k_scale_d(int n, int nrhs, const double* __restrict__ d, int pow
     out[r * ldo + i] = power > 0 ? cscale(v, d[i]) : cd{v.re / d[i], v.im / d[i]};
 }
 
-// partials[(v * nrhs + r) * gridDim.x + blk] = sum over the block's rows of conj(V_v[r]) w[r]
-__global__ void __launch_bounds__(256) k_dot_partial(int n, int nrhs, const cd* __restrict__ V, int64_t vstride,
+// A batch holds several problems side by side: problem s owns rows
+// [off[s], off[s + 1]) of every vector, and each (problem, right-hand side) pair
+// is a column of its own, c = r * nseg + s, with its own inner products and
+// coefficients. One problem: nseg = 1, off = {0, n}, row_seg = nullptr.
+__device__ __forceinline__ int seg_of(const int* __restrict__ row_seg, int i) { return row_seg ? row_seg[i] : 0; }
+
+// partials[(v * ncol + c) * gridDim.x + blk] = sum over the block's rows of
+// column c of conj(V_v) w; grid (parts, nvec, ncol)
+__global__ void __launch_bounds__(256) k_dot_partial(int n, int nrhs, int nseg, const int64_t* __restrict__ off,
+                                                     const cd* __restrict__ V, int64_t vstride,
                                                      const cd* __restrict__ w, cd* __restrict__ partials) {
     __shared__ double rre[256], rim[256];
-    const int v = blockIdx.y, r = blockIdx.z;
+    const int v = blockIdx.y, c = blockIdx.z;
+    const int r = c / nseg, sg = c % nseg;
     const cd* a = V + v * vstride + static_cast<int64_t>(r) * n;
     const cd* b = w + static_cast<int64_t>(r) * n;
+    const int lo = static_cast<int>(off[sg]), hi = static_cast<int>(off[sg + 1]);
     double sr = 0.0, si = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = lo + static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x); i < hi; i += gridDim.x * 256) {
         const cd x = a[i], y = b[i];
         sr += x.re * y.re + x.im * y.im;
         si += x.re * y.im - x.im * y.re;
@@ -543,37 +537,91 @@ __global__ void __launch_bounds__(256) k_dot_partial(int n, int nrhs, const cd* 
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) partials[(static_cast<int64_t>(v) * nrhs + r) * gridDim.x + blockIdx.x] = cd{rre[0], rim[0]};
+    if (threadIdx.x == 0)
+        partials[(static_cast<int64_t>(v) * nrhs * nseg + c) * gridDim.x + blockIdx.x] = cd{rre[0], rim[0]};
 }
 
-// w[r][i] -= sum_v h[v * nrhs + r] V_v[r][i]
-__global__ void k_sub_basis(int n, int nrhs, int nvec, const cd* __restrict__ V, int64_t vstride,
-                            const cd* __restrict__ h, cd* __restrict__ w) {
+// w[r][i] -= sum_v h[v * ncol + c] V_v[r][i], c = r * nseg + (problem of row i)
+__global__ void k_sub_basis(int n, int nrhs, int nseg, const int* __restrict__ row_seg, int nvec,
+                            const cd* __restrict__ V, int64_t vstride, const cd* __restrict__ h, cd* __restrict__ w) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (i >= n || r >= nrhs) return;
+    const int c = r * nseg + seg_of(row_seg, i), ncol = nrhs * nseg;
     cd acc = w[static_cast<int64_t>(r) * n + i];
-    for (int v = 0; v < nvec; ++v) acc = csub(acc, cmul(h[v * nrhs + r], V[v * vstride + static_cast<int64_t>(r) * n + i]));
+    for (int v = 0; v < nvec; ++v) acc = csub(acc, cmul(h[v * ncol + c], V[v * vstride + static_cast<int64_t>(r) * n + i]));
     w[static_cast<int64_t>(r) * n + i] = acc;
 }
 
-// out[r][i] = w[r][i] * s[r]
-__global__ void k_scale_cols(int n, int nrhs, const cd* __restrict__ w, const double* __restrict__ s, cd* __restrict__ out) {
+// out[r][i] = w[r][i] * s[c]
+__global__ void k_scale_cols(int n, int nrhs, int nseg, const int* __restrict__ row_seg, const cd* __restrict__ w,
+                             const double* __restrict__ s, cd* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (i >= n || r >= nrhs) return;
-    out[static_cast<int64_t>(r) * n + i] = cscale(w[static_cast<int64_t>(r) * n + i], s[r]);
+    out[static_cast<int64_t>(r) * n + i] = cscale(w[static_cast<int64_t>(r) * n + i], s[r * nseg + seg_of(row_seg, i)]);
 }
 
-// x[r][i] += sum_v y[v * nrhs + r] Z_v[r][i]
-__global__ void k_add_basis(int n, int nrhs, int nvec, const cd* __restrict__ Z, int64_t zstride,
-                            const cd* __restrict__ y, cd* __restrict__ x, int64_t ldx) {
+// x[r][i] += sum_v y[v * ncol + c] Z_v[r][i]
+__global__ void k_add_basis(int n, int nrhs, int nseg, const int* __restrict__ row_seg, int nvec,
+                            const cd* __restrict__ Z, int64_t zstride, const cd* __restrict__ y, cd* __restrict__ x,
+                            int64_t ldx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (i >= n || r >= nrhs) return;
+    const int c = r * nseg + seg_of(row_seg, i), ncol = nrhs * nseg;
     cd acc = x[r * ldx + i];
-    for (int v = 0; v < nvec; ++v) acc = cadd(acc, cmul(y[v * nrhs + r], Z[v * zstride + static_cast<int64_t>(r) * n + i]));
+    for (int v = 0; v < nvec; ++v) acc = cadd(acc, cmul(y[v * ncol + c], Z[v * zstride + static_cast<int64_t>(r) * n + i]));
     x[r * ldx + i] = acc;
+}
+
+// Per column c = r * nseg + s: the sum over the problem's rows of |v_i|^2
+// (power 0), (d_i |v_i|)^2 (power 1) or (|v_i| / d_i)^2 (power -1), as partials
+// partials[c * gridDim.x + blk] of a fixed-shape tree; grid (parts, ncol)
+__global__ void __launch_bounds__(256) k_seg_sumsq(int nseg, const int64_t* __restrict__ off, const cd* __restrict__ v,
+                                                   int64_t ldv, const double* __restrict__ d, int power,
+                                                   double* __restrict__ partials) {
+    __shared__ double red[256];
+    const int c = blockIdx.y;
+    const int r = c / nseg, sg = c % nseg;
+    const cd* col = v + r * ldv;
+    const int64_t lo = off[sg], hi = off[sg + 1];
+    double s = 0.0;
+    for (int64_t i = lo + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < hi; i += static_cast<int64_t>(gridDim.x) * 256) {
+        if (power == 0) {
+            s += col[i].re * col[i].re + col[i].im * col[i].im;
+        } else {
+            const double w = power > 0 ? d[i] : 1.0 / d[i];
+            const double re = col[i].re * w, im = col[i].im * w;
+            s += re * re + im * im;
+        }
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[c * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Per problem s: max |v_k| over its stored entries [off[s], off[s + 1]), as
+// partials[s * gridDim.x + blk]; grid (parts, nseg)
+__global__ void __launch_bounds__(256) k_seg_absmax(const int64_t* __restrict__ off, const cd* __restrict__ v,
+                                                    double* __restrict__ partials) {
+    __shared__ double red[256];
+    const int sg = blockIdx.y;
+    double s = 0.0;
+    for (int64_t i = off[sg] + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < off[sg + 1];
+         i += static_cast<int64_t>(gridDim.x) * 256)
+        s = fmax(s, hypot(v[i].re, v[i].im));
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[sg * gridDim.x + blockIdx.x] = red[0];
 }
 
 // ------------------------------------------------------------------ port current
@@ -621,10 +669,22 @@ __device__ __forceinline__ void centroid_e(const TetFieldPlan& tp, const cd* __r
 
 __global__ void __launch_bounds__(256) k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd* __restrict__ x,
                                                      int64_t ldx, cd minus_jw, cd scale, double ax, double ay, double az,
-                                                     cd* __restrict__ contrib) {
+                                                     int nseg, const int64_t* __restrict__ seg_tets,
+                                                     const cd* __restrict__ seg_factors, cd* __restrict__ contrib) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (t >= num_tets || r >= nrhs) return;
+    if (nseg > 1) {
+        // a batch: the problem of tet t brings its own -j w and Phi scale
+        int lo = 0, hi = nseg;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (seg_tets[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        minus_jw = seg_factors[2 * lo];
+        scale = seg_factors[2 * lo + 1];
+    }
     const double sigma = tp.body_sigma[tp.tet_body[t]];
     cd out = cd{0.0, 0.0};
     if (sigma > 0.0) {
@@ -714,15 +774,19 @@ __global__ void __launch_bounds__(256) k_gf_scatter(int num_nodes, const int* __
 // current[r] = -(sum over conducting tets, in tet order, of the flux terms) / length:
 // one wavefront per right-hand side adds 64 terms at a time lane by lane, so the
 // sum is the reference's sequential one.
-__global__ void __launch_bounds__(64) k_current_sum(int num_tets, TetFieldPlan tp, const cd* __restrict__ contrib,
-                                                    double length, cd* __restrict__ current) {
-    const int r = blockIdx.x;
+__global__ void __launch_bounds__(64) k_current_sum(int num_tets, int nseg, const int64_t* __restrict__ seg_tets,
+                                                    TetFieldPlan tp, const cd* __restrict__ contrib,
+                                                    const double* __restrict__ lengths, double length,
+                                                    cd* __restrict__ current) {
+    // block c = r * nseg + s sums problem s's tets in order for right-hand side r
+    const int c = blockIdx.x, r = c / nseg, sg = c % nseg;
     const int lane = threadIdx.x;
     const cd* cr = contrib + static_cast<int64_t>(r) * num_tets;
+    const int lo = static_cast<int>(seg_tets[sg]), hi = static_cast<int>(seg_tets[sg + 1]);
     double fre = 0.0, fim = 0.0;
-    for (int t0 = 0; t0 < num_tets; t0 += 64) {
+    for (int t0 = lo; t0 < hi; t0 += 64) {
         const int t = t0 + lane;
-        const bool on = t < num_tets && tp.body_sigma[tp.tet_body[t]] > 0.0;
+        const bool on = t < hi && tp.body_sigma[tp.tet_body[t]] > 0.0;
         const cd v = on ? cr[t] : cd{0.0, 0.0};
         const unsigned long long mask = __ballot(on);
 #pragma unroll
@@ -733,7 +797,8 @@ __global__ void __launch_bounds__(64) k_current_sum(int num_tets, TetFieldPlan t
             }
         }
     }
-    if (lane == 0) current[r] = cd{-fre / length, -fim / length};
+    const double len = lengths ? lengths[sg] : length;
+    if (lane == 0) current[c] = cd{-fre / len, -fim / len};
 }
 
 }  // namespace aphi_dev

@@ -67,10 +67,12 @@ struct FactorStatsDev {
 // assembly.hip
 __global__ void k_element(int num_tets, const double* coords, const int* tets, const int* tet_body,
                           const BodyCoef* coef, cd* E, int* error_flag);
-__global__ void k_assemble_reduce(int64_t num_targets, int num_slots, const int64_t* tptr, const uint32_t* entry,
-                                  const uint8_t* kind, const cd* E, FreqScales sc, cd* values, cd* rhs);
+__global__ void k_assemble_reduce(int64_t num_targets, int64_t num_slots, const int64_t* tptr, const uint32_t* entry,
+                                  const uint8_t* kind, const cd* E, const FreqScales* scales, int nseg,
+                                  const int64_t* seg_slots, const int64_t* seg_rows, cd* values, cd* rhs);
 __global__ void k_copy_cd(int64_t n, const cd* src, cd* dst);
 __global__ void k_fill_d(int64_t n, double* p, double v);
+__global__ void k_fill_flag(int* flag);
 __global__ void k_eq_rowmax16(int n, const int* row_ptr, const int* tptr, const int* tslot, const cd* w, double* rowmax);
 __global__ void k_eq_factor(int n, double* rowmax, double* factor, double* d);
 __global__ void k_eq_scale(int64_t nnz, const int* row_of, const int* col, const double* factor, cd* w);
@@ -80,8 +82,9 @@ __global__ void k_symv16(int n, int nrhs, const int* row_ptr, const int* col, co
                          const int* trow, const cd* a, const cd* xi, cd* y, int64_t ldy);
 __global__ void k_residual(int n, int nrhs, const cd* b, int64_t ldb, cd* r, int64_t ldr);
 __global__ void k_sumsq_partial(int n, const cd* v, int64_t ldv, double* partials);
-__global__ void k_absmax_partial(int64_t n, const cd* v, double* partials);
-__global__ void k_sumsq_weighted_partial(int n, const cd* v, int64_t ldv, const double* d, int power, double* partials);
+__global__ void k_seg_sumsq(int nseg, const int64_t* off, const cd* v, int64_t ldv, const double* d, int power,
+                            double* partials);
+__global__ void k_seg_absmax(const int64_t* off, const cd* v, double* partials);
 __global__ void k_scale_permute_in(int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
                                    cd* out, int64_t ldo);
 __global__ void k_permute_scale_out(int n, int nrhs, const int* perm, const double* d, const cd* xt, int64_t ldt,
@@ -89,10 +92,14 @@ __global__ void k_permute_scale_out(int n, int nrhs, const int* perm, const doub
 
 // GMRES refinement (FP64)
 __global__ void k_scale_d(int n, int nrhs, const double* d, int power, const cd* in, int64_t ldi, cd* out, int64_t ldo);
-__global__ void k_dot_partial(int n, int nrhs, const cd* V, int64_t vstride, const cd* w, cd* partials);
-__global__ void k_sub_basis(int n, int nrhs, int nvec, const cd* V, int64_t vstride, const cd* h, cd* w);
-__global__ void k_scale_cols(int n, int nrhs, const cd* w, const double* s, cd* out);
-__global__ void k_add_basis(int n, int nrhs, int nvec, const cd* Z, int64_t zstride, const cd* y, cd* x, int64_t ldx);
+// GMRES over columns c = r * nseg + s (problem s of a batch, right-hand side r)
+__global__ void k_dot_partial(int n, int nrhs, int nseg, const int64_t* off, const cd* V, int64_t vstride, const cd* w,
+                              cd* partials);
+__global__ void k_sub_basis(int n, int nrhs, int nseg, const int* row_seg, int nvec, const cd* V, int64_t vstride,
+                            const cd* h, cd* w);
+__global__ void k_scale_cols(int n, int nrhs, int nseg, const int* row_seg, const cd* w, const double* s, cd* out);
+__global__ void k_add_basis(int n, int nrhs, int nseg, const int* row_seg, int nvec, const cd* Z, int64_t zstride,
+                            const cd* y, cd* x, int64_t ldx);
 
 // Per-tet data of the field recovery (compute_fields, src/postprocess.cpp:312-340).
 struct TetFieldPlan {
@@ -105,9 +112,15 @@ struct TetFieldPlan {
     const double* phi_coeff;  // 10 per tet
     const cd* phi_fixed;      // 10 per tet
 };
+// nseg > 1 (a batch): tet t of problem s (seg_tets bounds) takes -j w and the Phi
+// scale from seg_factors[2 s], [2 s + 1] instead of minus_jw, scale
 __global__ void k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd* x, int64_t ldx, cd minus_jw, cd scale,
-                              double ax, double ay, double az, cd* contrib);
-__global__ void k_current_sum(int num_tets, TetFieldPlan tp, const cd* contrib, double length, cd* current);
+                              double ax, double ay, double az, int nseg, const int64_t* seg_tets, const cd* seg_factors,
+                              cd* contrib);
+// grid nrhs * nseg: problem s of right-hand side r at current[r * nseg + s];
+// lengths per problem, or `length` for all when null
+__global__ void k_current_sum(int num_tets, int nseg, const int64_t* seg_tets, TetFieldPlan tp, const cd* contrib,
+                              const double* lengths, double length, cd* current);
 __global__ void k_tet_efield(int num_tets, TetFieldPlan tp, const cd* x, cd minus_jw, cd scale, cd* e_tet);
 
 // random.hip: normal variates of the counter-based generator (philox.hpp)

@@ -238,9 +238,9 @@ void aphi_mesh_destroy(aphi_mesh* mesh);
 /* Per-launch event timing of the factorization's trailing-update kernel (adds
  * synchronisation-free event records around each launch; off by default). */
 int aphi_mesh_set_profiling(aphi_mesh* mesh, int on);
-/* The solve's per-level launches are captured into a HIP graph and replayed
- * (on by default; APHI_SOLVE_GRAPHS=0 in the environment turns it off for all
- * meshes). A capture fails while another host thread issues work on the legacy
+/* The solve's per-level launches, and the factorization's, are captured into
+ * HIP graphs and replayed (on by default; APHI_SOLVE_GRAPHS=0 and
+ * APHI_FACTOR_GRAPHS=0 in the environment turn them off for all meshes). A capture fails while another host thread issues work on the legacy
  * default stream (a synchronous copy, a library running there), so a mesh used
  * beside other threads' GPU work -- a serving slot -- turns it off here and
  * launches the same kernels one by one (bitwise the same results). */
@@ -252,6 +252,22 @@ int aphi_mesh_set_solve_graphs(aphi_mesh* mesh, int on);
  * give bitwise the same factor. No counterpart in the reference (tuning and
  * testing switch); other modes are refused (-1). */
 int aphi_mesh_set_update_tiles(aphi_mesh* mesh, int mode);
+/* Autotuner of the complex-float trailing update's tiles (no counterpart in the
+ * reference). Each update launch falls in a class by its tallest region (rows:
+ * <= 1024, <= 2048, ... , 8 buckets) and its widest region (columns: <= 64,
+ * <= 128, ..., 8 buckets); the table holds per class 0 (the default rule of
+ * aphi_mesh_set_update_tiles), 1 (64 x 64 tiles) or 2 (128 x 128 tiles), row-major
+ * [height class][width class], 64 entries. The tuner factors the mesh's assembled
+ * system in FP32 with every launch timed, once with each tiling (`repeats` times),
+ * picks the faster tiling per class, then times whole factorizations with the
+ * default rule and with that table and keeps the table only if it is faster.
+ * Every tiling gives the same factor bits, so tuning never changes results; the
+ * mesh holds a valid FP32 factor of its system afterwards. A table learnt on one
+ * mesh can be handed to others (aphi_mesh_set_update_tile_table). A mode set by
+ * aphi_mesh_set_update_tiles other than 0 overrides the table. */
+int aphi_mesh_tune_update_tiles(aphi_mesh* mesh, int repeats, int8_t* table_out, double* ms_default, double* ms_tuned);
+int aphi_mesh_get_update_tile_table(const aphi_mesh* mesh, int8_t* table);
+int aphi_mesh_set_update_tile_table(aphi_mesh* mesh, const int8_t* table);
 /* Device bytes currently held by this mesh (plan + numeric buffers). */
 int64_t aphi_mesh_device_bytes(const aphi_mesh* mesh);
 
@@ -266,6 +282,8 @@ int aphi_system_download(aphi_mesh* mesh, double* values, double* rhs);
 int aphi_system_upload(aphi_mesh* mesh, const double* values, const double* rhs);
 /* Device pointer to the assembled rhs (n complex), for solving the port system. */
 const void* aphi_rhs_device(const aphi_mesh* mesh);
+/* The assembled rhs copied into dst (device, n complex), device to device. */
+int aphi_rhs_copy(aphi_mesh* mesh, void* dst);
 
 /* Port current and impedance of each of nrhs solutions x (device, n complex per
  * column, leading dimension ldx): terminal_current (src/postprocess.cpp:526-547)
@@ -399,6 +417,44 @@ int aphi_solve_symmetric(aphi_mesh* mesh, int nrhs, const void* b, int64_t ldb, 
  * and relative_residual (:315-329) per right-hand side, on the ORIGINAL system. */
 int aphi_backward_error(aphi_mesh* mesh, int nrhs, const void* x, int64_t ldx, const void* b, int64_t ldb,
                         double* backward_errors, double* residuals);
+
+/* ------------------------------------------------------------------ batches
+ * Many independent problems in one device plan: the reference solves one problem
+ * at a time (tools/solve_mesh.cpp:207-236 calls refill, src/assembly.cpp:255-422,
+ * then solve_symmetric, src/factorization.cpp:332-398); a batch does the same for
+ * every problem with shared launches. It is the block-diagonal union of the
+ * problems' meshes: their unknowns side by side (problem p's at
+ * [row_offsets[p], row_offsets[p+1]) of every vector), one element and one
+ * scatter launch over all tets and slots, the fronts of equal elimination-tree
+ * height of every problem factored in the same launches, and the multi-RHS solves
+ * over all of them at once. Measures (backward error, refinement's inner
+ * products) stay per problem: each problem's result is what it would be alone.
+ * The generic mesh calls (aphi_equilibrate, aphi_factorize, aphi_solve,
+ * aphi_refill with one frequency for all, aphi_system_download) act on the whole
+ * batch; the calls whose outputs are per problem refuse a batch and have the
+ * aphi_batch_ forms below. */
+int aphi_batch_create(aphi_context* ctx, const aphi_mesh_desc* descs, int count, aphi_mesh** out);
+/* count (may be NULL); row_offsets, tet_offsets: count + 1 entries each (may be NULL). */
+int aphi_batch_info(const aphi_mesh* batch, int* count, int64_t* row_offsets, int64_t* tet_offsets);
+/* refill of every problem, problem p at angular frequency omegas[p]. */
+int aphi_batch_refill(aphi_mesh* batch, const double* omegas, int conditioning);
+/* solve_symmetric of every problem with the options of aphi_solve_symmetric; b, x
+ * span all problems' unknowns (n of the batch per right-hand side). reports:
+ * count entries, each with its problem's backward errors (the factorization's
+ * fields are the batch's). backward_errors (may be NULL): count * nrhs, problem-major. */
+int aphi_batch_solve_symmetric(aphi_mesh* batch, int nrhs, const void* b, int64_t ldb, void* x, int64_t ldx,
+                               const aphi_solve_options* options, aphi_solve_report* reports, double* backward_errors);
+/* backward_error and relative_residual per problem and right-hand side
+ * (count * nrhs each, problem-major). */
+int aphi_batch_backward_error(aphi_mesh* batch, int nrhs, const void* x, int64_t ldx, const void* b, int64_t ldb,
+                              double* backward_errors, double* residuals);
+/* terminal_current + impedance per problem: problem p at omegas[p] with length
+ * lengths[p] and drive voltage voltages[p], one axis for all; current 2 * count *
+ * nrhs doubles, resistance / inductance count * nrhs (problem-major). Each
+ * problem's sum runs over its own tets in order, as aphi_terminal_current does. */
+int aphi_batch_terminal_current(aphi_mesh* batch, int nrhs, const void* x, int64_t ldx, const double* omegas,
+                                int conditioning, const double* axis, const double* lengths, const double* voltages,
+                                double* current, double* resistance, double* inductance);
 
 #ifdef __cplusplus
 }

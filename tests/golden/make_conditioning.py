@@ -21,7 +21,7 @@ sys.path.insert(0, os.path.join(ROOT, "hybrid-aphi-solver_amd"))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-CASES = ["cube2", "cube3_offcentre", "config1", "cube8"]
+CASES = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw"]
 
 
 def main():

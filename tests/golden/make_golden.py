@@ -33,6 +33,10 @@ CASES = [
     ("cube3_offcentre", 3, 0.05, (1, 2, 0, 2), 1e6, 1e6, 1, True),
     ("config1", 6, 0.1, (2, 4, 2, 4), 5.8e7, 50.0, 1, False),
     ("cube8", 8, 0.2, (3, 5, 3, 5), 5.8e7, 1e5, 1, False),
+    # a draw of the batched workload: side, sigma and frequency with no short
+    # binary expansion (the scatter's 1/(j w) and its products round differently
+    # under other complex-arithmetic routines)
+    ("cube5_draw", 5, 0.10592777681478216, (2, 4, 1, 5), 27506162.131823607, 540.9701531845078, 1, False),
     ("config2", 24, 0.1, (10, 14, 10, 14), 5.8e7, 10.0, 0, False),
     ("config3", 32, 0.1, (6, 12, 20, 23), 5.8e7, 1.0, 0, False),
     # the 24^3 sweep's two end points (10 Hz and 10 MHz are exact in the log sweep)

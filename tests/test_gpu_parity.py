@@ -12,7 +12,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-NUMERIC = ["cube2", "cube3_offcentre", "config1", "cube8"]
+NUMERIC = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw"]
 
 
 def dev(a):
@@ -47,6 +47,22 @@ def test_refill_scaled_phi_bitwise(case):
     ov, orhs = O.refill(p, omega, conditioning=2)
     assert np.array_equal(vals.view(np.uint64), ov.view(np.uint64))
     assert np.array_equal(rhs.view(np.uint64), orhs.view(np.uint64))
+
+
+@pytest.mark.parametrize("index", [5, 23, 47, 63])
+def test_refill_bitwise_at_sweep_points(case, index):
+    """At the log sweep's points (expand_frequency_sweep, 1 Hz-100 MHz), whose
+    frequencies have no short binary expansion, the device refill is still the
+    oracle's bit for bit (the oracle is the reference's, pinned at such a point by
+    the cube5_draw fixture), in both symmetric conditionings."""
+    m, p = case["m"], case["p"]
+    omega = 2 * np.pi * A.frequency_sweep(1.0, 1e8, 64)[index]
+    for cond in (A.APHI_ROW_SCALED, A.APHI_SCALED_PHI):
+        m.refill(omega, cond)
+        vals, rhs = m.system()
+        ov, orhs = O.refill(p, omega, conditioning=cond)
+        assert np.array_equal(vals.view(np.uint64), ov.view(np.uint64)), cond
+        assert np.array_equal(rhs.view(np.uint64), orhs.view(np.uint64)), cond
 
 
 def test_refill_refusals(case):
