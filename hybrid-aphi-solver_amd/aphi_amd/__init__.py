@@ -155,6 +155,7 @@ class CubeConfig:
     mu_r: float = 1.0
     v_top: complex = 1.0
     v_bottom: complex = 0.0
+    i_top: complex = None               # a current (A) driving the z = side port instead of v_top
     ordering: int = APHI_ORDER_AMD
     relax_small: int = 0
     relax_fill: float = 0.0
@@ -168,6 +169,9 @@ class CubeConfig:
         vt, vb = complex(self.v_top), complex(self.v_bottom)
         s.v_top_re, s.v_top_im, s.v_bottom_re, s.v_bottom_im = vt.real, vt.imag, vb.real, vb.imag
         s.ordering, s.relax_small, s.relax_fill = self.ordering, self.relax_small, self.relax_fill
+        if self.i_top is not None:
+            it = complex(self.i_top)
+            s.top_current, s.i_top_re, s.i_top_im = 1, it.real, it.imag
         return s
 
 
@@ -236,6 +240,56 @@ class Problem:
         self._h = h
         self._read_counts()
         return self
+
+    @classmethod
+    def from_bound(cls, arrays, ordering=APHI_ORDER_AMD, relax_small=0, relax_fill=0.0):
+        """A problem from the reference host's bound objects, flattened
+        (aphi_problem_create_bound): ``arrays`` maps the aphi_bound_desc field
+        names to numpy arrays (the scatter_* arrays may be left out: the map is
+        then built from the pattern)."""
+        self = cls.__new__(cls)
+        self.config = None
+        dtypes = {_capi.I32P: np.int32, _capi.DP: np.float64, _capi.I64P: np.int64}
+        keep, d = {}, _capi.BoundDesc()
+        for f, t in _capi.BoundDesc._fields_:
+            if t is C.c_int32:
+                continue
+            if f not in arrays:
+                if f.startswith("scatter_"):
+                    continue  # built from the pattern
+                raise ValueError(f"from_bound: missing array {f}")
+            a = np.ascontiguousarray(arrays[f], dtype=dtypes.get(t, np.int8))
+            keep[f] = a
+            setattr(d, f, a.ctypes.data_as(t))
+        d.num_nodes = keep["coords"].size // 3
+        d.num_tets = keep["tet_nodes"].size // 4
+        d.num_bodies = keep["body_sigma"].size
+        d.num_total = keep["row_ptr"].size - 1
+        d.num_ports = keep["port_index"].size
+        h = C.c_void_p()
+        _check(lib().aphi_problem_create_bound(C.byref(d), int(ordering), int(relax_small), float(relax_fill),
+                                               C.byref(h)))
+        self._h = h
+        self._read_counts()
+        return self
+
+    def bound_arrays(self):
+        """This problem's bound objects flattened as aphi_problem_create_bound
+        takes them (what a reference host would hand over for its mesh)."""
+        nt = int(self.counts["tets"])
+        td = self.ints("tet_dofs").reshape(nt, 16)
+        bp = self.doubles("body_param").reshape(-1, 3)
+        fixed = self.ints("port_is_fixed")
+        amp = np.where(np.repeat(fixed, 2) == 1, self.doubles("port_value"), self.doubles("port_amplitude"))
+        return dict(coords=self.doubles("coords"), tet_nodes=self.ints("tet_nodes"),
+                    tet_edge_signs=self.ints("tet_edge_signs"), body_sigma=bp[:, 0], body_eps_r=bp[:, 1],
+                    body_mu_r=bp[:, 2], body_ptr=self.ints("body_ptr"), body_tets=self.ints("body_tets"),
+                    edge_index=td[:, :6], edge_coeff=self.doubles("tet_edge_coeff"), phi_index=td[:, 6:],
+                    phi_coeff=self.doubles("tet_phi_coeff"), phi_fixed=self.doubles("tet_phi_fixed"),
+                    port_index=self.ints("port_index"), port_current=1 - fixed, port_amplitude=amp,
+                    row_ptr=self.ints("row_ptr"), col_index=self.ints("col_index"),
+                    scatter_live=self.ints("scatter_live"), scatter_pos=self.ints("scatter_pos"),
+                    scatter_offset=self.ints("scatter_offset"), scatter_slot=self.ints("scatter_slot"))
 
     def describe_gauge_free(self):
         d = _capi.GaugeFreeDesc()

@@ -35,6 +35,7 @@ class CubeSpec(C.Structure):
         ("v_top_re", C.c_double), ("v_top_im", C.c_double),
         ("v_bottom_re", C.c_double), ("v_bottom_im", C.c_double),
         ("ordering", C.c_int), ("relax_small", C.c_int), ("relax_fill", C.c_double),
+        ("top_current", C.c_int), ("i_top_re", C.c_double), ("i_top_im", C.c_double),
     ]
 
 
@@ -77,6 +78,19 @@ class GaugeFreeDesc(C.Structure):
     ]
 
 
+class BoundDesc(C.Structure):
+    _fields_ = [
+        ("num_nodes", C.c_int32), ("num_tets", C.c_int32), ("num_bodies", C.c_int32), ("num_total", C.c_int32),
+        ("num_ports", C.c_int32),
+        ("coords", DP), ("tet_nodes", I32P), ("tet_edge_signs", C.POINTER(C.c_int8)),
+        ("body_sigma", DP), ("body_eps_r", DP), ("body_mu_r", DP), ("body_ptr", I32P), ("body_tets", I32P),
+        ("edge_index", I32P), ("edge_coeff", DP), ("phi_index", I32P), ("phi_coeff", DP), ("phi_fixed", DP),
+        ("port_index", I32P), ("port_current", I32P), ("port_amplitude", DP),
+        ("row_ptr", I32P), ("col_index", I32P),
+        ("scatter_live", I32P), ("scatter_pos", I32P), ("scatter_offset", I64P), ("scatter_slot", I32P),
+    ]
+
+
 class FactorStats(C.Structure):
     _fields_ = [("ok", C.c_int), ("columns_done", C.c_int), ("smallest_pivot", C.c_double),
                 ("largest_pivot", C.c_double), ("largest_multiplier", C.c_double), ("nnz", C.c_int64),
@@ -105,6 +119,8 @@ SIGNATURES = {
     "aphi_problem_create_cube": (C.c_int, [C.POINTER(CubeSpec), C.POINTER(C.c_void_p)]),
     "aphi_problem_create_csr": (C.c_int, [C.c_int32, I32P, I32P, C.c_int, C.c_int64, C.c_int, C.c_double,
                                           C.POINTER(C.c_void_p)]),
+    "aphi_problem_create_bound": (C.c_int, [C.POINTER(BoundDesc), C.c_int, C.c_int, C.c_double,
+                                            C.POINTER(C.c_void_p)]),
     "aphi_problem_destroy": (None, [C.c_void_p]),
     "aphi_problem_counts_get": (C.c_int, [C.c_void_p, C.POINTER(ProblemCounts)]),
     "aphi_problem_get_int": (C.c_int, [C.c_void_p, C.c_char_p, I64P, C.c_int64, I64P]),

@@ -40,7 +40,17 @@ struct aphi_problem {
     std::vector<int8_t> tet_signs;
     std::vector<double> tet_phi_coeff, tet_phi_fixed;
     double analyze_ms = 0.0, build_ms = 0.0;
+    // a problem made from a caller's bound objects (aphi_problem_create_bound):
+    // the tets' local DOFs as handed over, no mesh topology beyond the tets
+    bool from_bound = false;
+    std::vector<TetDofs> given_dofs;
 };
+
+namespace {
+TetDofs tet_dofs_of(const aphi_problem* p, int t) {
+    return p->from_bound ? p->given_dofs[static_cast<size_t>(t)] : p->dofs.local_dofs(t, p->mesh, p->bound);
+}
+}  // namespace
 
 namespace aphi_capi {
 namespace {
@@ -77,6 +87,53 @@ void finish_symbolic(aphi_problem* p, int ordering, size_t max_factor_nnz, int r
 
 }  // namespace
 
+namespace {
+// The flat copies aphi_problem_describe hands out: geometry, bodies, prescribed
+// values and every tet's local unknowns.
+void fill_flat(aphi_problem* p) {
+    const Mesh& m = p->mesh;
+    p->coords.resize(static_cast<size_t>(m.num_nodes()) * 3);
+    for (int i = 0; i < m.num_nodes(); ++i) {
+        p->coords[3 * static_cast<size_t>(i)] = m.nodes[static_cast<size_t>(i)].x;
+        p->coords[3 * static_cast<size_t>(i) + 1] = m.nodes[static_cast<size_t>(i)].y;
+        p->coords[3 * static_cast<size_t>(i) + 2] = m.nodes[static_cast<size_t>(i)].z;
+    }
+    p->tet_nodes.resize(static_cast<size_t>(m.num_tets()) * 4);
+    for (int t = 0; t < m.num_tets(); ++t)
+        for (int c = 0; c < 4; ++c) p->tet_nodes[4 * static_cast<size_t>(t) + c] = m.tets[static_cast<size_t>(t)][static_cast<size_t>(c)];
+    p->tet_body.assign(p->plan.tet_body.begin(), p->plan.tet_body.end());
+    for (const Body& b : p->bound.bodies) {
+        p->body_sigma.push_back(b.sigma);
+        p->body_eps_r.push_back(b.eps_r);
+        p->body_mu_r.push_back(b.mu_r);
+    }
+    for (const Complex& v : p->plan.fixed_value) {
+        p->values.push_back(v.real());
+        p->values.push_back(v.imag());
+    }
+    {
+        const size_t nt = static_cast<size_t>(m.num_tets());
+        p->tet_dofs.resize(nt * 16);
+        p->tet_signs.resize(nt * 6);
+        p->tet_phi_coeff.resize(nt * 10);
+        p->tet_phi_fixed.resize(nt * 20);
+        for (size_t t = 0; t < nt; ++t) {
+            const TetDofs d = tet_dofs_of(p, static_cast<int>(t));
+            for (int i = 0; i < 6; ++i) {
+                p->tet_dofs[16 * t + i] = d.edge_index[i];
+                p->tet_signs[6 * t + i] = m.tet_edge_signs[t][static_cast<size_t>(i)];
+            }
+            for (int i = 0; i < 10; ++i) {
+                p->tet_dofs[16 * t + 6 + i] = d.phi_index[i];
+                p->tet_phi_coeff[10 * t + i] = d.phi_coeff[i];
+                p->tet_phi_fixed[20 * t + 2 * i] = d.phi_fixed[i].real();
+                p->tet_phi_fixed[20 * t + 2 * i + 1] = d.phi_fixed[i].imag();
+            }
+        }
+    }
+}
+}  // namespace
+
 extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem** out) {
     return aphi_capi::guarded([&] {
         if (!spec || !out) throw std::invalid_argument("aphi_problem_create_cube: null argument");
@@ -98,6 +155,8 @@ extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem
         s.air_mu_r = spec->air_mu_r;
         s.v_top = Complex(spec->v_top_re, spec->v_top_im);
         s.v_bottom = Complex(spec->v_bottom_re, spec->v_bottom_im);
+        s.top_current = spec->top_current != 0;
+        s.i_top = Complex(spec->i_top_re, spec->i_top_im);
         if (!(s.side > 0.0)) throw std::invalid_argument("aphi_problem_create_cube: side must be positive");
         if (spec->ordering < 0 || spec->ordering > 2) throw std::invalid_argument("aphi_problem_create_cube: unknown ordering");
         p->mesh = make_kuhn_cube(s);
@@ -120,46 +179,7 @@ extern "C" int aphi_problem_create_cube(const aphi_cube_spec* spec, aphi_problem
         p->build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
                        p->analyze_ms - p->build_ms;
 
-        const Mesh& m = p->mesh;
-        p->coords.resize(static_cast<size_t>(m.num_nodes()) * 3);
-        for (int i = 0; i < m.num_nodes(); ++i) {
-            p->coords[3 * static_cast<size_t>(i)] = m.nodes[static_cast<size_t>(i)].x;
-            p->coords[3 * static_cast<size_t>(i) + 1] = m.nodes[static_cast<size_t>(i)].y;
-            p->coords[3 * static_cast<size_t>(i) + 2] = m.nodes[static_cast<size_t>(i)].z;
-        }
-        p->tet_nodes.resize(static_cast<size_t>(m.num_tets()) * 4);
-        for (int t = 0; t < m.num_tets(); ++t)
-            for (int c = 0; c < 4; ++c) p->tet_nodes[4 * static_cast<size_t>(t) + c] = m.tets[static_cast<size_t>(t)][static_cast<size_t>(c)];
-        p->tet_body.assign(p->plan.tet_body.begin(), p->plan.tet_body.end());
-        for (const Body& b : p->bound.bodies) {
-            p->body_sigma.push_back(b.sigma);
-            p->body_eps_r.push_back(b.eps_r);
-            p->body_mu_r.push_back(b.mu_r);
-        }
-        for (const Complex& v : p->plan.fixed_value) {
-            p->values.push_back(v.real());
-            p->values.push_back(v.imag());
-        }
-        {
-            const size_t nt = static_cast<size_t>(m.num_tets());
-            p->tet_dofs.resize(nt * 16);
-            p->tet_signs.resize(nt * 6);
-            p->tet_phi_coeff.resize(nt * 10);
-            p->tet_phi_fixed.resize(nt * 20);
-            for (size_t t = 0; t < nt; ++t) {
-                const TetDofs d = p->dofs.local_dofs(static_cast<int>(t), m, p->bound);
-                for (int i = 0; i < 6; ++i) {
-                    p->tet_dofs[16 * t + i] = d.edge_index[i];
-                    p->tet_signs[6 * t + i] = m.tet_edge_signs[t][static_cast<size_t>(i)];
-                }
-                for (int i = 0; i < 10; ++i) {
-                    p->tet_dofs[16 * t + 6 + i] = d.phi_index[i];
-                    p->tet_phi_coeff[10 * t + i] = d.phi_coeff[i];
-                    p->tet_phi_fixed[20 * t + 2 * i] = d.phi_fixed[i].real();
-                    p->tet_phi_fixed[20 * t + 2 * i + 1] = d.phi_fixed[i].imag();
-                }
-            }
-        }
+        fill_flat(p.get());
         const int n = p->sp.n;
         p->col_ref.resize(static_cast<size_t>(n));
         for (int j = 0; j < n; ++j)
@@ -248,10 +268,155 @@ extern "C" int aphi_problem_create_csr(int32_t n, const int32_t* row_ptr, const 
     });
 }
 
+extern "C" int aphi_problem_create_bound(const aphi_bound_desc* d, int ordering, int relax_small, double relax_fill,
+                                         aphi_problem** out) {
+    return aphi_capi::guarded([&] {
+        if (!d || !out) throw std::invalid_argument("aphi_problem_create_bound: null argument");
+        *out = nullptr;
+        const int nv = d->num_nodes, nt = d->num_tets, nb = d->num_bodies, n = d->num_total, np = d->num_ports;
+        if (nv <= 0 || nt <= 0 || nb <= 0 || n <= 0 || np < 0)
+            throw std::invalid_argument("aphi_problem_create_bound: empty mesh, bodies or unknowns");
+        if (!d->coords || !d->tet_nodes || !d->tet_edge_signs || !d->body_sigma || !d->body_eps_r || !d->body_mu_r ||
+            !d->body_ptr || !d->body_tets || !d->edge_index || !d->edge_coeff || !d->phi_index || !d->phi_coeff ||
+            !d->phi_fixed || !d->row_ptr || !d->col_index || (np > 0 && (!d->port_index || !d->port_current ||
+                                                                         !d->port_amplitude)))
+            throw std::invalid_argument("aphi_problem_create_bound: null array");
+        const bool given_map = d->scatter_live != nullptr;
+        if (given_map != (d->scatter_pos != nullptr) || given_map != (d->scatter_offset != nullptr) ||
+            given_map != (d->scatter_slot != nullptr))
+            throw std::invalid_argument("aphi_problem_create_bound: pass all four ScatterMap arrays or none");
+        if (ordering < 0 || ordering > 2) throw std::invalid_argument("aphi_problem_create_bound: unknown ordering");
+        auto p = std::make_unique<aphi_problem>();
+        p->from_bound = true;
+        const auto t0 = std::chrono::steady_clock::now();
+        // the pattern, with the checks of SparseSymmetric::from_pattern
+        if (d->row_ptr[0] != 0) throw std::invalid_argument("SparseSymmetric::from_pattern: row_ptr must run from 0 to col_index.size()");
+        for (int r = 0; r < n; ++r) {
+            if (d->row_ptr[r + 1] < d->row_ptr[r])
+                throw std::invalid_argument("SparseSymmetric::from_pattern: row_ptr is not non-decreasing");
+            for (int k = d->row_ptr[r]; k < d->row_ptr[r + 1]; ++k) {
+                const int c = d->col_index[k];
+                if (c < r || c >= n)
+                    throw std::invalid_argument("aphi_problem_create_bound: the pattern is not an upper-triangle CSR of "
+                                                "num_total rows");
+                if (k > d->row_ptr[r] && c <= d->col_index[k - 1])
+                    throw std::invalid_argument("SparseSymmetric::from_pattern: each row's columns must be strictly ascending");
+            }
+        }
+        p->pattern.rows = p->pattern.cols = n;
+        p->pattern.upper_only = true;
+        p->pattern.row_ptr.assign(d->row_ptr, d->row_ptr + n + 1);
+        p->pattern.col_index.assign(d->col_index, d->col_index + d->row_ptr[n]);
+        // the mesh: nodes, tets and their edge signs
+        Mesh& m = p->mesh;
+        m.nodes.resize(static_cast<size_t>(nv));
+        for (int i = 0; i < nv; ++i) m.nodes[static_cast<size_t>(i)] = Vec3{d->coords[3 * i], d->coords[3 * i + 1], d->coords[3 * i + 2]};
+        m.tets.resize(static_cast<size_t>(nt));
+        m.tet_edge_signs.resize(static_cast<size_t>(nt));
+        for (int t = 0; t < nt; ++t) {
+            for (int c = 0; c < 4; ++c) {
+                const int v = d->tet_nodes[4 * static_cast<int64_t>(t) + c];
+                if (v < 0 || v >= nv) throw std::invalid_argument("aphi_problem_create_bound: a tet names a missing node");
+                m.tets[static_cast<size_t>(t)][static_cast<size_t>(c)] = v;
+            }
+            for (int e = 0; e < 6; ++e) m.tet_edge_signs[static_cast<size_t>(t)][static_cast<size_t>(e)] = d->tet_edge_signs[6 * static_cast<int64_t>(t) + e];
+        }
+        // the bodies partition the tets (each body's tets ascending)
+        if (d->body_ptr[0] != 0 || d->body_ptr[nb] != nt)
+            throw std::invalid_argument("aphi_problem_create_bound: the bodies must cover every tet exactly once");
+        std::vector<uint8_t> seen(static_cast<size_t>(nt), 0);
+        for (int b = 0; b < nb; ++b) {
+            Body body;
+            body.sigma = d->body_sigma[b];
+            body.eps_r = d->body_eps_r[b];
+            body.mu_r = d->body_mu_r[b];
+            if (d->body_ptr[b + 1] < d->body_ptr[b]) throw std::invalid_argument("aphi_problem_create_bound: body_ptr decreases");
+            for (int q = d->body_ptr[b]; q < d->body_ptr[b + 1]; ++q) {
+                const int t = d->body_tets[q];
+                if (t < 0 || t >= nt || seen[static_cast<size_t>(t)] || (q > d->body_ptr[b] && t <= d->body_tets[q - 1]))
+                    throw std::invalid_argument("aphi_problem_create_bound: the bodies must cover every tet exactly once, "
+                                                "each body's tets ascending");
+                seen[static_cast<size_t>(t)] = 1;
+                body.tets.push_back(t);
+            }
+            p->bound.bodies.push_back(std::move(body));
+        }
+        // every tet's local unknowns (DofMap::local_dofs)
+        p->given_dofs.resize(static_cast<size_t>(nt));
+        for (int t = 0; t < nt; ++t) {
+            TetDofs& o = p->given_dofs[static_cast<size_t>(t)];
+            for (int i = 0; i < 6; ++i) {
+                o.edge_index[i] = d->edge_index[6 * static_cast<int64_t>(t) + i];
+                o.edge_coeff[i] = d->edge_coeff[6 * static_cast<int64_t>(t) + i];
+                if (o.edge_index[i] < -1 || o.edge_index[i] >= n)
+                    throw std::invalid_argument("aphi_problem_create_bound: an edge unknown is out of range");
+            }
+            for (int i = 0; i < 10; ++i) {
+                o.phi_index[i] = d->phi_index[10 * static_cast<int64_t>(t) + i];
+                o.phi_coeff[i] = d->phi_coeff[10 * static_cast<int64_t>(t) + i];
+                o.phi_fixed[i] = Complex(d->phi_fixed[20 * static_cast<int64_t>(t) + 2 * i],
+                                         d->phi_fixed[20 * static_cast<int64_t>(t) + 2 * i + 1]);
+                o.phi_fixed_port[i] = -1;
+                if (o.phi_index[i] < -1 || o.phi_index[i] >= n)
+                    throw std::invalid_argument("aphi_problem_create_bound: a Phi unknown is out of range");
+            }
+        }
+        // ports and the DofMap fields the scatter reads
+        p->dofs.num_total = n;
+        p->dofs.num_ports = np;
+        for (int k = 0; k < np; ++k) {
+            Port port;
+            const bool current = d->port_current[k] != 0;
+            port.type = current ? PortType::BoundaryCurrent : PortType::BoundaryVoltage;
+            const Complex a(d->port_amplitude[2 * k], d->port_amplitude[2 * k + 1]);
+            port.amplitude = a;
+            p->bound.ports.push_back(port);
+            if (d->port_index[k] < 0 || d->port_index[k] >= n)
+                throw std::invalid_argument("aphi_problem_create_bound: a port row is out of range");
+            p->dofs.port_index.push_back(d->port_index[k]);
+            p->dofs.port_is_fixed.push_back(current ? 0 : 1);
+            p->dofs.port_value.push_back(current ? Complex(0.0, 0.0) : a);
+        }
+        const TetDofsOf local = [&](int t) { return p->given_dofs[static_cast<size_t>(t)]; };
+        if (given_map) {
+            ScatterMap& sm = p->map;
+            sm.packed = true;
+            sm.live.assign(d->scatter_live, d->scatter_live + nt);
+            sm.pos.assign(d->scatter_pos, d->scatter_pos + 16 * static_cast<int64_t>(nt));
+            sm.offset.assign(d->scatter_offset, d->scatter_offset + nt + 1);
+            if (sm.offset[0] != 0) throw std::invalid_argument("aphi_problem_create_bound: scatter offsets must start at 0");
+            for (int t = 0; t < nt; ++t) {
+                const int live = sm.live[static_cast<size_t>(t)];
+                if (live < 0 || live > 16 ||
+                    sm.offset[static_cast<size_t>(t) + 1] - sm.offset[static_cast<size_t>(t)] != tet_block_size(live, true))
+                    throw std::invalid_argument("aphi_problem_create_bound: the ScatterMap blocks do not match their "
+                                                "live counts (built for a different pattern?)");
+                for (int i = 0; i < 16; ++i) {
+                    const int q = sm.pos[16 * static_cast<size_t>(t) + i];
+                    if (q < -1 || q >= live) throw std::invalid_argument("aphi_problem_create_bound: a scatter position is out of range");
+                }
+            }
+            sm.slot.assign(d->scatter_slot, d->scatter_slot + sm.offset[static_cast<size_t>(nt)]);
+            for (int v : sm.slot)
+                if (v < -1 || v >= static_cast<int>(p->pattern.nnz()))
+                    throw std::invalid_argument("aphi_problem_create_bound: a scatter slot lies outside the pattern");
+        } else {
+            p->map = build_scatter_map(p->pattern, nt, local);
+        }
+        p->plan = build_assembly_plan(p->pattern, p->map, nt, local, p->dofs, p->bound);
+        finish_symbolic(p.get(), ordering, 0, relax_small, relax_fill);
+        fill_flat(p.get());
+        p->build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() -
+                      p->analyze_ms;
+        *out = p.release();
+    });
+}
+
 extern "C" int aphi_problem_create_gauge_free(const aphi_problem* cube, int ordering, aphi_problem** out) {
     return aphi_capi::guarded([&] {
         if (!cube || !out) throw std::invalid_argument("aphi_problem_create_gauge_free: null argument");
-        if (cube->from_csr) throw std::invalid_argument("aphi_problem_create_gauge_free: the problem has no mesh");
+        if (cube->from_csr || cube->from_bound)
+            throw std::invalid_argument("aphi_problem_create_gauge_free: the problem has no mesh topology");
         if (ordering < 0 || ordering > 2) throw std::invalid_argument("aphi_problem_create_gauge_free: unknown ordering");
         *out = nullptr;
         auto p = std::make_unique<aphi_problem>();
@@ -315,7 +480,8 @@ extern "C" int aphi_write_potential(const aphi_problem* p, const char* path, con
                                     int conditioning, int format, int64_t* bytes) {
     return aphi_capi::guarded([&] {
         if (!p || !path || !x) throw std::invalid_argument("aphi_write_potential: null argument");
-        if (p->from_csr) throw std::invalid_argument("aphi_write_potential: the problem has no mesh");
+        if (p->from_csr || p->from_bound)
+            throw std::invalid_argument("aphi_write_potential: the problem has no mesh topology");
         if (conditioning < 0 || conditioning > 2) throw std::invalid_argument("aphi_write_potential: unknown conditioning");
         if (format != 0 && format != 1) throw std::invalid_argument("aphi_write_potential: format must be 0 (text) or 1 (VTK)");
         const NodalPotential np = potential_at_nodes(p->mesh, p->dofs, reinterpret_cast<const Complex*>(x),
@@ -433,7 +599,7 @@ extern "C" int aphi_problem_get_int(const aphi_problem* p, const char* name, int
             return copy_out(p->sp.perm, out, capacity, count);
         } else if (k == "tet_dofs") {
             for (int t = 0; t < m.num_tets(); ++t) {
-                const TetDofs d = p->dofs.local_dofs(t, m, p->bound);
+                const TetDofs d = tet_dofs_of(p, t);
                 for (int i = 0; i < 6; ++i) tmp.push_back(d.edge_index[i]);
                 for (int i = 0; i < 10; ++i) tmp.push_back(d.phi_index[i]);
             }
@@ -486,13 +652,20 @@ extern "C" int aphi_problem_get_double(const aphi_problem* p, const char* name, 
             src = &p->tet_phi_coeff;
         } else if (k == "tet_phi_fixed") {
             for (int t = 0; t < p->mesh.num_tets(); ++t) {
-                const TetDofs d = p->dofs.local_dofs(t, p->mesh, p->bound);
+                const TetDofs d = tet_dofs_of(p, t);
                 for (int i = 0; i < 10; ++i) { tmp.push_back(d.phi_fixed[i].real()); tmp.push_back(d.phi_fixed[i].imag()); }
             }
         } else if (k == "body_param") {
             for (const Body& b : p->bound.bodies) { tmp.push_back(b.sigma); tmp.push_back(b.eps_r); tmp.push_back(b.mu_r); }
         } else if (k == "port_value") {
             for (const Complex& v : p->dofs.port_value) { tmp.push_back(v.real()); tmp.push_back(v.imag()); }
+        } else if (k == "port_amplitude") {
+            for (const Port& q : p->bound.ports) { tmp.push_back(q.amplitude.real()); tmp.push_back(q.amplitude.imag()); }
+        } else if (k == "tet_edge_coeff") {
+            for (int t = 0; t < p->mesh.num_tets(); ++t) {
+                const TetDofs d = tet_dofs_of(p, t);
+                for (int i = 0; i < 6; ++i) tmp.push_back(d.edge_coeff[i]);
+            }
         } else if (k == "gf_prescribed") {
             src = &p->gf_prescribed;
         } else if (k == "gf_values") {

@@ -12,6 +12,7 @@
 #include <array>
 #include <complex>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -92,6 +93,8 @@ struct CubeSpec {
     double air_sigma = 0.0, air_eps_r = 1.0, air_mu_r = 1.0;
     Complex v_top{1.0, 0.0};
     Complex v_bottom{0.0, 0.0};
+    bool top_current = false;  // the z = side port current-driven with amplitude i_top
+    Complex i_top{1.0, 0.0};
 };
 
 Mesh make_kuhn_cube(const CubeSpec& spec);
@@ -210,6 +213,10 @@ struct ScatterMap {
 };
 ScatterMap build_scatter_map(const SparsityPattern& pattern, const DofMap& dofs, const Mesh& mesh,
                              const BoundProblem& bound);
+// The same from each tet's local DOFs as a caller hands them over (the reference's
+// DofMap::local_dofs for tet 0 .. num_tets - 1).
+using TetDofsOf = std::function<TetDofs(int)>;
+ScatterMap build_scatter_map(const SparsityPattern& pattern, int num_tets, const TetDofsOf& local);
 
 // ------------------------------------------------------------------ assembly plan
 //
@@ -242,6 +249,10 @@ struct AssemblyPlan {
 };
 AssemblyPlan build_assembly_plan(const SparsityPattern& pattern, const ScatterMap& map,
                                  const DofMap& dofs, const Mesh& mesh, const BoundProblem& bound);
+// The same with the tets' local DOFs from `local` (bodies, ports and the DofMap's
+// port rows and values as in the overload above).
+AssemblyPlan build_assembly_plan(const SparsityPattern& pattern, const ScatterMap& map, int num_tets,
+                                 const TetDofsOf& local, const DofMap& dofs, const BoundProblem& bound);
 
 // ------------------------------------------------------------------ ordering
 

@@ -231,6 +231,12 @@ BoundProblem bind_cube(const Mesh& mesh, const CubeSpec& spec) {
         return p;
     };
     b.ports.push_back(port_on("P1", 10, spec.v_top));
+    if (spec.top_current) {
+        // a boundary current port: its terminal potential is an unknown and the
+        // current drives its row (src/dof_map.cpp:91-103, src/assembly.cpp:396-403)
+        b.ports.back().type = PortType::BoundaryCurrent;
+        b.ports.back().amplitude = spec.i_top;
+    }
     b.ports.push_back(port_on("P2", 11, spec.v_bottom));
     std::vector<int> shared;
     std::set_intersection(b.ports[0].nodes.begin(), b.ports[0].nodes.end(), b.ports[1].nodes.begin(),

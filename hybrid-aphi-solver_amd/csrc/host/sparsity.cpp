@@ -82,7 +82,10 @@ SparsityPattern build_sparsity(const DofMap& dofs, const BoundProblem& bound, co
 
 ScatterMap build_scatter_map(const SparsityPattern& pattern, const DofMap& dofs, const Mesh& mesh,
                              const BoundProblem& bound) {
-    const int nt = mesh.num_tets();
+    return build_scatter_map(pattern, mesh.num_tets(), [&](int t) { return dofs.local_dofs(t, mesh, bound); });
+}
+
+ScatterMap build_scatter_map(const SparsityPattern& pattern, int nt, const TetDofsOf& local) {
     ScatterMap m;
     m.packed = pattern.upper_only;
     m.live.assign(static_cast<size_t>(nt), 0);
@@ -90,7 +93,7 @@ ScatterMap build_scatter_map(const SparsityPattern& pattern, const DofMap& dofs,
     m.offset.assign(static_cast<size_t>(nt) + 1, 0);
     std::vector<int> lives(static_cast<size_t>(nt) * 16);
     for (int t = 0; t < nt; ++t) {
-        const TetDofs d = dofs.local_dofs(t, mesh, bound);
+        const TetDofs d = local(t);
         int* live = &lives[static_cast<size_t>(t) * 16];
         const int n = live_dofs(d, live);
         m.live[static_cast<size_t>(t)] = n;
@@ -123,15 +126,21 @@ ScatterMap build_scatter_map(const SparsityPattern& pattern, const DofMap& dofs,
 
 AssemblyPlan build_assembly_plan(const SparsityPattern& pattern, const ScatterMap& map,
                                  const DofMap& dofs, const Mesh& mesh, const BoundProblem& bound) {
+    return build_assembly_plan(pattern, map, mesh.num_tets(), [&](int t) { return dofs.local_dofs(t, mesh, bound); },
+                               dofs, bound);
+}
+
+AssemblyPlan build_assembly_plan(const SparsityPattern& pattern, const ScatterMap& map, int num_tets,
+                                 const TetDofsOf& local, const DofMap& dofs, const BoundProblem& bound) {
     if (!pattern.upper_only || !map.packed)
         throw std::invalid_argument("build_assembly_plan: the device path stores the upper triangle only");
-    if (static_cast<int>(map.live.size()) != mesh.num_tets())
+    if (static_cast<int>(map.live.size()) != num_tets)
         throw std::invalid_argument("build_assembly_plan: the ScatterMap was built for a different mesh");
     AssemblyPlan plan;
     plan.num_slots = static_cast<int>(pattern.nnz());
     plan.num_rows = dofs.num_total;
     const size_t ntarget = static_cast<size_t>(plan.num_slots) + static_cast<size_t>(plan.num_rows);
-    plan.tet_body.assign(static_cast<size_t>(mesh.num_tets()), -1);
+    plan.tet_body.assign(static_cast<size_t>(num_tets), -1);
     for (size_t b = 0; b < bound.bodies.size(); ++b)
         for (int t : bound.bodies[b].tets) plan.tet_body[static_cast<size_t>(t)] = static_cast<int>(b);
 
@@ -162,7 +171,7 @@ AssemblyPlan build_assembly_plan(const SparsityPattern& pattern, const ScatterMa
         // Bodies, then each body's tets ascending: src/assembly.cpp:277-280.
         for (const Body& body : bound.bodies) {
             for (int t : body.tets) {
-                const TetDofs d = dofs.local_dofs(t, mesh, bound);
+                const TetDofs d = local(t);
                 const int* pos = &map.pos[static_cast<size_t>(t) * 16];
                 const uint32_t base = static_cast<uint32_t>(t) * 196u;
                 // (A, A): :300-315

@@ -69,6 +69,10 @@ typedef struct aphi_cube_spec {
     int ordering;          /* aphi_ordering */
     int relax_small;       /* supernode amalgamation: <= 0 selects the default */
     double relax_fill;     /* <= 0 selects the default */
+    /* 1: the z=side port is current-driven (a boundary-current port of amplitude
+     * i_top amperes) instead of held at v_top */
+    int top_current;
+    double i_top_re, i_top_im;
 } aphi_cube_spec;
 
 typedef struct aphi_problem_counts {
@@ -101,6 +105,52 @@ int aphi_problem_create_cubes(const aphi_cube_spec* specs, int count, int thread
  * solve_symmetric, e.g. the gauge-free potential (src/gauge_free_potential.cpp:177). */
 int aphi_problem_create_csr(int32_t n, const int32_t* row_ptr, const int32_t* col_index, int ordering,
                             int64_t max_factor_nnz, int relax_small, double relax_fill, aphi_problem** out);
+/* A problem from the reference host's own objects, flattened: what a caller of
+ * refill(SymmetricSystem&, const BoundProblem&, const Mesh&, const DofMap&,
+ * const SparsityPattern&, double omega, Conditioning, const ScatterMap*)
+ * (include/aphi_solver/assembly.hpp:197-199) holds, for any mesh it has bound
+ * (a .msh file, a generated cube). From it the library builds the device
+ * assembly plan (every contribution to every slot and rhs entry in the order
+ * src/assembly.cpp:255-422 accumulates them), then analyze(pattern, ordering)
+ * (include/aphi_solver/symbolic.hpp:104) and the supernodal plan; describe() gives
+ * the device mesh. The ScatterMap (ScatterMap::build, assembly.hpp:82-83) is taken
+ * as given, or built from the pattern when its arrays are NULL. Refused (with the
+ * reference's messages where it has them): indices out of range, a tet in no body
+ * or in two, a pattern that is not upper-triangle CSR, a scatter slot the pattern
+ * lacks. Such a problem has no mesh topology beyond its tets: the gauge-free
+ * potential and the result writers refuse it. */
+typedef struct aphi_bound_desc {
+    int32_t num_nodes, num_tets, num_bodies, num_total, num_ports;
+    const double* coords;          /* 3 * num_nodes: Mesh::nodes, metres */
+    const int32_t* tet_nodes;      /* 4 * num_tets: Mesh::tets */
+    const int8_t* tet_edge_signs;  /* 6 * num_tets: Mesh::tet_edge_signs */
+    const double* body_sigma;      /* num_bodies: BoundProblem::bodies, in order */
+    const double* body_eps_r;
+    const double* body_mu_r;
+    const int32_t* body_ptr;       /* num_bodies + 1 */
+    const int32_t* body_tets;      /* body_ptr[num_bodies] = num_tets: each body's tets, ascending */
+    /* DofMap::local_dofs(t, mesh, bound) of every tet: TetDofs::edge, ::phi, ::phi_fixed */
+    const int32_t* edge_index;     /* 6 * num_tets, -1 eliminated */
+    const double* edge_coeff;      /* 6 * num_tets */
+    const int32_t* phi_index;      /* 10 * num_tets, -1 eliminated */
+    const double* phi_coeff;       /* 10 * num_tets */
+    const double* phi_fixed;       /* 20 * num_tets: complex prescribed values */
+    /* ports, in BoundProblem::ports order */
+    const int32_t* port_index;     /* num_ports: DofMap::port_index */
+    const int32_t* port_current;   /* num_ports: 1 current-driven, 0 voltage-driven */
+    const double* port_amplitude;  /* 2 * num_ports: Port::amplitude (current ports) or
+                                      DofMap::port_value (voltage ports), complex */
+    /* SparsityPattern, upper triangle */
+    const int32_t* row_ptr;        /* num_total + 1 */
+    const int32_t* col_index;      /* row_ptr[num_total] */
+    /* ScatterMap (all four, or all NULL) */
+    const int32_t* scatter_live;   /* num_tets: ScatterMap::live_count */
+    const int32_t* scatter_pos;    /* 16 * num_tets: edge_at then phi_at */
+    const int64_t* scatter_offset; /* num_tets + 1: start of each tet's packed block */
+    const int32_t* scatter_slot;   /* scatter_offset[num_tets]: ScatterMap::slots */
+} aphi_bound_desc;
+int aphi_problem_create_bound(const aphi_bound_desc* desc, int ordering, int relax_small, double relax_fill,
+                              aphi_problem** out);
 void aphi_problem_destroy(aphi_problem* p);
 int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out);
 /* Named integer arrays, copied into out[0..capacity); *count receives the full
@@ -115,7 +165,9 @@ int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out);
  * gf_target_ptr, gf_entry, gf_kind. */
 int aphi_problem_get_int(const aphi_problem* p, const char* name, int64_t* out, int64_t capacity, int64_t* count);
 /* Named double arrays: coords (3 per node), tet_phi_fixed (10 complex per tet),
- * body_param (sigma, eps_r, mu_r per body), port_value (complex per port); of a
+ * body_param (sigma, eps_r, mu_r per body), port_value (complex per port: the
+ * prescribed value of a voltage port, 0 for a current port), port_amplitude
+ * (complex per port: Port::amplitude), tet_edge_coeff (6 per tet); of a
  * gauge-free problem also gf_prescribed (complex per vertex), gf_values. */
 int aphi_problem_get_double(const aphi_problem* p, const char* name, double* out, int64_t capacity, int64_t* count);
 /* Result files of a solution x (host, num_total complex, e.g. downloaded from

@@ -66,7 +66,8 @@ def refill(problem, omega, conditioning=1):
     rhs = np.zeros(2 * n)
     port_row = _i32(problem.ints("port_index"))
     nports = len(port_row)
-    port_cur = np.zeros(nports, dtype=np.int32)  # the cube's ports are voltage-driven
+    fixed = problem.ints("port_is_fixed")
+    port_cur = _i32(1 - fixed)  # a current-driven port's row takes row * I, a voltage port's V / column
     args = [
         C.c_int(n), C.c_int64(len(ci)), _p(rp, C.c_int32), _p(ci, C.c_int32),
         _p(_f64(problem.doubles("coords")), C.c_double), _p(_i32(problem.ints("tet_nodes")), C.c_int32),
@@ -82,7 +83,7 @@ def refill(problem, omega, conditioning=1):
     types = [C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_int32,
              C.c_int64, C.c_int32]
     args += [C.c_int(len(body_ptr) - 1)] + [_p(a, t) for a, t in zip(arrays, types)]
-    pv = _f64(problem.doubles("port_value"))
+    pv = _f64(np.where(np.repeat(fixed, 2) == 1, problem.doubles("port_value"), problem.doubles("port_amplitude")))
     keep += [port_row, port_cur, pv]
     args += [C.c_double(omega), C.c_int(conditioning), C.c_int(nports), _p(port_row, C.c_int32),
              _p(port_cur, C.c_int32), _p(pv, C.c_double), _p(vals, C.c_double), _p(rhs, C.c_double)]

@@ -10,6 +10,7 @@
 // usage: ref_driver OUTDIR n side bar_i0 bar_i1 bar_j0 bar_j1 sigma freq_hz mode
 //   mode 0: symbolic only (topology, gauge, DOF map, pattern, scatter map, ordering)
 //   mode 2: + refill at freq_hz and one solve_symmetric, with I, R and L (large meshes)
+//   mode 3: mode 1 with the z = side port driven by a 1 A boundary current
 //   mode 1: + refill at freq_hz, equilibration, factorization, solve, R and L,
 //           the fields' e_tet, the gauge-free potential recovered from them, and
 //           the potential.out / VTK files of the solution (no run information)
@@ -113,6 +114,11 @@ int main(int argc, char** argv) {
     top.surface = {"10"};
     top.amplitude = 1.0;
     Port bottom = top;
+    if (mode == 3) {
+        // the z = side terminal driven by a 1 A boundary current instead of 1 V
+        top.type = PortType::BoundaryCurrent;
+        top.amplitude = 1.0;
+    }
     bottom.name = "P2";
     bottom.surface = {"11"};
     bottom.amplitude = 0.0;
@@ -197,7 +203,7 @@ int main(int argc, char** argv) {
                      rep.equilibration_min, rep.equilibration_max, I.real(), I.imag(), Zin.real(),
                      Zin.imag() / omega);
     }
-    if (mode == 1) {
+    if (mode == 1 || mode == 3) {
         const double omega = 2.0 * 3.14159265358979323846 * freq;
         SymmetricSystem sys = make_symmetric_system(pattern, dofs.num_total);
         refill(sys, bound, mesh, dofs, pattern, omega, Conditioning::RowScaled);

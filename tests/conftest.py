@@ -30,7 +30,9 @@ def conditioning(name):
 
 def cube_config(meta):
     import aphi_amd as A
-    return A.CubeConfig(n=meta["n"], side=meta["side"], bar=tuple(meta["bar"]), sigma=meta["sigma"])
+    # mode 3 fixtures drive the z = side port with a 1 A boundary current
+    return A.CubeConfig(n=meta["n"], side=meta["side"], bar=tuple(meta["bar"]), sigma=meta["sigma"],
+                        i_top=1.0 if meta.get("mode") == 3 else None)
 
 
 @pytest.fixture(scope="session")

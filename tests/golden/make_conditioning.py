@@ -7,7 +7,9 @@ They bound the forward error a backward-stable solution may carry:
 the solution-agreement tests allow. Needs only the committed fixtures and the
 oracle:
 
-    python tests/golden/make_conditioning.py
+    python tests/golden/make_conditioning.py [--all]
+
+Cases already in the file are kept unless --all is given.
 """
 import json
 import os
@@ -21,7 +23,7 @@ sys.path.insert(0, os.path.join(ROOT, "hybrid-aphi-solver_amd"))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
 
-CASES = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw"]
+CASES = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw", "cube4_current"]
 
 
 def main():
@@ -29,8 +31,11 @@ def main():
     from conftest import cube_config, golden
     from oracle import oracle as O
 
-    out = {}
+    path = os.path.join(HERE, "conditioning.json")
+    out = json.load(open(path)) if os.path.exists(path) else {}
     for name in CASES:
+        if name in out and "--all" not in sys.argv:
+            continue  # kept from an earlier run (the dense SVDs take minutes)
         meta, g = golden(name)
         p = A.Problem(cube_config(meta))
         vals, _ = O.refill(p, meta["omega"])
@@ -46,7 +51,7 @@ def main():
         se = np.linalg.svd(d[:, None] * M * d[None, :], compute_uv=False)
         out[name] = {"n": n, "cond2": float(s[0] / s[-1]), "cond2_equilibrated": float(se[0] / se[-1])}
         print(name, out[name], flush=True)
-    with open(os.path.join(HERE, "conditioning.json"), "w") as f:
+    with open(path, "w") as f:
         json.dump(out, f, indent=1)
 
 

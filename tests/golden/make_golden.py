@@ -37,6 +37,8 @@ CASES = [
     # binary expansion (the scatter's 1/(j w) and its products round differently
     # under other complex-arithmetic routines)
     ("cube5_draw", 5, 0.10592777681478216, (2, 4, 1, 5), 27506162.131823607, 540.9701531845078, 1, False),
+    # the z = side port driven by a 1 A boundary current (mode 3)
+    ("cube4_current", 4, 0.08, (1, 3, 1, 2), 3.0e7, 2.5e3, 3, True),
     ("config2", 24, 0.1, (10, 14, 10, 14), 5.8e7, 10.0, 0, False),
     ("config3", 32, 0.1, (6, 12, 20, 23), 5.8e7, 1.0, 0, False),
     # the 24^3 sweep's two end points (10 Hz and 10 MHz are exact in the log sweep)
@@ -59,7 +61,7 @@ def run(case, outdir):
     for a in INT_ARRAYS:
         v = np.fromfile(os.path.join(outdir, a + ".bin"), dtype=np.int64)
         digests[a] = digest(v)
-        if full or (a in FULL_INTS and mode == 1):
+        if full or (a in FULL_INTS and mode in (1, 3)):
             arrays[a] = v.astype(np.int32)
     if mode == 2:
         for a, dt in (("values", np.complex128), ("rhs", np.complex128), ("x", np.complex128)):
@@ -67,7 +69,7 @@ def run(case, outdir):
             digests[a] = digest(v)
             if a == "x":
                 arrays[a] = v
-    if mode == 1:
+    if mode in (1, 3):
         for a, dt in (("values", np.complex128), ("rhs", np.complex128), ("equil_d", np.float64),
                       ("diag", np.complex128), ("x", np.complex128), ("e_tet", np.complex128),
                       ("gf_phi_vertex", np.complex128), ("gf_port_potential", np.complex128)):

@@ -12,7 +12,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-NUMERIC = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw"]
+NUMERIC = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw", "cube4_current"]
 
 
 def dev(a):
@@ -63,6 +63,31 @@ def test_refill_bitwise_at_sweep_points(case, index):
         ov, orhs = O.refill(p, omega, conditioning=cond)
         assert np.array_equal(vals.view(np.uint64), ov.view(np.uint64)), cond
         assert np.array_equal(rhs.view(np.uint64), orhs.view(np.uint64)), cond
+
+
+def test_bound_problem_mesh_equals_cube_mesh(case, gpu_context):
+    """A device mesh made from the cube's bound objects flattened
+    (aphi_problem_create_bound, the drop-in boundary for any bound mesh): the same
+    refill, solution and port current bit for bit as the cube's own mesh."""
+    m, p, meta = case["m"], case["p"], case["meta"]
+    q = A.Problem.from_bound(p.bound_arrays())
+    mq = A.Mesh(gpu_context, q)
+    try:
+        m.refill(meta["omega"])
+        mq.refill(meta["omega"])
+        v1, r1 = m.system()
+        v2, r2 = mq.system()
+        assert np.array_equal(v1.view(np.uint64), v2.view(np.uint64))
+        assert np.array_equal(r1.view(np.uint64), r2.view(np.uint64))
+        b = dev(np.vstack([r1, r1 * (2 - 1j)]))
+        x1, _ = m.solve_symmetric(b, precision=A.APHI_FP64)
+        x2, _ = mq.solve_symmetric(b, precision=A.APHI_FP64)
+        assert np.array_equal(x1.cpu().numpy().view(np.uint64), x2.cpu().numpy().view(np.uint64))
+        I1, R1, L1 = m.terminal_current(x1, meta["omega"], meta["side"])
+        I2, R2, L2 = mq.terminal_current(x2, meta["omega"], meta["side"])
+        assert np.array_equal(I1, I2) and np.array_equal(R1, R2) and np.array_equal(L1, L2)
+    finally:
+        mq.close()
 
 
 def test_refill_refusals(case):

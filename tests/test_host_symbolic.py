@@ -11,7 +11,7 @@ from conftest import cube_config, golden
 
 import aphi_amd as A
 
-CASES = ["cube2", "cube3_offcentre", "config1", "cube8", "config2", "config3"]
+CASES = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw", "cube4_current", "config2", "config3"]
 ARRAYS = ["edges", "tet_edges", "tet_edge_signs", "tree_edge", "dirichlet_edge", "edge_index", "phi_index",
           "row_ptr", "col_index", "perm", "parent", "row_count", "col_count", "scatter_live", "scatter_pos",
           "scatter_offset", "scatter_slot"]

@@ -13,7 +13,7 @@ from conftest import cube_config, golden
 import aphi_amd as A
 from oracle import oracle as O
 
-NUMERIC = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw"]
+NUMERIC = ["cube2", "cube3_offcentre", "config1", "cube8", "cube5_draw", "cube4_current"]
 
 
 def sha(a):
