@@ -15,6 +15,39 @@ namespace aphi_dev {
 
 namespace {
 
+// |a + j b| as the reference's std::abs(std::complex<double>) computes it on its
+// host (glibc's hypot): Borges' method ("An improved algorithm for hypot(a, b)",
+// 2019) without fused multiply-adds -- sqrt(ax^2 + ay^2) followed by one
+// correction step -- with the inputs scaled by 2^-600 / 2^600 away from overflow
+// and underflow, and ax + ay once ay is below ax's last bit. Equilibration row
+// maxima and max |A_ij| then carry the reference's bits (a correctly rounded or
+// differently rounded modulus moves them by an ulp now and then).
+__device__ __forceinline__ double hypot_kernel(double ax, double ay) {
+    double h = sqrt(ax * ax + ay * ay), t1, t2;
+    if (h <= 2.0 * ay) {
+        const double delta = h - ay;
+        t1 = ax * (2.0 * delta - ax);
+        t2 = (delta - 2.0 * (ax - ay)) * delta;
+    } else {
+        const double delta = h - ax;
+        t1 = 2.0 * delta * (ax - 2.0 * ay);
+        t2 = (4.0 * delta - ay) * ay + delta * delta;
+    }
+    h -= (t1 + t2) / (2.0 * h);
+    return h;
+}
+__device__ __forceinline__ double abs_like_host(double a, double b) {
+    if (!isfinite(a) || !isfinite(b)) return (isinf(a) || isinf(b)) ? INFINITY : a + b;
+    a = fabs(a);
+    b = fabs(b);
+    const double ax = a < b ? b : a, ay = a < b ? a : b;
+    constexpr double kScale = 0x1p-600, kLarge = 0x1p+511, kTiny = 0x1p-459, kEps = 0x1p-54;
+    if (ax > kLarge) return ay <= ax * kEps ? ax + ay : hypot_kernel(ax * kScale, ay * kScale) / kScale;
+    if (ay < kTiny) return ax >= ay / kEps ? ax + ay : hypot_kernel(ax / kScale, ay / kScale) * kScale;
+    if (ay <= ax * kEps) return ax + ay;
+    return hypot_kernel(ax, ay);
+}
+
 __constant__ int kEV[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {2, 3}, {1, 3}};
 
 constexpr int kTetsPerBlock = 16;
@@ -331,10 +364,10 @@ __global__ void __launch_bounds__(256) k_eq_rowmax16(int n, const int* __restric
     const int q = static_cast<int>(threadIdx.x) % 16;
     double mx = 0.0;
     if (i < n) {
-        for (int k = row_ptr[i] + q; k < row_ptr[i + 1]; k += 16) mx = fmax(mx, hypot(w[k].re, w[k].im));
+        for (int k = row_ptr[i] + q; k < row_ptr[i + 1]; k += 16) mx = fmax(mx, abs_like_host(w[k].re, w[k].im));
         for (int t = tptr[i] + q; t < tptr[i + 1]; t += 16) {
             const cd v = w[tslot[t]];
-            mx = fmax(mx, hypot(v.re, v.im));
+            mx = fmax(mx, abs_like_host(v.re, v.im));
         }
     }
 #pragma unroll
@@ -614,7 +647,7 @@ __global__ void __launch_bounds__(256) k_seg_absmax(const int64_t* __restrict__ 
     double s = 0.0;
     for (int64_t i = off[sg] + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < off[sg + 1];
          i += static_cast<int64_t>(gridDim.x) * 256)
-        s = fmax(s, hypot(v[i].re, v[i].im));
+        s = fmax(s, abs_like_host(v[i].re, v[i].im));
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
