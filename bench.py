@@ -51,6 +51,70 @@ def update_traffic(n):
         return json.load(f)["traffic_bytes_per_launch"]
 
 
+def cpu_baseline(A, prob, mesh, freq):
+    """The oracle's restatement of the reference's scalar up-looking LDL^T
+    (src/factorization.cpp:109-218), one thread, on this box's host: the whole
+    factorization of the 6^3 configuration and of a 12^3 cube (no extrapolation
+    inside either), the rate of the latter applied to this workload's flop count,
+    and the same factorization run on P threads at once for the throughput of P
+    independent frequency points (capped by host memory for this workload's factor)."""
+    import concurrent.futures as cf
+    import platform
+    from oracle import oracle as O
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+    except OSError:
+        pass
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+
+    def full(n):
+        p = A.Problem(A.CubeConfig(n=n, bar=(n // 3, n // 3 + 2, n // 3, n // 3 + 2)))
+        v, _ = O.refill(p, 2 * np.pi * freq)
+        return p, v
+
+    p6, v6 = full(6)
+    s6 = O.factor_sample(p6, v6, 1e9)
+    p12, v12 = full(12)
+    s12 = O.factor_sample(p12, v12, 1e9)
+    rate = s12["flops_done"] / s12["seconds"]
+    vals, _ = mesh.system()
+    s32 = O.factor_sample(prob, vals, 3.0)  # its equilibration, and a first look at its leaf columns
+    est = s32["flops_total"] / rate + s32["equilibrate_s"]
+    # P independent points at once (the C factorization runs outside the
+    # interpreter lock): aggregate rate over the 8^3 cube
+    P = max(1, min(16, cores))
+    p8, v8 = full(8)
+    single = O.factor_sample(p8, v8, 1e9)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=P) as ex:
+        runs = list(ex.map(lambda _: O.factor_sample(p8, v8, 1e9), range(P)))
+    wall = time.perf_counter() - t0
+    eff = sum(r["flops_done"] for r in runs) / wall / (single["flops_done"] / single["seconds"])
+    factor_bytes = 20.0 * prob.counts["predicted_nnz"] + 40.0 * prob.counts["nnz"]  # values + row indices, matrix
+    try:
+        host = os.sysconf("SC_PAGE_SIZE") * os.sysconf("SC_PHYS_PAGES")
+    except (ValueError, OSError):
+        host = 256e9
+    p_mem = max(1, int(min(host, 270 * 2 ** 30) // factor_bytes))
+    parallel = min(eff, p_mem)
+    return {"value": 1.0 / est, "unit": "frequency points/s", "cores": 1, "kind": "port",
+            "cpu_model": model, "nproc": cores,
+            "config1_point_s": s6["seconds"] + s6["equilibrate_s"],
+            "rate_gflops_12cube": rate / 1e9, "est_point_s": est,
+            "parallel_value": parallel / est, "parallel_processes": P, "parallel_efficiency": eff / P,
+            "parallel_memory_cap": p_mem,
+            "sample": f"oracle (restated reference up-looking LDL^T, 1 thread, {model}, {cores} cores visible): "
+                      f"the 6^3 configuration factored whole in {s6['seconds']:.2f} s; a 12^3 cube ({p12.n} "
+                      f"unknowns, {s12['flops_total']:.3e} flops) whole in {s12['seconds']:.1f} s = "
+                      f"{rate / 1e9:.2f} GFLOP/s; this workload's {s32['flops_total']:.3e} flops at that rate plus "
+                      f"its equilibration ({s32['equilibrate_s']:.1f} s): {est:.0f} s per frequency point (solve "
+                      f"excluded). {P} threads factoring the 8^3 cube at once run at {eff:.1f}x one thread; host "
+                      f"memory holds {p_mem} of this workload's factors at once, so {parallel:.1f} points at a time: "
+                      f"{parallel / est:.2e} points/s"}
+
+
 BATCH_SEED = 20261204
 
 
@@ -124,7 +188,7 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling (0 skips)")
-    ap.add_argument("--serving-requests", type=int, default=32,
+    ap.add_argument("--serving-requests", type=int, default=200,
                     help="requests of the serving run (mixed sizes 8..32, Poisson arrivals; 0 skips)")
     ap.add_argument("--serving-rate", type=float, default=2.0, help="arrival rate of the serving run (requests/s)")
     ap.add_argument("--serving-slots", type=int, default=2)
@@ -135,6 +199,8 @@ def main():
                     help="only run a device copy of known size (HBM counter calibration) and exit")
     ap.add_argument("--factor-only", choices=("fp32", "fp64"), default=None,
                     help="only refill, equilibrate and factor the workload once at one frequency (counter runs) and exit")
+    ap.add_argument("--solve-reps", type=int, default=0,
+                    help="with --factor-only: that many 16-RHS solves after the factorization")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the sweep's frequency points are independent and would shard across GPUs "
@@ -169,8 +235,17 @@ def main():
         mesh.equilibrate(10)
         st = mesh.factorize_ldlt(prec)
         ctx.synchronize()
+        solve_ms = []
+        if a.solve_reps > 0:
+            Bs = A.random_complex_rhs(ctx, RHS_SEED, a.nrhs, prob.n)
+            Xs = torch.empty_like(Bs)
+            for _ in range(a.solve_reps):
+                s0 = time.perf_counter()
+                mesh.solve(Bs, Xs)
+                ctx.synchronize()
+                solve_ms.append((time.perf_counter() - s0) * 1e3)
         print(json.dumps({"factor_only": a.factor_only, "frequency": float(freqs[37]),
-                          "milliseconds": st["milliseconds"], "ok": bool(st["ok"])}))
+                          "milliseconds": st["milliseconds"], "ok": bool(st["ok"]), "solve_ms": solve_ms}))
         return
     n = prob.n
     # right-hand sides of frequency f: the library's counter-based generator
@@ -183,7 +258,10 @@ def main():
     # (tools/solve_mesh.cpp:207); --steps below 64 takes every (64/steps)-th point
     nsteps = max(1, min(a.steps, len(freqs)))
     order = [int(round(i * len(freqs) / nsteps)) for i in range(nsteps)]
-    order64 = [order[int(round(i * nsteps / a.fp64_steps))] for i in range(a.fp64_steps)] if a.fp64_steps else []
+    # FP64 points spread over the whole band, its top included (their cost does
+    # not depend on the frequency)
+    k64 = min(a.fp64_steps, nsteps)
+    order64 = ([order[int(round(i * (nsteps - 1) / max(k64 - 1, 1)))] for i in range(k64)] if k64 else [])
 
     def step(fi, precision):
         mesh.refill(2 * np.pi * freqs[fi])
@@ -239,7 +317,11 @@ def main():
     # error of the FP32+IR solution against the complex128 path, per frequency
     # (untimed; both solved again on the same points)
     err_vs_fp64 = []
-    for fi in order64:
+    # the FP64 points and the FP32 point that needed the most refinement solves
+    fp32_reps = [(r["refinement_steps"], fi) for r, fi in zip(reps, order) if r["precision_used"] == A.APHI_FP32]
+    worst32 = max(fp32_reps)[1] if fp32_reps else None
+    err_points = order64 + ([worst32] if worst32 is not None and worst32 not in order64 else [])
+    for fi in err_points:
         mesh.refill(2 * np.pi * freqs[fi])
         x64, _ = mesh.solve_symmetric(B[fi], precision=A.APHI_FP64, equilibration_iterations=10)
         mesh.refill(2 * np.pi * freqs[fi])
@@ -278,17 +360,7 @@ def main():
 
     cpu = None
     if a.cpu_budget > 0:
-        from oracle import oracle as O
-        vals, _ = mesh.system()
-        s = O.factor_sample(prob, vals, a.cpu_budget)
-        rate = s["flops_done"] / s["seconds"]
-        est = s["flops_total"] / rate + s["equilibrate_s"]
-        cpu = {"value": 1.0 / est, "unit": "frequency points/s", "cores": 1, "kind": "port",
-               "sample": f"the oracle's restatement of the reference's scalar up-looking LDL^T on this workload's "
-                         f"equilibrated matrix at {freqs[order[0]]:.3g} Hz, run {s['seconds']:.1f} s "
-                         f"({s['rows']} of {n} rows, {100 * s['flops_done'] / s['flops_total']:.2f}% of its "
-                         f"{s['flops_total']:.3e} flops at {rate / 1e9:.2f} GFLOP/s); whole-factorization time "
-                         f"extrapolated at that rate: {est:.0f} s per frequency point (solve excluded)"}
+        cpu = cpu_baseline(A, prob, mesh, freqs[order[0]])
         log("cpu baseline:", cpu["sample"])
 
     batched = None
@@ -349,6 +421,7 @@ def main():
         "fp64_p99_ms": float(np.percentile(lat64, 99) * 1e3) if lat64 else None,
         "fp64_backward_error_max": max(r["backward_error"] for r in reps64) if reps64 else None,
         "error_vs_complex128": err_vs_fp64,
+        "error_vs_complex128_frequencies": [float(freqs[fi]) for fi in err_points],
         "speedup_vs_fp64": (value / value64) if value64 else None,
         "factor_ms_fp32": st32["milliseconds"],
         "factor_ms_fp64": st64["milliseconds"],
