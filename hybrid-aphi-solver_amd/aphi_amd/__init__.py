@@ -62,6 +62,22 @@ def _destroy(fn_name, h):
         pass
 
 
+def _own(obj, fn_name, h):
+    """Gives ``obj`` the handle ``h`` and a finalizer that destroys it exactly once:
+    on ``close()``, when ``obj`` is collected, or at interpreter exit (pending
+    finalizers then run in reverse order of creation, so a mesh goes before the
+    context it was made on)."""
+    obj._h = h
+    obj._fin = weakref.finalize(obj, _destroy, fn_name, C.c_void_p(h.value))
+
+
+def _release(obj):
+    fin = getattr(obj, "_fin", None)
+    if fin is not None:
+        fin()
+    obj._h = None
+
+
 def _check(status):
     if status == _capi.APHI_OK:
         return
@@ -184,7 +200,7 @@ class Problem:
         h = C.c_void_p()
         spec = config.spec()
         _check(lib().aphi_problem_create_cube(C.byref(spec), C.byref(h)))
-        self._h = h
+        _own(self, "aphi_problem_destroy", h)
         self._read_counts()
 
     @classmethod
@@ -199,7 +215,7 @@ class Problem:
         for c, h in zip(configs, hs):
             p = cls.__new__(cls)
             p.config = c
-            p._h = C.c_void_p(h)
+            _own(p, "aphi_problem_destroy", C.c_void_p(h))
             p._read_counts()
             out.append(p)
         return out
@@ -223,7 +239,7 @@ class Problem:
         _check(lib().aphi_problem_create_csr(int(n), rp.ctypes.data_as(_capi.I32P), ci.ctypes.data_as(_capi.I32P),
                                              int(ordering), int(max_factor_nnz), int(relax_small), float(relax_fill),
                                              C.byref(h)))
-        self._h = h
+        _own(self, "aphi_problem_destroy", h)
         self._read_counts()
         return self
 
@@ -237,7 +253,7 @@ class Problem:
         self.cube = cube
         h = C.c_void_p()
         _check(lib().aphi_problem_create_gauge_free(cube._h, int(ordering), C.byref(h)))
-        self._h = h
+        _own(self, "aphi_problem_destroy", h)
         self._read_counts()
         return self
 
@@ -269,7 +285,7 @@ class Problem:
         h = C.c_void_p()
         _check(lib().aphi_problem_create_bound(C.byref(d), int(ordering), int(relax_small), float(relax_fill),
                                                C.byref(h)))
-        self._h = h
+        _own(self, "aphi_problem_destroy", h)
         self._read_counts()
         return self
 
@@ -301,9 +317,8 @@ class Problem:
         _check(lib().aphi_problem_counts_get(self._h, C.byref(c)))
         self.counts = {k: getattr(c, k) for k, _ in _capi.ProblemCounts._fields_}
 
-    def __del__(self):
-        _destroy("aphi_problem_destroy", getattr(self, "_h", None))
-        self._h = None
+    def close(self):
+        _release(self)
 
     @property
     def n(self):
@@ -350,7 +365,7 @@ class Context:
     def __init__(self, device=0):
         h = C.c_void_p()
         _check(lib().aphi_context_create(device, C.byref(h)))
-        self._h = h
+        _own(self, "aphi_context_destroy", h)
         self.device = device
         self._meshes = weakref.WeakSet()
 
@@ -361,11 +376,7 @@ class Context:
         """Destroys the context; meshes created on it are destroyed first."""
         for m in list(getattr(self, "_meshes", ())):
             m.close()
-        _destroy("aphi_context_destroy", getattr(self, "_h", None))
-        self._h = None
-
-    def __del__(self):
-        self.close()
+        _release(self)
 
 
 def _dev_ptr(t):
@@ -387,17 +398,13 @@ class Mesh:
         if not ctx._h:
             raise ValueError("Mesh: the context is closed")
         _check(lib().aphi_mesh_create(ctx._h, C.byref(desc), C.byref(h)))
-        self._h = h
+        _own(self, "aphi_mesh_destroy", h)
         ctx._meshes.add(self)
         self.n = problem.n
         self.nnz = int(problem.counts["nnz"])
 
     def close(self):
-        _destroy("aphi_mesh_destroy", getattr(self, "_h", None))
-        self._h = None
-
-    def __del__(self):
-        self.close()
+        _release(self)
 
     def set_solve_graphs(self, on=True):
         """Graph capture and replay of the solve's level loops (off for meshes used
@@ -433,8 +440,11 @@ class Mesh:
         tab = np.ascontiguousarray(np.asarray(table, dtype=np.int8).reshape(64))
         _check(lib().aphi_mesh_set_update_tile_table(self._h, tab.ctypes.data_as(C.POINTER(C.c_int8))))
 
-    def set_profiling(self, on=True):
-        _check(lib().aphi_mesh_set_profiling(self._h, int(on)))
+    def set_profiling(self, on=True, serial=False):
+        """Per-launch timing of the trailing update (``factorize_ldlt`` then reports
+        ``update_ms``). ``serial``: every launch on one stream, timed alone, instead
+        of the look-ahead schedule whose overlapping launches share the device."""
+        _check(lib().aphi_mesh_set_profiling(self._h, (2 if serial else 1) if on else 0))
 
     def device_bytes(self):
         return int(lib().aphi_mesh_device_bytes(self._h))
@@ -584,7 +594,7 @@ class Batch(Mesh):
         if not ctx._h:
             raise ValueError("Batch: the context is closed")
         _check(lib().aphi_batch_create(ctx._h, descs, len(self.problems), C.byref(h)))
-        self._h = h
+        _own(self, "aphi_mesh_destroy", h)
         ctx._meshes.add(self)
         self.count = len(self.problems)
         self.rows = np.zeros(self.count + 1, dtype=np.int64)

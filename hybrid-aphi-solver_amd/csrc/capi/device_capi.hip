@@ -195,6 +195,7 @@ struct aphi_mesh {
     double eq_min = 1.0, eq_max = 1.0;
     aphi_factor_stats last_stats{};
     bool profiling = false;
+    bool serial_updates = false;  // profiling with the look-ahead's updates on the main stream
     std::vector<hipEvent_t> prof_events;
 
     aphi_dev::FrontPlan plan() const {
@@ -425,7 +426,12 @@ int64_t factor_sequence(aphi_mesh* m, double pivot_floor, double& update_flops, 
             // the previous block's rest update must be done before this block's
             // next-block part (they share the columns [bo + kOuter, bo + 2 kOuter))
             if (side_busy) HIPCHK(hipStreamWaitEvent(s, m->ctx->side_done, 0));
-            if (ob + 1 < nouter) {
+            if (ob + 1 < nouter && log && m->serial_updates) {
+                // timed alone: the same two launches, one after the other
+                update(s, bo, aphi_dev::kOuter, bo + 2 * aphi_dev::kOuter, -1);
+                update(s, bo, aphi_dev::kOuter, 0, bo + 2 * aphi_dev::kOuter);
+                side_busy = false;
+            } else if (ob + 1 < nouter) {
                 // the rest only needs this block's panel (final here): it starts now,
                 // beside the next-block part, whose columns it does not touch
                 HIPCHK(hipEventRecord(m->ctx->main_done, s));
@@ -503,15 +509,30 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out, std::
     std::vector<aphi_dev::StatSlot> slots(aphi_dev::kStatSlots);
     HIPCHK(hipMemcpyAsync(slots.data(), m->stats.p, sizeof(aphi_dev::FactorStatsDev), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    aphi_dev::StatSlot h = slots[0];
-    for (const aphi_dev::StatSlot& q : slots) {
-        h.smallest_pivot = std::min(h.smallest_pivot, q.smallest_pivot);
-        h.largest_pivot = std::max(h.largest_pivot, q.largest_pivot);
-        h.largest_multiplier = std::max(h.largest_multiplier, q.largest_multiplier);
-        h.fail_col = std::min(h.fail_col, q.fail_col);
-    }
+    auto reduce = [&slots]() {
+        aphi_dev::StatSlot h = slots[0];
+        for (const aphi_dev::StatSlot& q : slots) {
+            h.smallest_pivot = std::min(h.smallest_pivot, q.smallest_pivot);
+            h.largest_pivot = std::max(h.largest_pivot, q.largest_pivot);
+            h.largest_multiplier = std::max(h.largest_multiplier, q.largest_multiplier);
+            h.fail_col = std::min(h.fail_col, q.fail_col);
+        }
+        return h;
+    };
+    aphi_dev::StatSlot h = reduce();
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, m->ctx->ev0, m->ctx->ev1));
+    if (h.fail_col != INT_MAX) {
+        // the extremes over what the reference's loop reaches before it stops
+        const int fail = h.fail_col;
+        aphi_dev::launch_failure_stats<T>(s, m->num_super, m->factor_entries, m->plan(), m->P.p, fail,
+                                          m->stats.as<aphi_dev::FactorStatsDev>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(slots.data(), m->stats.p, sizeof(aphi_dev::FactorStatsDev), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        h = reduce();
+        h.fail_col = fail;
+    }
     aphi_factor_stats r{};
     r.ok = h.fail_col == INT_MAX ? 1 : 0;
     r.columns_done = r.ok ? m->n : h.fail_col + 1;
@@ -1316,8 +1337,9 @@ extern "C" int aphi_mesh_tune_update_tiles(aphi_mesh* m, int repeats, int8_t* ta
         }
         m->factor_precision = APHI_FP32;
         const int saved_mode = m->update_tiles;
-        const bool saved_prof = m->profiling;
+        const bool saved_prof = m->profiling, saved_serial = m->serial_updates;
         m->profiling = false;
+        m->serial_updates = false;  // tuned on the schedule the factorization runs
         constexpr int H = aphi_mesh::kHeightClasses, W = aphi_mesh::kWidthClasses;
         // 1. every launch timed with the small tiles, then with the large ones:
         //    the faster tiling per launch class (its summed launch time)
@@ -1361,6 +1383,7 @@ extern "C" int aphi_mesh_tune_update_tiles(aphi_mesh* m, int repeats, int8_t* ta
             ++m->tiles_version;
         }
         m->profiling = saved_prof;
+        m->serial_updates = saved_serial;
         if (table_out) std::memcpy(table_out, m->tile_table, sizeof(m->tile_table));
         if (ms_default) *ms_default = t_default;
         if (ms_tuned) *ms_tuned = t_tuned;
@@ -1370,7 +1393,9 @@ extern "C" int aphi_mesh_tune_update_tiles(aphi_mesh* m, int repeats, int8_t* ta
 extern "C" int aphi_mesh_set_profiling(aphi_mesh* m, int on) {
     return aphi_capi::guarded([&] {
         require(m != nullptr, "aphi_mesh_set_profiling: null mesh");
+        require(on >= 0 && on <= 2, "aphi_mesh_set_profiling: on must be 0, 1 or 2");
         m->profiling = on != 0;
+        m->serial_updates = on == 2;
     });
 }
 

@@ -133,6 +133,50 @@ __global__ void k_scatter_panel(int64_t nnz, const int64_t* __restrict__ s2p, co
     if (k < nnz) P[s2p[k]] = to_c<T>(a[k]);
 }
 
+// Statistics of a factorization stopped by a pivot at or below the floor. The
+// reference's up-looking loop (src/factorization.cpp:124-210) stops after row
+// `fail` of its own order: its extremes cover the pivots of reference columns
+// 0..fail and the multipliers l_kj of rows k <= fail. Every one of those entries
+// depends only on columns before `fail` in that order (descendants come first),
+// so the device's values of them are the finished ones even though it factored
+// on past the failure. A flat grid over the panels (P[panel_off[s] + c*m + r]:
+// r == c the pivot, r > c a multiplier of row rows[row_ptr[s] + r]) gathers
+// the extremes over exactly those entries into freshly initialised slots.
+template <typename T>
+__global__ void k_failure_stats(int ns, FrontPlan fp, const C<T>* __restrict__ P, int fail,
+                                FactorStatsDev* st) {
+    PivotAcc acc;
+    double mult = 0.0;
+    const int64_t total = fp.panel_off[ns];
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < total; e += stride) {
+        int lo = 0, hi = ns - 1;  // the supernode s with panel_off[s] <= e < panel_off[s + 1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (fp.panel_off[mid] <= e) lo = mid;
+            else hi = mid - 1;
+        }
+        const int k = fp.first[lo + 1] - fp.first[lo];
+        const int64_t m = fp.row_ptr[lo + 1] - fp.row_ptr[lo];
+        const int64_t off = e - fp.panel_off[lo];
+        if (off >= m * k) continue;
+        const int c = static_cast<int>(off / m);
+        const int64_t r = off - static_cast<int64_t>(c) * m;
+        if (r < c) continue;  // above the diagonal: unused
+        const int row = fp.rows[fp.row_ptr[lo] + r];
+        if (fp.col_ref[row] > fail) continue;
+        const double mag = cmag<T>(P[e]);
+        if (r == c) {
+            acc.mn = fmin(acc.mn, mag);
+            acc.mx = fmax(acc.mx, mag);
+        } else {
+            mult = fmax(mult, mag);
+        }
+    }
+    flush_pivots(st, acc);
+    block_max_multiplier(st, mult);
+}
+
 // ------------------------------------------------------------------ small fronts
 
 // One workgroup per front: assemble (panel + children's update blocks, children
@@ -858,6 +902,16 @@ void launch_stats_init(hipStream_t st, FactorStatsDev* stats) {
 }
 
 template <typename T>
+void launch_failure_stats(hipStream_t st, int ns, int64_t entries, const FrontPlan& fp, const void* P, int fail,
+                          FactorStatsDev* stats) {
+    hipLaunchKernelGGL(k_stats_init, dim3(1), dim3(kStatSlots), 0, st, stats);
+    if (ns <= 0 || entries <= 0) return;
+    const int64_t blocks = std::min<int64_t>((entries + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_failure_stats<T>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, ns, fp,
+                       static_cast<const C<T>*>(P), fail, stats);
+}
+
+template <typename T>
 size_t small_lds_bytes(int max_m) {
     return static_cast<size_t>(max_m) * (max_m + 1) * sizeof(C<T>);
 }
@@ -937,6 +991,10 @@ int update_tile(bool small) {
     return UpdTile<T>::TS;
 }
 
+template void launch_failure_stats<double>(hipStream_t, int, int64_t, const FrontPlan&, const void*, int,
+                                           FactorStatsDev*);
+template void launch_failure_stats<float>(hipStream_t, int, int64_t, const FrontPlan&, const void*, int,
+                                          FactorStatsDev*);
 template void launch_zero<double>(hipStream_t, int64_t, void*);
 template void launch_zero<float>(hipStream_t, int64_t, void*);
 template void launch_scatter_panel<double>(hipStream_t, int64_t, const int64_t*, const cd*, void*);

@@ -25,6 +25,12 @@ constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a 
 constexpr int kBwdRows = 256;     // rows per backward partial sum
 
 void launch_stats_init(hipStream_t st, FactorStatsDev* stats);  // every slot to its identity
+// Re-initialises the slots and gathers the pivot and multiplier extremes of the
+// entries the reference's loop reaches before it stops at reference column
+// `fail` (pivots of columns <= fail, multipliers of rows <= fail).
+template <typename T>
+void launch_failure_stats(hipStream_t st, int ns, int64_t entries, const FrontPlan& fp, const void* P, int fail,
+                          FactorStatsDev* stats);
 template <typename T> void launch_zero(hipStream_t st, int64_t n, void* p);
 template <typename T> void launch_scatter_panel(hipStream_t st, int64_t nnz, const int64_t* s2p, const cd* a, void* P);
 template <typename T> size_t small_lds_bytes(int max_m);

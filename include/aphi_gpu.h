@@ -288,7 +288,11 @@ int aphi_context_synchronize(aphi_context* ctx);
 int aphi_mesh_create(aphi_context* ctx, const aphi_mesh_desc* desc, aphi_mesh** out);
 void aphi_mesh_destroy(aphi_mesh* mesh);
 /* Per-launch event timing of the factorization's trailing-update kernel (adds
- * synchronisation-free event records around each launch; off by default). */
+ * synchronisation-free event records around each launch; off by default).
+ * on = 1: the usual schedule, whose look-ahead runs part of the updates on a second
+ * stream beside the next block's (overlapping launches share the compute units, so
+ * each is timed slower than it runs alone); on = 2: every launch on the library
+ * stream, one after the other (the kernel's rate alone; the factor is the same). */
 int aphi_mesh_set_profiling(aphi_mesh* mesh, int on);
 /* The solve's per-level launches, and the factorization's, are captured into
  * HIP graphs and replayed (on by default; APHI_SOLVE_GRAPHS=0 and

@@ -51,3 +51,24 @@ def test_no_device_fails_loudly():
         pytest.skip("a device is present")
     with pytest.raises(A.DeviceError):
         A.Context(0)
+
+
+def test_handles_are_released_once():
+    """close() destroys the host handle once; later calls see a closed handle
+    (invalid argument), a second close() is a no-op, and a collected object's
+    finalizer has run."""
+    import gc
+    import weakref
+    p = A.Problem(A.CubeConfig(n=2, bar=(0, 1, 0, 1)))
+    fin = p._fin
+    assert fin.alive and p.ints("row_ptr").size == p.n + 1
+    p.close()
+    assert not fin.alive and p._h is None
+    p.close()
+    with pytest.raises(ValueError):
+        p.ints("row_ptr")
+    q = A.Problem(A.CubeConfig(n=2, bar=(0, 1, 0, 1)))
+    fin, ref = q._fin, weakref.ref(q)
+    del q
+    gc.collect()
+    assert ref() is None and not fin.alive

@@ -187,6 +187,29 @@ def test_pivot_floor_refusal(case):
         m.solve(dev(case["rhs"]))  # no usable factor
 
 
+def test_pivot_failure_statistics(case):
+    """A floor between the pivots: the factorization stops at the first reference
+    column whose pivot is at or below it (src/factorization.cpp:191-210), and the
+    report's extremes cover only what the reference's loop reached by then -- the
+    oracle's figures for the same floor, not those of the whole factor."""
+    m, p = case["m"], case["p"]
+    mags = np.sort(np.abs(case["D"]))
+    i = len(mags) // 2
+    while i < len(mags) - 2 and mags[i + 1] < mags[i] * (1 + 1e-6):
+        i += 1  # a clear gap, so both sides classify every pivot alike
+    floor = 0.5 * (mags[i] + mags[i + 1])
+    _, _, o = O.solve_symmetric(p, case["vals"], case["rhs"], pivot_floor=floor)
+    assert not o["ok"] and o["columns_done"] < p.n
+    m.refill(case["meta"]["omega"])
+    m.equilibrate(10)
+    st = m.factorize_ldlt(A.APHI_FP64, pivot_floor=floor)
+    assert not st["ok"] and st["columns_done"] == o["columns_done"]
+    for k in ("smallest_pivot", "largest_pivot", "largest_multiplier"):
+        assert np.isfinite(st[k]) and abs(st[k] / o[k] - 1) < 1e-8, (k, st[k], o[k])
+    # the same mesh factors in full again afterwards
+    assert m.factorize_ldlt(A.APHI_FP64)["ok"]
+
+
 def test_deterministic_bits(case):
     m = case["m"]
     m.refill(case["meta"]["omega"])
