@@ -358,6 +358,27 @@ class Problem:
         _check(lib().aphi_problem_describe(self._h, C.byref(d)))
         return d
 
+    def describe_fields(self):
+        """compute_fields' node plan (aphi_problem_describe_fields)."""
+        d = _capi.FieldsDesc()
+        _check(lib().aphi_problem_describe_fields(self._h, C.byref(d)))
+        return d
+
+    def write_vtk_fields(self, path, x, omega, fields, conditioning=APHI_ROW_SCALED):
+        """write_vtk with compute_fields' data (FieldSet::All, no run information) of
+        the solution x and the FieldOutput ``fields`` (Mesh.compute_fields);
+        returns the byte count."""
+        if hasattr(x, "cpu"):
+            x = x.reshape(-1).cpu().numpy()
+        xx = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        if xx.size != self.n:
+            raise ValueError("write_vtk_fields: x must hold n values")
+        f, keep = _fields_struct(fields)
+        nb = C.c_int64()
+        _check(lib().aphi_write_vtk_fields(self._h, str(path).encode(), xx.view(np.float64).ctypes.data_as(_capi.DP),
+                                           float(omega), int(conditioning), C.byref(f), C.byref(nb)))
+        return nb.value
+
 
 class Context:
     """One HIP device and one stream (default priority)."""
@@ -377,6 +398,29 @@ class Context:
         for m in list(getattr(self, "_meshes", ())):
             m.close()
         _release(self)
+
+
+_FIELD_ARRAYS = ("a_tet", "b_tet", "e_tet", "j_tet", "a_node", "b_node", "h_node", "e_node", "j_node")
+
+
+def _fields_struct(fields):
+    """An aphi_fields over the arrays of a field dict (complex (count, 3) arrays,
+    sigma_tet, vertex_weight, on_material_interface); returns it and the arrays it
+    points into."""
+    keep = {}
+    f = _capi.Fields()
+    for k in _FIELD_ARRAYS:
+        a = np.ascontiguousarray(fields[k], dtype=np.complex128)
+        keep[k] = a
+        setattr(f, k, a.view(np.float64).ctypes.data_as(_capi.DP))
+    for k in ("sigma_tet", "vertex_weight"):
+        a = np.ascontiguousarray(fields[k], dtype=np.float64)
+        keep[k] = a
+        setattr(f, k, a.ctypes.data_as(_capi.DP))
+    a = np.ascontiguousarray(fields["on_material_interface"], dtype=np.int8)
+    keep["on_material_interface"] = a
+    f.on_material_interface = a.ctypes.data_as(C.POINTER(C.c_int8))
+    return f, keep
 
 
 def _dev_ptr(t):
@@ -568,6 +612,31 @@ class Mesh:
                                            cur.ctypes.data_as(_capi.DP), res.ctypes.data_as(_capi.DP),
                                            ind.ctypes.data_as(_capi.DP)))
         return cur.view(np.complex128), res, ind
+
+    # compute_fields -- postprocess.hpp (FieldOutput), src/postprocess.cpp:223-481
+    def compute_fields(self, x, omega, conditioning=APHI_ROW_SCALED):
+        """compute_fields of the solution x (device complex128, n) on the device:
+        a dict of the FieldOutput arrays (per tet a_tet, b_tet, e_tet, j_tet of
+        shape (num_tets, 3) and sigma_tet; per P2 node a_node ... j_node of shape
+        (num_p2, 3), vertex_weight, on_material_interface) and the interface and
+        orphan counts."""
+        if not getattr(self, "_fields_attached", False):
+            d = self.problem.describe_fields()
+            _check(lib().aphi_fields_attach(self._h, C.byref(d)))
+            self._fields_np = int(d.num_p2)
+            self._fields_attached = True
+        nt, npn = int(self.problem.counts["tets"]), self._fields_np
+        out = {k: np.zeros((nt if k.endswith("_tet") else npn, 3), dtype=np.complex128) for k in _FIELD_ARRAYS}
+        out["sigma_tet"] = np.zeros(nt)
+        out["vertex_weight"] = np.zeros(npn)
+        out["on_material_interface"] = np.zeros(npn, dtype=np.int8)
+        f, keep = _fields_struct(out)
+        _check(lib().aphi_compute_fields(self._h, _dev_ptr(x.reshape(-1).contiguous()), float(omega),
+                                         int(conditioning), C.byref(f)))
+        out.update(keep)
+        out["num_interface_nodes"] = int(f.num_interface_nodes)
+        out["num_orphan_vertices"] = int(f.num_orphan_vertices)
+        return out
 
     # backward_error / relative_residual -- factorization.hpp:140, :130
     def backward_error(self, x, b):

@@ -78,6 +78,18 @@ class GaugeFreeDesc(C.Structure):
     ]
 
 
+class FieldsDesc(C.Structure):
+    _fields_ = [("num_tets", C.c_int32), ("num_p2", C.c_int32), ("tet_p2", I32P), ("node_ptr", I64P),
+                ("node_entry", I32P)]
+
+
+class Fields(C.Structure):
+    _fields_ = [("a_tet", DP), ("b_tet", DP), ("e_tet", DP), ("j_tet", DP), ("sigma_tet", DP),
+                ("a_node", DP), ("b_node", DP), ("h_node", DP), ("e_node", DP), ("j_node", DP),
+                ("vertex_weight", DP), ("on_material_interface", C.POINTER(C.c_int8)),
+                ("num_interface_nodes", C.c_int32), ("num_orphan_vertices", C.c_int32)]
+
+
 class BoundDesc(C.Structure):
     _fields_ = [
         ("num_nodes", C.c_int32), ("num_tets", C.c_int32), ("num_bodies", C.c_int32), ("num_total", C.c_int32),
@@ -131,6 +143,10 @@ SIGNATURES = {
     "aphi_problem_create_gauge_free": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "aphi_problem_describe_gauge_free": (C.c_int, [C.c_void_p, C.POINTER(GaugeFreeDesc)]),
     "aphi_tet_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p]),
+    "aphi_problem_describe_fields": (C.c_int, [C.c_void_p, C.POINTER(FieldsDesc)]),
+    "aphi_write_vtk_fields": (C.c_int, [C.c_void_p, C.c_char_p, DP, C.c_double, C.c_int, C.POINTER(Fields), I64P]),
+    "aphi_fields_attach": (C.c_int, [C.c_void_p, C.POINTER(FieldsDesc)]),
+    "aphi_compute_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.POINTER(Fields)]),
     "aphi_philox4x32_10": (C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "aphi_random_normal_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, DP]),
     "aphi_random_uniform_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, DP]),

@@ -137,6 +137,10 @@ struct aphi_mesh {
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
     DevBuf ea_bound_off, ea_bound, x256_off, inv_pairs;
     DevBuf tdofs, tsign, tcoef, tfixed, bsigma, tcontrib, tcurrent;  // port-current recovery
+    // compute_fields: the node plan (aphi_fields_attach) and its work arrays
+    int fields_np = 0;
+    DevBuf fptr, fentry, bmu, fval, fvol, fmu, fsig, fa_tet, fb_tet, fe_tet, fj_tet;
+    DevBuf fa_node, fb_node, fh_node, fe_node, fj_node, fweight, fflag;
     int64_t x256_entries = 0;
     int num_inv_pairs = 0;
     int64_t xinv_entries = 0;
@@ -1404,7 +1408,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->fptr, &m->fentry, &m->bmu, &m->fval, &m->fvol, &m->fmu, &m->fsig, &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
                            &m->bpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv, &m->gf.coords,
                            &m->gf.tets, &m->gf.tet_body, &m->gf.tet_on, &m->gf.column, &m->gf.prescribed, &m->gf.tptr,
@@ -1753,6 +1757,88 @@ extern "C" int aphi_tet_fields(aphi_mesh* m, const void* xv, double omega, int c
                            static_cast<const cd*>(xv), minus_jw, scale, static_cast<cd*>(e_tet));
         HIPCHK(hipStreamSynchronize(m->st()));
         HIPCHK(hipGetLastError());
+    });
+}
+
+extern "C" int aphi_fields_attach(aphi_mesh* m, const aphi_fields_desc* d) {
+    return aphi_capi::guarded([&] {
+        require(m && d, "aphi_fields_attach: null argument");
+        require_single(m, "aphi_fields_attach");
+        require(m->tdofs.p != nullptr, "aphi_fields_attach: the mesh was created without per-tet unknowns");
+        require(d->num_tets == m->num_tets && d->num_p2 > 0, "aphi_fields_attach: the plan is not this mesh's");
+        require(d->node_ptr && d->node_entry, "aphi_fields_attach: null array");
+        const int np = d->num_p2, nt = d->num_tets;
+        // the device indexes with these: check them here
+        require(d->node_ptr[0] == 0 && d->node_ptr[np] == 10 * static_cast<int64_t>(nt),
+                "aphi_fields_attach: node_ptr must run from 0 to 10 * num_tets");
+        for (int u = 0; u < np; ++u)
+            require(d->node_ptr[u] <= d->node_ptr[u + 1], "aphi_fields_attach: node_ptr must not decrease");
+        for (int64_t e = 0; e < 10 * static_cast<int64_t>(nt); ++e)
+            require(d->node_entry[e] >= 0 && d->node_entry[e] < 10 * nt, "aphi_fields_attach: an entry is out of range");
+        HIPCHK(hipSetDevice(m->ctx->device));
+        upload(m->st(), m->fptr, d->node_ptr, static_cast<size_t>(np) + 1);
+        upload(m->st(), m->fentry, d->node_entry, 10 * static_cast<size_t>(nt));
+        upload(m->st(), m->bmu, m->mu_r.data(), m->mu_r.size());
+        m->fields_np = np;
+    });
+}
+
+extern "C" int aphi_compute_fields(aphi_mesh* m, const void* xv, double omega, int conditioning, aphi_fields* out) {
+    return aphi_capi::guarded([&] {
+        require(m && xv && out, "aphi_compute_fields: null argument");
+        require_single(m, "aphi_compute_fields");
+        require(conditioning >= APHI_NATURAL && conditioning <= APHI_SCALED_PHI, "aphi_compute_fields: unknown conditioning");
+        require(m->fields_np > 0, "aphi_compute_fields: attach the field plan first (aphi_fields_attach)");
+        const double* host[] = {out->a_tet,  out->b_tet,  out->e_tet,  out->j_tet,  out->sigma_tet,    out->a_node,
+                                out->b_node, out->h_node, out->e_node, out->j_node, out->vertex_weight};
+        for (const double* h : host) require(h != nullptr, "aphi_compute_fields: a field array is null");
+        require(out->on_material_interface != nullptr, "aphi_compute_fields: a field array is null");
+        after_caller(m->ctx);
+        HIPCHK(hipSetDevice(m->ctx->device));
+        const int nt = m->num_tets, np = m->fields_np;
+        const size_t ntc = 3 * static_cast<size_t>(nt) * sizeof(cd), npc = 3 * static_cast<size_t>(np) * sizeof(cd);
+        m->fval.reserve(60 * static_cast<size_t>(nt) * sizeof(cd));
+        for (DevBuf* b : {&m->fvol, &m->fmu, &m->fsig}) b->reserve(static_cast<size_t>(nt) * sizeof(double));
+        for (DevBuf* b : {&m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet}) b->reserve(ntc);
+        for (DevBuf* b : {&m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node}) b->reserve(npc);
+        m->fweight.reserve(static_cast<size_t>(np) * sizeof(double));
+        m->fflag.reserve(static_cast<size_t>(np));
+        aphi_dev::TetFieldPlan tp{m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(), m->bsigma.as<double>(),
+                                  m->tdofs.as<int>(), m->tsign.as<int8_t>(), m->tcoef.as<double>(), m->tfixed.as<cd>()};
+        const cd minus_jw{-0.0, -omega};
+        const cd scale = conditioning == APHI_SCALED_PHI ? cd{0.0, omega} : cd{1.0, 0.0};
+        hipStream_t s = m->st();
+        hipLaunchKernelGGL(aphi_dev::k_fields_tet, dim3(blocks_for(nt)), dim3(256), 0, s, nt, tp, m->bmu.as<double>(),
+                           static_cast<const cd*>(xv), minus_jw, scale, m->fa_tet.as<cd>(), m->fb_tet.as<cd>(),
+                           m->fe_tet.as<cd>(), m->fj_tet.as<cd>(), m->fsig.as<double>(), m->fvol.as<double>(),
+                           m->fmu.as<double>(), m->fval.as<cd>());
+        hipLaunchKernelGGL(aphi_dev::k_fields_node, dim3(blocks_for(np)), dim3(256), 0, s, np, m->fptr.as<int64_t>(),
+                           m->fentry.as<int>(), m->fsig.as<double>(), m->fmu.as<double>(), m->fvol.as<double>(),
+                           m->fb_tet.as<cd>(), m->fval.as<cd>(), m->fa_node.as<cd>(), m->fb_node.as<cd>(),
+                           m->fh_node.as<cd>(), m->fe_node.as<cd>(), m->fj_node.as<cd>(), m->fweight.as<double>(),
+                           m->fflag.as<int8_t>());
+        HIPCHK(hipGetLastError());
+        const struct {
+            double* dst;
+            const DevBuf* src;
+            size_t bytes;
+        } copies[] = {{out->a_tet, &m->fa_tet, ntc},   {out->b_tet, &m->fb_tet, ntc},
+                      {out->e_tet, &m->fe_tet, ntc},   {out->j_tet, &m->fj_tet, ntc},
+                      {out->sigma_tet, &m->fsig, static_cast<size_t>(nt) * sizeof(double)},
+                      {out->a_node, &m->fa_node, npc}, {out->b_node, &m->fb_node, npc},
+                      {out->h_node, &m->fh_node, npc}, {out->e_node, &m->fe_node, npc},
+                      {out->j_node, &m->fj_node, npc},
+                      {out->vertex_weight, &m->fweight, static_cast<size_t>(np) * sizeof(double)}};
+        for (const auto& c : copies) HIPCHK(hipMemcpyAsync(c.dst, c.src->p, c.bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(out->on_material_interface, m->fflag.p, static_cast<size_t>(np), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        int32_t interface_nodes = 0, orphans = 0;
+        for (int u = 0; u < np; ++u) {
+            if (out->on_material_interface[u]) ++interface_nodes;
+            if (!(out->vertex_weight[u] > 0.0)) ++orphans;
+        }
+        out->num_interface_nodes = interface_nodes;
+        out->num_orphan_vertices = orphans;
     });
 }
 

@@ -44,11 +44,43 @@ struct aphi_problem {
     // the tets' local DOFs as handed over, no mesh topology beyond the tets
     bool from_bound = false;
     std::vector<TetDofs> given_dofs;
+    // compute_fields' node plan, built on first use: each tet's ten P2 nodes and,
+    // per P2 node, its (tet * 10 + local node) entries in tet order
+    std::vector<int32_t> tet_p2, field_entry;
+    std::vector<int64_t> field_ptr;
 };
 
 namespace {
 TetDofs tet_dofs_of(const aphi_problem* p, int t) {
     return p->from_bound ? p->given_dofs[static_cast<size_t>(t)] : p->dofs.local_dofs(t, p->mesh, p->bound);
+}
+
+// The P2 node of local node k of tet t is tets[t][k] (k < 4) or the midpoint node
+// of local edge k - 4 (src/postprocess.cpp:283-287, 405-414); a node's entries
+// come out in tet order because the tets are visited in order.
+void build_field_plan(aphi_problem* p) {
+    if (!p->field_ptr.empty()) return;
+    if (p->from_csr || p->from_bound || p->gauge_free)
+        throw std::invalid_argument("compute_fields: the problem has no mesh topology");
+    const Mesh& m = p->mesh;
+    const int nt = m.num_tets(), np = p->dofs.num_p2_nodes;
+    p->tet_p2.resize(10 * static_cast<size_t>(nt));
+    std::vector<int64_t> ptr(static_cast<size_t>(np) + 1, 0);
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 10; ++k) {
+            const int u = k < 4 ? p->dofs.vertex_p2(m.tets[static_cast<size_t>(t)][static_cast<size_t>(k)])
+                                : p->dofs.edge_p2(m.tet_edges[static_cast<size_t>(t)][static_cast<size_t>(k - 4)]);
+            p->tet_p2[10 * static_cast<size_t>(t) + static_cast<size_t>(k)] = u;
+            ++ptr[static_cast<size_t>(u) + 1];
+        }
+    for (int u = 0; u < np; ++u) ptr[static_cast<size_t>(u) + 1] += ptr[static_cast<size_t>(u)];
+    p->field_entry.assign(10 * static_cast<size_t>(nt), 0);
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < 10 * nt; ++e) {
+        const int u = p->tet_p2[static_cast<size_t>(e)];
+        p->field_entry[static_cast<size_t>(fill[static_cast<size_t>(u)]++)] = e;
+    }
+    p->field_ptr = std::move(ptr);
 }
 }  // namespace
 
@@ -492,6 +524,41 @@ extern "C" int aphi_write_potential(const aphi_problem* p, const char* path, con
     });
 }
 
+extern "C" int aphi_problem_describe_fields(const aphi_problem* p, aphi_fields_desc* out) {
+    return aphi_capi::guarded([&] {
+        if (!p || !out) throw std::invalid_argument("aphi_problem_describe_fields: null argument");
+        aphi_problem* q = const_cast<aphi_problem*>(p);  // the plan is a cache built on first use
+        build_field_plan(q);
+        out->num_tets = p->mesh.num_tets();
+        out->num_p2 = p->dofs.num_p2_nodes;
+        out->tet_p2 = q->tet_p2.data();
+        out->node_ptr = q->field_ptr.data();
+        out->node_entry = q->field_entry.data();
+    });
+}
+
+extern "C" int aphi_write_vtk_fields(const aphi_problem* p, const char* path, const double* x, double omega,
+                                     int conditioning, const aphi_fields* fields, int64_t* bytes) {
+    return aphi_capi::guarded([&] {
+        if (!p || !path || !x || !fields) throw std::invalid_argument("aphi_write_vtk_fields: null argument");
+        if (p->from_csr || p->from_bound || p->gauge_free)
+            throw std::invalid_argument("aphi_write_vtk_fields: the problem has no mesh topology");
+        if (conditioning < 0 || conditioning > 2) throw std::invalid_argument("aphi_write_vtk_fields: unknown conditioning");
+        const double* arrays[] = {fields->a_tet, fields->b_tet, fields->e_tet, fields->j_tet, fields->sigma_tet,
+                                  fields->a_node, fields->b_node, fields->h_node, fields->e_node, fields->j_node};
+        for (const double* a : arrays)
+            if (!a) throw std::invalid_argument("aphi_write_vtk_fields: a field array is null");
+        if (!fields->on_material_interface) throw std::invalid_argument("aphi_write_vtk_fields: a field array is null");
+        const NodalPotential np = potential_at_nodes(p->mesh, p->dofs, reinterpret_cast<const Complex*>(x),
+                                                     p->dofs.num_total, omega, conditioning);
+        FieldsView f{fields->a_tet,  fields->b_tet,  fields->e_tet,  fields->j_tet,  fields->sigma_tet,
+                     fields->a_node, fields->b_node, fields->h_node, fields->e_node, fields->j_node,
+                     fields->on_material_interface};
+        const size_t n = write_vtk_fields(path, p->mesh, np, omega, conditioning, f);
+        if (bytes) *bytes = static_cast<int64_t>(n);
+    });
+}
+
 extern "C" void aphi_problem_destroy(aphi_problem* p) { delete p; }
 
 extern "C" int aphi_problem_counts_get(const aphi_problem* p, aphi_problem_counts* out) {
@@ -632,6 +699,15 @@ extern "C" int aphi_problem_get_int(const aphi_problem* p, const char* name, int
             return copy_out(p->gf.entry, out, capacity, count);
         } else if (k == "gf_kind") {
             return copy_out(p->gf.kind, out, capacity, count);
+        } else if (k == "tet_p2") {
+            build_field_plan(const_cast<aphi_problem*>(p));
+            return copy_out(p->tet_p2, out, capacity, count);
+        } else if (k == "field_node_ptr") {
+            build_field_plan(const_cast<aphi_problem*>(p));
+            return copy_out(p->field_ptr, out, capacity, count);
+        } else if (k == "field_node_entry") {
+            build_field_plan(const_cast<aphi_problem*>(p));
+            return copy_out(p->field_entry, out, capacity, count);
         } else {
             throw std::invalid_argument("aphi_problem_get_int: unknown array '" + k + "'");
         }

@@ -746,6 +746,188 @@ __global__ void __launch_bounds__(256) k_tet_efield(int num_tets, TetFieldPlan t
     for (int c = 0; c < 3; ++c) e_tet[3 * static_cast<int64_t>(t) + c] = e[c];
 }
 
+// ------------------------------------------------------------------ compute_fields
+//
+// FieldOutput of compute_fields (src/postprocess.cpp:223-481). First one thread
+// per tet: the edge unknowns and resolved Phi coefficients, B = curl A, A and E at
+// the centroid (J = sigma E) and A and E at each of the ten P2 nodes, every term
+// in the reference's operation order (contraction off in this file). Then one
+// thread per P2 node: the volume-weighted sums over its (tet, local node)
+// entries in tet order -- the order the reference's tet loop adds them -- with
+// the side selection of the most conducting / most permeable tets, and the
+// normalisation and constitutive relations at the node.
+namespace {
+constexpr double kMu0 = 4.0e-7 * 3.14159265358979323846;
+
+// compute_fields' evaluate (src/postprocess.cpp:364-386) at barycentric point L
+__device__ __forceinline__ void eval_ae(const Geo& G, const double* L, const cd a[6], const double sg[6],
+                                        const cd ph[10], cd minus_jw, cd av[3], cd ev[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) av[c] = cd{0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double w[3];
+        whitney(G, k, L, w);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) av[c] = cadd(av[c], cscale(a[k], w[c] * sg[k]));
+    }
+    cd gp[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+        double gr[3];
+        p2grad(G, b, L, gr);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gp[c] = cadd(gp[c], cscale(ph[b], gr[c]));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ev[c] = csub(cmul(minus_jw, av[c]), gp[c]);
+}
+}  // namespace
+
+__global__ void __launch_bounds__(256) k_fields_tet(int num_tets, TetFieldPlan tp, const double* __restrict__ body_mu_r,
+                                                    const cd* __restrict__ x, cd minus_jw, cd scale, cd* __restrict__ a_tet,
+                                                    cd* __restrict__ b_tet, cd* __restrict__ e_tet, cd* __restrict__ j_tet,
+                                                    double* __restrict__ sigma_tet, double* __restrict__ tet_vol,
+                                                    double* __restrict__ tet_mu, cd* __restrict__ node_val) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= num_tets) return;
+    Geo G;
+    tet_geometry(tp.coords, tp.tets, t, G);
+    const int* d = tp.dofs + 16 * static_cast<int64_t>(t);
+    cd a[6], ph[10];
+    double sg[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        a[k] = d[k] >= 0 ? x[d[k]] : cd{0.0, 0.0};
+        sg[k] = static_cast<double>(tp.signs[6 * static_cast<int64_t>(t) + k]);
+    }
+#pragma unroll
+    for (int b = 0; b < 10; ++b) {
+        const int ib = d[6 + b];
+        ph[b] = ib >= 0 ? cmul(cscale(scale, tp.phi_coeff[10 * static_cast<int64_t>(t) + b]), x[ib])
+                        : tp.phi_fixed[10 * static_cast<int64_t>(t) + b];
+    }
+    // B = curl A: (grad L_vi x grad L_vj) * 2, then the edge sign (src/basis_functions.cpp:101-116)
+    cd bc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double* gi = G.g[kEV[k][0]];
+        const double* gj = G.g[kEV[k][1]];
+        const double cr[3] = {(gi[1] * gj[2] - gi[2] * gj[1]) * 2.0, (gi[2] * gj[0] - gi[0] * gj[2]) * 2.0,
+                              (gi[0] * gj[1] - gi[1] * gj[0]) * 2.0};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bc[c] = cadd(bc[c], cscale(a[k], cr[c] * sg[k]));
+    }
+    const int body = tp.tet_body[t];
+    const double sigma = tp.body_sigma[body];
+    sigma_tet[t] = sigma;
+    tet_vol[t] = G.vol;
+    tet_mu[t] = kMu0 * body_mu_r[body];
+    const double Lc[4] = {0.25, 0.25, 0.25, 0.25};
+    cd av[3], ev[3];
+    eval_ae(G, Lc, a, sg, ph, minus_jw, av, ev);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        b_tet[3 * static_cast<int64_t>(t) + c] = bc[c];
+        a_tet[3 * static_cast<int64_t>(t) + c] = av[c];
+        e_tet[3 * static_cast<int64_t>(t) + c] = ev[c];
+        j_tet[3 * static_cast<int64_t>(t) + c] = cscale(ev[c], sigma);
+    }
+    for (int k = 0; k < 10; ++k) {
+        double L[4] = {0.0, 0.0, 0.0, 0.0};
+        if (k < 4) {
+            L[k] = 1.0;
+        } else {
+            L[kEV[k - 4][0]] = 0.5;
+            L[kEV[k - 4][1]] = 0.5;
+        }
+        eval_ae(G, L, a, sg, ph, minus_jw, av, ev);
+        cd* o = node_val + 6 * (10 * static_cast<int64_t>(t) + k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            o[c] = av[c];
+            o[3 + c] = ev[c];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_fields_node(int num_p2, const int64_t* __restrict__ node_ptr,
+                                                     const int* __restrict__ node_entry,
+                                                     const double* __restrict__ sigma_tet,
+                                                     const double* __restrict__ tet_mu, const double* __restrict__ tet_vol,
+                                                     const cd* __restrict__ b_tet, const cd* __restrict__ node_val,
+                                                     cd* __restrict__ a_node, cd* __restrict__ b_node,
+                                                     cd* __restrict__ h_node, cd* __restrict__ e_node,
+                                                     cd* __restrict__ j_node, double* __restrict__ vertex_weight,
+                                                     int8_t* __restrict__ on_interface) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= num_p2) return;
+    const int64_t e0 = node_ptr[u], e1 = node_ptr[u + 1];
+    // std::min(a, b) = b < a ? b : a; std::max(a, b) = a < b ? b : a
+    double sig_min = __builtin_inf(), sig_max = -__builtin_inf(), mu_min = __builtin_inf(), mu_max = -__builtin_inf();
+    for (int64_t e = e0; e < e1; ++e) {
+        const int t = node_entry[e] / 10;
+        const double s = sigma_tet[t], m = tet_mu[t];
+        sig_min = s < sig_min ? s : sig_min;
+        sig_max = sig_max < s ? s : sig_max;
+        mu_min = m < mu_min ? m : mu_min;
+        mu_max = mu_max < m ? m : mu_max;
+    }
+    const bool sigma_uniform = sig_min == sig_max, mu_uniform = mu_min == mu_max;
+    double vw = 0.0, ew = 0.0, bw = 0.0;
+    cd a[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}}, ef[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}},
+       bf[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    int8_t interface_flag = 0;
+    for (int64_t e = e0; e < e1; ++e) {
+        const int ent = node_entry[e], t = ent / 10;
+        const double vol = tet_vol[t];
+        const cd* o = node_val + 6 * static_cast<int64_t>(ent);
+        vw = vw + vol;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[c] = cadd(a[c], cscale(o[c], vol));
+        if (sigma_uniform || sigma_tet[t] == sig_max) {
+            ew = ew + vol;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ef[c] = cadd(ef[c], cscale(o[3 + c], vol));
+        }
+        if (mu_uniform || tet_mu[t] == mu_max) {
+            bw = bw + vol;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bf[c] = cadd(bf[c], cscale(b_tet[3 * static_cast<int64_t>(t) + c], vol));
+        }
+        if (!sigma_uniform || !mu_uniform) interface_flag = 1;
+    }
+    vertex_weight[u] = vw;
+    on_interface[u] = interface_flag;
+    cd h[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}}, j[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    if (vw > 0.0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a[c] = cd{a[c].re / vw, a[c].im / vw};
+        if (ew > 0.0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ef[c] = cd{ef[c].re / ew, ef[c].im / ew};
+        }
+        if (bw > 0.0) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bf[c] = cd{bf[c].re / bw, bf[c].im / bw};
+        }
+        const double nu_node = 1.0 / mu_max;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            j[c] = cscale(ef[c], sig_max);
+            h[c] = cscale(bf[c], nu_node);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a_node[3 * static_cast<int64_t>(u) + c] = a[c];
+        e_node[3 * static_cast<int64_t>(u) + c] = ef[c];
+        b_node[3 * static_cast<int64_t>(u) + c] = bf[c];
+        j_node[3 * static_cast<int64_t>(u) + c] = j[c];
+        h_node[3 * static_cast<int64_t>(u) + c] = h[c];
+    }
+}
+
 // ------------------------------------------------------------------ gauge-free potential
 //
 // Per tet of a participating body (src/gauge_free_potential.cpp:112-151), the

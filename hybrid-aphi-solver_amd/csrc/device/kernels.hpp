@@ -122,6 +122,15 @@ __global__ void k_tet_current(int num_tets, TetFieldPlan tp, int nrhs, const cd*
 __global__ void k_current_sum(int num_tets, int nseg, const int64_t* seg_tets, TetFieldPlan tp, const cd* contrib,
                               const double* lengths, double length, cd* current);
 __global__ void k_tet_efield(int num_tets, TetFieldPlan tp, const cd* x, cd minus_jw, cd scale, cd* e_tet);
+// compute_fields: per tet (node_val: A then E at each of the ten P2 nodes, 6
+// complex per (tet, local node)), then per P2 node over its entries in tet order
+__global__ void k_fields_tet(int num_tets, TetFieldPlan tp, const double* body_mu_r, const cd* x, cd minus_jw, cd scale,
+                             cd* a_tet, cd* b_tet, cd* e_tet, cd* j_tet, double* sigma_tet, double* tet_vol,
+                             double* tet_mu, cd* node_val);
+__global__ void k_fields_node(int num_p2, const int64_t* node_ptr, const int* node_entry, const double* sigma_tet,
+                              const double* tet_mu, const double* tet_vol, const cd* b_tet, const cd* node_val,
+                              cd* a_node, cd* b_node, cd* h_node, cd* e_node, cd* j_node, double* vertex_weight,
+                              int8_t* on_interface);
 
 // random.hip: normal variates of the counter-based generator (philox.hpp)
 __global__ void k_random_normal(uint64_t seed, uint64_t offset, int64_t count, double* out);

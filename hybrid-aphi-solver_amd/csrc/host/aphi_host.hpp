@@ -394,6 +394,16 @@ size_t write_potential(const std::string& path, const NodalPotential& p, double 
 // write_vtk (src/postprocess.cpp:1055-1276) without field data or run information
 size_t write_vtk_potential(const std::string& path, const Mesh& mesh, const NodalPotential& p, double omega,
                            int conditioning);
+// compute_fields' FieldOutput as flat arrays (complex interleaved re, im; 3 complex
+// per tet or node), owned by the caller
+struct FieldsView {
+    const double *a_tet, *b_tet, *e_tet, *j_tet, *sigma_tet;
+    const double *a_node, *b_node, *h_node, *e_node, *j_node;
+    const int8_t* on_material_interface;
+};
+// write_vtk with field data (FieldSet::All), no run information
+size_t write_vtk_fields(const std::string& path, const Mesh& mesh, const NodalPotential& p, double omega,
+                        int conditioning, const FieldsView& f);
 
 // log-spaced sweep, src/input_file.cpp:19-50
 std::vector<double> expand_frequency_sweep(double start, double stop, int points, bool logarithmic);

@@ -1,7 +1,8 @@
 // Result files in the reference's formats, from a solution handed over by the
 // device path: the nodal potential (potential_at_nodes + write_potential,
 // src/postprocess.cpp:42-123, 549-606) and the legacy-VTK quadratic-tet mesh with
-// the potential (write_vtk without field data, src/postprocess.cpp:1055-1276).
+// the potential and, when given, compute_fields' data computed on the device
+// (write_vtk, src/postprocess.cpp:1055-1276).
 // Numbers are written as the reference writes them (std::to_chars: the shortest
 // text that reads back to the same double), so the same inputs give the same
 // bytes.
@@ -143,8 +144,52 @@ size_t write_potential(const std::string& path, const NodalPotential& p, double 
     return t.s.size();
 }
 
+namespace {
+// put_vectors / put_magnitude of the reference's writer: the real or imaginary
+// parts of 3 complex per item, and sqrt(sum |c|^2) per item
+void put_vectors(Text& t, const char* name, const double* v, int n, bool real_part) {
+    t.put("\nVECTORS ");
+    t.put(name);
+    t.put(" double\n");
+    for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) {
+            if (k != 0) t.put(' ');
+            t.put(v[6 * static_cast<size_t>(i) + 2 * k + (real_part ? 0 : 1)]);
+        }
+        t.put('\n');
+    }
+}
+
+void put_magnitude(Text& t, const char* name, const double* v, int n) {
+    t.put("\nSCALARS ");
+    t.put(name);
+    t.put(" double 1\nLOOKUP_TABLE default\n");
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += std::norm(Complex(v[6 * static_cast<size_t>(i) + 2 * k],
+                                                           v[6 * static_cast<size_t>(i) + 2 * k + 1]));
+        t.put(std::sqrt(s));
+        t.put('\n');
+    }
+}
+
+size_t write_vtk_impl(const std::string& path, const Mesh& mesh, const NodalPotential& p, double omega,
+                      int conditioning, const FieldsView* f);
+}  // namespace
+
 size_t write_vtk_potential(const std::string& path, const Mesh& mesh, const NodalPotential& p, double omega,
                            int conditioning) {
+    return write_vtk_impl(path, mesh, p, omega, conditioning, nullptr);
+}
+
+size_t write_vtk_fields(const std::string& path, const Mesh& mesh, const NodalPotential& p, double omega,
+                        int conditioning, const FieldsView& f) {
+    return write_vtk_impl(path, mesh, p, omega, conditioning, &f);
+}
+
+namespace {
+size_t write_vtk_impl(const std::string& path, const Mesh& mesh, const NodalPotential& p, double omega,
+                      int conditioning, const FieldsView* f) {
     if (static_cast<int>(p.position.size()) != mesh.num_nodes() + mesh.num_edges())
         throw std::invalid_argument("write_vtk: the potential has " + std::to_string(p.position.size()) +
                                     " nodes for a mesh of " + std::to_string(mesh.num_nodes()) + " vertices and " +
@@ -197,6 +242,19 @@ size_t write_vtk_potential(const std::string& path, const Mesh& mesh, const Noda
         t.put(c < static_cast<int>(mesh.tet_tags.size()) ? mesh.tet_tags[static_cast<size_t>(c)] : -1);
         t.put('\n');
     }
+    if (f) {
+        t.put("\nSCALARS sigma double 1\nLOOKUP_TABLE default\n");
+        for (int c = 0; c < nc; ++c) {
+            t.put(f->sigma_tet[c]);
+            t.put('\n');
+        }
+        put_vectors(t, "B_cell_real", f->b_tet, nc, true);
+        put_vectors(t, "B_cell_imag", f->b_tet, nc, false);
+        put_vectors(t, "E_cell_real", f->e_tet, nc, true);
+        put_vectors(t, "E_cell_imag", f->e_tet, nc, false);
+        put_vectors(t, "J_cell_real", f->j_tet, nc, true);
+        put_vectors(t, "J_cell_imag", f->j_tet, nc, false);
+    }
     t.put("\nPOINT_DATA ");
     t.put(np);
     t.put("\nSCALARS phi_real double 1\nLOOKUP_TABLE default\n");
@@ -219,8 +277,29 @@ size_t write_vtk_potential(const std::string& path, const Mesh& mesh, const Noda
         t.put(p.status[static_cast<size_t>(i)] == 'A' ? 0 : 1);
         t.put('\n');
     }
+    if (f) {
+        const struct {
+            const char *re, *im, *mag;
+            const double* v;
+        } node_fields[] = {{"A_real", "A_imag", "A_magnitude", f->a_node},
+                           {"B_real", "B_imag", "B_magnitude", f->b_node},
+                           {"H_real", "H_imag", "H_magnitude", f->h_node},
+                           {"E_real", "E_imag", "E_magnitude", f->e_node},
+                           {"J_real", "J_imag", "J_magnitude", f->j_node}};
+        for (const auto& nf : node_fields) {
+            put_vectors(t, nf.re, nf.v, np, true);
+            put_vectors(t, nf.im, nf.v, np, false);
+            put_magnitude(t, nf.mag, nf.v, np);
+        }
+        t.put("\nSCALARS material_interface int 1\nLOOKUP_TABLE default\n");
+        for (int i = 0; i < np; ++i) {
+            t.put(static_cast<int>(f->on_material_interface[i]));
+            t.put('\n');
+        }
+    }
     save(path, t.s, "write_vtk");
     return t.s.size();
 }
+}  // namespace
 
 }  // namespace aphi_amd

@@ -180,6 +180,42 @@ int aphi_write_potential(const aphi_problem* p, const char* path, const double* 
 /* The log sweep of src/input_file.cpp:19-50. */
 int aphi_frequency_sweep(double start, double stop, int points, int logarithmic, double* out);
 
+/* ------------------------------------------------------------------ field recovery
+ * compute_fields (src/postprocess.cpp:223-481) and write_vtk with field data
+ * (src/postprocess.cpp:1055-1276). The host plan says where each tet's ten P2
+ * nodes (4 vertices, then the 6 local edges in kTetLocalEdgeVerts order) land and,
+ * per P2 node, which (tet, local node) pairs contribute to its averages, tets
+ * ascending -- the reference's accumulation order. */
+typedef struct aphi_fields_desc {
+    int32_t num_tets, num_p2;      /* P2 nodes: vertices, then edge midpoints */
+    const int32_t* tet_p2;         /* 10 * num_tets */
+    const int64_t* node_ptr;       /* num_p2 + 1 */
+    const int32_t* node_entry;     /* 10 * num_tets: tet * 10 + local node, grouped by node */
+} aphi_fields_desc;
+/* Pointers into the problem's arrays (built on first call); a problem with no
+ * mesh topology (pattern-only, bound or gauge-free) is refused. */
+int aphi_problem_describe_fields(const aphi_problem* p, aphi_fields_desc* out);
+
+/* The FieldOutput of compute_fields in caller-owned host arrays, complex values
+ * interleaved (re, im): per tet A, B, E, J (3 complex each) and sigma; per P2
+ * node the averages A, B, H, E, J (3 complex each), the A weight (the node's
+ * incident volume) and the material-interface flag. */
+typedef struct aphi_fields {
+    double *a_tet, *b_tet, *e_tet, *j_tet;   /* 6 * num_tets each */
+    double* sigma_tet;                       /* num_tets */
+    double *a_node, *b_node, *h_node, *e_node, *j_node;  /* 6 * num_p2 each */
+    double* vertex_weight;                   /* num_p2 */
+    int8_t* on_material_interface;           /* num_p2 */
+    int32_t num_interface_nodes, num_orphan_vertices;  /* filled on return */
+} aphi_fields;
+/* write_vtk(path, mesh, potential_at_nodes(x), solution, &fields, FieldSet::All)
+ * without run information: the potential file of format 1 plus the cell data
+ * (sigma, B, E, J per tet) and the point data (A, B, H, E, J and their phasor
+ * magnitudes, the material-interface flag), in the reference's order and number
+ * formatting. x: host, num_total complex. */
+int aphi_write_vtk_fields(const aphi_problem* p, const char* path, const double* x, double omega, int conditioning,
+                          const aphi_fields* fields, int64_t* bytes);
+
 /* ------------------------------------------------------------------ device boundary
  * What the reference host hands the GPU once per mesh: geometry, the assembly
  * plan (every contribution to every slot / rhs entry, in the reference's
@@ -390,6 +426,18 @@ int aphi_random_normal(aphi_context* ctx, uint64_t seed, uint64_t offset, int64_
  * in its operation order, so bit for bit the reference's for the same x. e_tet:
  * device, 3 complex per tet. */
 int aphi_tet_fields(aphi_mesh* mesh, const void* x, double omega, int conditioning, void* e_tet);
+
+/* compute_fields (src/postprocess.cpp:223-481) on the device for a solution x
+ * (device, n complex): per tet A and E at the centroid, J = sigma E and B = curl A;
+ * per P2 node the volume-weighted averages of A (all incident tets), E (the most
+ * conducting side) and B (the most permeable side), each node's sum in the
+ * reference's tet order and every term in its operation order, so bit for bit
+ * the reference's FieldOutput for the same x; then J = sigma_max E and
+ * H = B / mu_max at the nodes. The mesh takes the field plan once
+ * (aphi_fields_attach, from aphi_problem_describe_fields); the results land in
+ * the caller's host arrays. */
+int aphi_fields_attach(aphi_mesh* mesh, const aphi_fields_desc* desc);
+int aphi_compute_fields(aphi_mesh* mesh, const void* x, double omega, int conditioning, aphi_fields* out);
 
 /* Hands a mesh made from a gauge-free problem its assembly plan (once). */
 int aphi_gauge_free_attach(aphi_mesh* mesh, const aphi_gauge_free_desc* desc);

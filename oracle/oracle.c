@@ -791,6 +791,195 @@ int oracle_tet_fields(int num_tets, const double* coords, const int32_t* tet_nod
     return 1;
 }
 
+/* A and E = -jw A - grad Phi at the barycentric point L of tet t (compute_fields'
+ * evaluate, src/postprocess.cpp:364-386), in its operation order; a and ph are
+ * the tet's raw edge unknowns and resolved Phi coefficients. */
+static void tet_eval(double g[4][3], const double L[4], const double* sg, const double a_re[6],
+                     const double a_im[6], const double ph_re[10], const double ph_im[10], double omega,
+                     double av_re[3], double av_im[3], double e_re[3], double e_im[3]) {
+    const double mjw_re = -0.0, mjw_im = -omega;
+    for (int c = 0; c < 3; ++c) { av_re[c] = 0.0; av_im[c] = 0.0; }
+    for (int k = 0; k < 6; ++k) {
+        double w[3];
+        whitney(g, k, L, w);
+        for (int c = 0; c < 3; ++c) {
+            const double wc = w[c] * sg[k];
+            av_re[c] = av_re[c] + a_re[k] * wc;
+            av_im[c] = av_im[c] + a_im[k] * wc;
+        }
+    }
+    double gp_re[3] = {0.0, 0.0, 0.0}, gp_im[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < 10; ++b) {
+        double gr[3];
+        p2grad(g, b, L, gr);
+        for (int c = 0; c < 3; ++c) {
+            gp_re[c] = gp_re[c] + ph_re[b] * gr[c];
+            gp_im[c] = gp_im[c] + ph_im[b] * gr[c];
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        e_re[c] = (mjw_re * av_re[c] - mjw_im * av_im[c]) - gp_re[c];
+        e_im[c] = (mjw_re * av_im[c] + mjw_im * av_re[c]) - gp_im[c];
+    }
+}
+
+/* compute_fields (src/postprocess.cpp:223-481) without phi_node: per tet A, E, J at
+ * the centroid, B = curl A and sigma; per P2 node the volume-weighted averages of A
+ * (every incident tet), E (the most conducting side) and B (the most permeable
+ * side), then J = sigma_max E and H = B / mu_max, with the interface flags and
+ * counts. tet_p2 gives each tet's ten P2 nodes (vertices, then local edges in
+ * kTetLocalEdgeVerts order). Complex outputs interleaved (re, im); counts[0] =
+ * interface nodes, counts[1] = orphan nodes. Returns 0 for a degenerate tet. */
+int oracle_compute_fields(int num_tets, int num_p2, const double* coords, const int32_t* tet_nodes,
+                          const int32_t* tet_p2, const int32_t* tet_body, const double* body_sigma,
+                          const double* body_mu_r, const int32_t* tet_dofs, const double* tet_signs,
+                          const double* tet_phi_coeff, const double* tet_phi_fixed, const double* x, double omega,
+                          int conditioning, double* a_tet, double* b_tet, double* e_tet, double* j_tet,
+                          double* sigma_tet, double* a_node, double* b_node, double* h_node, double* e_node,
+                          double* j_node, double* vertex_weight, int8_t* on_interface, int32_t* counts) {
+    const double kMu0 = 4.0e-7 * 3.14159265358979323846;
+    const size_t np = (size_t)num_p2;
+    double* sig_min = malloc(np * sizeof(double));
+    double* sig_max = malloc(np * sizeof(double));
+    double* mu_min = malloc(np * sizeof(double));
+    double* mu_max = malloc(np * sizeof(double));
+    double* e_weight = calloc(np, sizeof(double));
+    double* b_weight = calloc(np, sizeof(double));
+    int ok = 1;
+    for (size_t u = 0; u < np; ++u) {
+        sig_min[u] = INFINITY; sig_max[u] = -INFINITY; mu_min[u] = INFINITY; mu_max[u] = -INFINITY;
+        vertex_weight[u] = 0.0; on_interface[u] = 0;
+    }
+    memset(a_node, 0, 6 * np * sizeof(double));
+    memset(b_node, 0, 6 * np * sizeof(double));
+    memset(h_node, 0, 6 * np * sizeof(double));
+    memset(e_node, 0, 6 * np * sizeof(double));
+    memset(j_node, 0, 6 * np * sizeof(double));
+    for (int t = 0; t < num_tets; ++t) {
+        const double sgm = body_sigma[tet_body[t]], m = kMu0 * body_mu_r[tet_body[t]];
+        for (int k = 0; k < 10; ++k) {
+            const int u = tet_p2[10 * (int64_t)t + k];
+            /* std::min(a, b) = b < a ? b : a; std::max(a, b) = a < b ? b : a */
+            sig_min[u] = sgm < sig_min[u] ? sgm : sig_min[u];
+            sig_max[u] = sig_max[u] < sgm ? sgm : sig_max[u];
+            mu_min[u] = m < mu_min[u] ? m : mu_min[u];
+            mu_max[u] = mu_max[u] < m ? m : mu_max[u];
+        }
+    }
+    const double sc_re = conditioning == 2 ? 0.0 : 1.0, sc_im = conditioning == 2 ? omega : 0.0;
+    for (int t = 0; t < num_tets && ok; ++t) {
+        double p[4][3], g[4][3], vol;
+        for (int j = 0; j < 4; ++j)
+            for (int c = 0; c < 3; ++c) p[j][c] = coords[3 * tet_nodes[4 * t + j] + c];
+        if (!tet_geometry(p, g, &vol)) { ok = 0; break; }
+        const int32_t* d = tet_dofs + 16 * (int64_t)t;
+        const double* sg = tet_signs + 6 * (int64_t)t;
+        double a_re[6], a_im[6], ph_re[10], ph_im[10];
+        for (int k = 0; k < 6; ++k) {
+            a_re[k] = d[k] >= 0 ? x[2 * d[k]] : 0.0;
+            a_im[k] = d[k] >= 0 ? x[2 * d[k] + 1] : 0.0;
+        }
+        for (int b = 0; b < 10; ++b) {
+            const int ib = d[6 + b];
+            if (ib >= 0) {
+                const double cf = tet_phi_coeff[10 * (int64_t)t + b];
+                const double s_re = sc_re * cf, s_im = sc_im * cf;
+                const double xr = x[2 * ib], xi = x[2 * ib + 1];
+                ph_re[b] = s_re * xr - s_im * xi;
+                ph_im[b] = s_re * xi + s_im * xr;
+            } else {
+                ph_re[b] = tet_phi_fixed[20 * (int64_t)t + 2 * b];
+                ph_im[b] = tet_phi_fixed[20 * (int64_t)t + 2 * b + 1];
+            }
+        }
+        /* B = curl A: whitney_edge_curl_global = (grad L_vi x grad L_vj) * 2 * sign */
+        double bc_re[3] = {0.0, 0.0, 0.0}, bc_im[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < 6; ++k) {
+            const double* gi = g[kEV[k][0]];
+            const double* gj = g[kEV[k][1]];
+            const double cr[3] = {(gi[1] * gj[2] - gi[2] * gj[1]) * 2.0, (gi[2] * gj[0] - gi[0] * gj[2]) * 2.0,
+                                  (gi[0] * gj[1] - gi[1] * gj[0]) * 2.0};
+            for (int c = 0; c < 3; ++c) {
+                const double cc = cr[c] * sg[k];
+                bc_re[c] = bc_re[c] + a_re[k] * cc;
+                bc_im[c] = bc_im[c] + a_im[k] * cc;
+            }
+        }
+        const int body = tet_body[t];
+        const double mu = kMu0 * body_mu_r[body];
+        const double sigma = body_sigma[body];
+        sigma_tet[t] = sigma;
+        const double Lc[4] = {0.25, 0.25, 0.25, 0.25};
+        double av_re[3], av_im[3], ev_re[3], ev_im[3];
+        tet_eval(g, Lc, sg, a_re, a_im, ph_re, ph_im, omega, av_re, av_im, ev_re, ev_im);
+        for (int c = 0; c < 3; ++c) {
+            b_tet[6 * (int64_t)t + 2 * c] = bc_re[c];
+            b_tet[6 * (int64_t)t + 2 * c + 1] = bc_im[c];
+            a_tet[6 * (int64_t)t + 2 * c] = av_re[c];
+            a_tet[6 * (int64_t)t + 2 * c + 1] = av_im[c];
+            j_tet[6 * (int64_t)t + 2 * c] = sigma * ev_re[c];
+            j_tet[6 * (int64_t)t + 2 * c + 1] = sigma * ev_im[c];
+            e_tet[6 * (int64_t)t + 2 * c] = ev_re[c];
+            e_tet[6 * (int64_t)t + 2 * c + 1] = ev_im[c];
+        }
+        for (int k = 0; k < 10; ++k) {
+            double L[4] = {0.0, 0.0, 0.0, 0.0};
+            if (k < 4) {
+                L[k] = 1.0;
+            } else {
+                L[kEV[k - 4][0]] = 0.5;
+                L[kEV[k - 4][1]] = 0.5;
+            }
+            const int u = tet_p2[10 * (int64_t)t + k];
+            tet_eval(g, L, sg, a_re, a_im, ph_re, ph_im, omega, av_re, av_im, ev_re, ev_im);
+            vertex_weight[u] = vertex_weight[u] + vol;
+            for (int c = 0; c < 3; ++c) {
+                a_node[6 * (size_t)u + 2 * c] = a_node[6 * (size_t)u + 2 * c] + vol * av_re[c];
+                a_node[6 * (size_t)u + 2 * c + 1] = a_node[6 * (size_t)u + 2 * c + 1] + vol * av_im[c];
+            }
+            const int sigma_uniform = sig_min[u] == sig_max[u];
+            const int mu_uniform = mu_min[u] == mu_max[u];
+            if (sigma_uniform || sigma == sig_max[u]) {
+                e_weight[u] = e_weight[u] + vol;
+                for (int c = 0; c < 3; ++c) {
+                    e_node[6 * (size_t)u + 2 * c] = e_node[6 * (size_t)u + 2 * c] + vol * ev_re[c];
+                    e_node[6 * (size_t)u + 2 * c + 1] = e_node[6 * (size_t)u + 2 * c + 1] + vol * ev_im[c];
+                }
+            }
+            if (mu_uniform || mu == mu_max[u]) {
+                b_weight[u] = b_weight[u] + vol;
+                for (int c = 0; c < 3; ++c) {
+                    b_node[6 * (size_t)u + 2 * c] = b_node[6 * (size_t)u + 2 * c] + vol * bc_re[c];
+                    b_node[6 * (size_t)u + 2 * c + 1] = b_node[6 * (size_t)u + 2 * c + 1] + vol * bc_im[c];
+                }
+            }
+            if (!sigma_uniform || !mu_uniform) on_interface[u] = 1;
+        }
+    }
+    counts[0] = 0;
+    counts[1] = 0;
+    for (size_t u = 0; u < np && ok; ++u) {
+        if (on_interface[u]) ++counts[0];
+        if (vertex_weight[u] > 0.0) {
+            for (int c = 0; c < 6; ++c) a_node[6 * u + c] = a_node[6 * u + c] / vertex_weight[u];
+        } else {
+            ++counts[1];
+            continue;
+        }
+        if (e_weight[u] > 0.0)
+            for (int c = 0; c < 6; ++c) e_node[6 * u + c] = e_node[6 * u + c] / e_weight[u];
+        if (b_weight[u] > 0.0)
+            for (int c = 0; c < 6; ++c) b_node[6 * u + c] = b_node[6 * u + c] / b_weight[u];
+        const double nu_node = 1.0 / mu_max[u];
+        for (int c = 0; c < 6; ++c) {
+            j_node[6 * u + c] = sig_max[u] * e_node[6 * u + c];
+            h_node[6 * u + c] = nu_node * b_node[6 * u + c];
+        }
+    }
+    free(sig_min); free(sig_max); free(mu_min); free(mu_max); free(e_weight); free(b_weight);
+    return ok;
+}
+
 int oracle_terminal_current(int num_tets, const double* coords, const int32_t* tet_nodes, const int32_t* tet_body,
                             const double* body_sigma, const int32_t* tet_dofs, const double* tet_signs,
                             const double* tet_phi_coeff, const double* tet_phi_fixed, const double* x, double omega,

@@ -230,6 +230,40 @@ def tet_fields(problem, x, omega, conditioning=1):
     return out.view(np.complex128).reshape(nt, 3)
 
 
+def compute_fields(problem, x, omega, conditioning=1):
+    """compute_fields (src/postprocess.cpp:223-481) of one solution x, without
+    phi_node: a dict with the same keys as Mesh.compute_fields."""
+    nt = int(problem.counts["tets"])
+    tet_p2 = _i32(problem.ints("tet_p2"))
+    npn = int(tet_p2.max()) + 1 if tet_p2.size else 0
+    npn = max(npn, int(problem.counts["nodes"]) + int(problem.counts["edges"]))
+    bp = _f64(problem.doubles("body_param")).reshape(-1, 3)
+    sig, mu = np.ascontiguousarray(bp[:, 0]), np.ascontiguousarray(bp[:, 2])
+    xx = np.ascontiguousarray(x, dtype=np.complex128)
+    out = {k: np.zeros((nt if k.endswith("_tet") else npn, 3), dtype=np.complex128)
+           for k in ("a_tet", "b_tet", "e_tet", "j_tet", "a_node", "b_node", "h_node", "e_node", "j_node")}
+    out["sigma_tet"] = np.zeros(nt)
+    out["vertex_weight"] = np.zeros(npn)
+    out["on_material_interface"] = np.zeros(npn, dtype=np.int8)
+    counts = np.zeros(2, dtype=np.int32)
+    d = lambda k: _p(out[k].view(np.float64), C.c_double)  # noqa: E731
+    ok = lib().oracle_compute_fields(
+        C.c_int(nt), C.c_int(npn), _p(_f64(problem.doubles("coords")), C.c_double),
+        _p(_i32(problem.ints("tet_nodes")), C.c_int32), _p(tet_p2, C.c_int32),
+        _p(_i32(problem.ints("tet_body")), C.c_int32), _p(sig, C.c_double), _p(mu, C.c_double),
+        _p(_i32(problem.ints("tet_dofs")), C.c_int32), _p(_f64(problem.ints("tet_edge_signs")), C.c_double),
+        _p(_f64(problem.doubles("tet_phi_coeff")), C.c_double), _p(_f64(problem.doubles("tet_phi_fixed")), C.c_double),
+        _p(xx.view(np.float64), C.c_double), C.c_double(omega), C.c_int(conditioning),
+        d("a_tet"), d("b_tet"), d("e_tet"), d("j_tet"), _p(out["sigma_tet"], C.c_double),
+        d("a_node"), d("b_node"), d("h_node"), d("e_node"), d("j_node"), _p(out["vertex_weight"], C.c_double),
+        _p(out["on_material_interface"], C.c_int8), _p(counts, C.c_int32))
+    if not ok:
+        raise ValueError("compute_fields: a tet is degenerate")
+    out["num_interface_nodes"] = int(counts[0])
+    out["num_orphan_vertices"] = int(counts[1])
+    return out
+
+
 def gauge_free_system(problem, e_tet, omega):
     """The vertex map and P1 system of recover_gauge_free_potential for the cube
     problem's mesh and the field e_tet ((num_tets, 3) complex). Returns a dict

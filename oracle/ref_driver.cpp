@@ -13,7 +13,8 @@
 //   mode 3: mode 1 with the z = side port driven by a 1 A boundary current
 //   mode 1: + refill at freq_hz, equilibration, factorization, solve, R and L,
 //           the fields' e_tet, the gauge-free potential recovered from them, and
-//           the potential.out / VTK files of the solution (no run information)
+//           the potential.out / VTK files of the solution (no run information), and the
+//           VTK with compute_fields' data (fields.vtk)
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -243,6 +244,7 @@ int main(int argc, char** argv) {
         const NodalPotential pot = potential_at_nodes(mesh, bound, dofs, sol);
         write_potential(g_out + "/potential.out", pot, sol, nullptr);
         write_vtk(g_out + "/solution.vtk", mesh, pot, sol);
+        write_vtk(g_out + "/fields.vtk", mesh, pot, sol, &fields, FieldSet::All);
         const Z Zin = impedance(1.0, I);
         std::fprintf(meta,
                      ", \"freq\": %.17g, \"omega\": %.17g, \"scatter_map_bitwise_equal\": %s, \"factor_ok\": %s, "

@@ -77,7 +77,7 @@ def run(case, outdir):
             digests[a] = digest(v)
             if full or a in ("x", "diag", "equil_d", "rhs", "gf_phi_vertex", "gf_port_potential"):
                 arrays[a] = v
-        for fname in ("potential.out", "solution.vtk"):
+        for fname in ("potential.out", "solution.vtk", "fields.vtk"):
             data = open(os.path.join(outdir, fname), "rb").read()
             digests[fname] = hashlib.sha256(data).hexdigest()
             meta["bytes_" + fname] = len(data)

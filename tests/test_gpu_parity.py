@@ -210,6 +210,27 @@ def test_pivot_failure_statistics(case):
     assert m.factorize_ldlt(A.APHI_FP64)["ok"]
 
 
+def test_compute_fields_bitwise_and_vtk_bytes(case, tmp_path):
+    """compute_fields on the device from the reference's own solution: every
+    FieldOutput array bit for bit the oracle's (whose VTK is the reference's, byte
+    for byte), and the VTK written from the device's fields is the reference's file."""
+    import hashlib
+    m, p, meta = case["m"], case["p"], case["meta"]
+    _, arrays = golden(meta["case"])
+    x = arrays["x"]
+    f = m.compute_fields(dev(x), meta["omega"])
+    o = O.compute_fields(p, x, meta["omega"])
+    for k in ("a_tet", "b_tet", "e_tet", "j_tet", "sigma_tet", "a_node", "b_node", "h_node", "e_node", "j_node",
+              "vertex_weight"):
+        assert np.array_equal(np.asarray(f[k]).view(np.uint64), np.asarray(o[k]).view(np.uint64)), k
+    assert np.array_equal(f["on_material_interface"], o["on_material_interface"])
+    assert f["num_interface_nodes"] == o["num_interface_nodes"] > 0
+    assert f["num_orphan_vertices"] == o["num_orphan_vertices"]
+    out = tmp_path / "fields.vtk"
+    p.write_vtk_fields(out, x, meta["omega"], f)
+    assert hashlib.sha256(out.read_bytes()).hexdigest() == meta["sha256"]["fields.vtk"]
+
+
 def test_deterministic_bits(case):
     m = case["m"]
     m.refill(case["meta"]["omega"])
