@@ -192,6 +192,8 @@ def main():
                     help="requests of the serving run (mixed sizes 8..32, Poisson arrivals; 0 skips)")
     ap.add_argument("--serving-rate", type=float, default=2.0, help="arrival rate of the serving run (requests/s)")
     ap.add_argument("--serving-slots", type=int, default=2)
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving workload and print its summary (with latency by mesh size) as JSON")
     ap.add_argument("--batch-problems", type=int, default=256,
                     help="problems of the batched run (seeded draws of 4..12 hexes per edge; 0 skips)")
     ap.add_argument("--batch-steps", type=int, default=3, help="timed passes over the batch")
@@ -219,6 +221,32 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4}))
+        return
+    if a.serving_only:
+        # the serving workload alone (a long stream: progress on stderr every 50 requests)
+        from aphi_amd.serving import Server, serving_requests
+
+        def progress(done, total):
+            if done % 50 == 0 or done == total:
+                log(f"serving: {done}/{total} requests done")
+        reqs = serving_requests(RHS_SEED, a.serving_requests, a.serving_rate)
+        srv = Server(slots=a.serving_slots, hbm_budget=200e9)
+        try:
+            results, summary = srv.run(reqs, progress=progress)
+        finally:
+            srv.close()
+        by_n = {}
+        for r in reqs:
+            res = results[r.id]
+            by_n.setdefault(r.config.n, []).append(res["latency_ms"])
+        summary.update(rate_per_s=a.serving_rate, seed=RHS_SEED,
+                       distinct_meshes=len({r.key() for r in reqs}),
+                       latency_by_hexes_per_edge={str(k): {"requests": len(v), "p50_ms": float(np.percentile(v, 50)),
+                                                           "p99_ms": float(np.percentile(v, 99))}
+                                                  for k, v in sorted(by_n.items())},
+                       workload="Poisson stream, meshes n in {8,16,24,32} (p = 0.7 repeat), 1 Hz-100 MHz, 1-8 RHS, "
+                                "FP32-MFMA + FP64 refinement to 1e-12")
+        print(json.dumps({"serving": summary}), flush=True)
         return
     bar = (6, 12, 20, 23) if a.n == 32 else (a.n // 5, a.n // 5 + 6, (2 * a.n) // 3, (2 * a.n) // 3 + 3)
     cfg = A.CubeConfig(n=a.n, side=0.1, bar=bar, sigma=5.8e7)

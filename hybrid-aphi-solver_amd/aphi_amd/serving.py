@@ -218,9 +218,10 @@ class Server:
             out["x"] = xh
         return out
 
-    def run(self, requests):
+    def run(self, requests, progress=None):
         """Serves ``requests`` in real time (each becomes available at its arrival
-        time after the call starts). Returns (results by id, summary)."""
+        time after the call starts). Returns (results by id, summary). ``progress``,
+        if given, is called as progress(done, total) after each request."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         queue, results = [], {}
         state = dict(done=False)
@@ -251,9 +252,11 @@ class Server:
                             return
                         self.lock.wait()
                 start = time.perf_counter()
+                build_ms = 0.0
                 try:
                     if e["mesh"] is None:
                         self._build(req, e)
+                        build_ms = (time.perf_counter() - start) * 1e3
                     res = self._serve(req, e)
                 except Exception as ex:  # reported per request; the loop goes on
                     res = dict(id=req.id, n=0, nrhs=req.nrhs, ok=False, error=f"{type(ex).__name__}: {ex}",
@@ -263,9 +266,13 @@ class Server:
                     self._release(req.key(), e)
                 end = time.perf_counter()
                 res.update(slot=slot, arrival=req.arrival, start=start - t0, end=end - t0,
-                           latency_ms=(end - t0 - req.arrival) * 1e3)
+                           latency_ms=(end - t0 - req.arrival) * 1e3, build_ms=build_ms,
+                           serve_ms=(end - start) * 1e3 - build_ms)
                 with self.lock:
                     results[req.id] = res
+                    done = len(results)
+                if progress is not None:
+                    progress(done, len(reqs))
 
         threads = [threading.Thread(target=worker, args=(s,)) for s in range(self.slots)]
         for th in threads:
@@ -298,5 +305,8 @@ class Server:
                        max_scaled_backward_error=max(results[r.id]["scaled_backward_error"] for r in reqs),
                        all_ok=all(results[r.id]["ok"] for r in reqs), evictions=self.evictions,
                        host_builds=self.host_builds, device_uploads=self.uploads, fp64_requests=fp64,
-                       fp32_attempts_refactored=stalled, hbm_budget_gb=self.budget / 1e9)
+                       fp32_attempts_refactored=stalled, hbm_budget_gb=self.budget / 1e9,
+                       mean_build_ms=float(np.mean([results[r.id].get("build_ms", 0.0) for r in reqs])),
+                       mean_serve_ms=float(np.mean([results[r.id].get("serve_ms", 0.0) for r in reqs])),
+                       mean_wait_ms=float(np.mean([results[r.id]["start"] - r.arrival for r in reqs]) * 1e3))
         return results, summary
