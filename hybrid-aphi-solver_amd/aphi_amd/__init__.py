@@ -493,6 +493,10 @@ class Mesh:
     def device_bytes(self):
         return int(lib().aphi_mesh_device_bytes(self._h))
 
+    def release_workspace(self):
+        """Frees the per-frequency state and keeps the plan (aphi_mesh_release_workspace)."""
+        _check(lib().aphi_mesh_release_workspace(self._h))
+
     # refill(SymmetricSystem&, ..., omega, conditioning) -- assembly.hpp:197
     def refill(self, omega, conditioning=APHI_ROW_SCALED):
         _check(lib().aphi_refill(self._h, float(omega), int(conditioning)))

@@ -147,6 +147,7 @@ SIGNATURES = {
     "aphi_write_vtk_fields": (C.c_int, [C.c_void_p, C.c_char_p, DP, C.c_double, C.c_int, C.POINTER(Fields), I64P]),
     "aphi_fields_attach": (C.c_int, [C.c_void_p, C.POINTER(FieldsDesc)]),
     "aphi_compute_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.POINTER(Fields)]),
+    "aphi_mesh_release_workspace": (C.c_int, [C.c_void_p]),
     "aphi_philox4x32_10": (C.c_int, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "aphi_random_normal_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, DP]),
     "aphi_random_uniform_host": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, DP]),

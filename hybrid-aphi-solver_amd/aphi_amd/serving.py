@@ -73,12 +73,22 @@ def serving_requests(seed, count, rate, sizes=(8, 16, 24, 32), repeat=0.7, max_r
     return out
 
 
-def _estimate_bytes(p):
-    """Device bytes of a mesh with an FP32 factor (a fallback to FP64 is accounted
-    from the mesh's actual size once its request is done)."""
+def _workspace_bytes(p, precision):
+    """Device bytes of one request's numeric state on the mesh of problem p:
+    factor and contribution blocks in the factor's precision, assembled values,
+    element matrices, solve and refinement vectors."""
     c = p.counts
-    return int(8 * (c["factor_entries"] + c["cb_pool_entries"]) + 16 * 196 * c["tets"] + 256 * c["nnz"]
+    cs = 16 if precision == APHI_FP64 else 8
+    return int(cs * (c["factor_entries"] + c["cb_pool_entries"]) + 16 * 196 * c["tets"] + 256 * c["nnz"]
                + 4096 * c["num_total"])
+
+
+def _plan_bytes(p):
+    """An upper estimate of a mesh's plan on the device (assembly plan, pattern,
+    supernodal plan, per-tet unknowns), replaced by the measured size once it is
+    uploaded."""
+    c = p.counts
+    return int(16 * c["contributions"] + 64 * c["nnz"] + 256 * c["num_total"] + 256 * c["tets"] + (64 << 20))
 
 
 class Server:
@@ -103,6 +113,7 @@ class Server:
         self.lock = threading.Condition()
         self.reserved = 0
         self.evictions = 0
+        self.trims = 0
         self.host_builds = 0
         self.uploads = 0
 
@@ -123,12 +134,24 @@ class Server:
 
     # ---------------------------------------------------------------- memory
     def _admit(self, e, need):
-        """Reserves ``need`` bytes for the new mesh of entry ``e`` (busy, in the
-        cache), evicting idle meshes (least recently used first); waits while busy
+        """Reserves ``need`` more bytes for entry ``e`` (busy, in the cache): its
+        request's numeric state, and its plan when the mesh is not resident. Idle
+        meshes give memory back least recently used first: their per-frequency
+        state (aphi_mesh_release_workspace) before their plans, so the symbolic
+        factors stay on the device as long as the budget allows; waits while busy
         meshes hold the memory. Every reserved byte belongs to an entry."""
         with self.lock:
             while self.reserved + need > self.budget:
                 idle = sorted((x["last"], k) for k, x in self.meshes.items() if not x["busy"])
+                trim = [k for _, k in idle
+                        if self.meshes[k]["mesh"] is not None and self.meshes[k]["bytes"] > self.meshes[k]["plan"]]
+                if trim:
+                    x = self.meshes[trim[0]]
+                    x["mesh"].release_workspace()
+                    self.reserved -= x["bytes"] - x["plan"]
+                    x["bytes"] = x["plan"]
+                    self.trims += 1
+                    continue
                 if not idle:
                     if not any(x["busy"] and x is not e for x in self.meshes.values()):
                         raise MemoryError(f"a mesh needs {need / 1e9:.1f} GB, over the HBM budget")
@@ -140,7 +163,7 @@ class Server:
                 self.reserved -= x["bytes"]
                 self.evictions += 1
             self.reserved += need
-            e["bytes"] = need
+            e["bytes"] += need
 
     def _problem(self, req):
         key = req.key()
@@ -152,14 +175,13 @@ class Server:
                 self.host_builds += 1
         return fut
 
-    def _build(self, req, e):
+    def _build(self, req, e, p):
         """Uploads the request's mesh into the cache entry ``e`` (a busy
-        placeholder put there when the request was taken)."""
+        placeholder put there when the request was taken, its memory admitted)."""
         try:
-            p = self._problem(req).result()
-            self._admit(e, _estimate_bytes(p))
             e["ctx"] = Context(self.device)
             e["mesh"] = Mesh(e["ctx"], p)
+            e["plan"] = e["mesh"].device_bytes()
             # other slots' threads issue GPU work at any time: no graph capture
             e["mesh"].set_solve_graphs(False)
             with self.lock:
@@ -177,7 +199,7 @@ class Server:
     def _release(self, key, e):
         with self.lock:
             if self.meshes.get(key) is e:
-                actual = e["mesh"].device_bytes()
+                actual = e["mesh"].device_bytes() if e.get("mesh") is not None else 0
                 self.reserved += actual - e["bytes"]
                 e["bytes"] = actual
                 e["busy"] = False
@@ -198,12 +220,11 @@ class Server:
             self.policies[req.key()].record(req.frequency, prec, rep)
 
     # ---------------------------------------------------------------- requests
-    def _serve(self, req, e):
+    def _serve(self, req, e, prec):
         m = e["mesh"]
         omega = 2.0 * math.pi * req.frequency
         m.refill(omega, APHI_ROW_SCALED)
         B = random_complex_rhs(e["ctx"], req.seed, req.nrhs, m.n)
-        prec = self._choose(req)
         X, rep = m.solve_symmetric(B, precision=prec, equilibration_iterations=10,
                                    max_refinements=self.max_refinements, tolerance=self.tolerance)
         self._record(req, prec, rep)
@@ -235,7 +256,7 @@ class Server:
                 if e is None or not e["busy"]:
                     queue.remove(r)
                     if e is None:
-                        e = dict(mesh=None, ctx=None, bytes=0, last=time.perf_counter(), busy=True)
+                        e = dict(mesh=None, ctx=None, bytes=0, plan=0, last=time.perf_counter(), busy=True)
                         self.meshes[r.key()] = e
                     e["busy"] = True
                     return r, e
@@ -254,10 +275,17 @@ class Server:
                 start = time.perf_counter()
                 build_ms = 0.0
                 try:
+                    p = self._problem(req).result()
+                    prec = self._choose(req)
                     if e["mesh"] is None:
-                        self._build(req, e)
-                        build_ms = (time.perf_counter() - start) * 1e3
-                    res = self._serve(req, e)
+                        need = _plan_bytes(p) + _workspace_bytes(p, prec)
+                    else:  # what its numeric state may still grow by
+                        need = max(0, e["plan"] + _workspace_bytes(p, prec) - e["bytes"])
+                    self._admit(e, need)
+                    if e["mesh"] is None:
+                        self._build(req, e, p)
+                    build_ms = (time.perf_counter() - start) * 1e3
+                    res = self._serve(req, e, prec)
                 except Exception as ex:  # reported per request; the loop goes on
                     res = dict(id=req.id, n=0, nrhs=req.nrhs, ok=False, error=f"{type(ex).__name__}: {ex}",
                                backward_error=float("inf"), scaled_backward_error=float("inf"), refinement_steps=0,
@@ -304,6 +332,7 @@ class Server:
                        max_backward_error=max(results[r.id]["backward_error"] for r in reqs),
                        max_scaled_backward_error=max(results[r.id]["scaled_backward_error"] for r in reqs),
                        all_ok=all(results[r.id]["ok"] for r in reqs), evictions=self.evictions,
+                       workspace_releases=self.trims,
                        host_builds=self.host_builds, device_uploads=self.uploads, fp64_requests=fp64,
                        fp32_attempts_refactored=stalled, hbm_budget_gb=self.budget / 1e9,
                        mean_build_ms=float(np.mean([results[r.id].get("build_ms", 0.0) for r in reqs])),

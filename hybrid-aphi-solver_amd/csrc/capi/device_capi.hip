@@ -1293,6 +1293,35 @@ extern "C" void aphi_mesh_destroy(aphi_mesh* m) {
     delete m;
 }
 
+extern "C" int aphi_mesh_release_workspace(aphi_mesh* m) {
+    return aphi_capi::guarded([&] {
+        require(m != nullptr, "aphi_mesh_release_workspace: null mesh");
+        after_caller(m->ctx);
+        HIPCHK(hipSetDevice(m->ctx->device));
+        HIPCHK(hipStreamSynchronize(m->ctx->stream));
+        HIPCHK(hipStreamSynchronize(m->ctx->side));
+        // the captured graphs name the buffers released below
+        if (m->sgraph.exec) HIPCHK(hipGraphExecDestroy(m->sgraph.exec));
+        m->sgraph = SolveGraph{};
+        if (m->fgraph.exec) HIPCHK(hipGraphExecDestroy(m->fgraph.exec));
+        m->fgraph.exec = nullptr;
+        m->fgraph.key.clear();
+        m->fgraph.precision = -1;
+        // (the assembly error flag stays: it is allocated with the plan)
+        DevBuf* numeric[] = {&m->coef, &m->E,     &m->A,     &m->rhs,  &m->Aeq,      &m->work,  &m->rowmax,
+                             &m->fac,  &m->d,     &m->partials, &m->stats, &m->P, &m->CB,       &m->mws,   &m->Xinv,
+                             &m->X256, &m->XT256, &m->dinv,  &m->bt,    &m->W,    &m->X,        &m->Yv,    &m->bpart,
+                             &m->tsum, &m->colscale, &m->res, &m->norms, &m->xil, &m->gV,       &m->gZ,    &m->gW,
+                             &m->gU,   &m->gDots, &m->gSmall, &m->tcontrib, &m->tcurrent, &m->fval, &m->fvol,
+                             &m->fmu,  &m->fsig,  &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node,
+                             &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag};
+        for (DevBuf* b : numeric) b->release();
+        m->assembled = m->equilibrated = m->factored = false;
+        m->factor_precision = -1;
+        m->solve_nrhs_cap = 0;
+    });
+}
+
 extern "C" int aphi_mesh_set_solve_graphs(aphi_mesh* m, int on) {
     return aphi_capi::guarded([&] {
         require(m != nullptr, "aphi_mesh_set_solve_graphs: null mesh");
@@ -1503,6 +1532,7 @@ void refill_all(aphi_mesh* m, const double* omegas, int conditioning) {
     m->E.reserve(static_cast<size_t>(m->num_tets) * 196 * sizeof(cd));
     m->A.reserve(static_cast<size_t>(m->nnz) * sizeof(cd));
     m->rhs.reserve(static_cast<size_t>(m->n) * sizeof(cd));
+    m->flag.reserve(sizeof(int));
     hipLaunchKernelGGL(aphi_dev::k_fill_flag, dim3(1), dim3(64), 0, s, m->flag.as<int>());
     hipLaunchKernelGGL(aphi_dev::k_element, dim3(blocks_for(m->num_tets, 16)), dim3(256), 0, s, m->num_tets,
                        m->coords.as<double>(), m->tets.as<int>(), m->tet_body.as<int>(),

@@ -389,7 +389,10 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__
     const int p = lane % 16, kq = lane / 16, k0 = 64 * wave;
     const cf* Ps = P + fp.panel_off[s];
     const cf* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
-    VF r0 = VF{0, 0, 0, 0}, i0a = VF{0, 0, 0, 0}, r1 = VF{0, 0, 0, 0}, i1a = VF{0, 0, 0, 0};
+    // three-product complex form (k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi);
+    // re = k1 - k3, im = k1 + k2): three MFMAs per complex product instead of four
+    VF s1a = VF{0, 0, 0, 0}, s2a = VF{0, 0, 0, 0}, s3a = VF{0, 0, 0, 0};
+    VF s1b = VF{0, 0, 0, 0}, s2b = VF{0, 0, 0, 0}, s3b = VF{0, 0, 0, 0};
     if (k0 < nb) {
         typedef float f4 __attribute__((ext_vector_type(4)));
         const int row = min(i0 + 2 * p, x.m - 1);  // rows row and row + 1 (the second may lie past m: never stored)
@@ -407,16 +410,16 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__
         __builtin_amdgcn_sched_barrier(0);  // every operand load in flight before the first MFMA
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            r0 = Mfma<float>::mac(a[u].x, b[u].re, r0);
-            r0 = Mfma<float>::mac(-a[u].y, b[u].im, r0);
-            i0a = Mfma<float>::mac(a[u].x, b[u].im, i0a);
-            i0a = Mfma<float>::mac(a[u].y, b[u].re, i0a);
-            r1 = Mfma<float>::mac(a[u].z, b[u].re, r1);
-            r1 = Mfma<float>::mac(-a[u].w, b[u].im, r1);
-            i1a = Mfma<float>::mac(a[u].z, b[u].im, i1a);
-            i1a = Mfma<float>::mac(a[u].w, b[u].re, i1a);
+            const float bd = b[u].im - b[u].re, bs = b[u].re + b[u].im;
+            s1a = Mfma<float>::mac(a[u].x + a[u].y, b[u].re, s1a);
+            s2a = Mfma<float>::mac(a[u].x, bd, s2a);
+            s3a = Mfma<float>::mac(a[u].y, bs, s3a);
+            s1b = Mfma<float>::mac(a[u].z + a[u].w, b[u].re, s1b);
+            s2b = Mfma<float>::mac(a[u].z, bd, s2b);
+            s3b = Mfma<float>::mac(a[u].w, bs, s3b);
         }
     }
+    const VF r0 = s1a - s3a, i0a = s1a + s2a, r1 = s1b - s3b, i1a = s1b + s2b;
     cf* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
     if constexpr (NW == 1) {
         // tile row r of tile h is front row i0 + 2 r + h
@@ -529,7 +532,8 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_partial2(const int* __restrict_
     const cf* Ps = P + fp.panel_off[s];
     const int* rows = fp.rows + x.rp;
     const cf* Xg = X + static_cast<int64_t>(g) * n * kQ;
-    VF accr = VF{0, 0, 0, 0}, acci = VF{0, 0, 0, 0};
+    // three-product complex form, as in k_fwd_update2
+    VF s1 = VF{0, 0, 0, 0}, s2 = VF{0, 0, 0, 0}, s3 = VF{0, 0, 0, 0};
     if (iw < x.m) {
         typedef float f4 __attribute__((ext_vector_type(4)));
         const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
@@ -555,16 +559,15 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_partial2(const int* __restrict_
             const int r = iw + 8 * u + 2 * kq;
             const cf b0 = r < x.m ? b[2 * u] : cf{0, 0};
             const cf b1 = r + 1 < x.m ? b[2 * u + 1] : cf{0, 0};
-            accr = Mfma<float>::mac(a[u].x, b0.re, accr);
-            accr = Mfma<float>::mac(-a[u].y, b0.im, accr);
-            acci = Mfma<float>::mac(a[u].x, b0.im, acci);
-            acci = Mfma<float>::mac(a[u].y, b0.re, acci);
-            accr = Mfma<float>::mac(a[u].z, b1.re, accr);
-            accr = Mfma<float>::mac(-a[u].w, b1.im, accr);
-            acci = Mfma<float>::mac(a[u].z, b1.im, acci);
-            acci = Mfma<float>::mac(a[u].w, b1.re, acci);
+            s1 = Mfma<float>::mac(a[u].x + a[u].y, b0.re, s1);
+            s2 = Mfma<float>::mac(a[u].x, b0.im - b0.re, s2);
+            s3 = Mfma<float>::mac(a[u].y, b0.re + b0.im, s3);
+            s1 = Mfma<float>::mac(a[u].z + a[u].w, b1.re, s1);
+            s2 = Mfma<float>::mac(a[u].z, b1.im - b1.re, s2);
+            s3 = Mfma<float>::mac(a[u].w, b1.re + b1.im, s3);
         }
     }
+    const VF accr = s1 - s3, acci = s1 + s2;
     cf* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
     reduce_waves<float, NW>(part, accr, acci, [&](int p, int q, cf t) {
         if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;

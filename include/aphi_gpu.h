@@ -362,6 +362,15 @@ int aphi_mesh_get_update_tile_table(const aphi_mesh* mesh, int8_t* table);
 int aphi_mesh_set_update_tile_table(aphi_mesh* mesh, const int8_t* table);
 /* Device bytes currently held by this mesh (plan + numeric buffers). */
 int64_t aphi_mesh_device_bytes(const aphi_mesh* mesh);
+/* Frees the mesh's per-frequency state -- assembled values, equilibration, factor,
+ * contribution blocks, solve and refinement work arrays, captured graphs -- and
+ * keeps the plan (what ScatterMap::build and analyze produced: the symbolic factor),
+ * so the mesh holds only its plan until the next aphi_refill / aphi_system_upload.
+ * For callers that keep many meshes resident under a memory budget (a serving
+ * loop): the next frequency on the mesh reuses the plan and allocates its numeric
+ * state again. Calls that need an assembled system or a factor are refused until
+ * then. Waits for the mesh's streams first. */
+int aphi_mesh_release_workspace(aphi_mesh* mesh);
 
 /* refill(SymmetricSystem&, bound, mesh, dofs, pattern, omega, conditioning, map)
  * -- include/aphi_solver/assembly.hpp:197, src/assembly.cpp:468-494. Clears and

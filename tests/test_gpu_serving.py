@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import aphi_amd as A
-from aphi_amd.serving import Request, Server, _estimate_bytes, serving_requests
+from aphi_amd.serving import Request, Server, _plan_bytes, _workspace_bytes, serving_requests
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -54,7 +54,7 @@ def test_budget_forces_evictions_without_changing_results(gpu_context):
     finally:
         big.close()
     # room for the largest mesh and a little more: the meshes keep evicting each other
-    need = max(_estimate_bytes(A.Problem(r.config)) for r in reqs)
+    need = max(_workspace_bytes(p, A.APHI_FP32) + _plan_bytes(p) for p in (A.Problem(r.config) for r in reqs))
     tight = Server(slots=2, hbm_budget=1.2 * need, precision=A.APHI_FP32)
     try:
         got, summary = tight.run(reqs)
@@ -64,6 +64,32 @@ def test_budget_forces_evictions_without_changing_results(gpu_context):
     assert summary["evictions"] > 0
     for r in reqs:
         assert got[r.id]["digest"] == ref[r.id]["digest"], r.id
+
+
+def test_release_workspace_keeps_the_plan(gpu_context):
+    """After aphi_mesh_release_workspace the mesh holds only its plan, refuses what
+    needs a factor, and its next refill and solve give the same bits as before."""
+    import torch
+    p = A.Problem(A.CubeConfig(n=6, bar=(2, 4, 2, 4)))
+    m = A.Mesh(gpu_context, p)
+    try:
+        omega = 2 * np.pi * 1e4
+        m.refill(omega)
+        b = m.rhs_device()[None, :].contiguous()
+        x1, _ = m.solve_symmetric(b, precision=A.APHI_FP32, max_refinements=10)
+        full = m.device_bytes()
+        m.release_workspace()
+        plan = m.device_bytes()
+        assert 0 < plan < full
+        with pytest.raises(ValueError):
+            m.solve(b, torch.empty_like(b))
+        with pytest.raises(ValueError):
+            m.backward_error(x1, b)
+        m.refill(omega)
+        x2, _ = m.solve_symmetric(b, precision=A.APHI_FP32, max_refinements=10)
+        assert np.array_equal(x1.cpu().numpy().view(np.uint64), x2.cpu().numpy().view(np.uint64))
+    finally:
+        m.close()
 
 
 def test_serving_workload_sizes_against_the_oracle(gpu_context):
