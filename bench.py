@@ -116,6 +116,9 @@ def cpu_baseline(A, prob, mesh, freq):
 
 
 BATCH_SEED = 20261204
+# FP32 refinement budget of a sweep point: the GMRES restart length, so an attempt
+# runs one Krylov cycle at most
+REFINE_BUDGET = 40
 
 
 def batched_leg(A, ctx, count, steps):
@@ -295,7 +298,8 @@ def main():
         omega = 2 * np.pi * freqs[fi]
         mesh.refill(omega)
         _, rep = mesh.solve_symmetric(B[fi], X, precision=precision, equilibration_iterations=10,
-                                      max_refinements=(30 if precision == A.APHI_FP32 else 0), tolerance=1e-12)
+                                      max_refinements=(REFINE_BUDGET if precision == A.APHI_FP32 else 0),
+                                      tolerance=1e-12)
         # the tets' fields (compute_fields, tools/solve_mesh.cpp:296) reduced to the port
         # current, R and L of every right-hand side (postprocess.hpp:348-358)
         rep["current"], rep["R"], rep["L"] = mesh.terminal_current(X, omega, cfg.side)
@@ -303,7 +307,7 @@ def main():
 
     # mixed-precision sweep: FP32 factor + GMRES refinement, FP64 factor at the
     # frequencies where the points solved so far show FP32 would be slower
-    policy = A.PrecisionPolicy()
+    policy = A.PrecisionPolicy(budget=REFINE_BUDGET)
 
     def step_mixed(fi):
         prec = policy.choose(freqs[fi])
@@ -316,7 +320,7 @@ def main():
     # so the timed sweep starts from nothing learnt
     for i in range(a.warmup):
         fi = order[i % len(order)]
-        warm = A.PrecisionPolicy()
+        warm = A.PrecisionPolicy(budget=REFINE_BUDGET)
         step(fi, warm.choose(freqs[fi]))
     ctx.synchronize()
     torch.cuda.synchronize()
