@@ -131,7 +131,10 @@ struct aphi_mesh {
     std::vector<double> sigma, eps_r, mu_r;
     std::vector<Level> levels;
     int max_level_count = 0;
-    int64_t max_bwd_partials = 0;  // count * backward row chunks (aphi_dev::bwd_chunks) over levels, per kSB * padded nrhs
+    // count * rect_chunks * rect_cols and count * rect_cols over levels (the
+    // backward sums of the rows below the pivots, per padded nrhs)
+    int64_t max_rect_partials = 0, max_rect_sums = 0;
+    int64_t max_fwd_partials = 0;  // count * forward splits * rows below the pivots over levels, per padded nrhs
     // plan on the device
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
@@ -147,7 +150,7 @@ struct aphi_mesh {
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
     DevBuf P, CB, mws, Xinv, X256, XT256, dinv;
-    DevBuf bt, W, X, Yv, bpart, tsum, colscale, res, norms, xil;
+    DevBuf bt, W, X, Yv, rpart, trect, fpart, colscale, res, norms, xil;
     DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
     int solve_nrhs_cap = 0;
     SolveGraph sgraph;
@@ -589,8 +592,9 @@ void ensure_solve_buffers(aphi_mesh* m, int nrhs) {
     m->Yv.reserve(static_cast<size_t>(m->n) * q * cs);
     m->W.reserve(static_cast<size_t>(m->rows_total) * q * cs);
     m->xil.reserve(static_cast<size_t>(m->n) * q * sizeof(cd));
-    m->bpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_bwd_partials, 1)) * aphi_dev::kSB * q * cs);
-    m->tsum.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kSB * q * cs);
+    m->rpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_rect_partials, 1)) * q * cs);
+    m->trect.reserve(static_cast<size_t>(std::max<int64_t>(m->max_rect_sums, 1)) * q * cs);
+    m->fpart.reserve(static_cast<size_t>(std::max<int64_t>(m->max_fwd_partials, 1)) * q * cs);
     m->colscale.reserve(nrhs * sizeof(double));
     m->res.reserve(static_cast<size_t>(m->n) * nrhs * sizeof(cd));
     const size_t cols = static_cast<size_t>(nrhs) * m->seg.count;  // (problem, right-hand side) pairs
@@ -610,14 +614,20 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
     aphi_dev::launch_perm_in<T>(s, n, nrhs, m->perm.as<int>(), m->d.as<double>(), b, ldb, m->colscale.as<double>(), m->bt.p);
     const int* nodes = m->nodes.as<int>();
     constexpr int SB = aphi_dev::kSB;
-    // rows below block B of the level's supernodes (max over the level)
-    const auto rows_below = [](const Level& L, int B) {
+    // triangle rows below block B of the level's supernodes, and rows below the
+    // pivots (max over the level)
+    const auto tri_rows = [](const Level& L, int B) {
         int rb = 0;
         for (int i = 0; i < L.count(); ++i) {
             const int kk = L.k[static_cast<size_t>(i)];
             if (kk <= B) continue;
-            rb = std::max(rb, L.m[static_cast<size_t>(i)] - B - std::min(aphi_dev::kSB, kk - B));
+            rb = std::max(rb, kk - B - std::min(aphi_dev::kSB, kk - B));
         }
+        return rb;
+    };
+    const auto rect_rows = [](const Level& L) {
+        int rb = 0;
+        for (int i = 0; i < L.count(); ++i) rb = std::max(rb, L.m[static_cast<size_t>(i)] - L.k[static_cast<size_t>(i)]);
         return rb;
     };
     const auto levels = [&]() {
@@ -628,8 +638,10 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         int maxk = 0;
         for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
         for (int B = 0; B < maxk; B += SB)
-            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), rows_below(L, B), nrhs, fp,
+            aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), tri_rows(L, B), nrhs, fp,
                                          m->P.p, m->X256.p, m->W.p, m->Yv.p, n, m->rows_total);
+        aphi_dev::launch_fwd_rect<T>(s, nodes + L.off, cnt, maxk, rect_rows(L), nrhs, fp, m->P.p, m->W.p, m->Yv.p, n,
+                                     m->rows_total, m->fpart.p);
     }
     for (int li = m->num_levels - 1; li >= 0; --li) {
         const Level& L = m->levels[static_cast<size_t>(li)];
@@ -637,16 +649,20 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         int maxk = 0;
         for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
         if (maxk == 0) continue;
+        const bool all = maxk > SB;  // the chain has updates
+        const void* t = aphi_dev::launch_bwd_rect<T>(s, nodes + L.off, cnt, maxk, rect_rows(L), nrhs, all, fp, m->P.p,
+                                                     m->X.p, n, m->rpart.p, m->trect.p);
+        const int rk = aphi_dev::rect_cols(maxk);
         for (int B = ((maxk - 1) / SB) * SB; B >= 0; B -= SB)
-            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), rows_below(L, B), nrhs, fp,
-                                         m->P.p, m->dinv.p, m->XT256.p, m->Yv.p, m->X.p, n, m->bpart.p, m->tsum.p);
+            aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), nrhs, fp, m->P.p, m->dinv.p,
+                                         m->XT256.p, m->Yv.p, m->X.p, n, const_cast<void*>(t), rk, all);
     }
     };
     if (solve_graphs() && m->solve_graphs_on) {
         const int prec = std::is_same<T, float>::value ? APHI_FP32 : APHI_FP64;
         const int groups = aphi_dev::rhs_groups(nrhs);
         const std::vector<const void*> key = {m->P.p,  m->X256.p, m->XT256.p, m->dinv.p, m->bt.p, m->W.p,
-                                              m->Yv.p, m->X.p,    m->bpart.p, m->tsum.p, m->nodes.p};
+                                              m->Yv.p, m->X.p,    m->rpart.p, m->trect.p, m->fpart.p, m->nodes.p};
         SolveGraph& sg = m->sgraph;
         if (!(sg.exec && sg.precision == prec && sg.groups == groups && sg.key == key)) {
             if (sg.exec) HIPCHK(hipGraphExecDestroy(sg.exec));
@@ -1158,7 +1174,7 @@ std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d
             lv.ea_off = static_cast<int64_t>(ea.size());
             lv.small_count = static_cast<int>(small[static_cast<size_t>(L)].size());
             lv.large_count = static_cast<int>(large[static_cast<size_t>(L)].size());
-            int blk = 0, maxrb = 0;
+            int blk = 0, maxk = 0, maxrect = 0;
             for (int pass = 0; pass < 2; ++pass)
                 for (int s : (pass == 0 ? small : large)[static_cast<size_t>(L)]) {
                     node_list.push_back(s);
@@ -1172,12 +1188,23 @@ std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d
                         ea.push_back(blk);
                         blk += (mm + 63) / 64;
                     }
-                    maxrb = std::max(maxrb, aphi_dev::bwd_chunks(mm - std::min(kk, aphi_dev::kSB)));
+                    maxk = std::max(maxk, kk);
+                    maxrect = std::max(maxrect, mm - kk);
                 }
             lv.ea_blocks = blk;
             ea.push_back(blk);
             m->max_level_count = std::max(m->max_level_count, lv.count());
-            m->max_bwd_partials = std::max<int64_t>(m->max_bwd_partials, static_cast<int64_t>(lv.count()) * std::max(maxrb, 1));
+            const int64_t rc = aphi_dev::rect_cols(maxk);
+            m->max_rect_partials = std::max<int64_t>(m->max_rect_partials,
+                                                     static_cast<int64_t>(lv.count()) * aphi_dev::rect_chunks(maxrect) * rc);
+            m->max_rect_sums = std::max<int64_t>(m->max_rect_sums, static_cast<int64_t>(lv.count()) * rc);
+            if (maxrect > 0)
+                for (const bool f32 : {true, false}) {
+                    const int ks = aphi_dev::fwd_rect_split(lv.count(), maxk, maxrect, f32, nullptr);
+                    if (ks > 1)
+                        m->max_fwd_partials = std::max<int64_t>(
+                            m->max_fwd_partials, static_cast<int64_t>(lv.count()) * ks * maxrect);
+                }
         }
         upload(m->st(), m->nodes, node_list.data(), node_list.size());
         {
@@ -1310,8 +1337,8 @@ extern "C" int aphi_mesh_release_workspace(aphi_mesh* m) {
         // (the assembly error flag stays: it is allocated with the plan)
         DevBuf* numeric[] = {&m->coef, &m->E,     &m->A,     &m->rhs,  &m->Aeq,      &m->work,  &m->rowmax,
                              &m->fac,  &m->d,     &m->partials, &m->stats, &m->P, &m->CB,       &m->mws,   &m->Xinv,
-                             &m->X256, &m->XT256, &m->dinv,  &m->bt,    &m->W,    &m->X,        &m->Yv,    &m->bpart,
-                             &m->tsum, &m->colscale, &m->res, &m->norms, &m->xil, &m->gV,       &m->gZ,    &m->gW,
+                             &m->X256, &m->XT256, &m->dinv,  &m->bt,    &m->W,    &m->X,        &m->Yv,
+                             &m->rpart, &m->trect, &m->fpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->gV,       &m->gZ,    &m->gW,
                              &m->gU,   &m->gDots, &m->gSmall, &m->tcontrib, &m->tcurrent, &m->fval, &m->fvol,
                              &m->fmu,  &m->fsig,  &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node,
                              &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag};
@@ -1437,9 +1464,9 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->tsum, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->fptr, &m->fentry, &m->bmu, &m->fval, &m->fvol, &m->fmu, &m->fsig, &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->rpart, &m->trect, &m->fpart, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->fptr, &m->fentry, &m->bmu, &m->fval, &m->fvol, &m->fmu, &m->fsig, &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
-                           &m->bpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv, &m->gf.coords,
+                           &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv, &m->gf.coords,
                            &m->gf.tets, &m->gf.tet_body, &m->gf.tet_on, &m->gf.column, &m->gf.prescribed, &m->gf.tptr,
                            &m->gf.entry, &m->gf.kind, &m->gf.values, &m->gf.beta, &m->gf.terms};
     int64_t total = 0;

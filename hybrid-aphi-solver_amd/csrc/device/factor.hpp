@@ -22,7 +22,8 @@ constexpr int kSmallTileCols = 512;
 constexpr int kSB = 256;        // solve block: columns per forward/backward step (multiple of kNB)
 constexpr int kQ = 16;          // right-hand sides per interleaved solve group
 constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a multiple of 64)
-constexpr int kBwdRows = 256;     // rows per backward partial sum
+constexpr int kRectRows = 1024;   // rows per backward partial sum below a front's pivots
+constexpr int kFwdRectWgs = 4096; // workgroups a forward rectangle launch aims for (row tiles x column splits)
 
 void launch_stats_init(hipStream_t st, FactorStatsDev* stats);  // every slot to its identity
 // Re-initialises the slots and gathers the pivot and multiplier extremes of the
@@ -60,7 +61,6 @@ void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int m
 // solve.hip: vectors in groups of kQ right-hand sides, entry (i, q) of group g at
 // [(g * rows + i) * kQ + q] (rows = n, or the total front row count for W)
 int rhs_groups(int nrhs);
-int bwd_chunks(int rows_below);  // row chunks of a backward step's partial sums
 template <typename T>
 void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
                     const double* colscale, void* out);
@@ -75,16 +75,40 @@ void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int max_m, i
 template <typename T>
 void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const FrontPlan& fp, const void* P,
                        const void* xinv, void* X, void* XT);
-// max_rows_below: the most rows below block B among the level's supernodes
+// Step B of the block chain: y_B = X_B w_B, then the triangle rows [B + nb, k)
+// take W -= L y_B; max_tri_rows: the most such rows among the level's supernodes
 template <typename T>
-void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
+void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_tri_rows, int nrhs,
                      const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R);
-// partial: count * bwd_chunks(max_rows_below) * kSB * kQ * rhs_groups(nrhs) entries;
-// tsum: count * kSB * kQ * rhs_groups(nrhs)
+// After a level's chain: the rows [k, m) below every front's pivots take
+// W -= L[k:m, 0:k] y[0:k] (max_rect_rows: the most such rows, max_k the most
+// pivots). The fronts' blocks are split fwd_rect_split ways when the row tiles
+// alone would leave the device short of workgroups; the splits' partial sums go
+// to part (count * split * max_rect_rows * kQ * rhs_groups(nrhs) entries) and are
+// subtracted in split order.
+int fwd_rect_split(int count, int max_k, int max_rect_rows, bool fp32, int* bps);
 template <typename T>
-void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* dinv, const void* XT256, const void* Y, void* X,
-                     int n, void* partial, void* tsum);
+void launch_fwd_rect(hipStream_t st, const int* nodes, int count, int max_k, int max_rect_rows, int nrhs,
+                     const FrontPlan& fp, const void* P, void* W, const void* Y, int n, int64_t R, void* part);
+// Before a level's backward chain: t = L[k:m, 0:k]^T x[rows k..m) for every front,
+// in partial sums over kRectRows rows (rpart: count * rect_chunks(max_rect_rows) *
+// rect_cols(max_k) * kQ * rhs_groups(nrhs) entries) reduced into t (count *
+// rect_cols(max_k) * kQ * rhs_groups(nrhs), column stride rect_cols(max_k)).
+// all: the level's chain has updates (a front with more than kSB pivots), so t is
+// formed for every front, zero where there are no rows below the pivots. Returns
+// the buffer holding t (rpart itself when one chunk suffices), null when there is
+// nothing to add
+int rect_chunks(int rect_rows);
+int rect_cols(int max_k);
+template <typename T>
+const void* launch_bwd_rect(hipStream_t st, const int* nodes, int count, int max_k, int max_rect_rows, int nrhs,
+                            bool all, const FrontPlan& fp, const void* P, const void* X, int n, void* rpart, void* t);
+// Step B of the backward chain: x_B = X_B^T (D_B^-1 y_B - t[B:B+nb]), then
+// t[0:B] += L[B:B+nb, 0:B]^T x_B (t from launch_bwd_rect, column stride rk)
+template <typename T>
+void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int nrhs, const FrontPlan& fp,
+                     const void* P, const void* dinv, const void* XT256, const void* Y, void* X, int n, void* t, int rk,
+                     bool all);
 // dinv[j] = 1 / D[j] for every column (n entries), after a factorization
 template <typename T> void launch_dinv(hipStream_t st, int ns, const FrontPlan& fp, const void* P, void* dinv);
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,

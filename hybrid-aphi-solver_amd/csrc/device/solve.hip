@@ -329,38 +329,80 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_diag(const int* __restrict__ no
     });
 }
 
-// W[i] -= L[i, B:B+nb] y_B for the 16 rows i0.. below the block; wave w takes the
-// block columns [64 w, 64 w + 64)
+// Rows of a forward update: the triangle rows [B + nb, k) below block B (RECT
+// false: the block chain), or the rows [k, m) below the front's pivots with the
+// blocks [ks bps, (ks + 1) bps) of the front (RECT true: one launch per level after
+// its chain, so that the bulk of L streams in large launches instead of one small
+// launch per block; the front's blocks split ksplit ways so that a level of few
+// fronts still has enough workgroups)
+struct FwdRows {
+    int lo, hi, b0, b1;  // rows [lo, hi), columns [b0, b1) in blocks of kSB
+};
+
+template <bool RECT>
+__device__ __forceinline__ bool fwd_rows(const Sn& x, int B, int ks, int bps, FwdRows& r) {
+    if constexpr (RECT) {
+        r = FwdRows{x.k, x.m, ks * bps * kSB, min(x.k, (ks + 1) * bps * kSB)};
+        if (r.b0 >= r.b1) return false;
+    } else {
+        if (x.k <= B) return false;
+        r = FwdRows{B + min(kSB, x.k - B), x.k, B, B + 1};
+    }
+    return r.lo < r.hi;
+}
+
+// Where a forward update's sums go: W -= sum directly (one split), or the split's
+// partial sums part[((f * ksplit + ks) * G + g) * prow + (i - k)][q], subtracted by
+// k_fwd_rect_reduce in split order
 template <typename T>
+struct FwdOut {
+    C<T>* part;
+    int ksplit, bps, prow;
+};
+
+// W[i] -= L[i, blocks] y_blocks for the 16 rows i0.. of the update rows; wave w
+// takes the columns [64 w, 64 w + 64) of each block
+template <typename T, bool RECT>
 __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ nodes, int B, FrontPlan fp,
                                                     const C<T>* __restrict__ P, C<T>* W, const C<T>* __restrict__ Y,
-                                                    int n, int64_t R) {
+                                                    int n, int64_t R, FwdOut<T> fo) {
     __shared__ C<T> part[4][256];
     const int s = nodes[blockIdx.y];
-    const int g = static_cast<int>(blockIdx.z);
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
-    if (x.k <= B) return;
-    const int nb = min(kSB, x.k - B);
-    const int i0 = B + nb + static_cast<int>(blockIdx.x) * 16;
-    if (i0 >= x.m) return;
+    const int tiles = static_cast<int>(gridDim.x) / fo.ksplit;
+    const int ks = static_cast<int>(blockIdx.x) / tiles;
+    FwdRows rr;
+    if (!fwd_rows<RECT>(x, B, ks, fo.bps, rr)) return;
+    const int i0 = rr.lo + (static_cast<int>(blockIdx.x) % tiles) * 16;
+    if (i0 >= rr.hi) return;
     const int k0 = 64 * (static_cast<int>(threadIdx.x) / 64);
     const C<T>* Ps = P + fp.panel_off[s];
-    const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
     V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-    if (k0 < nb)
-        cmma64<T>(accr, acci,
-                  [&](int p, int k) {
-                      return Ps[static_cast<int64_t>(B + min(k0 + k, nb - 1)) * x.m + min(i0 + p, x.m - 1)];
-                  },
-                  [&](int k, int q) {
-                      const C<T> v = Yg[min(k0 + k, nb - 1) * kQ + q];  // clamped read, then select
-                      return k0 + k < nb ? v : C<T>{0, 0};
-                  });
+    for (int Bb = rr.b0; Bb < rr.b1; Bb += kSB) {
+        const int nb = min(kSB, x.k - Bb);
+        const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + Bb) * kQ;
+        if (k0 < nb)
+            cmma64<T>(accr, acci,
+                      [&](int p, int k) {
+                          return Ps[static_cast<int64_t>(Bb + min(k0 + k, nb - 1)) * x.m + min(i0 + p, x.m - 1)];
+                      },
+                      [&](int k, int q) {
+                          const C<T> v = Yg[min(k0 + k, nb - 1) * kQ + q];  // clamped read, then select
+                          return k0 + k < nb ? v : C<T>{0, 0};
+                      });
+    }
     C<T>* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
+    C<T>* Pg = fo.ksplit > 1 ? fo.part + ((static_cast<int64_t>(blockIdx.y) * fo.ksplit + ks) * G + g) * fo.prow * kQ
+                             : nullptr;  // rows i - k
     reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
-        if (i0 + p < x.m) {
-            C<T>& dst = Wg[static_cast<int64_t>(i0 + p) * kQ + q];
-            dst = gsub(dst, t);
+        if (i0 + p < rr.hi) {
+            if (Pg) {
+                Pg[static_cast<int64_t>(i0 + p - x.k) * kQ + q] = t;
+            } else {
+                C<T>& dst = Wg[static_cast<int64_t>(i0 + p) * kQ + q];
+                dst = gsub(dst, t);
+            }
         }
     });
 }
@@ -369,58 +411,95 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
 // rows i0 + 2p and i0 + 2p + 1 of a column with one 16-byte load (each load
 // instruction covers 256 contiguous bytes of four columns, twice the run of the
 // one-row-per-lane mapping), and the two rows go to two 16-row MFMA tiles (even
-// rows, odd rows). Each output sums the same products, grouped and ordered as
-// in k_fwd_update. NW = 1: blocks of at most 64 columns, each lane updating its
-// own eight entries.
-template <int NW = 4>
+// rows, odd rows). A wave's operands for its slice of block b + 1 are loaded
+// before it multiplies those of block b, and a slice narrower than 64 columns
+// issues only the steps its columns need. NW = 1: blocks of at most 64 columns,
+// each lane updating its own eight entries.
+template <int NW, bool RECT>
 __global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__ nodes, int B, FrontPlan fp,
                                                          const cf* __restrict__ P, cf* W, const cf* __restrict__ Y, int n,
-                                                         int64_t R) {
+                                                         int64_t R, FwdOut<float> fo) {
     using VF = V4<float>;
+    typedef float f4 __attribute__((ext_vector_type(4)));
     __shared__ cf part[2][NW][256];
     const int s = nodes[blockIdx.y];
-    const int g = static_cast<int>(blockIdx.z);
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
-    if (x.k <= B) return;
-    const int nb = min(kSB, x.k - B);
-    const int i0 = B + nb + static_cast<int>(blockIdx.x) * 32;
-    if (i0 >= x.m) return;
+    const int tiles = static_cast<int>(gridDim.x) / fo.ksplit;
+    const int ks = static_cast<int>(blockIdx.x) / tiles;
+    FwdRows rr;
+    if (!fwd_rows<RECT>(x, B, ks, fo.bps, rr)) return;
+    const int i0 = rr.lo + (static_cast<int>(blockIdx.x) % tiles) * 32;
+    if (i0 >= rr.hi) return;
     const int tid = static_cast<int>(threadIdx.x), wave = tid / 64, lane = tid % 64;
     const int p = lane % 16, kq = lane / 16, k0 = 64 * wave;
     const cf* Ps = P + fp.panel_off[s];
-    const cf* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
+    const cf* Yg = Y + (static_cast<int64_t>(g) * n + x.first) * kQ;
     // three-product complex form (k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi);
     // re = k1 - k3, im = k1 + k2): three MFMAs per complex product instead of four
     VF s1a = VF{0, 0, 0, 0}, s2a = VF{0, 0, 0, 0}, s3a = VF{0, 0, 0, 0};
     VF s1b = VF{0, 0, 0, 0}, s2b = VF{0, 0, 0, 0}, s3b = VF{0, 0, 0, 0};
-    if (k0 < nb) {
-        typedef float f4 __attribute__((ext_vector_type(4)));
-        const int row = min(i0 + 2 * p, x.m - 1);  // rows row and row + 1 (the second may lie past m: never stored)
-        f4 a[16];
-        cf b[16];
+    const int row = min(i0 + 2 * p, x.m - 1);  // rows row and row + 1 (the second may lie past m: never stored)
+    // The wave's columns in stages of 32: stage j is the k steps 8 (j % 2) ..
+    // 8 (j % 2) + 7 of its 64-column slice of block b0 + (j / 2) kSB; steps(j) is
+    // how many of them lie inside the block (uniform over the wave)
+    const int nstage = 2 * ((rr.b1 - rr.b0 + kSB - 1) / kSB);
+    const auto steps = [&](int j) {
+        if (j >= nstage) return 0;
+        const int Bb = rr.b0 + (j / 2) * kSB, u0 = 8 * (j % 2);
+        return max(0, min(8, (min(kSB, x.k - Bb) - k0 - 4 * u0 + 3) / 4));
+    };
+    // Four-wave workgroups load every step of a stage (clamped columns past the
+    // block or past the last stage), so that the loads in flight are known at
+    // compile time and a stage waits only for its own; one-wave workgroups (blocks
+    // of at most 64 columns) load only the steps they use.
+    const auto load = [&](int j, int nu, f4 (&a)[8], cf (&b)[8]) {
+        if constexpr (NW > 1) j = min(j, nstage - 1);
+        const int Bb = rr.b0 + (j / 2) * kSB, kb = k0 + 32 * (j % 2);
+        const int last = min(kSB, x.k - Bb) - 1;
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            a[u] = *reinterpret_cast<const f4*>(Ps + static_cast<int64_t>(B + min(k0 + 4 * u + kq, nb - 1)) * x.m + row);
+        for (int u = 0; u < 8; ++u)
+            if (NW > 1 || u < nu)
+                a[u] = *reinterpret_cast<const f4*>(Ps + static_cast<int64_t>(Bb + min(kb + 4 * u + kq, last)) * x.m + row);
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const int k = k0 + 4 * u + kq;
-            const cf v = Yg[min(k, nb - 1) * kQ + p];  // clamped read, then select
-            b[u] = k < nb ? v : cf{0, 0};
-        }
-        __builtin_amdgcn_sched_barrier(0);  // every operand load in flight before the first MFMA
+        for (int u = 0; u < 8; ++u)
+            if (NW > 1 || u < nu) {
+                const int k = kb + 4 * u + kq;
+                const cf v = Yg[(Bb + min(k, last)) * kQ + p];  // clamped read, then select
+                b[u] = k <= last ? v : cf{0, 0};
+            }
+    };
+    const auto mma = [&](int nu, const f4 (&a)[8], const cf (&b)[8]) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            const float bd = b[u].im - b[u].re, bs = b[u].re + b[u].im;
-            s1a = Mfma<float>::mac(a[u].x + a[u].y, b[u].re, s1a);
-            s2a = Mfma<float>::mac(a[u].x, bd, s2a);
-            s3a = Mfma<float>::mac(a[u].y, bs, s3a);
-            s1b = Mfma<float>::mac(a[u].z + a[u].w, b[u].re, s1b);
-            s2b = Mfma<float>::mac(a[u].z, bd, s2b);
-            s3b = Mfma<float>::mac(a[u].w, bs, s3b);
-        }
+        for (int u = 0; u < 8; ++u)
+            if (u < nu) {
+                const float bd = b[u].im - b[u].re, bs = b[u].re + b[u].im;
+                s1a = Mfma<float>::mac(a[u].x + a[u].y, b[u].re, s1a);
+                s2a = Mfma<float>::mac(a[u].x, bd, s2a);
+                s3a = Mfma<float>::mac(a[u].y, bs, s3a);
+                s1b = Mfma<float>::mac(a[u].z + a[u].w, b[u].re, s1b);
+                s2b = Mfma<float>::mac(a[u].z, bd, s2b);
+                s3b = Mfma<float>::mac(a[u].w, bs, s3b);
+            }
+    };
+    f4 a0[8], a1[8];
+    cf b0[8], b1[8];
+    int nu0 = steps(0);
+    load(0, nu0, a0, b0);
+    for (int j = 0; j < nstage; j += 2) {
+        const int nu1 = steps(j + 1);
+        load(j + 1, nu1, a1, b1);
+        __builtin_amdgcn_sched_barrier(0);  // the next stage's loads in flight before these MFMAs
+        mma(nu0, a0, b0);
+        nu0 = steps(j + 2);
+        load(j + 2, nu0, a0, b0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(nu1, a1, b1);
     }
     const VF r0 = s1a - s3a, i0a = s1a + s2a, r1 = s1b - s3b, i1a = s1b + s2b;
     cf* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
+    cf* Pg = fo.ksplit > 1 ? fo.part + ((static_cast<int64_t>(blockIdx.y) * fo.ksplit + ks) * G + g) * fo.prow * kQ
+                           : nullptr;  // rows i - k
     if constexpr (NW == 1) {
         // tile row r of tile h is front row i0 + 2 r + h
 #pragma unroll
@@ -428,9 +507,14 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int i = i0 + 2 * Mfma<float>::row(lane, v) + h;
-                if (i < x.m) {
-                    cf& dst = Wg[static_cast<int64_t>(i) * kQ + p];
-                    dst = gsub(dst, h ? cf{r1[v], i1a[v]} : cf{r0[v], i0a[v]});
+                if (i < rr.hi) {
+                    const cf t = h ? cf{r1[v], i1a[v]} : cf{r0[v], i0a[v]};
+                    if (Pg) {
+                        Pg[static_cast<int64_t>(i - x.k) * kQ + p] = t;
+                    } else {
+                        cf& dst = Wg[static_cast<int64_t>(i) * kQ + p];
+                        dst = gsub(dst, t);
+                    }
                 }
             }
     } else {
@@ -447,170 +531,268 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__
             t = gadd(t, part[h][2][tid]);
             t = gadd(t, part[h][3][tid]);
             const int i = i0 + 2 * (tid / 16) + h;  // tile row tid / 16 of tile h is front row i0 + 2 (tid / 16) + h
-            if (i < x.m) {
-                cf& dst = Wg[static_cast<int64_t>(i) * kQ + tid % 16];
-                dst = gsub(dst, t);
+            if (i < rr.hi) {
+                if (Pg) {
+                    Pg[static_cast<int64_t>(i - x.k) * kQ + tid % 16] = t;
+                } else {
+                    cf& dst = Wg[static_cast<int64_t>(i) * kQ + tid % 16];
+                    dst = gsub(dst, t);
+                }
             }
         }
     }
 }
 
+// W[k + r] -= the forward splits' partial sums of row r, in split order; one thread
+// per (row, right-hand side)
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_rect_reduce(const int* __restrict__ nodes, FrontPlan fp, C<T>* W,
+                                                         int64_t R, FwdOut<T> fo) {
+    const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
+    const Sn x = sn(fp, s);
+    const int idx = static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x);
+    const int r = idx / kQ, q = idx % kQ;
+    if (r >= x.m - x.k) return;
+    const int nsplit = ((x.k + kSB - 1) / kSB + fo.bps - 1) / fo.bps;  // splits that hold blocks of this front
+    const C<T>* pp = fo.part + (static_cast<int64_t>(blockIdx.y) * fo.ksplit * G + g) * fo.prow * kQ +
+                     static_cast<int64_t>(r) * kQ + q;
+    const int64_t sstride = static_cast<int64_t>(G) * fo.prow * kQ;
+    C<T> t = pp[0];
+    for (int ks = 1; ks < nsplit; ++ks) t = gadd(t, pp[ks * sstride]);
+    C<T>& dst = W[(static_cast<int64_t>(g) * R + x.rp + x.k + r) * kQ + q];
+    dst = gsub(dst, t);
+}
+
 // ------------------------------------------------------------------ backward steps
 //
-// Step B (blocks in decreasing order): t_B = L[rows below, B:B+nb]^T x[rows below]
-// in partial sums over kBwdRows rows, reduced in chunk order, then
-// x_B = X_B^T (D_B^-1 y_B - t_B).
+// Per level, first the sums over the rows below the pivots, t = L[k:m, 0:k]^T
+// x[rows k..m), which need only the ancestors' x: partial sums over kRectRows rows
+// per workgroup, reduced in chunk order into the level's t (column stride rk).
+// Then the block chain, blocks in decreasing order: x_B = X_B^T (D_B^-1 y_B - t_B),
+// and right after it the columns before the block take the rows of block B:
+// t[0:B] += L[B:B+nb, 0:B]^T x_B. Each entry of t therefore sums its rows' terms in
+// a fixed order: the rows below the pivots by chunk, then the blocks from the
+// last to its own.
 
-// partial[((f * stride + chunk) * G + g)][c][q] for the 16 block columns c0.. over
-// the rows [i0, i0 + kBwdRows) below the block; wave w takes 64 of the rows
-template <typename T>
+// Rows and columns of a backward product: the rows [k, m) below the pivots with
+// all k columns, in chunks of kRectRows rows (RECT), or the rows of block B with
+// the columns [0, B) before it, in one chunk (the chain's update)
+struct BwdRows {
+    int lo, hi, chunk_rows, ncol;
+};
+
+template <bool RECT>
+__device__ __forceinline__ bool bwd_rows(const Sn& x, int B, BwdRows& r) {
+    if constexpr (RECT) {
+        r = BwdRows{x.k, x.m, kRectRows, x.k};
+    } else {
+        if (x.k <= B) return false;
+        r = BwdRows{B, B + min(kSB, x.k - B), kSB, B};
+    }
+    return r.lo < r.hi && r.ncol > 0;
+}
+
+// out[c][q] for the 16 columns c0.. over the rows [i0, i0 + chunk_rows) of the
+// product's rows (wave w takes rows i0 + 64 w + 256 j): RECT writes
+// partial[((f * stride + chunk) * G + g)][c][q], the chain's update adds to
+// t[(f * G + g)][c][q] (column stride cstride both)
+template <typename T, bool RECT>
 __global__ void __launch_bounds__(256) k_bwd_partial(const int* __restrict__ nodes, int B, int stride, int ctiles,
-                                                     FrontPlan fp, const C<T>* __restrict__ P,
-                                                     const C<T>* __restrict__ X, int n, C<T>* __restrict__ partial) {
+                                                     int cstride, FrontPlan fp, const C<T>* __restrict__ P,
+                                                     const C<T>* __restrict__ X, int n, C<T>* __restrict__ out) {
     __shared__ C<T> part[4][256];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
-    if (x.k <= B) return;
-    const int nb = min(kSB, x.k - B);
+    BwdRows rr;
+    if (!bwd_rows<RECT>(x, B, rr)) return;
     const int chunk = static_cast<int>(blockIdx.x) / ctiles, c0 = (static_cast<int>(blockIdx.x) % ctiles) * 16;
-    const int i0 = B + nb + chunk * kBwdRows;
-    if (i0 >= x.m || c0 >= nb) return;
-    const int iw = i0 + 64 * (static_cast<int>(threadIdx.x) / 64);
+    const int i0 = rr.lo + chunk * rr.chunk_rows;
+    if (i0 >= rr.hi || c0 >= rr.ncol) return;
+    const int iend = min(i0 + rr.chunk_rows, rr.hi);
     const C<T>* Ps = P + fp.panel_off[s];
     const int* rows = fp.rows + x.rp;
     const C<T>* Xg = X + static_cast<int64_t>(g) * n * kQ;
+    const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
+    const C<T>* col = Ps + static_cast<int64_t>(min(c0 + p, rr.ncol - 1)) * x.m;
     V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-    if (iw < x.m) {
+    for (int iw = i0 + 64 * (static_cast<int>(threadIdx.x) / 64); iw < iend; iw += 256) {
         // the row indices first (one wait), then every operand load before the MFMAs
-        const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
         int gr[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) gr[u] = rows[min(iw + 4 * u + kq, x.m - 1)];
         C<T> a[16], b[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
-            a[u] = Ps[static_cast<int64_t>(B + min(c0 + p, nb - 1)) * x.m + min(iw + 4 * u + kq, x.m - 1)];
+        for (int u = 0; u < 16; ++u) a[u] = col[min(iw + 4 * u + kq, x.m - 1)];
 #pragma unroll
         for (int u = 0; u < 16; ++u) b[u] = Xg[static_cast<int64_t>(gr[u]) * kQ + p];
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const C<T> bb = iw + 4 * u + kq < x.m ? b[u] : C<T>{0, 0};
+            const C<T> bb = iw + 4 * u + kq < iend ? b[u] : C<T>{0, 0};
             accr = Mfma<T>::mac(a[u].re, bb.re, accr);
             accr = Mfma<T>::mac(-a[u].im, bb.im, accr);
             acci = Mfma<T>::mac(a[u].re, bb.im, acci);
             acci = Mfma<T>::mac(a[u].im, bb.re, acci);
         }
     }
-    C<T>* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
-    reduce4<T>(part, accr, acci, [&](int p, int q, C<T> t) {
-        if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;
-    });
+    if constexpr (RECT) {
+        C<T>* o = out + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * cstride * kQ;
+        reduce4<T>(part, accr, acci, [&](int pp, int q, C<T> t) {
+            if (c0 + pp < rr.ncol) o[(c0 + pp) * kQ + q] = t;
+        });
+    } else {
+        C<T>* o = out + (static_cast<int64_t>(blockIdx.y) * G + g) * cstride * kQ;
+        reduce4<T>(part, accr, acci, [&](int pp, int q, C<T> t) {
+            if (c0 + pp < rr.ncol) o[(c0 + pp) * kQ + q] = gadd(o[(c0 + pp) * kQ + q], t);
+        });
+    }
 }
 
-// The same partial sums for complex64 with one 16-byte load per lane for two
+// The same products for complex64 with one 16-byte load per lane for two
 // adjacent rows of a column (eight contiguous rows of each of 16 columns per load
 // instruction instead of four): within each group of eight rows, the even rows
-// form one MFMA k-group and the odd rows the next. NW = 1: at most 64 rows below
-// the block (one chunk, one wave).
-template <int NW = 4>
+// form one MFMA k-group and the odd rows the next. The chunk's row indices are
+// staged in LDS first, so that the x loads they feed do not wait on the memory
+// counter, and each wave loads its next 32 rows' operands before it multiplies the
+// current ones. NW = 1: one wave walks the chunk 64 rows at a time (levels whose
+// fronts have at most 64 such rows).
+template <int NW, bool RECT>
 __global__ void __launch_bounds__(64 * NW) k_bwd_partial2(const int* __restrict__ nodes, int B, int stride, int ctiles,
-                                                          FrontPlan fp, const cf* __restrict__ P, const cf* __restrict__ X,
-                                                          int n, cf* __restrict__ partial) {
+                                                          int cstride, FrontPlan fp, const cf* __restrict__ P,
+                                                          const cf* __restrict__ X, int n, cf* __restrict__ out) {
     using VF = V4<float>;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int kIdx = RECT ? kRectRows : kSB;  // rows of a chunk
     __shared__ cf part[NW][256];
+    __shared__ int sidx[kIdx];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
-    if (x.k <= B) return;
-    const int nb = min(kSB, x.k - B);
+    BwdRows rr;
+    if (!bwd_rows<RECT>(x, B, rr)) return;
     const int chunk = static_cast<int>(blockIdx.x) / ctiles, c0 = (static_cast<int>(blockIdx.x) % ctiles) * 16;
-    const int i0 = B + nb + chunk * kBwdRows;
-    if (i0 >= x.m || c0 >= nb) return;
-    const int iw = i0 + 64 * (static_cast<int>(threadIdx.x) / 64);
+    const int i0 = rr.lo + chunk * rr.chunk_rows;
+    if (i0 >= rr.hi || c0 >= rr.ncol) return;
+    const int iend = min(i0 + rr.chunk_rows, rr.hi);
     const cf* Ps = P + fp.panel_off[s];
     const int* rows = fp.rows + x.rp;
     const cf* Xg = X + static_cast<int64_t>(g) * n * kQ;
+    const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
+    const cf* col = Ps + static_cast<int64_t>(min(c0 + p, rr.ncol - 1)) * x.m;
+    // the chunk's row indices (positions past its end repeat the last one)
+    for (int i = static_cast<int>(threadIdx.x); i < kIdx; i += 64 * NW) sidx[i] = rows[min(i0 + i, iend - 1)];
+    __syncthreads();
     // three-product complex form, as in k_fwd_update2
     VF s1 = VF{0, 0, 0, 0}, s2 = VF{0, 0, 0, 0}, s3 = VF{0, 0, 0, 0};
-    if (iw < x.m) {
-        typedef float f4 __attribute__((ext_vector_type(4)));
-        const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
-        const cf* col = Ps + static_cast<int64_t>(B + min(c0 + p, nb - 1)) * x.m;
-        // rows iw + 8u + 2kq and the next one; the row indices first (one wait)
-        int gr[16];
+    // The wave's rows in stages of 32: rows iw .. iw + 31 (four groups of eight);
+    // steps(iw) is how many groups hold rows of the product (uniform over the
+    // wave). Four-wave workgroups load every group (clamped rows), so that the loads
+    // in flight are known at compile time; one-wave workgroups load only the
+    // groups they use.
+    const auto steps = [&](int iw) { return iw < iend ? min(4, (iend - iw + 7) / 8) : 0; };
+    const auto load = [&](int iw, int nu, f4 (&a)[4], cf (&b)[8]) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int r = iw + 8 * u + 2 * kq;
-            gr[2 * u] = rows[min(r, x.m - 1)];
-            gr[2 * u + 1] = rows[min(r + 1, x.m - 1)];
-        }
-        f4 a[8];
-        cf b[16];
+        for (int u = 0; u < 4; ++u)  // the second row of the pair may lie past the rows: its product is masked
+            if (NW > 1 || u < nu) a[u] = *reinterpret_cast<const f4*>(col + min(iw + 8 * u + 2 * kq, iend - 1));
 #pragma unroll
-        for (int u = 0; u < 8; ++u)  // the second row of the pair may lie past m: its product is masked
-            a[u] = *reinterpret_cast<const f4*>(col + min(iw + 8 * u + 2 * kq, x.m - 1));
+        for (int u = 0; u < 4; ++u)
+            if (NW > 1 || u < nu) {
+                const int r = min(iw + 8 * u + 2 * kq - i0, kIdx - 2);
+                b[2 * u] = Xg[static_cast<int64_t>(sidx[r]) * kQ + p];
+                b[2 * u + 1] = Xg[static_cast<int64_t>(sidx[r + 1]) * kQ + p];
+            }
+    };
+    const auto mma = [&](int iw, int nu, const f4 (&a)[4], const cf (&b)[8]) {
 #pragma unroll
-        for (int u = 0; u < 16; ++u) b[u] = Xg[static_cast<int64_t>(gr[u]) * kQ + p];
+        for (int u = 0; u < 4; ++u)
+            if (u < nu) {
+                const int r = iw + 8 * u + 2 * kq;
+                const cf b0 = r < iend ? b[2 * u] : cf{0, 0};
+                const cf b1 = r + 1 < iend ? b[2 * u + 1] : cf{0, 0};
+                s1 = Mfma<float>::mac(a[u].x + a[u].y, b0.re, s1);
+                s2 = Mfma<float>::mac(a[u].x, b0.im - b0.re, s2);
+                s3 = Mfma<float>::mac(a[u].y, b0.re + b0.im, s3);
+                s1 = Mfma<float>::mac(a[u].z + a[u].w, b1.re, s1);
+                s2 = Mfma<float>::mac(a[u].z, b1.im - b1.re, s2);
+                s3 = Mfma<float>::mac(a[u].w, b1.re + b1.im, s3);
+            }
+    };
+    // stage s of a wave: rows i0 + 64 w + 256 (s / 2) * NW / 4 ... in order: the
+    // wave's 64 rows of each pass, first half then second half
+    const int w = static_cast<int>(threadIdx.x) / 64;
+    const auto stage_row = [&](int st) { return i0 + 64 * w + 64 * NW * (st / 2) + 32 * (st % 2); };
+    f4 a0[4], a1[4];
+    cf b0[8], b1[8];
+    int st = 0;
+    int nu0 = steps(stage_row(0));
+    load(stage_row(0), nu0, a0, b0);
+    for (; stage_row(st) < iend; st += 2) {
+        const int r0 = stage_row(st), r1 = stage_row(st + 1), r2 = stage_row(st + 2);
+        const int nu1 = steps(r1);
+        load(r1, nu1, a1, b1);
+        __builtin_amdgcn_sched_barrier(0);  // the next rows' loads in flight before these MFMAs
+        mma(r0, nu0, a0, b0);
+        nu0 = steps(r2);
+        load(r2, nu0, a0, b0);
         __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int r = iw + 8 * u + 2 * kq;
-            const cf b0 = r < x.m ? b[2 * u] : cf{0, 0};
-            const cf b1 = r + 1 < x.m ? b[2 * u + 1] : cf{0, 0};
-            s1 = Mfma<float>::mac(a[u].x + a[u].y, b0.re, s1);
-            s2 = Mfma<float>::mac(a[u].x, b0.im - b0.re, s2);
-            s3 = Mfma<float>::mac(a[u].y, b0.re + b0.im, s3);
-            s1 = Mfma<float>::mac(a[u].z + a[u].w, b1.re, s1);
-            s2 = Mfma<float>::mac(a[u].z, b1.im - b1.re, s2);
-            s3 = Mfma<float>::mac(a[u].w, b1.re + b1.im, s3);
-        }
+        mma(r1, nu1, a1, b1);
     }
     const VF accr = s1 - s3, acci = s1 + s2;
-    cf* out = partial + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * kSB * kQ;
-    reduce_waves<float, NW>(part, accr, acci, [&](int p, int q, cf t) {
-        if (c0 + p < nb) out[(c0 + p) * kQ + q] = t;
-    });
+    if constexpr (RECT) {
+        cf* o = out + ((static_cast<int64_t>(blockIdx.y) * stride + chunk) * G + g) * cstride * kQ;
+        reduce_waves<float, NW>(part, accr, acci, [&](int pp, int q, cf t) {
+            if (c0 + pp < rr.ncol) o[(c0 + pp) * kQ + q] = t;
+        });
+    } else {
+        cf* o = out + (static_cast<int64_t>(blockIdx.y) * G + g) * cstride * kQ;
+        reduce_waves<float, NW>(part, accr, acci, [&](int pp, int q, cf t) {
+            if (c0 + pp < rr.ncol) o[(c0 + pp) * kQ + q] = gadd(o[(c0 + pp) * kQ + q], t);
+        });
+    }
 }
 
-// tsum[(f * G + g)][c][q] = sum over chunks of partial, one block column c per
-// workgroup: 16 summers per right-hand side, combined in a fixed order
+// t[(f * G + g)][c][q] (column stride cstride) = sum over the chunks of the rows
+// below the pivots, for every column c < k (zero for a front without such rows),
+// one column per workgroup: 16 summers per right-hand side, combined in a fixed
+// order
 template <typename T>
-__global__ void __launch_bounds__(256) k_bwd_reduce(const int* __restrict__ nodes, int B, int stride, FrontPlan fp,
-                                                    const C<T>* __restrict__ partial, C<T>* __restrict__ tsum) {
+__global__ void __launch_bounds__(256) k_rect_reduce(const int* __restrict__ nodes, int stride, int cstride,
+                                                     FrontPlan fp, const C<T>* __restrict__ partial,
+                                                     C<T>* __restrict__ t) {
     __shared__ C<T> red[16][17];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
     const Sn x = sn(fp, s);
-    if (x.k <= B) return;
-    const int nb = min(kSB, x.k - B);
-    const int nch = x.m > B + nb ? (x.m - B - nb + kBwdRows - 1) / kBwdRows : 0;
     const int c = static_cast<int>(blockIdx.x);
-    if (c >= nb) return;
+    if (c >= x.k) return;
+    const int nch = x.m > x.k ? (x.m - x.k + kRectRows - 1) / kRectRows : 0;
     const int tid = threadIdx.x, q = tid % 16, p = tid / 16;
-    const int64_t cstride = static_cast<int64_t>(G) * kSB * kQ;
-    const C<T>* pp = partial + (static_cast<int64_t>(blockIdx.y) * stride * G + g) * kSB * kQ + c * kQ + q;
-    C<T> t = C<T>{0, 0};
+    const int64_t chstride = static_cast<int64_t>(G) * cstride * kQ;
+    const C<T>* pp = partial + (static_cast<int64_t>(blockIdx.y) * stride * G + g) * cstride * kQ + c * kQ + q;
+    C<T> v = C<T>{0, 0};
 #pragma unroll 8
-    for (int ch = p; ch < nch; ch += 16) t = gadd(t, pp[ch * cstride]);
-    red[p][q] = t;
+    for (int ch = p; ch < nch; ch += 16) v = gadd(v, pp[ch * chstride]);
+    red[p][q] = v;
     __syncthreads();
     if (p == 0) {
-        C<T> v = red[0][q];
-        for (int j = 1; j < 16; ++j) v = gadd(v, red[j][q]);
-        tsum[((static_cast<int64_t>(blockIdx.y) * G + g) * kSB + c) * kQ + q] = v;
+        C<T> w = red[0][q];
+        for (int j = 1; j < 16; ++j) w = gadd(w, red[j][q]);
+        t[((static_cast<int64_t>(blockIdx.y) * G + g) * cstride + c) * kQ + q] = w;
     }
 }
 
 // x rows [16 bx, 16 bx + 16) of the block: x[c] = sum_{k >= c} X[k][c] z[k],
-// z = y D^-1 - t (D^-1 from k_dinv); wave w takes k in [c0 + 64 w, c0 + 64 w + 64)
-// (NW = 1: blocks of at most 64 columns)
+// z = y D^-1 - t[B + k] (t column stride rk; read for every front when all is set,
+// else only for fronts with rows below their pivots, zero for the others or when t
+// is null); D^-1 from k_dinv; wave w takes k in [c0 + 64 w, c0 + 64 w + 64) (NW = 1:
+// blocks of at most 64 columns)
 template <typename T, int NW = 4>
 __global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
                                                       const C<T>* __restrict__ dinv, const C<T>* __restrict__ XT256,
-                                                      const C<T>* __restrict__ Y, const C<T>* __restrict__ tsum,
-                                                      C<T>* __restrict__ X, int n) {
+                                                      const C<T>* __restrict__ Y, const C<T>* __restrict__ tl, int rk,
+                                                      int all, C<T>* __restrict__ X, int n) {
     __shared__ C<T> part[NW][256];
     const int s = nodes[blockIdx.y];
     const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
@@ -620,11 +802,11 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ no
     const int c0 = static_cast<int>(blockIdx.x) * 16;
     if (c0 >= nb) return;
     const int k0 = c0 + 64 * (static_cast<int>(threadIdx.x) / 64);
-    const bool below = x.m > B + nb;
+    const bool has_t = tl != nullptr && (all || x.m > x.k);
     const C<T>* dv = dinv + x.first + B;
     const C<T>* XTs = XT256 + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
     const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
-    const C<T>* tg = tsum + (static_cast<int64_t>(blockIdx.y) * G + g) * kSB * kQ;
+    const C<T>* tg = has_t ? tl + ((static_cast<int64_t>(blockIdx.y) * G + g) * rk + B) * kQ : nullptr;
     V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
     if (k0 < nb) {
         // every load (clamped indices) issued before the first wait, then z, then the MFMAs
@@ -636,7 +818,7 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ no
             a[u] = XTs[static_cast<int64_t>(kk) * nb + min(c0 + p, nb - 1)];
             yv[u] = Yg[kk * kQ + p];
             dw[u] = dv[kk];
-            tv[u] = below ? tg[kk * kQ + p] : C<T>{0, 0};
+            tv[u] = has_t ? tg[kk * kQ + p] : C<T>{0, 0};
         }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -672,7 +854,6 @@ __global__ void k_colscale_final(int nrhs, int nparts, const double* __restrict_
 // ------------------------------------------------------------------ launchers
 
 int rhs_groups(int nrhs) { return (nrhs + kQ - 1) / kQ; }
-int bwd_chunks(int rows_below) { return std::max(1, (rows_below + kBwdRows - 1) / kBwdRows); }
 
 template <typename T>
 void launch_perm_in(hipStream_t st, int n, int nrhs, const int* perm, const double* d, const cd* b, int64_t ldb,
@@ -692,8 +873,9 @@ template <typename T>
 void launch_fwd_gather(hipStream_t st, const int* nodes, int count, int max_m, int nrhs, const FrontPlan& fp,
                        const void* b, int n, int64_t R, void* W) {
     if (count <= 0 || max_m <= 0) return;
-    const size_t lds = static_cast<size_t>(kGatherRows) * kQ * sizeof(C<T>);
-    allow_large_lds(k_fwd_gather<T>, lds);
+    // fronts of at most kGatherRows rows use only their own rows of the chunk
+    const size_t lds = static_cast<size_t>(std::min(kGatherRows, max_m)) * kQ * sizeof(C<T>);
+    allow_large_lds(k_fwd_gather<T>, static_cast<size_t>(kGatherRows) * kQ * sizeof(C<T>));
     hipLaunchKernelGGL(k_fwd_gather<T>, dim3((max_m + kGatherRows - 1) / kGatherRows, count, rhs_groups(nrhs)),
                        dim3(256), lds, st, nodes, fp, static_cast<const C<T>*>(b), n, R, static_cast<C<T>*>(W));
 }
@@ -710,7 +892,7 @@ void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const Fron
 }
 
 template <typename T>
-void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
+void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_tri_rows, int nrhs,
                      const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R) {
     if (count <= 0 || max_nb <= 0) return;
     const int G = rhs_groups(nrhs);
@@ -722,60 +904,126 @@ void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max
     else
         hipLaunchKernelGGL((k_fwd_diag<T, 4>), dim3((max_nb + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
                            static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
-    if (max_rows_below > 0) {
+    if (max_tri_rows > 0) {
         if constexpr (std::is_same<T, float>::value) {
             if (one)
-                hipLaunchKernelGGL(k_fwd_update2<1>, dim3((max_rows_below + 31) / 32, count, G), dim3(64), 0, st, nodes,
-                                   B, fp, static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y), n, R);
+                hipLaunchKernelGGL((k_fwd_update2<1, false>), dim3((max_tri_rows + 31) / 32, count, G), dim3(64), 0, st,
+                                   nodes, B, fp, static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y),
+                                   n, R, FwdOut<float>{nullptr, 1, 1, 0});
             else
-                hipLaunchKernelGGL(k_fwd_update2<4>, dim3((max_rows_below + 31) / 32, count, G), dim3(256), 0, st, nodes,
-                                   B, fp, static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y), n, R);
+                hipLaunchKernelGGL((k_fwd_update2<4, false>), dim3((max_tri_rows + 31) / 32, count, G), dim3(256), 0, st,
+                                   nodes, B, fp, static_cast<const cf*>(P), static_cast<cf*>(W), static_cast<const cf*>(Y),
+                                   n, R, FwdOut<float>{nullptr, 1, 1, 0});
         } else
-            hipLaunchKernelGGL(k_fwd_update<T>, dim3((max_rows_below + 15) / 16, count, G), dim3(256), 0, st, nodes, B,
-                               fp, static_cast<const C<T>*>(P), static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n, R);
+            hipLaunchKernelGGL((k_fwd_update<T, false>), dim3((max_tri_rows + 15) / 16, count, G), dim3(256), 0, st, nodes,
+                               B, fp, static_cast<const C<T>*>(P), static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n,
+                               R, FwdOut<T>{nullptr, 1, 1, 0});
     }
 }
 
+// Splits of a level's forward rectangle: enough workgroups to fill the device
+// (about kFwdRectWgs) from row tiles x column splits, each split whole blocks
+int fwd_rect_split(int count, int max_k, int max_rect_rows, bool fp32, int* bps) {
+    const int nbk = (std::max(max_k, 1) + kSB - 1) / kSB;
+    const int rows_per_wg = fp32 ? 32 : 16;
+    const int64_t wgs = static_cast<int64_t>(count) * ((std::max(max_rect_rows, 1) + rows_per_wg - 1) / rows_per_wg);
+    int ks = static_cast<int>(std::min<int64_t>(nbk, std::max<int64_t>(1, (kFwdRectWgs + wgs - 1) / wgs)));
+    const int b = (nbk + ks - 1) / ks;
+    if (bps) *bps = b;
+    return (nbk + b - 1) / b;
+}
+
 template <typename T>
-void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_rows_below, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* dinv, const void* XT256, const void* Y, void* X,
-                     int n, void* partial, void* tsum) {
+void launch_fwd_rect(hipStream_t st, const int* nodes, int count, int max_k, int max_rect_rows, int nrhs,
+                     const FrontPlan& fp, const void* P, void* W, const void* Y, int n, int64_t R, void* part) {
+    if (count <= 0 || max_k <= 0 || max_rect_rows <= 0) return;
+    const int G = rhs_groups(nrhs);
+    constexpr bool f32 = std::is_same<T, float>::value;
+    int bps = 1;
+    const int ks = fwd_rect_split(count, max_k, max_rect_rows, f32, &bps);
+    const FwdOut<T> fo{static_cast<C<T>*>(part), ks, bps, max_rect_rows};
+    const int tiles = (max_rect_rows + (f32 ? 31 : 15)) / (f32 ? 32 : 16);
+    const dim3 grid(tiles * ks, count, G);
+    if constexpr (f32) {
+        if (max_k <= 64)
+            hipLaunchKernelGGL((k_fwd_update2<1, true>), grid, dim3(64), 0, st, nodes, 0, fp, static_cast<const cf*>(P),
+                               static_cast<cf*>(W), static_cast<const cf*>(Y), n, R, fo);
+        else
+            hipLaunchKernelGGL((k_fwd_update2<4, true>), grid, dim3(256), 0, st, nodes, 0, fp, static_cast<const cf*>(P),
+                               static_cast<cf*>(W), static_cast<const cf*>(Y), n, R, fo);
+    } else {
+        hipLaunchKernelGGL((k_fwd_update<T, true>), grid, dim3(256), 0, st, nodes, 0, fp, static_cast<const C<T>*>(P),
+                           static_cast<C<T>*>(W), static_cast<const C<T>*>(Y), n, R, fo);
+    }
+    if (ks > 1)
+        hipLaunchKernelGGL(k_fwd_rect_reduce<T>, dim3((max_rect_rows * kQ + 255) / 256, count, G), dim3(256), 0, st,
+                           nodes, fp, static_cast<C<T>*>(W), R, fo);
+}
+
+int rect_chunks(int rect_rows) { return std::max(1, (rect_rows + kRectRows - 1) / kRectRows); }
+int rect_cols(int max_k) { return ((std::max(max_k, 1) + 15) / 16) * 16; }
+
+template <typename T>
+const void* launch_bwd_rect(hipStream_t st, const int* nodes, int count, int max_k, int max_rect_rows, int nrhs,
+                            bool all, const FrontPlan& fp, const void* P, const void* X, int n, void* rpart,
+                            void* t) {
+    if (count <= 0 || max_k <= 0) return nullptr;
+    if (max_rect_rows <= 0 && !all) return nullptr;
+    const int G = rhs_groups(nrhs);
+    const int ctiles = (max_k + 15) / 16, rk = rect_cols(max_k);
+    const int stride = rect_chunks(max_rect_rows);
+    if (max_rect_rows > 0) {
+        const dim3 grid(stride * ctiles, count, G);
+        if constexpr (std::is_same<T, float>::value) {
+            if (max_rect_rows <= 64)
+                hipLaunchKernelGGL((k_bwd_partial2<1, true>), grid, dim3(64), 0, st, nodes, 0, stride, ctiles, rk, fp,
+                                   static_cast<const cf*>(P), static_cast<const cf*>(X), n, static_cast<cf*>(rpart));
+            else
+                hipLaunchKernelGGL((k_bwd_partial2<4, true>), grid, dim3(256), 0, st, nodes, 0, stride, ctiles, rk, fp,
+                                   static_cast<const cf*>(P), static_cast<const cf*>(X), n, static_cast<cf*>(rpart));
+        } else {
+            hipLaunchKernelGGL((k_bwd_partial<T, true>), grid, dim3(256), 0, st, nodes, 0, stride, ctiles, rk, fp,
+                               static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(rpart));
+        }
+        // with one chunk of rows the partial sums are already t (same layout) for
+        // the fronts that have such rows, which is all a diagonal step reads when
+        // the chain has no updates
+        if (stride == 1 && !all) return rpart;
+    }
+    hipLaunchKernelGGL(k_rect_reduce<T>, dim3(max_k, count, G), dim3(256), 0, st, nodes, stride, rk, fp,
+                       static_cast<const C<T>*>(rpart), static_cast<C<T>*>(t));
+    return t;
+}
+
+template <typename T>
+void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int nrhs, const FrontPlan& fp,
+                     const void* P, const void* dinv, const void* XT256, const void* Y, void* X, int n, void* t, int rk,
+                     bool all) {
     if (count <= 0 || max_nb <= 0) return;
     const int G = rhs_groups(nrhs);
     const int ctiles = (max_nb + 15) / 16;
-    const int stride = bwd_chunks(max_rows_below);
-    // the sums t the diagonal step reads: with one chunk of rows the partial sums
-    // are already t (same layout), and the reduction is skipped
-    const void* t = tsum;
-    if (max_rows_below > 0) {
-        if constexpr (std::is_same<T, float>::value) {
-            if (max_rows_below <= 64)
-                hipLaunchKernelGGL(k_bwd_partial2<1>, dim3(stride * ctiles, count, G), dim3(64), 0, st, nodes, B, stride,
-                                   ctiles, fp, static_cast<const cf*>(P), static_cast<const cf*>(X), n,
-                                   static_cast<cf*>(partial));
-            else
-                hipLaunchKernelGGL(k_bwd_partial2<4>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride,
-                                   ctiles, fp, static_cast<const cf*>(P), static_cast<const cf*>(X), n,
-                                   static_cast<cf*>(partial));
-        } else {
-            hipLaunchKernelGGL(k_bwd_partial<T>, dim3(stride * ctiles, count, G), dim3(256), 0, st, nodes, B, stride,
-                               ctiles, fp, static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n,
-                               static_cast<C<T>*>(partial));
-        }
-        if (stride == 1)
-            t = partial;
-        else
-            hipLaunchKernelGGL(k_bwd_reduce<T>, dim3(max_nb, count, G), dim3(256), 0, st, nodes, B, stride, fp,
-                               static_cast<const C<T>*>(partial), static_cast<C<T>*>(tsum));
-    }
     if (max_nb <= 64)
         hipLaunchKernelGGL((k_bwd_diag<T, 1>), dim3(ctiles, count, G), dim3(64), 0, st, nodes, B, fp,
                            static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
-                           static_cast<const C<T>*>(t), static_cast<C<T>*>(X), n);
+                           static_cast<const C<T>*>(t), rk, all ? 1 : 0, static_cast<C<T>*>(X), n);
     else
         hipLaunchKernelGGL((k_bwd_diag<T, 4>), dim3(ctiles, count, G), dim3(256), 0, st, nodes, B, fp,
                            static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
-                           static_cast<const C<T>*>(t), static_cast<C<T>*>(X), n);
+                           static_cast<const C<T>*>(t), rk, all ? 1 : 0, static_cast<C<T>*>(X), n);
+    if (B == 0) return;
+    // the columns before the block take its rows: t[0:B] += L[B:B+nb, 0:B]^T x_B
+    const dim3 grid((B + 15) / 16, count, G);
+    if constexpr (std::is_same<T, float>::value) {
+        if (max_nb <= 64)
+            hipLaunchKernelGGL((k_bwd_partial2<1, false>), grid, dim3(64), 0, st, nodes, B, 1, (B + 15) / 16, rk, fp,
+                               static_cast<const cf*>(P), static_cast<const cf*>(X), n, static_cast<cf*>(t));
+        else
+            hipLaunchKernelGGL((k_bwd_partial2<4, false>), grid, dim3(256), 0, st, nodes, B, 1, (B + 15) / 16, rk, fp,
+                               static_cast<const cf*>(P), static_cast<const cf*>(X), n, static_cast<cf*>(t));
+    } else {
+        hipLaunchKernelGGL((k_bwd_partial<T, false>), grid, dim3(256), 0, st, nodes, B, 1, (B + 15) / 16, rk, fp,
+                           static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(t));
+    }
 }
 
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
@@ -794,8 +1042,12 @@ template void launch_fwd_gather<double>(hipStream_t, const int*, int, int, int, 
 template void launch_fwd_gather<float>(hipStream_t, const int*, int, int, int, const FrontPlan&, const void*, int, int64_t, void*);
 template void launch_fwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int, int64_t);
 template void launch_fwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, void*, void*, int, int64_t);
-template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, const void*, void*, int, void*, void*);
-template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, const void*, void*, int, void*, void*);
+template void launch_fwd_rect<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, void*, const void*, int, int64_t, void*);
+template void launch_fwd_rect<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, void*, const void*, int, int64_t, void*);
+template const void* launch_bwd_rect<double>(hipStream_t, const int*, int, int, int, int, bool, const FrontPlan&, const void*, const void*, int, void*, void*);
+template const void* launch_bwd_rect<float>(hipStream_t, const int*, int, int, int, int, bool, const FrontPlan&, const void*, const void*, int, void*, void*);
+template void launch_bwd_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, const void*, void*, int, void*, int, bool);
+template void launch_bwd_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, const void*, const void*, const void*, const void*, void*, int, void*, int, bool);
 template <typename T>
 void launch_dinv(hipStream_t st, int ns, const FrontPlan& fp, const void* P, void* dinv) {
     if (ns <= 0) return;
