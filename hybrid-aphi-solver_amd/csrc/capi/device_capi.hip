@@ -633,10 +633,16 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
     const auto levels = [&]() {
     for (const Level& L : m->levels) {
         const int cnt = L.count();
-        aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, std::max(L.small_max_m, L.large_max_m), nrhs, fp, m->bt.p,
-                                       n, m->rows_total, m->W.p);
         int maxk = 0;
         for (int i = 0; i < cnt; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(i)]);
+        const int maxm = std::max(L.small_max_m, L.large_max_m);
+        if (aphi_dev::small_level<T>(maxk, maxm)) {
+            // gather, diagonal step and update vector of every front in one launch
+            aphi_dev::launch_fwd_small<T>(s, nodes + L.off, cnt, maxm, nrhs, fp, m->P.p, m->X256.p, m->bt.p, n,
+                                          m->rows_total, m->W.p, m->Yv.p);
+            continue;
+        }
+        aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, maxm, nrhs, fp, m->bt.p, n, m->rows_total, m->W.p);
         for (int B = 0; B < maxk; B += SB)
             aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), tri_rows(L, B), nrhs, fp,
                                          m->P.p, m->X256.p, m->W.p, m->Yv.p, n, m->rows_total);

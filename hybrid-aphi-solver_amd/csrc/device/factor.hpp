@@ -24,6 +24,8 @@ constexpr int kQ = 16;          // right-hand sides per interleaved solve group
 constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a multiple of 64)
 constexpr int kRectRows = 1024;   // rows per backward partial sum below a front's pivots
 constexpr int kFwdRectWgs = 4096; // workgroups a forward rectangle launch aims for (row tiles x column splits)
+constexpr int kSmallK = 64;       // pivots per front of a level solved by the one-launch small-front kernels
+constexpr int kSmallLds = 64 * 1024;  // LDS of a small-front forward workgroup: (max m + kSmallK) x kQ entries
 
 void launch_stats_init(hipStream_t st, FactorStatsDev* stats);  // every slot to its identity
 // Re-initialises the slots and gathers the pivot and multiplier extremes of the
@@ -109,6 +111,13 @@ template <typename T>
 void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int nrhs, const FrontPlan& fp,
                      const void* P, const void* dinv, const void* XT256, const void* Y, void* X, int n, void* t, int rk,
                      bool all);
+// Levels whose fronts have at most kSmallK pivots and whose front vectors fit in
+// kSmallLds: the forward step in one launch, one workgroup per front (gather,
+// y = X w, update vector)
+template <typename T> bool small_level(int max_k, int max_m);
+template <typename T>
+void launch_fwd_small(hipStream_t st, const int* nodes, int count, int max_m, int nrhs, const FrontPlan& fp,
+                      const void* P, const void* X256, const void* b, int n, int64_t R, void* W, void* Y);
 // dinv[j] = 1 / D[j] for every column (n entries), after a factorization
 template <typename T> void launch_dinv(hipStream_t st, int ns, const FrontPlan& fp, const void* P, void* dinv);
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,

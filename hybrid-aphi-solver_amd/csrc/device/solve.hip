@@ -407,46 +407,34 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
     });
 }
 
-// The same update for complex64 with 32 rows per workgroup: lane (p, kq) loads
+// One wave's sums of the complex-float forward update for the 32 rows i0.. over
+// the columns [c0, c1) of a front (whole blocks of kSB, the last possibly short):
+// wave w takes the columns [64 w, 64 w + 64) of each block. Lane (p, kq) loads
 // rows i0 + 2p and i0 + 2p + 1 of a column with one 16-byte load (each load
-// instruction covers 256 contiguous bytes of four columns, twice the run of the
-// one-row-per-lane mapping), and the two rows go to two 16-row MFMA tiles (even
-// rows, odd rows). A wave's operands for its slice of block b + 1 are loaded
-// before it multiplies those of block b, and a slice narrower than 64 columns
-// issues only the steps its columns need. NW = 1: blocks of at most 64 columns,
-// each lane updating its own eight entries.
-template <int NW, bool RECT>
-__global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__ nodes, int B, FrontPlan fp,
-                                                         const cf* __restrict__ P, cf* W, const cf* __restrict__ Y, int n,
-                                                         int64_t R, FwdOut<float> fo) {
+// instruction covers 256 contiguous bytes of four columns), the two rows going to
+// two 16-row MFMA tiles (even rows r0/i0a, odd rows r1/i1a; tile row r of tile h
+// is front row i0 + 2 r + h). A wave's operands for its next 32 columns are loaded
+// before it multiplies the current ones.
+template <int NW>
+__device__ __forceinline__ void fwd2_sums(const Sn& x, const cf* __restrict__ Ps, const cf* __restrict__ Yg, int i0,
+                                          int c0, int c1, V4<float>& r0, V4<float>& i0a, V4<float>& r1,
+                                          V4<float>& i1a) {
     using VF = V4<float>;
     typedef float f4 __attribute__((ext_vector_type(4)));
-    __shared__ cf part[2][NW][256];
-    const int s = nodes[blockIdx.y];
-    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
-    const Sn x = sn(fp, s);
-    const int tiles = static_cast<int>(gridDim.x) / fo.ksplit;
-    const int ks = static_cast<int>(blockIdx.x) / tiles;
-    FwdRows rr;
-    if (!fwd_rows<RECT>(x, B, ks, fo.bps, rr)) return;
-    const int i0 = rr.lo + (static_cast<int>(blockIdx.x) % tiles) * 32;
-    if (i0 >= rr.hi) return;
     const int tid = static_cast<int>(threadIdx.x), wave = tid / 64, lane = tid % 64;
     const int p = lane % 16, kq = lane / 16, k0 = 64 * wave;
-    const cf* Ps = P + fp.panel_off[s];
-    const cf* Yg = Y + (static_cast<int64_t>(g) * n + x.first) * kQ;
     // three-product complex form (k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi);
     // re = k1 - k3, im = k1 + k2): three MFMAs per complex product instead of four
     VF s1a = VF{0, 0, 0, 0}, s2a = VF{0, 0, 0, 0}, s3a = VF{0, 0, 0, 0};
     VF s1b = VF{0, 0, 0, 0}, s2b = VF{0, 0, 0, 0}, s3b = VF{0, 0, 0, 0};
     const int row = min(i0 + 2 * p, x.m - 1);  // rows row and row + 1 (the second may lie past m: never stored)
     // The wave's columns in stages of 32: stage j is the k steps 8 (j % 2) ..
-    // 8 (j % 2) + 7 of its 64-column slice of block b0 + (j / 2) kSB; steps(j) is
+    // 8 (j % 2) + 7 of its 64-column slice of block c0 + (j / 2) kSB; steps(j) is
     // how many of them lie inside the block (uniform over the wave)
-    const int nstage = 2 * ((rr.b1 - rr.b0 + kSB - 1) / kSB);
+    const int nstage = 2 * ((c1 - c0 + kSB - 1) / kSB);
     const auto steps = [&](int j) {
         if (j >= nstage) return 0;
-        const int Bb = rr.b0 + (j / 2) * kSB, u0 = 8 * (j % 2);
+        const int Bb = c0 + (j / 2) * kSB, u0 = 8 * (j % 2);
         return max(0, min(8, (min(kSB, x.k - Bb) - k0 - 4 * u0 + 3) / 4));
     };
     // Four-wave workgroups load every step of a stage (clamped columns past the
@@ -455,7 +443,7 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__
     // of at most 64 columns) load only the steps they use.
     const auto load = [&](int j, int nu, f4 (&a)[8], cf (&b)[8]) {
         if constexpr (NW > 1) j = min(j, nstage - 1);
-        const int Bb = rr.b0 + (j / 2) * kSB, kb = k0 + 32 * (j % 2);
+        const int Bb = c0 + (j / 2) * kSB, kb = k0 + 32 * (j % 2);
         const int last = min(kSB, x.k - Bb) - 1;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -496,7 +484,41 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__
         __builtin_amdgcn_sched_barrier(0);
         mma(nu1, a1, b1);
     }
-    const VF r0 = s1a - s3a, i0a = s1a + s2a, r1 = s1b - s3b, i1a = s1b + s2b;
+    r0 = s1a - s3a;
+    i0a = s1a + s2a;
+    r1 = s1b - s3b;
+    i1a = s1b + s2b;
+}
+
+// The same update for complex64 with 32 rows per workgroup: lane (p, kq) loads
+// rows i0 + 2p and i0 + 2p + 1 of a column with one 16-byte load (each load
+// instruction covers 256 contiguous bytes of four columns, twice the run of the
+// one-row-per-lane mapping), and the two rows go to two 16-row MFMA tiles (even
+// rows, odd rows). A wave's operands for its slice of block b + 1 are loaded
+// before it multiplies those of block b, and a slice narrower than 64 columns
+// issues only the steps its columns need. NW = 1: blocks of at most 64 columns,
+// each lane updating its own eight entries.
+template <int NW, bool RECT>
+__global__ void __launch_bounds__(64 * NW) k_fwd_update2(const int* __restrict__ nodes, int B, FrontPlan fp,
+                                                         const cf* __restrict__ P, cf* W, const cf* __restrict__ Y, int n,
+                                                         int64_t R, FwdOut<float> fo) {
+    using VF = V4<float>;
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    __shared__ cf part[2][NW][256];
+    const int s = nodes[blockIdx.y];
+    const int g = static_cast<int>(blockIdx.z), G = static_cast<int>(gridDim.z);
+    const Sn x = sn(fp, s);
+    const int tiles = static_cast<int>(gridDim.x) / fo.ksplit;
+    const int ks = static_cast<int>(blockIdx.x) / tiles;
+    FwdRows rr;
+    if (!fwd_rows<RECT>(x, B, ks, fo.bps, rr)) return;
+    const int i0 = rr.lo + (static_cast<int>(blockIdx.x) % tiles) * 32;
+    if (i0 >= rr.hi) return;
+    const int tid = static_cast<int>(threadIdx.x), wave = tid / 64, lane = tid % 64, p = lane % 16;
+    const cf* Ps = P + fp.panel_off[s];
+    const cf* Yg = Y + (static_cast<int64_t>(g) * n + x.first) * kQ;
+    VF r0, i0a, r1, i1a;
+    fwd2_sums<NW>(x, Ps, Yg, i0, rr.b0, rr.b1, r0, i0a, r1, i1a);
     cf* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
     cf* Pg = fo.ksplit > 1 ? fo.part + ((static_cast<int64_t>(blockIdx.y) * fo.ksplit + ks) * G + g) * fo.prow * kQ
                            : nullptr;  // rows i - k
@@ -837,6 +859,107 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ no
     });
 }
 
+// ------------------------------------------------------------------ levels of small fronts
+//
+// A level whose fronts have at most kSmallK pivots and whose front vectors fit in
+// kSmallLds runs its forward step in one launch, one workgroup per (front, group
+// of 16 right-hand sides): the front vector w (m x 16) is gathered in LDS (own
+// right-hand side rows, then the children's update vectors in order), y = X w[0:k]
+// (X the inverse of the unit-lower diagonal block) goes to Y and to LDS, and the
+// update vector w[k:m] - L21 y goes to W. Each output tile is one wave's 16 x 16
+// complex MFMA tile; the inner dimension runs in chunks of 32, every operand of a
+// chunk loaded before its MFMAs. (The backward step of such levels stays with the
+// general kernels, whose row-chunked partial sums measured faster.)
+
+// acc += A (16 x K) B (K x 16), K = 4 steps; a_at(p, j) must be finite and b_at(j, q)
+// zero for j >= K (both are called for j up to K rounded up to 32)
+template <typename T, typename FA, typename FB>
+__device__ __forceinline__ void cmma_k(V4<T>& accr, V4<T>& acci, int steps, FA a_at, FB b_at) {
+    const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
+    for (int s0 = 0; s0 < steps; s0 += 8) {
+        C<T> a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = a_at(p, 4 * (s0 + u) + kq);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) b[u] = b_at(4 * (s0 + u) + kq, p);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            accr = Mfma<T>::mac(a[u].re, b[u].re, accr);
+            accr = Mfma<T>::mac(-a[u].im, b[u].im, accr);
+            acci = Mfma<T>::mac(a[u].re, b[u].im, acci);
+            acci = Mfma<T>::mac(a[u].im, b[u].re, acci);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd_small(const int* __restrict__ nodes, FrontPlan fp,
+                                                   const C<T>* __restrict__ P, const C<T>* __restrict__ X256,
+                                                   const C<T>* __restrict__ bt, int n, int64_t R, C<T>* W,
+                                                   C<T>* __restrict__ Y) {
+    extern __shared__ unsigned char smem_raw[];
+    C<T>* w = reinterpret_cast<C<T>*>(smem_raw);  // m x kQ
+    const int s = nodes[blockIdx.x];
+    const int g = static_cast<int>(blockIdx.y);
+    const Sn x = sn(fp, s);
+    C<T>* yl = w + static_cast<int64_t>(x.m) * kQ;  // k x kQ
+    const int tid = static_cast<int>(threadIdx.x), wave = tid / 64, lane = tid % 64;
+    // w = [b rows of s ; 0] + the children's update vectors, children in order
+    for (int idx = tid; idx < x.m * kQ; idx += 256) {
+        const int i = idx / kQ;
+        w[idx] = i < x.k ? bt[(static_cast<int64_t>(g) * n + x.first + i) * kQ + idx % kQ] : C<T>{0, 0};
+    }
+    for (int c = fp.child_ptr[s]; c < fp.child_ptr[s + 1]; ++c) {
+        const int ch = fp.child[c];
+        const Sn y = sn(fp, ch);
+        const int* rel = fp.rel + y.rp + y.k;
+        const C<T>* Wc = W + (static_cast<int64_t>(g) * R + y.rp + y.k) * kQ;
+        __syncthreads();
+        for (int idx = tid; idx < (y.m - y.k) * kQ; idx += 256) {
+            const int i = idx / kQ, q = idx % kQ;
+            C<T>& a = w[rel[i] * kQ + q];
+            a = gadd(a, Wc[static_cast<int64_t>(i) * kQ + q]);
+        }
+    }
+    __syncthreads();
+    // y = X w[0:k] (X[r][j] is stored as zero for j > r)
+    const int nb = x.k;
+    const C<T>* Xs = X256 + fp.x256_off[s];
+    C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first) * kQ;
+    for (int t = wave; 16 * t < nb; t += 4) {
+        const int r0 = 16 * t;
+        V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
+        cmma_k<T>(accr, acci, (min(nb, r0 + 16) + 3) / 4,
+                  [&](int pp, int j) { return Xs[static_cast<int64_t>(min(j, nb - 1)) * nb + min(r0 + pp, nb - 1)]; },
+                  [&](int j, int q) { return j < nb ? w[j * kQ + q] : C<T>{0, 0}; });
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + Mfma<T>::row(lane, v);
+            if (r < nb) {
+                const C<T> val = C<T>{accr[v], acci[v]};
+                yl[r * kQ + lane % 16] = val;
+                Yg[r * kQ + lane % 16] = val;
+            }
+        }
+    }
+    __syncthreads();
+    // the update vector: W[k:m] = w[k:m] - L21 y
+    const C<T>* Ps = P + fp.panel_off[s];
+    C<T>* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
+    for (int t = wave; 16 * t < x.m - x.k; t += 4) {
+        const int i0 = x.k + 16 * t;
+        V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
+        cmma_k<T>(accr, acci, (nb + 3) / 4,
+                  [&](int pp, int j) { return Ps[static_cast<int64_t>(min(j, nb - 1)) * x.m + min(i0 + pp, x.m - 1)]; },
+                  [&](int j, int q) { return j < nb ? yl[j * kQ + q] : C<T>{0, 0}; });
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = i0 + Mfma<T>::row(lane, v);
+            if (r < x.m) Wg[static_cast<int64_t>(r) * kQ + lane % 16] = gsub(w[r * kQ + lane % 16], C<T>{accr[v], acci[v]});
+        }
+    }
+}
+
 // colscale[r] = 1 (mode 0) or 1/||b_r|| (mode 1, 1 for a zero column)
 __global__ void k_colscale_final(int nrhs, int nparts, const double* __restrict__ partials, double* __restrict__ colscale,
                                  int mode) {
@@ -1025,6 +1148,28 @@ void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max
                            static_cast<const C<T>*>(P), static_cast<const C<T>*>(X), n, static_cast<C<T>*>(t));
     }
 }
+
+template <typename T>
+bool small_level(int max_k, int max_m) {
+    return max_k <= kSmallK &&
+           static_cast<size_t>(max_m + kSmallK) * kQ * sizeof(C<T>) <= static_cast<size_t>(kSmallLds);
+}
+
+template <typename T>
+void launch_fwd_small(hipStream_t st, const int* nodes, int count, int max_m, int nrhs, const FrontPlan& fp,
+                      const void* P, const void* X256, const void* b, int n, int64_t R, void* W, void* Y) {
+    if (count <= 0) return;
+    const size_t lds = static_cast<size_t>(max_m + kSmallK) * kQ * sizeof(C<T>);
+    allow_large_lds(k_fwd_small<T>, lds);
+    hipLaunchKernelGGL(k_fwd_small<T>, dim3(count, rhs_groups(nrhs)), dim3(256), lds, st, nodes, fp,
+                       static_cast<const C<T>*>(P), static_cast<const C<T>*>(X256), static_cast<const C<T>*>(b), n, R,
+                       static_cast<C<T>*>(W), static_cast<C<T>*>(Y));
+}
+
+template bool small_level<double>(int, int);
+template bool small_level<float>(int, int);
+template void launch_fwd_small<double>(hipStream_t, const int*, int, int, int, const FrontPlan&, const void*, const void*, const void*, int, int64_t, void*, void*);
+template void launch_fwd_small<float>(hipStream_t, const int*, int, int, int, const FrontPlan&, const void*, const void*, const void*, int, int64_t, void*, void*);
 
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,
                          int mode) {
