@@ -13,12 +13,20 @@ instead (aphi_amd.PrecisionPolicy); the counts are reported. The plain FP64 path
 (complex128 factor, no refinement) is timed on --fp64-steps of the sweep's points,
 spread over the range (its cost does not depend on the frequency).
 
+--pipeline points are in flight at once (default 2), each on its own context and
+copy of the mesh, so that one point's refinement solves (latency-bound chains of
+small launches) run beside the next point's factorization (MFMA-bound). `value`
+and the plain FP64 rate are measured that way; the same sweep one point at a time
+is reported beside them (`sequential`), with the latency of a point that has the
+device to itself.
+
 Prints one JSON line. Inputs are resident in HBM before the timed region.
 """
 import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -188,6 +196,8 @@ def main():
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--fp64-steps", type=int, default=4)
+    ap.add_argument("--pipeline", type=int, default=2,
+                    help="frequency points in flight, each on its own context and copy of the mesh (1: one at a time)")
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--nrhs", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline sampling (0 skips)")
@@ -285,6 +295,18 @@ def main():
     B = torch.stack([A.random_complex_rhs(ctx, RHS_SEED, a.nrhs, n, offset=2 * fi * a.nrhs * n)
                      for fi in range(len(freqs))])
     X = torch.empty(a.nrhs, n, dtype=torch.complex128, device="cuda")
+    # --pipeline workers: a context, a copy of the mesh and a solution buffer each.
+    # Their solves launch kernel by kernel (graph capture on one thread's stream does
+    # not tolerate another thread's work on the legacy stream).
+    ctxs, meshes, Xs = [ctx], [mesh], [X]
+    for _ in range(1, max(1, a.pipeline)):
+        c = A.Context(0)
+        ctxs.append(c)
+        meshes.append(A.Mesh(c, prob))
+        Xs.append(torch.empty_like(X))
+    if len(meshes) > 1:
+        for m in meshes:
+            m.set_solve_graphs(False)
     # the sweep in ascending frequency, as the reference's solve_mesh loop runs it
     # (tools/solve_mesh.cpp:207); --steps below 64 takes every (64/steps)-th point
     nsteps = max(1, min(a.steps, len(freqs)))
@@ -294,61 +316,111 @@ def main():
     k64 = min(a.fp64_steps, nsteps)
     order64 = ([order[int(round(i * (nsteps - 1) / max(k64 - 1, 1)))] for i in range(k64)] if k64 else [])
 
-    def step(fi, precision):
+    def step(fi, precision, w=0):
+        m, Xw = meshes[w], Xs[w]
         omega = 2 * np.pi * freqs[fi]
-        mesh.refill(omega)
-        _, rep = mesh.solve_symmetric(B[fi], X, precision=precision, equilibration_iterations=10,
-                                      max_refinements=(REFINE_BUDGET if precision == A.APHI_FP32 else 0),
-                                      tolerance=1e-12)
+        m.refill(omega)
+        _, rep = m.solve_symmetric(B[fi], Xw, precision=precision, equilibration_iterations=10,
+                                   max_refinements=(REFINE_BUDGET if precision == A.APHI_FP32 else 0),
+                                   tolerance=1e-12)
         # the tets' fields (compute_fields, tools/solve_mesh.cpp:296) reduced to the port
         # current, R and L of every right-hand side (postprocess.hpp:348-358)
-        rep["current"], rep["R"], rep["L"] = mesh.terminal_current(X, omega, cfg.side)
+        rep["current"], rep["R"], rep["L"] = m.terminal_current(Xw, omega, cfg.side)
         return rep
+
+    def sweep(points, choose, record=None, workers=None):
+        # the points in order over --pipeline workers, each with its own context and
+        # copy of the mesh: a worker takes the next point as soon as its previous
+        # one is done, so one point's refinement solves run beside the next point's
+        # factorization. The precision of a point is chosen when it is taken, from
+        # the points finished by then; latency = a point's own start to finish.
+        lat, reps = [None] * len(points), [None] * len(points)
+        lock = threading.Lock()
+        nxt = [0]
+        errors = []
+
+        def worker(w):
+            try:
+                while True:
+                    with lock:
+                        i = nxt[0]
+                        if i >= len(points) or errors:
+                            return
+                        nxt[0] += 1
+                        fi = points[i]
+                        prec = choose(freqs[fi])
+                    s0 = time.perf_counter()
+                    rep = step(fi, prec, w)
+                    lat[i] = time.perf_counter() - s0
+                    rep["requested"] = prec
+                    reps[i] = rep
+                    if record is not None:
+                        with lock:
+                            record(freqs[fi], prec, rep)
+            except BaseException as e:  # noqa: BLE001 (re-raised below)
+                errors.append(e)
+
+        nw = len(meshes) if workers is None else workers
+        for c in ctxs:
+            c.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=worker, args=(w,)) for w in range(nw)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        for c in ctxs:
+            c.synchronize()
+        torch.cuda.synchronize()
+        if errors:
+            raise errors[0]
+        return time.perf_counter() - t0, lat, reps
 
     # mixed-precision sweep: FP32 factor + GMRES refinement, FP64 factor at the
     # frequencies where the points solved so far show FP32 would be slower
     policy = A.PrecisionPolicy(budget=REFINE_BUDGET)
 
-    def step_mixed(fi):
-        prec = policy.choose(freqs[fi])
-        rep = step(fi, prec)
-        policy.record(freqs[fi], prec, rep)
-        rep["requested"] = prec
-        return rep
-
-    # warm-up points: the sweep's first frequencies, through a policy of their own
-    # so the timed sweep starts from nothing learnt
-    for i in range(a.warmup):
-        fi = order[i % len(order)]
-        warm = A.PrecisionPolicy(budget=REFINE_BUDGET)
-        step(fi, warm.choose(freqs[fi]))
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    lat, reps = [], []
-    t_start = time.perf_counter()
-    for fi in order:
-        s0 = time.perf_counter()
-        reps.append(step_mixed(fi))
-        lat.append(time.perf_counter() - s0)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
+    # warm-up points: the sweep's first frequencies on every worker, through a
+    # policy of their own so the timed sweep starts from nothing learnt
+    for w in range(len(meshes)):
+        for i in range(a.warmup):
+            fi = order[i % len(order)]
+            warm = A.PrecisionPolicy(budget=REFINE_BUDGET)
+            step(fi, warm.choose(freqs[fi]), w)
+    elapsed, lat, reps = sweep(order, policy.choose, policy.record)
     value = len(order) / elapsed
     log(f"fp32+IR: {value:.3f} points/s  p50 {np.percentile(lat, 50) * 1e3:.0f} ms  "
-        f"p99 {np.percentile(lat, 99) * 1e3:.0f} ms  max be {max(r['backward_error'] for r in reps):.2e}")
+        f"p99 {np.percentile(lat, 99) * 1e3:.0f} ms  max be {max(r['backward_error'] for r in reps):.2e}  "
+        f"({len(meshes)} in flight)")
 
-    # plain FP64 path on the same frequency points
-    lat64, reps64 = [], []
-    ctx.synchronize()
-    t64 = time.perf_counter()
-    for fi in order64:
-        s0 = time.perf_counter()
-        reps64.append(step(fi, A.APHI_FP64))
-        lat64.append(time.perf_counter() - s0)
-    ctx.synchronize()
-    value64 = a.fp64_steps / (time.perf_counter() - t64) if a.fp64_steps else None
-    if value64:
+    # plain FP64 path on the same frequency points, in flight the same way
+    value64, lat64, reps64 = None, [], []
+    if order64:
+        el64, lat64, reps64 = sweep(order64, lambda f: A.APHI_FP64)
+        value64 = len(order64) / el64
         log(f"fp64: {value64:.3f} points/s  max be {max(r['backward_error'] for r in reps64):.2e}")
+    # the measurements below use the first worker's mesh alone
+    pipeline = len(meshes)
+    for m in meshes[1:]:
+        m.close()
+    for c in ctxs[1:]:
+        c.close()
+    del meshes[1:], Xs[1:], ctxs[1:]
+    torch.cuda.empty_cache()
+    mesh.set_solve_graphs(True)
+    sequential = None
+    if pipeline > 1:
+        # the same sweep one point at a time (a fresh policy): the latency of a point
+        # that has the device to itself
+        pol1 = A.PrecisionPolicy(budget=REFINE_BUDGET)
+        el1, lat1, reps1 = sweep(order, pol1.choose, pol1.record, workers=1)
+        sequential = {"value": len(order) / el1, "p50_ms": float(np.percentile(lat1, 50) * 1e3),
+                      "p99_ms": float(np.percentile(lat1, 99) * 1e3),
+                      "fp32_points": sum(1 for r in reps1 if r["precision_used"] == A.APHI_FP32),
+                      "backward_error_max": max(r["backward_error"] for r in reps1)}
+        log(f"fp32+IR one at a time: {sequential['value']:.3f} points/s  p50 {sequential['p50_ms']:.0f} ms  "
+            f"p99 {sequential['p99_ms']:.0f} ms")
 
     # error of the FP32+IR solution against the complex128 path, per frequency
     # (untimed; both solved again on the same points)
@@ -447,6 +519,7 @@ def main():
         "config": {"workload": f"{a.n}^3-hex Kuhn cube, 6x3 off-centre bar, 64 log frequencies 1 Hz-100 MHz swept in "
                                f"ascending order ({len(order)} timed), {a.nrhs} RHS per point, FP32-MFMA LDL^T + "
                                f"FP64 GMRES-IR to 1e-12",
+                   "points_in_flight": pipeline,
                    "unknowns": n, "nnz_L": int(prob.counts["predicted_nnz"]),
                    "factor_flops": prob.counts["flops"], "frequencies_timed": [float(freqs[fi]) for fi in order]},
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
@@ -458,6 +531,7 @@ def main():
         "precision_requested": [r["requested"] for r in reps],
         "precision_used": [r["precision_used"] for r in reps],
         "fp32_points": sum(1 for r in reps if r["precision_used"] == A.APHI_FP32),
+        "sequential": sequential,
         "fp64_value": value64,
         "fp64_p99_ms": float(np.percentile(lat64, 99) * 1e3) if lat64 else None,
         "fp64_backward_error_max": max(r["backward_error"] for r in reps64) if reps64 else None,
