@@ -48,20 +48,22 @@ def test_cost_model_threshold():
 
 
 def test_failure_risk_near_the_budget():
-    """The FP32 cost is weighted by the likelihood of a stalled attempt: none up to
-    60 % of the budget, certain at the budget."""
+    """The FP32 cost is weighted by the likelihood of a stalled attempt,
+    (expected solves / budget)^2: none with no refinement, certain at the budget."""
     p = A.PrecisionPolicy(budget=30)
     p.record(1e2, A.APHI_FP32, rep(2, A.APHI_FP32, 615.0, 3 * 19.0))
     p.record(1e5, A.APHI_FP64, rep(0, A.APHI_FP64, 1364.0, 29.0))
     fp64 = 1364.0 + 29.0
-    assert p.fp32_cost(18.0, fp64) == 615.0 + 19.0 * 19.0                 # no risk up to 18
-    r = ((24.0 - 18.0) / 12.0) ** 2
-    assert abs(p.fp32_cost(24.0, fp64) - ((1 - r) * (615.0 + 25.0 * 19.0) + r * (615.0 + 4 * 19.0 + fp64))) < 1e-9
-    assert p.fp32_cost(30.0, fp64) == 615.0 + 4 * 19.0 + fp64             # certain failure
-    k = 25.9                                                                # a rising trend near the budget
+    failed = 615.0 + 4 * 19.0 + fp64
+    assert p.fp32_cost(0.0, fp64) == 615.0 + 19.0                           # nothing to refine: no risk
+    r = (15.0 / 30.0) ** 2
+    assert abs(p.fp32_cost(15.0, fp64) - ((1 - r) * (615.0 + 16.0 * 19.0) + r * failed)) < 1e-9
+    assert p.fp32_cost(30.0, fp64) == failed                               # certain failure
+    assert p.fp32_cost(45.0, fp64) == failed
+    k = 20.6                                                                # a rising trend two thirds of the way
     plain = 615.0 + (1.0 + k) * 19.0
     assert plain < fp64 < p.fp32_cost(k, fp64)                              # cheaper if it converged, not worth the risk
-    k = 20.6
+    k = 12.0
     assert p.fp32_cost(k, fp64) < fp64                                      # small risk: FP32 still pays
 
 
