@@ -139,6 +139,9 @@ struct aphi_mesh {
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
     DevBuf ea_bound_off, ea_bound, x256_off, inv_pairs;
+    DevBuf woff;            // per supernode: its W window's offset within the level (-1: none)
+    int64_t lw_entries = 0;  // the zero column and the largest level's W windows (complex float entries)
+    int64_t lw_zero = 0;     // entries of the zero column at the start of the W buffer
     DevBuf tdofs, tsign, tcoef, tfixed, bsigma, tcontrib, tcurrent;  // port-current recovery
     // compute_fields: the node plan (aphi_fields_attach) and its work arrays
     int fields_np = 0;
@@ -150,6 +153,7 @@ struct aphi_mesh {
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
     DevBuf P, CB, mws, Xinv, X256, XT256, dinv;
+    DevBuf LW;  // complex-float factorization: W = -L D windows of the current level (FrontPlan::woff)
     DevBuf bt, W, X, Yv, rpart, trect, fpart, colscale, res, norms, xil;
     DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
     int solve_nrhs_cap = 0;
@@ -220,6 +224,7 @@ struct aphi_mesh {
         fp.ea_bound_off = ea_bound_off.as<int64_t>();
         fp.ea_bound = ea_bound.as<int>();
         fp.x256_off = x256_off.as<int64_t>();
+        fp.woff = woff.as<int64_t>();
         return fp;
     }
     hipStream_t st() const { return ctx->stream; }
@@ -333,6 +338,9 @@ int64_t factor_sequence(aphi_mesh* m, double pivot_floor, double& update_flops, 
     aphi_dev::launch_scatter_panel<T>(s, m->nnz, m->s2p.as<int64_t>(), m->Aeq.as<cd>(), m->P.p);
     const int* nodes = m->nodes.as<int>();
     auto* st = m->stats.as<aphi_dev::FactorStatsDev>();
+    // complex float: the panels also write W = -L D into the level's W windows,
+    // which the trailing updates read as their second operand
+    void* wbuf = std::is_same<T, float>::value ? m->LW.p : nullptr;
     int64_t launches = 0;
     update_flops = 0.0;
     const auto next_event = [&]() {
@@ -400,7 +408,8 @@ int64_t factor_sequence(aphi_mesh* m, double pivot_floor, double& update_flops, 
                 log->push_back(UpdateLaunch{hc, wc, small, fl, ev_used, 0.0});
                 HIPCHK(hipEventRecord(next_event(), on));
             }
-            aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, 0, fp, m->P.p, m->CB.p, small);
+            aphi_dev::launch_update<T>(on, lg, L.large_count, c0, width, tlo, thi, tiles, 0, fp, m->P.p, m->CB.p, wbuf,
+                                       small);
             if (log) HIPCHK(hipEventRecord(next_event(), on));
             ++launches;
             update_flops += fl;
@@ -427,7 +436,7 @@ int64_t factor_sequence(aphi_mesh* m, double pivot_floor, double& update_flops, 
                     if (rem > 0) rb = std::max(rb, (rem + 63) / 64);
                 }
                 aphi_dev::launch_block_step<T>(s, lg, L.large_count, b, rb, 0, fp, m->P.p, m->CB.p, m->mws.p, m->Xinv.p,
-                                               pivot_floor, st);
+                                               pivot_floor, st, wbuf);
                 update(s, b, aphi_dev::kNB, 0, bo + aphi_dev::kOuter);
             }
             // the previous block's rest update must be done before this block's
@@ -471,6 +480,11 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out, std::
     m->X256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
     m->XT256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
     m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
+    if (std::is_same<T, float>::value) {
+        m->LW.reserve(static_cast<size_t>(std::max<int64_t>(m->lw_entries, 1) + aphi_dev::kPanelTail) * cs);
+        // the zero column (never written by the factorization; cleared outside any capture)
+        HIPCHK(hipMemsetAsync(m->LW.p, 0, static_cast<size_t>(m->lw_zero) * cs, s));
+    }
     int64_t launches = 0;
     double update_flops = 0.0;
     size_t ev_used = 0;
@@ -480,8 +494,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out, std::
     HIPCHK(hipEventRecord(m->ctx->ev0, s));
     if (!timed && factor_graphs() && m->graphs_on) {
         const int prec = std::is_same<T, float>::value ? APHI_FP32 : APHI_FP64;
-        const std::vector<const void*> key = {m->P.p,    m->CB.p,    m->mws.p,   m->Xinv.p, m->dinv.p, m->X256.p,
-                                              m->XT256.p, m->stats.p, m->Aeq.p,  m->s2p.p,  m->nodes.p};
+        const std::vector<const void*> key = {m->P.p,     m->CB.p,    m->mws.p, m->Xinv.p, m->dinv.p, m->X256.p,
+                                              m->XT256.p, m->stats.p, m->Aeq.p, m->s2p.p,  m->nodes.p, m->LW.p};
         auto& g = m->fgraph;
         if (!(g.exec && g.precision == prec && g.pivot_floor == pivot_floor && g.key == key &&
               g.tiles_version == m->tiles_version)) {
@@ -1174,6 +1188,12 @@ std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d
             (mm <= small_max ? small : large)[static_cast<size_t>(L)].push_back(s);
         }
         std::vector<int> node_list, ea;
+        std::vector<int64_t> wo(static_cast<size_t>(ns), -1);
+        {
+            int64_t maxm = 0;
+            for (int s = 0; s < ns; ++s) maxm = std::max<int64_t>(maxm, d->super_row_ptr[s + 1] - d->super_row_ptr[s]);
+            m->lw_zero = (maxm + aphi_dev::kPanelTail + 63) / 64 * 64;
+        }
         for (int L = 0; L < d->num_levels; ++L) {
             Level& lv = m->levels[static_cast<size_t>(L)];
             lv.off = static_cast<int64_t>(node_list.size());
@@ -1181,6 +1201,7 @@ std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d
             lv.small_count = static_cast<int>(small[static_cast<size_t>(L)].size());
             lv.large_count = static_cast<int>(large[static_cast<size_t>(L)].size());
             int blk = 0, maxk = 0, maxrect = 0;
+            int64_t wlevel = m->lw_zero;  // this level's W windows, after the zero column
             for (int pass = 0; pass < 2; ++pass)
                 for (int s : (pass == 0 ? small : large)[static_cast<size_t>(L)]) {
                     node_list.push_back(s);
@@ -1193,12 +1214,15 @@ std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d
                         lv.large_max_m = std::max(lv.large_max_m, mm);
                         ea.push_back(blk);
                         blk += (mm + 63) / 64;
+                        wo[static_cast<size_t>(s)] = wlevel;
+                        wlevel += static_cast<int64_t>(std::min(kk, aphi_dev::kWCols)) * mm;
                     }
                     maxk = std::max(maxk, kk);
                     maxrect = std::max(maxrect, mm - kk);
                 }
             lv.ea_blocks = blk;
             ea.push_back(blk);
+            m->lw_entries = std::max(m->lw_entries, wlevel);
             m->max_level_count = std::max(m->max_level_count, lv.count());
             const int64_t rc = aphi_dev::rect_cols(maxk);
             m->max_rect_partials = std::max<int64_t>(m->max_rect_partials,
@@ -1213,6 +1237,7 @@ std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d
                 }
         }
         upload(m->st(), m->nodes, node_list.data(), node_list.size());
+        upload(m->st(), m->woff, wo.data(), wo.size());
         {
             // diagonal-block inverses: full kNB x kNB blocks, then one nb x nb
             std::vector<int64_t> xo(static_cast<size_t>(ns));
@@ -1343,7 +1368,7 @@ extern "C" int aphi_mesh_release_workspace(aphi_mesh* m) {
         // (the assembly error flag stays: it is allocated with the plan)
         DevBuf* numeric[] = {&m->coef, &m->E,     &m->A,     &m->rhs,  &m->Aeq,      &m->work,  &m->rowmax,
                              &m->fac,  &m->d,     &m->partials, &m->stats, &m->P, &m->CB,       &m->mws,   &m->Xinv,
-                             &m->X256, &m->XT256, &m->dinv,  &m->bt,    &m->W,    &m->X,        &m->Yv,
+                             &m->X256, &m->XT256, &m->dinv,  &m->bt,    &m->W,    &m->X,        &m->Yv,    &m->LW,
                              &m->rpart, &m->trect, &m->fpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->gV,       &m->gZ,    &m->gW,
                              &m->gU,   &m->gDots, &m->gSmall, &m->tcontrib, &m->tcurrent, &m->fval, &m->fvol,
                              &m->fmu,  &m->fsig,  &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node,
@@ -1470,7 +1495,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->rpart, &m->trect, &m->fpart, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->fptr, &m->fentry, &m->bmu, &m->fval, &m->fvol, &m->fmu, &m->fsig, &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->woff, &m->LW, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->rpart, &m->trect, &m->fpart, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->fptr, &m->fentry, &m->bmu, &m->fval, &m->fvol, &m->fmu, &m->fsig, &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
                            &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv, &m->gf.coords,
                            &m->gf.tets, &m->gf.tet_body, &m->gf.tet_on, &m->gf.column, &m->gf.prescribed, &m->gf.tptr,

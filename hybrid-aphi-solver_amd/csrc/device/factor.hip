@@ -19,6 +19,7 @@
 // front (the only O(m^2 k) step).
 #include <algorithm>
 #include <climits>
+#include <stdexcept>
 
 #include "common.hpp"
 #include "factor.hpp"
@@ -504,7 +505,7 @@ __global__ void __launch_bounds__(1024) k_diag(const int* __restrict__ nodes, in
 // trailing-update workgroups that fill the device meanwhile.
 template <typename T>
 __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, int b, FrontPlan fp, C<T>* P, C<T>* CB,
-                                               const C<T>* __restrict__ mws, FactorStatsDev* st) {
+                                               const C<T>* __restrict__ mws, FactorStatsDev* st, C<T>* W) {
     using V4 = typename Mfma<T>::v4;
     const int s = nodes[blockIdx.y];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
@@ -521,9 +522,19 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
             const int q = 4 * u + kq;
             a[u] = q < nb ? fr.panel[static_cast<int64_t>(b + q) * fr.m + min(rw + p, fr.m - 1)] : C<T>{0, 0};
         }
+
 #pragma unroll 1
         for (int ct = 0; ct < kNB; ct += 16) {
             if (ct >= nb) break;
+            // complex float: -D of the lane's column, loaded with this tile's M (the W
+            // rows are -L D)
+            C<T> dn = C<T>{0, 0};
+            if constexpr (std::is_same<T, float>::value) {
+                if (W != nullptr && ct + p < nb) {
+                    const C<T> d = fr.panel[static_cast<int64_t>(b + ct + p) * fr.m + b + ct + p];
+                    dn = C<T>{-d.re, -d.im};
+                }
+            }
             C<T> g[16];
 #pragma unroll
             for (int u = 0; u < 16; ++u) {
@@ -538,13 +549,18 @@ __global__ void __launch_bounds__(256) k_panel(const int* __restrict__ nodes, in
                 acci = Mfma<T>::mac(a[u].re, g[u].im, acci);
                 acci = Mfma<T>::mac(a[u].im, g[u].re, acci);
             }
-            // D[row][col]: row = Mfma<T>::row(lane, v) (front row), col = lane % 16 (column)
+            // D[row][col]: row = Mfma<T>::row(lane, v) (front row), col = lane % 16 (column);
+            // complex float also writes the update's W operand into the front's W window
+            C<T>* wcol = nullptr;
+            if constexpr (std::is_same<T, float>::value)
+                if (W != nullptr) wcol = W + fp.woff[s] + static_cast<int64_t>((b + ct + p) % kWCols) * fr.m;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = rw + Mfma<T>::row(lane, v), c = ct + p;
                 if (i < fr.m && c < nb) {
                     const C<T> l = C<T>{accr[v], acci[v]};
                     fr.panel[static_cast<int64_t>(b + c) * fr.m + i] = l;
+                    if (wcol != nullptr) wcol[i] = gmul(l, dn);
                     lmax = fmax(lmax, static_cast<double>(hypot(l.re, l.im)));
                 }
             }
@@ -614,9 +630,11 @@ __device__ __forceinline__ void lds_dma16(const void* src, void* lds_dst) {
 // zeroed tail for them.
 template <typename T, bool Small = false>
 __global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Small>::MINB))
-    k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi, FrontPlan fp, C<T>* P, C<T>* CB) {
+    k_update(const int* __restrict__ nodes, int c0, int width, int tlo, int thi, FrontPlan fp, C<T>* P, C<T>* CB,
+             const C<T>* __restrict__ W) {
     using V4 = typename Mfma<T>::v4;
     using UT = UpdTile<T, Small>;
+    constexpr bool k3 = std::is_same<T, float>::value;  // complex float: W operand from the W windows
     constexpr int TS = UT::TS, WX = UT::WX, WY = UT::WY, KC = UT::KC, NW = UT::NW;
     static_assert(UT::GX * WX * 16 == TS && UT::GY * WY * 16 == TS, "waves tile the update tile");
     constexpr int RPL = 16 / static_cast<int>(sizeof(C<T>));  // rows per lane per DMA
@@ -626,7 +644,7 @@ __global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Smal
     // LDS row stride (complex): padded, except where one DMA fills two adjacent rows
     constexpr int LDR = CPD == 1 ? TS + 16 : TS;
     __shared__ C<T> Lc[2][KC][LDR], Wc[2][KC][LDR];
-    __shared__ C<T> Dsh[kOuter];
+    __shared__ C<T> Dsh[k3 ? 1 : kOuter];
     const int s = nodes[blockIdx.y];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
     if (fr.k <= c0) return;
@@ -664,9 +682,11 @@ __global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Smal
     };
 
     const int tid = threadIdx.x;
-    const int wave = tid / 64, lane = tid % 64;
+    // the wave index is uniform: address arithmetic derived from it stays scalar
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;
     const int wi = (wave % UT::GX) * (16 * WX), wj = (wave / UT::GX) * (16 * WY);  // wave offsets (i, j)
     const C<T>* L0 = fr.panel + static_cast<int64_t>(c0) * fr.m;
+    const C<T>* W0 = k3 ? W + fp.woff[s] : nullptr;
     const int nst = (K + KC - 1) / KC;
 
     // stage st of the tile at (i0, j0): 2*KC column segments (L rows j0.. then rows i0..), 2*KC/NW per wave
@@ -677,28 +697,35 @@ __global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Smal
         for (int u = 0; u < 2 * KC / CPD / NW; ++u) {
             const int seg = (wave + NW * u) * CPD;
             const int mat = seg / KC, q = seg % KC;
-            const int c = min(st * KC + q + sub, K - 1);
+            const int cu = st * KC + q + sub, c = min(cu, K - 1);
             const int row = min((mat ? i0 : j0) + RPL * lr, fr.m - 1);
-            lds_dma16(L0 + static_cast<int64_t>(c) * fr.m + row, mat ? &Wc[buf][q][0] : &Lc[buf][q][0]);
+            const C<T>* src = L0 + static_cast<int64_t>(c) * fr.m + row;
+            // complex float: W of column c; past the update's last column the zero
+            // column at the start of the W buffer, so those steps add nothing
+            if (k3 && mat) src = cu < K ? W0 + static_cast<int64_t>((c0 + c) % kWCols) * fr.m + row : W + row;
+            lds_dma16(src, mat ? &Wc[buf][q][0] : &Lc[buf][q][0]);
         }
     };
     int i0, j0;
     tile_origin(t, i0, j0);
     int gs = 0;  // stage counter across tiles: LDS buffer gs & 1
     issue_stage(0, 0, i0, j0);
-    for (int q = tid; q < nst * KC; q += 64 * NW) {
-        const int qq = min(q, K - 1);
-        const C<T> d = L0[static_cast<int64_t>(qq) * fr.m + c0 + qq];
-        Dsh[q] = q < K ? C<T>{-d.re, -d.im} : C<T>{0, 0};
+    if constexpr (!k3) {
+        for (int q = tid; q < nst * KC; q += 64 * NW) {
+            const int qq = min(q, K - 1);
+            const C<T> d = L0[static_cast<int64_t>(qq) * fr.m + c0 + qq];
+            Dsh[q] = q < K ? C<T>{-d.re, -d.im} : C<T>{0, 0};
+        }
     }
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
-    // FP32 tiles use the three-product form of the complex product (a = ar + i ai,
-    // b = br + i bi): k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi),
-    // re = k1 - k3, im = k1 + k2, summed in S1, S2 and S3: three MFMAs per product
-    // instead of four. FP64 tiles use the four-product form (accr, acci).
-    constexpr bool k3 = std::is_same<T, float>::value;
+    // FP32 tiles use a three-product form of the complex product a w (a = L[j] =
+    // ar + i ai, w = W[i] = wr + i wi, W = -L D prepared by the panel kernel):
+    // t1 = ar wr, t2 = ai wi, t3 = (ar + ai)(wr + wi), re = t1 - t2,
+    // im = (t3 - t1) - t2, summed in S1, S2 and S3: three MFMAs per product instead
+    // of four, and two VALU additions per step for the operand sums. FP64 tiles use
+    // the four-product form (accr, acci) with D applied at read time.
     if constexpr (k3) {
         // FP32: each wave's (16 WX) x (16 WY) block as XB x YB tiles of 32 x 32 on
         // v_mfma_f32_32x32x2f32 (16 passes: half the MFMA instructions of 16x16x4
@@ -725,10 +752,9 @@ __global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Smal
                 else if (tn < tlast) issue_stage(buf ^ 1, 0, ni0, nj0);
                 // the LDS operands of a step are read one step ahead, so a step's
                 // MFMAs do not wait on its own LDS reads
-                C<T> ra[2][YB], rw[2][XB], rd[2];
+                C<T> ra[2][YB], rw[2][XB];
                 const auto fetch = [&](int slot, int k2) {
                     const int kq = k2 + k2q;
-                    rd[slot] = Dsh[st * KC + kq];
 #pragma unroll
                     for (int y = 0; y < YB; ++y) ra[slot][y] = Lc[buf][kq][wj + 32 * y + r32];
 #pragma unroll
@@ -740,22 +766,17 @@ __global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Smal
                     const int cur = (k2 / 2) & 1;
                     if (k2 + 2 < KC) fetch(cur ^ 1, k2 + 2);
                     __builtin_amdgcn_sched_barrier(0);  // keep the next step's reads ahead of this step's MFMAs
-                    float br[XB], bd[XB], bs[XB];
+                    float bs[XB];
 #pragma unroll
-                    for (int x = 0; x < XB; ++x) {
-                        const C<T> w = gmul(rw[cur][x], rd[cur]);
-                        br[x] = w.re;
-                        bd[x] = w.im - w.re;
-                        bs[x] = w.re + w.im;
-                    }
+                    for (int x = 0; x < XB; ++x) bs[x] = rw[cur][x].re + rw[cur][x].im;
 #pragma unroll
                     for (int y = 0; y < YB; ++y) {
                         const float ar = ra[cur][y].re, ai = ra[cur][y].im, as = ar + ai;
 #pragma unroll
                         for (int x = 0; x < XB; ++x) {
-                            s1[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(as, br[x], s1[x][y], 0, 0, 0);
-                            s2[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, bd[x], s2[x][y], 0, 0, 0);
-                            s3[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, bs[x], s3[x][y], 0, 0, 0);
+                            s1[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ar, rw[cur][x].re, s1[x][y], 0, 0, 0);
+                            s2[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(ai, rw[cur][x].im, s2[x][y], 0, 0, 0);
+                            s3[x][y] = __builtin_amdgcn_mfma_f32_32x32x2f32(as, bs[x], s3[x][y], 0, 0, 0);
                         }
                     }
                 }
@@ -779,7 +800,8 @@ __global__ void __launch_bounds__((64 * UpdTile<T, Small>::NW), (UpdTile<T, Smal
 #pragma unroll
                     for (int e = 0; e < 16; ++e) {
                         const int j = j0 + wj + 32 * y + 8 * (e / 4) + 4 * k2q + e % 4;
-                        const C<T> out = C<T>{f[e].re + (s1[x][y][e] - s3[x][y][e]), f[e].im + (s1[x][y][e] + s2[x][y][e])};
+                        const C<T> out =
+                            C<T>{f[e].re + (s1[x][y][e] - s2[x][y][e]), f[e].im + ((s3[x][y][e] - s1[x][y][e]) - s2[x][y][e])};
                         if (i < fr.m && j < tend && i >= j) *fr.at(i, j) = out;
                     }
                 }
@@ -936,14 +958,15 @@ void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, i
 template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,
-                       FactorStatsDev* stats) {
+                       FactorStatsDev* stats, void* W) {
     const size_t diag_lds = static_cast<size_t>(kNB * (kNB + 1) + 5 * kNB) * sizeof(C<T>);
     allow_large_lds(k_diag<T>, diag_lds);
     hipLaunchKernelGGL(k_diag<T>, dim3(count), dim3(1024), diag_lds, st, nodes, b, fp, static_cast<C<T>*>(P),
                        static_cast<C<T>*>(CB), static_cast<C<T>*>(mws), static_cast<C<T>*>(xinv), pivot_floor, stats);
     if (max_row_blocks > 0) {
         hipLaunchKernelGGL(k_panel<T>, dim3(max_row_blocks, count), dim3(256), 0, st, nodes, b, fp,
-                           static_cast<C<T>*>(P), static_cast<C<T>*>(CB), static_cast<const C<T>*>(mws), stats);
+                           static_cast<C<T>*>(P), static_cast<C<T>*>(CB), static_cast<const C<T>*>(mws), stats,
+                           static_cast<C<T>*>(W));
     }
     (void)max_tiles;
 }
@@ -965,12 +988,14 @@ int device_cus() {
 
 template <typename T>
 void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
-                   int max_wgs, const FrontPlan& fp, void* P, void* CB, bool small) {
+                   int max_wgs, const FrontPlan& fp, void* P, void* CB, const void* W, bool small) {
     if (count <= 0 || max_tiles <= 0) return;
     if constexpr (std::is_same<T, float>::value) {
+        if (W == nullptr) throw std::invalid_argument("launch_update: complex float needs the W windows");
         if (small) {
             hipLaunchKernelGGL((k_update<float, true>), dim3(max_tiles, count), dim3(64 * UpdTile<float, true>::NW), 0,
-                               st, nodes, c0, width, tlo, thi, fp, static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
+                               st, nodes, c0, width, tlo, thi, fp, static_cast<C<T>*>(P), static_cast<C<T>*>(CB),
+                               static_cast<const C<T>*>(W));
             return;
         }
     }
@@ -982,7 +1007,7 @@ void launch_update(hipStream_t st, const int* nodes, int count, int c0, int widt
     int g = max_tiles;
     if (max_wgs > 0) g = std::min(max_tiles, std::max(8, (max_wgs / count) / 8 * 8));
     hipLaunchKernelGGL(k_update<T>, dim3(g, count), dim3(64 * UpdTile<T>::NW), 0, st, nodes, c0, width, tlo, thi, fp,
-                       static_cast<C<T>*>(P), static_cast<C<T>*>(CB));
+                       static_cast<C<T>*>(P), static_cast<C<T>*>(CB), static_cast<const C<T>*>(W));
 }
 
 template <typename T>
@@ -1008,17 +1033,17 @@ template void launch_front_small<float>(hipStream_t, const int*, int, int, const
 template void launch_extend_add<double>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
 template void launch_extend_add<float>(hipStream_t, const int*, const int*, int, int, int, const FrontPlan&, void*, void*);
 template void launch_update<double>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*,
-                                   bool);
+                                   const void*, bool);
 template void launch_update<float>(hipStream_t, const int*, int, int, int, int, int, int, int, const FrontPlan&, void*, void*,
-                                  bool);
+                                  const void*, bool);
 template <typename T> int update_resident_wgs() { return device_cus() * UpdTile<T>::WPC; }
 template int update_resident_wgs<double>();
 template int update_resident_wgs<float>();
 template int update_tile<double>(bool);
 template int update_tile<float>(bool);
 template void launch_block_step<double>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,
-                                        void*, void*, double, FactorStatsDev*);
+                                        void*, void*, double, FactorStatsDev*, void*);
 template void launch_block_step<float>(hipStream_t, const int*, int, int, int, int, const FrontPlan&, void*, void*,
-                                       void*, void*, double, FactorStatsDev*);
+                                       void*, void*, double, FactorStatsDev*, void*);
 
 }  // namespace aphi_dev

@@ -12,6 +12,7 @@ constexpr int kNB = 64;       // block-column width of the blocked front factori
 constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64)
 constexpr int kOuter = 512;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
+constexpr int kWCols = 2 * kOuter;  // columns of a front's W window (complex float)
 // complex float updates whose tallest region has at most kSmallTileRows rows, or
 // whose widest region has at most kSmallTileCols columns (the block chain's own
 // updates: four times the workgroups of the large tiles on a narrow region), use
@@ -50,15 +51,19 @@ void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, i
 // workgroups (for an update that has the device to itself). small: the 64 x 64
 // tiles of complex float (ignored for complex double); max_tiles counts tiles of
 // update_tile<T>(small).
+// Complex float reads the W = -L D operand from the level's W windows (W, see
+// FrontPlan::woff); complex double forms it from the panel and D.
 template <typename T>
 void launch_update(hipStream_t st, const int* nodes, int count, int c0, int width, int tlo, int thi, int max_tiles,
-                   int max_wgs, const FrontPlan& fp, void* P, void* CB, bool small = false);
+                   int max_wgs, const FrontPlan& fp, void* P, void* CB, const void* W, bool small = false);
 template <typename T> int update_tile(bool small = false);  // tile side of k_update
 template <typename T> int update_resident_wgs();  // k_update workgroups the device holds at once
+// Diagonal block and panel of kNB columns at b; complex float also writes the
+// panel's W = -L D rows into the W windows (W non-null).
 template <typename T>
 void launch_block_step(hipStream_t st, const int* nodes, int count, int b, int max_row_blocks, int max_tiles,
                        const FrontPlan& fp, void* P, void* CB, void* mws, void* xinv, double pivot_floor,
-                       FactorStatsDev* stats);
+                       FactorStatsDev* stats, void* W);
 
 // solve.hip: vectors in groups of kQ right-hand sides, entry (i, q) of group g at
 // [(g * rows + i) * kQ + q] (rows = n, or the total front row count for W)

@@ -47,6 +47,13 @@ struct FrontPlan {
     const int* ea_bound;
     // Inverses of the kSB-column diagonal blocks (the last one nb x nb), for the solves.
     const int64_t* x256_off;   // ns
+    // Complex-float factorization: W = -L D of a front's columns, column c at
+    // W + woff[s] + (c mod kWCols) * m (two outer blocks of columns, so the update of
+    // one outer block can read its window while the next block's panels fill the
+    // other); offsets restart at every level, -1 for fronts factored whole in LDS.
+    // The buffer starts with a zero column at least as long as any front (the
+    // operand of the steps past an update's last column).
+    const int64_t* woff;  // ns
 };
 
 // Factorization statistics, spread over kStatSlots cache-line slots so that the
