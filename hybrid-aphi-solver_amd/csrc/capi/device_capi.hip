@@ -1392,6 +1392,7 @@ extern "C" int aphi_mesh_set_update_tiles(aphi_mesh* m, int mode) {
     return aphi_capi::guarded([&] {
         require(m != nullptr, "aphi_mesh_set_update_tiles: null mesh");
         require(mode >= 0 && mode <= 2, "aphi_mesh_set_update_tiles: mode must be 0, 1 or 2");
+        if (m->update_tiles != mode) ++m->tiles_version;  // a captured factorization holds the old tiling
         m->update_tiles = mode;
     });
 }

@@ -8,7 +8,7 @@ autotuner: neither may change a bit of the factor.
 * The tile table (per launch class: default rule, 64 x 64 or 128 x 128 tiles)
   changes only speed: all-small, all-large and tuned tables give the same D; the
   tuner returns a table of valid entries and two positive timings; bad tables
-  are refused.
+  are refused. Switching the tiling mode between replays keeps the bits too.
 """
 import numpy as np
 import pytest
@@ -87,3 +87,19 @@ def test_tile_tables_keep_the_bits(mesh):
     with pytest.raises(ValueError):
         m.set_update_tile_table(np.full((8, 8), 3, dtype=np.int8))
     m.set_update_tile_table(np.zeros((8, 8), dtype=np.int8))
+
+
+def test_tile_modes_between_graph_replays(mesh):
+    # switching the tiling mode between replays of the captured factorization
+    # recaptures it (the mode is part of what the graph holds) and keeps the bits
+    p, m, meta = mesh
+    m.set_update_tiles("auto")
+    assert m.factorize_ldlt(A.APHI_FP32)["ok"]
+    D0 = m.factor_diagonal()
+    for mode in ("small", "large", "small", "auto"):
+        m.set_update_tiles(mode)
+        for _ in range(2):
+            assert m.factorize_ldlt(A.APHI_FP32)["ok"]
+            assert np.array_equal(bits(m.factor_diagonal()), bits(D0)), mode
+    with pytest.raises(ValueError):
+        m.set_update_tiles("medium")
