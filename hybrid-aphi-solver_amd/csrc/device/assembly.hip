@@ -987,8 +987,12 @@ __global__ void __launch_bounds__(256) k_gf_scatter(int num_nodes, const int* __
 }
 
 // current[r] = -(sum over conducting tets, in tet order, of the flux terms) / length:
-// one wavefront per right-hand side adds 64 terms at a time lane by lane, so the
-// sum is the reference's sequential one.
+// one wavefront per right-hand side adds the terms of 64 tets at a time lane by
+// lane (the conducting lanes only, in ascending order), so the sum is the
+// reference's sequential one. The terms of the next kCurBatch chunks of 64 are
+// loaded while the current ones are added, so the memory latency overlaps the
+// additions.
+constexpr int kCurBatch = 4;
 __global__ void __launch_bounds__(64) k_current_sum(int num_tets, int nseg, const int64_t* __restrict__ seg_tets,
                                                     TetFieldPlan tp, const cd* __restrict__ contrib,
                                                     const double* __restrict__ lengths, double length,
@@ -999,17 +1003,34 @@ __global__ void __launch_bounds__(64) k_current_sum(int num_tets, int nseg, cons
     const cd* cr = contrib + static_cast<int64_t>(r) * num_tets;
     const int lo = static_cast<int>(seg_tets[sg]), hi = static_cast<int>(seg_tets[sg + 1]);
     double fre = 0.0, fim = 0.0;
-    for (int t0 = lo; t0 < hi; t0 += 64) {
-        const int t = t0 + lane;
-        const bool on = t < hi && tp.body_sigma[tp.tet_body[t]] > 0.0;
-        const cd v = on ? cr[t] : cd{0.0, 0.0};
-        const unsigned long long mask = __ballot(on);
+    cd v[kCurBatch], vn[kCurBatch];
+    bool on[kCurBatch], onn[kCurBatch];
+    const auto load = [&](int tb, cd (&w)[kCurBatch], bool (&f)[kCurBatch]) {
 #pragma unroll
-        for (int l = 0; l < 64; ++l) {
-            if ((mask >> l) & 1ull) {
-                fre = fre + lane_read(v.re, l);
-                fim = fim + lane_read(v.im, l);
+        for (int u = 0; u < kCurBatch; ++u) {
+            const int t = tb + 64 * u + lane, tc = max(lo, min(t, hi - 1));  // an index inside the problem
+            f[u] = t < hi && tp.body_sigma[tp.tet_body[tc]] > 0.0;
+            w[u] = cr[tc];
+        }
+    };
+    if (lo < hi) load(lo, v, on);
+    for (int tb = lo; tb < hi; tb += 64 * kCurBatch) {
+        if (tb + 64 * kCurBatch < hi) load(tb + 64 * kCurBatch, vn, onn);
+#pragma unroll
+        for (int u = 0; u < kCurBatch; ++u) {
+            // the conducting lanes in ascending order (most chunks have none)
+            unsigned long long mask = __ballot(on[u]);
+            while (mask != 0ull) {
+                const int l = __builtin_ctzll(mask);
+                fre = fre + lane_read(v[u].re, l);
+                fim = fim + lane_read(v[u].im, l);
+                mask &= mask - 1ull;
             }
+        }
+#pragma unroll
+        for (int u = 0; u < kCurBatch; ++u) {
+            v[u] = vn[u];
+            on[u] = onn[u];
         }
     }
     const double len = lengths ? lengths[sg] : length;
