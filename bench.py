@@ -126,7 +126,7 @@ def cpu_baseline(A, prob, mesh, freq):
 BATCH_SEED = 20261204
 # FP32 refinement budget of a sweep point: the GMRES restart length, so an attempt
 # runs one Krylov cycle at most
-REFINE_BUDGET = 40
+REFINE_BUDGET = 48
 
 
 def batched_leg(A, ctx, count, steps):

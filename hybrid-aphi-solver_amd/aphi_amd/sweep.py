@@ -31,17 +31,18 @@ class PrecisionPolicy:
     nearest neighbours all failed is left to FP64.
 
     An FP32 attempt whose expected refinement runs towards the budget may fail:
-    refinement stalls (the stall rule stops it after a few solves) and the point
-    is refactored in FP64. The FP32 cost is therefore an expectation: with the
-    failure likelihood taken as (expected solves / budget)^2, rising from 0 to 1
-    at the budget, a failure costing the FP32 factor, ``stall_solves`` solves and
-    the FP64 point on top. Refinement counts climb steeply just before refinement
-    breaks down: in the ascending 32^3 sweep an FP32 attempt converged after 21
-    solves, about half of a 40-solve budget, and the next point stalled, so the
-    likelihood is not taken as zero anywhere below the budget.
+    refinement runs out of budget or stalls, and the point is refactored in FP64.
+    The FP32 cost is therefore an expectation: with the failure likelihood taken
+    as ((expected solves - budget/2) / (budget/2))^2, zero up to half the budget
+    and rising to 1 at the budget, a failure costing the FP32 factor,
+    ``stall_solves`` solves (by default half the budget) and the FP64 point on top.
+    On the 32^3 cube, FP32 attempts without a stall rule converged after 21, 26
+    and 33 solves at 9.6, 12.9 and 17.3 MHz, reached 3e-12 after 40 at 23.2 MHz and
+    did not converge at 41.6 MHz; the refinement's stall rule judges the rate
+    late, where GMRES has sped up.
     """
 
-    def __init__(self, fp64_factor_ratio=2.3, budget=30, stall_solves=4):
+    def __init__(self, fp64_factor_ratio=2.3, budget=30, stall_solves=None):
         self.points = {}          # log10(f) -> refinement solves (inf: FP32 not viable)
         self.factor32 = None
         self.factor64 = None
@@ -49,7 +50,7 @@ class PrecisionPolicy:
         self.solve64 = None
         self.ratio = fp64_factor_ratio
         self.budget = budget
-        self.stall_solves = stall_solves
+        self.stall_solves = 0.5 * budget if stall_solves is None else stall_solves
 
     def expected_solves(self, freq):
         if not self.points:
@@ -96,7 +97,8 @@ class PrecisionPolicy:
     def fp32_cost(self, k, fp64_point):
         """Expected cost of an FP32 attempt expecting k refinement solves."""
         ok = self.factor32 + (1.0 + k) * self.solve32
-        risk = min(1.0, max(0.0, k / self.budget)) ** 2
+        half = 0.5 * self.budget
+        risk = min(1.0, max(0.0, (k - half) / half)) ** 2
         failed = self.factor32 + self.stall_solves * self.solve32 + fp64_point
         return (1.0 - risk) * ok + risk * failed
 

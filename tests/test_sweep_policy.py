@@ -41,30 +41,35 @@ def test_cost_model_threshold():
     p.record(1e6, A.APHI_FP32, rep(30, A.APHI_FP32, 800.0, 31 * 25.0))   # slow but converged
     # just above it the rising trend predicts more than the budget: 800 + 32.5*25 =
     # 1612 would beat 1685 if it converged, but failure is then certain and costs
-    # 800 + 4*25 + 1685, so FP64
+    # 800 + 15*25 + 1685 (half the budget in solves), so FP64
     assert p.choose(1.2e6) == A.APHI_FP64
     p.record(2e6, A.APHI_FP32, rep(36, A.APHI_FP32, 800.0, 37 * 25.0))
     assert p.choose(3e6) == A.APHI_FP64
 
 
 def test_failure_risk_near_the_budget():
-    """The FP32 cost is weighted by the likelihood of a stalled attempt,
-    (expected solves / budget)^2: none with no refinement, certain at the budget."""
-    p = A.PrecisionPolicy(budget=30)
-    p.record(1e2, A.APHI_FP32, rep(2, A.APHI_FP32, 615.0, 3 * 19.0))
-    p.record(1e5, A.APHI_FP64, rep(0, A.APHI_FP64, 1364.0, 29.0))
-    fp64 = 1364.0 + 29.0
-    failed = 615.0 + 4 * 19.0 + fp64
-    assert p.fp32_cost(0.0, fp64) == 615.0 + 19.0                           # nothing to refine: no risk
-    r = (15.0 / 30.0) ** 2
-    assert abs(p.fp32_cost(15.0, fp64) - ((1 - r) * (615.0 + 16.0 * 19.0) + r * failed)) < 1e-9
-    assert p.fp32_cost(30.0, fp64) == failed                               # certain failure
-    assert p.fp32_cost(45.0, fp64) == failed
-    k = 20.6                                                                # a rising trend two thirds of the way
-    plain = 615.0 + (1.0 + k) * 19.0
-    assert plain < fp64 < p.fp32_cost(k, fp64)                              # cheaper if it converged, not worth the risk
-    k = 12.0
-    assert p.fp32_cost(k, fp64) < fp64                                      # small risk: FP32 still pays
+    """The FP32 cost is weighted by the likelihood of a failed attempt,
+    ((expected solves - budget/2) / (budget/2))^2: none up to half the budget,
+    certain at the budget; a failure costs half the budget in solves."""
+    p = A.PrecisionPolicy(budget=48)
+    assert p.stall_solves == 24.0
+    p.record(1e2, A.APHI_FP32, rep(2, A.APHI_FP32, 560.0, 3 * 16.6))
+    p.record(1e5, A.APHI_FP64, rep(0, A.APHI_FP64, 1330.0, 30.0))
+    fp64 = 1330.0 + 30.0
+    failed = 560.0 + 24.0 * 16.6 + fp64
+    ok = lambda k: 560.0 + (1.0 + k) * 16.6  # noqa: E731
+    assert abs(p.fp32_cost(0.0, fp64) - ok(0.0)) < 1e-9                    # nothing to refine: no risk
+    assert abs(p.fp32_cost(24.0, fp64) - ok(24.0)) < 1e-9                  # half the budget: still no risk
+    r = (12.0 / 24.0) ** 2
+    assert abs(p.fp32_cost(36.0, fp64) - ((1 - r) * ok(36.0) + r * failed)) < 1e-9
+    assert abs(p.fp32_cost(48.0, fp64) - failed) < 1e-9                    # certain failure
+    assert abs(p.fp32_cost(60.0, fp64) - failed) < 1e-9
+    # the measured counts around the breakdown of the 32^3 sweep: 26 and 33 solves
+    # converged (12.9 and 17.3 MHz), the next point needed about the budget
+    assert p.fp32_cost(26.0, fp64) < fp64
+    assert p.fp32_cost(31.9, fp64) < fp64                                   # predicted from 21 and 26
+    assert p.fp32_cost(41.8, fp64) > fp64                                   # predicted from 26 and 33
+    assert ok(41.8) < fp64                                                  # cheaper only if it converged
 
 
 def test_interpolates_between_an_easy_and_a_failed_neighbour():
