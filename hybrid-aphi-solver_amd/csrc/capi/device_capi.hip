@@ -138,7 +138,7 @@ struct aphi_mesh {
     // plan on the device
     DevBuf coords, tets, tet_body, tptr, entry, kind, row_ptr, col, row_of, ttptr, tslot, trow, perm, col_ref;
     DevBuf first, srow_ptr, srows, rel, child_ptr, child, panel_off, cb_off, s2p, nodes, ea_start, xinv_off;
-    DevBuf ea_bound_off, ea_bound, x256_off, inv_pairs;
+    DevBuf ea_bound_off, ea_bound, xsb_off, inv_pairs;
     DevBuf woff;            // per supernode: its W window's offset within the level (-1: none)
     int64_t lw_entries = 0;  // the zero column and the largest level's W windows (complex float entries)
     int64_t lw_zero = 0;     // entries of the zero column at the start of the W buffer
@@ -147,12 +147,12 @@ struct aphi_mesh {
     int fields_np = 0;
     DevBuf fptr, fentry, bmu, fval, fvol, fmu, fsig, fa_tet, fb_tet, fe_tet, fj_tet;
     DevBuf fa_node, fb_node, fh_node, fe_node, fj_node, fweight, fflag;
-    int64_t x256_entries = 0;
+    int64_t xsb_entries = 0;
     int num_inv_pairs = 0;
     int64_t xinv_entries = 0;
     // numeric state
     DevBuf coef, E, flag, A, rhs, Aeq, work, rowmax, fac, d, partials, stats;
-    DevBuf P, CB, mws, Xinv, X256, XT256, dinv;
+    DevBuf P, CB, mws, Xinv, Xsb, XTsb, dinv;
     DevBuf LW;  // complex-float factorization: W = -L D windows of the current level (FrontPlan::woff)
     DevBuf bt, W, X, Yv, rpart, trect, fpart, colscale, res, norms, xil;
     DevBuf gV, gZ, gW, gU, gDots, gSmall;  // GMRES refinement
@@ -223,7 +223,7 @@ struct aphi_mesh {
         fp.xinv_off = xinv_off.as<int64_t>();
         fp.ea_bound_off = ea_bound_off.as<int64_t>();
         fp.ea_bound = ea_bound.as<int>();
-        fp.x256_off = x256_off.as<int64_t>();
+        fp.xsb_off = xsb_off.as<int64_t>();
         fp.woff = woff.as<int64_t>();
         return fp;
     }
@@ -462,8 +462,8 @@ int64_t factor_sequence(aphi_mesh* m, double pivot_floor, double& update_flops, 
             }
         }
     }
-    aphi_dev::launch_inv_blocks<T>(s, m->inv_pairs.as<int2>(), m->num_inv_pairs, fp, m->P.p, m->Xinv.p, m->X256.p,
-                                   m->XT256.p);
+    aphi_dev::launch_inv_blocks<T>(s, m->inv_pairs.as<int2>(), m->num_inv_pairs, fp, m->P.p, m->Xinv.p, m->Xsb.p,
+                                   m->XTsb.p);
     aphi_dev::launch_dinv<T>(s, m->num_super, fp, m->P.p, m->dinv.p);
     return launches;
 }
@@ -477,8 +477,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out, std::
     m->mws.reserve(static_cast<size_t>(std::max(m->max_level_count, 1)) * aphi_dev::kNB * aphi_dev::kNB * cs);
     m->Xinv.reserve(static_cast<size_t>(std::max<int64_t>(m->xinv_entries, 1)) * cs);
     m->dinv.reserve(static_cast<size_t>(m->n) * cs);
-    m->X256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
-    m->XT256.reserve(static_cast<size_t>(std::max<int64_t>(m->x256_entries, 1)) * cs);
+    m->Xsb.reserve(static_cast<size_t>(std::max<int64_t>(m->xsb_entries, 1)) * cs);
+    m->XTsb.reserve(static_cast<size_t>(std::max<int64_t>(m->xsb_entries, 1)) * cs);
     m->stats.reserve(sizeof(aphi_dev::FactorStatsDev));
     if (std::is_same<T, float>::value) {
         m->LW.reserve(static_cast<size_t>(std::max<int64_t>(m->lw_entries, 1) + aphi_dev::kPanelTail) * cs);
@@ -494,8 +494,8 @@ void factorize_t(aphi_mesh* m, double pivot_floor, aphi_factor_stats* out, std::
     HIPCHK(hipEventRecord(m->ctx->ev0, s));
     if (!timed && factor_graphs() && m->graphs_on) {
         const int prec = std::is_same<T, float>::value ? APHI_FP32 : APHI_FP64;
-        const std::vector<const void*> key = {m->P.p,     m->CB.p,    m->mws.p, m->Xinv.p, m->dinv.p, m->X256.p,
-                                              m->XT256.p, m->stats.p, m->Aeq.p, m->s2p.p,  m->nodes.p, m->LW.p};
+        const std::vector<const void*> key = {m->P.p,     m->CB.p,    m->mws.p, m->Xinv.p, m->dinv.p, m->Xsb.p,
+                                              m->XTsb.p, m->stats.p, m->Aeq.p, m->s2p.p,  m->nodes.p, m->LW.p};
         auto& g = m->fgraph;
         if (!(g.exec && g.precision == prec && g.pivot_floor == pivot_floor && g.key == key &&
               g.tiles_version == m->tiles_version)) {
@@ -652,14 +652,14 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         const int maxm = std::max(L.small_max_m, L.large_max_m);
         if (aphi_dev::small_level<T>(maxk, maxm)) {
             // gather, diagonal step and update vector of every front in one launch
-            aphi_dev::launch_fwd_small<T>(s, nodes + L.off, cnt, maxm, nrhs, fp, m->P.p, m->X256.p, m->bt.p, n,
+            aphi_dev::launch_fwd_small<T>(s, nodes + L.off, cnt, maxm, nrhs, fp, m->P.p, m->Xsb.p, m->bt.p, n,
                                           m->rows_total, m->W.p, m->Yv.p);
             continue;
         }
         aphi_dev::launch_fwd_gather<T>(s, nodes + L.off, cnt, maxm, nrhs, fp, m->bt.p, n, m->rows_total, m->W.p);
         for (int B = 0; B < maxk; B += SB)
             aphi_dev::launch_fwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), tri_rows(L, B), nrhs, fp,
-                                         m->P.p, m->X256.p, m->W.p, m->Yv.p, n, m->rows_total);
+                                         m->P.p, m->Xsb.p, m->W.p, m->Yv.p, n, m->rows_total);
         aphi_dev::launch_fwd_rect<T>(s, nodes + L.off, cnt, maxk, rect_rows(L), nrhs, fp, m->P.p, m->W.p, m->Yv.p, n,
                                      m->rows_total, m->fpart.p);
     }
@@ -675,13 +675,13 @@ void solve_t(aphi_mesh* m, int nrhs, const cd* b, int64_t ldb, cd* x, int64_t ld
         const int rk = aphi_dev::rect_cols(maxk);
         for (int B = ((maxk - 1) / SB) * SB; B >= 0; B -= SB)
             aphi_dev::launch_bwd_step<T>(s, nodes + L.off, cnt, B, std::min(SB, maxk - B), nrhs, fp, m->P.p, m->dinv.p,
-                                         m->XT256.p, m->Yv.p, m->X.p, n, const_cast<void*>(t), rk, all);
+                                         m->XTsb.p, m->Yv.p, m->X.p, n, const_cast<void*>(t), rk, all);
     }
     };
     if (solve_graphs() && m->solve_graphs_on) {
         const int prec = std::is_same<T, float>::value ? APHI_FP32 : APHI_FP64;
         const int groups = aphi_dev::rhs_groups(nrhs);
-        const std::vector<const void*> key = {m->P.p,  m->X256.p, m->XT256.p, m->dinv.p, m->bt.p, m->W.p,
+        const std::vector<const void*> key = {m->P.p,  m->Xsb.p, m->XTsb.p, m->dinv.p, m->bt.p, m->W.p,
                                               m->Yv.p, m->X.p,    m->rpart.p, m->trect.p, m->fpart.p, m->nodes.p};
         SolveGraph& sg = m->sgraph;
         if (!(sg.exec && sg.precision == prec && sg.groups == groups && sg.key == key)) {
@@ -1282,9 +1282,9 @@ std::unique_ptr<aphi_mesh> build_mesh(aphi_context* ctx, const aphi_mesh_desc* d
                 off += (kk / aphi_dev::kSB) * aphi_dev::kSB * aphi_dev::kSB + rest * rest;
                 for (int64_t B = 0; B < kk; B += aphi_dev::kSB) pairs.push_back(int2{s, static_cast<int>(B)});
             }
-            m->x256_entries = off;
+            m->xsb_entries = off;
             m->num_inv_pairs = static_cast<int>(pairs.size());
-            upload(m->st(), m->x256_off, xb.data(), xb.size());
+            upload(m->st(), m->xsb_off, xb.data(), xb.size());
             upload(m->st(), m->inv_pairs, pairs.data(), pairs.size());
         }
         upload(m->st(), m->ea_start, ea.data(), ea.size());
@@ -1389,7 +1389,7 @@ extern "C" int aphi_mesh_release_workspace(aphi_mesh* m) {
         // (the assembly error flag stays: it is allocated with the plan)
         DevBuf* numeric[] = {&m->coef, &m->E,     &m->A,     &m->rhs,  &m->Aeq,      &m->work,  &m->rowmax,
                              &m->fac,  &m->d,     &m->partials, &m->stats, &m->P, &m->CB,       &m->mws,   &m->Xinv,
-                             &m->X256, &m->XT256, &m->dinv,  &m->bt,    &m->W,    &m->X,        &m->Yv,    &m->LW,
+                             &m->Xsb, &m->XTsb, &m->dinv,  &m->bt,    &m->W,    &m->X,        &m->Yv,    &m->LW,
                              &m->rpart, &m->trect, &m->fpart, &m->colscale, &m->res, &m->norms, &m->xil, &m->gV,       &m->gZ,    &m->gW,
                              &m->gU,   &m->gDots, &m->gSmall, &m->tcontrib, &m->tcurrent, &m->fval, &m->fvol,
                              &m->fmu,  &m->fsig,  &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node,
@@ -1517,7 +1517,7 @@ extern "C" int64_t aphi_mesh_device_bytes(const aphi_mesh* m) {
     const DevBuf* all[] = {&m->coords, &m->tets, &m->tet_body, &m->tptr, &m->entry, &m->kind, &m->row_ptr, &m->col,
                            &m->row_of, &m->ttptr, &m->tslot, &m->trow, &m->perm, &m->col_ref, &m->first, &m->srow_ptr,
                            &m->srows, &m->rel, &m->child_ptr, &m->child, &m->panel_off, &m->cb_off, &m->s2p, &m->nodes,
-                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->x256_off, &m->woff, &m->LW, &m->inv_pairs, &m->Xinv, &m->X256, &m->XT256, &m->rpart, &m->trect, &m->fpart, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->fptr, &m->fentry, &m->bmu, &m->fval, &m->fvol, &m->fmu, &m->fsig, &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
+                           &m->ea_start, &m->xinv_off, &m->ea_bound_off, &m->ea_bound, &m->xsb_off, &m->woff, &m->LW, &m->inv_pairs, &m->Xinv, &m->Xsb, &m->XTsb, &m->rpart, &m->trect, &m->fpart, &m->tdofs, &m->tsign, &m->tcoef, &m->tfixed, &m->bsigma, &m->tcontrib, &m->tcurrent, &m->fptr, &m->fentry, &m->bmu, &m->fval, &m->fvol, &m->fmu, &m->fsig, &m->fa_tet, &m->fb_tet, &m->fe_tet, &m->fj_tet, &m->fa_node, &m->fb_node, &m->fh_node, &m->fe_node, &m->fj_node, &m->fweight, &m->fflag, &m->gV, &m->gZ, &m->gW, &m->gU, &m->gDots, &m->gSmall, &m->Yv, &m->coef, &m->E, &m->flag, &m->A, &m->rhs, &m->Aeq, &m->work, &m->rowmax,
                            &m->fac, &m->d, &m->partials, &m->stats, &m->P, &m->CB, &m->mws, &m->bt, &m->W, &m->X,
                            &m->colscale, &m->res, &m->norms, &m->xil, &m->dinv, &m->gf.coords,
                            &m->gf.tets, &m->gf.tet_body, &m->gf.tet_on, &m->gf.column, &m->gf.prescribed, &m->gf.tptr,

@@ -20,7 +20,7 @@ constexpr int kWCols = 2 * kOuter;  // columns of a front's W window (complex fl
 // height at 8192, 577 ms adding width at 512, 623 ms with all)
 constexpr int kSmallTileRows = 8192;
 constexpr int kSmallTileCols = 512;
-constexpr int kSB = 256;        // solve block: columns per forward/backward step (multiple of kNB)
+constexpr int kSB = 512;        // solve block: columns per forward/backward step (multiple of 128)
 constexpr int kQ = 16;          // right-hand sides per interleaved solve group
 constexpr int kGatherRows = 256;  // front rows per forward-gather workgroup (a multiple of 64)
 constexpr int kRectRows = 1024;   // rows per backward partial sum below a front's pivots
@@ -86,7 +86,7 @@ void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const Fron
 // take W -= L y_B; max_tri_rows: the most such rows among the level's supernodes
 template <typename T>
 void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_tri_rows, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R);
+                     const FrontPlan& fp, const void* P, const void* Xsb, void* W, void* Y, int n, int64_t R);
 // After a level's chain: the rows [k, m) below every front's pivots take
 // W -= L[k:m, 0:k] y[0:k] (max_rect_rows: the most such rows, max_k the most
 // pivots). The fronts' blocks are split fwd_rect_split ways when the row tiles
@@ -114,7 +114,7 @@ const void* launch_bwd_rect(hipStream_t st, const int* nodes, int count, int max
 // t[0:B] += L[B:B+nb, 0:B]^T x_B (t from launch_bwd_rect, column stride rk)
 template <typename T>
 void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int nrhs, const FrontPlan& fp,
-                     const void* P, const void* dinv, const void* XT256, const void* Y, void* X, int n, void* t, int rk,
+                     const void* P, const void* dinv, const void* XTsb, const void* Y, void* X, int n, void* t, int rk,
                      bool all);
 // Levels whose fronts have at most kSmallK pivots and whose front vectors fit in
 // kSmallLds: the forward step in one launch, one workgroup per front (gather,
@@ -122,7 +122,7 @@ void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max
 template <typename T> bool small_level(int max_k, int max_m);
 template <typename T>
 void launch_fwd_small(hipStream_t st, const int* nodes, int count, int max_m, int nrhs, const FrontPlan& fp,
-                      const void* P, const void* X256, const void* b, int n, int64_t R, void* W, void* Y);
+                      const void* P, const void* Xsb, const void* b, int n, int64_t R, void* W, void* Y);
 // dinv[j] = 1 / D[j] for every column (n entries), after a factorization
 template <typename T> void launch_dinv(hipStream_t st, int ns, const FrontPlan& fp, const void* P, void* dinv);
 void launch_column_scale(hipStream_t st, int n, int nrhs, const cd* b, int64_t ldb, double* partials, double* colscale,

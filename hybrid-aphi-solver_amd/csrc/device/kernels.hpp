@@ -46,7 +46,7 @@ struct FrontPlan {
     const int64_t* ea_bound_off;  // ns
     const int* ea_bound;
     // Inverses of the kSB-column diagonal blocks (the last one nb x nb), for the solves.
-    const int64_t* x256_off;   // ns
+    const int64_t* xsb_off;   // ns
     // Complex-float factorization: W = -L D of a front's columns, column c at
     // W + woff[s] + (c mod kWCols) * m (two outer blocks of columns, so the update of
     // one outer block can read its window while the next block's panels fill the

@@ -210,8 +210,8 @@ __global__ void __launch_bounds__(256) k_inv_blocks(const int2* __restrict__ pai
     const int jb = static_cast<int>(blockIdx.y);
     if (jb >= nblk) return;
     const C<T>* Ps = P + fp.panel_off[s];
-    C<T>* Xs = X + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
-    C<T>* XTs = XT + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
+    C<T>* Xs = X + fp.xsb_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
+    C<T>* XTs = XT + fp.xsb_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
     const int tid = threadIdx.x;
     const int row = tid % 64, g = tid / 64;  // output row and a group of 16 output columns
     const int c0 = jb * kNB, ncj = min(kNB, nb - c0);
@@ -297,11 +297,11 @@ __global__ void __launch_bounds__(64) k_dinv(FrontPlan fp, const C<T>* __restric
 // rows below the block take W[i] -= L[i, B:B+nb] y_B.
 
 // y rows [16 bx, 16 bx + 16) of the block: y = X w (X lower triangular, stored
-// with zeros above the diagonal); wave w takes the columns [64 w, 64 w + 64)
-// (NW = 1: blocks of at most 64 columns)
+// with zeros above the diagonal); wave w takes the columns 64 w + 256 j, 64 at a
+// time (NW = 1: blocks of at most 64 columns)
 template <typename T, int NW = 4>
 __global__ void __launch_bounds__(64 * NW) k_fwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
-                                                      const C<T>* __restrict__ X256, const C<T>* __restrict__ W,
+                                                      const C<T>* __restrict__ Xsb, const C<T>* __restrict__ W,
                                                       C<T>* __restrict__ Y, int n, int64_t R) {
     __shared__ C<T> part[NW][256];
     const int s = nodes[blockIdx.y];
@@ -313,15 +313,15 @@ __global__ void __launch_bounds__(64 * NW) k_fwd_diag(const int* __restrict__ no
     if (c0 >= nb) return;
     const int kmax = min(nb, c0 + 16);  // X[c][k] = 0 for k > c
     const int k0 = 64 * (static_cast<int>(threadIdx.x) / 64);
-    const C<T>* Xs = X256 + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
+    const C<T>* Xs = Xsb + fp.xsb_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
     const C<T>* Ws = W + (static_cast<int64_t>(g) * R + x.rp + B) * kQ;
     V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-    if (k0 < kmax)
+    for (int kc = k0; kc < kmax; kc += 64 * NW)
         cmma64<T>(accr, acci,
-                  [&](int p, int k) { return Xs[static_cast<int64_t>(min(k0 + k, nb - 1)) * nb + min(c0 + p, nb - 1)]; },
+                  [&](int p, int k) { return Xs[static_cast<int64_t>(min(kc + k, nb - 1)) * nb + min(c0 + p, nb - 1)]; },
                   [&](int k, int q) {
-                      const C<T> v = Ws[min(k0 + k, kmax - 1) * kQ + q];  // clamped read, then select
-                      return k0 + k < kmax ? v : C<T>{0, 0};
+                      const C<T> v = Ws[min(kc + k, kmax - 1) * kQ + q];  // clamped read, then select
+                      return kc + k < kmax ? v : C<T>{0, 0};
                   });
     C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
     reduce_waves<T, NW>(part, accr, acci, [&](int p, int q, C<T> t) {
@@ -361,7 +361,7 @@ struct FwdOut {
 };
 
 // W[i] -= L[i, blocks] y_blocks for the 16 rows i0.. of the update rows; wave w
-// takes the columns [64 w, 64 w + 64) of each block
+// takes the columns 64 w + 256 j of each block (64 at a time)
 template <typename T, bool RECT>
 __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ nodes, int B, FrontPlan fp,
                                                     const C<T>* __restrict__ P, C<T>* W, const C<T>* __restrict__ Y,
@@ -382,14 +382,14 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
     for (int Bb = rr.b0; Bb < rr.b1; Bb += kSB) {
         const int nb = min(kSB, x.k - Bb);
         const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + Bb) * kQ;
-        if (k0 < nb)
+        for (int kc = k0; kc < nb; kc += 256)
             cmma64<T>(accr, acci,
                       [&](int p, int k) {
-                          return Ps[static_cast<int64_t>(Bb + min(k0 + k, nb - 1)) * x.m + min(i0 + p, x.m - 1)];
+                          return Ps[static_cast<int64_t>(Bb + min(kc + k, nb - 1)) * x.m + min(i0 + p, x.m - 1)];
                       },
                       [&](int k, int q) {
-                          const C<T> v = Yg[min(k0 + k, nb - 1) * kQ + q];  // clamped read, then select
-                          return k0 + k < nb ? v : C<T>{0, 0};
+                          const C<T> v = Yg[min(kc + k, nb - 1) * kQ + q];  // clamped read, then select
+                          return kc + k < nb ? v : C<T>{0, 0};
                       });
     }
     C<T>* Wg = W + (static_cast<int64_t>(g) * R + x.rp) * kQ;
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(256) k_fwd_update(const int* __restrict__ node
 
 // One wave's sums of the complex-float forward update for the 32 rows i0.. over
 // the columns [c0, c1) of a front (whole blocks of kSB, the last possibly short):
-// wave w takes the columns [64 w, 64 w + 64) of each block. Lane (p, kq) loads
+// wave w takes the columns [w kSB / 4, (w + 1) kSB / 4) of each block. Lane (p, kq) loads
 // rows i0 + 2p and i0 + 2p + 1 of a column with one 16-byte load (each load
 // instruction covers 256 contiguous bytes of four columns), the two rows going to
 // two 16-row MFMA tiles (even rows r0/i0a, odd rows r1/i1a; tile row r of tile h
@@ -422,19 +422,22 @@ __device__ __forceinline__ void fwd2_sums(const Sn& x, const cf* __restrict__ Ps
     using VF = V4<float>;
     typedef float f4 __attribute__((ext_vector_type(4)));
     const int tid = static_cast<int>(threadIdx.x), wave = tid / 64, lane = tid % 64;
-    const int p = lane % 16, kq = lane / 16, k0 = 64 * wave;
+    // a block's columns split into four wave slices of kSB / 4, each in stages of 32
+    // (one-wave workgroups: blocks of at most 64 columns, in the first slice)
+    constexpr int SPB = kSB / 128;  // stages per wave slice
+    const int p = lane % 16, kq = lane / 16, k0 = (kSB / 4) * wave;
     // three-product complex form (k1 = (ar + ai) br, k2 = ar (bi - br), k3 = ai (br + bi);
     // re = k1 - k3, im = k1 + k2): three MFMAs per complex product instead of four
     VF s1a = VF{0, 0, 0, 0}, s2a = VF{0, 0, 0, 0}, s3a = VF{0, 0, 0, 0};
     VF s1b = VF{0, 0, 0, 0}, s2b = VF{0, 0, 0, 0}, s3b = VF{0, 0, 0, 0};
     const int row = min(i0 + 2 * p, x.m - 1);  // rows row and row + 1 (the second may lie past m: never stored)
-    // The wave's columns in stages of 32: stage j is the k steps 8 (j % 2) ..
-    // 8 (j % 2) + 7 of its 64-column slice of block c0 + (j / 2) kSB; steps(j) is
-    // how many of them lie inside the block (uniform over the wave)
-    const int nstage = 2 * ((c1 - c0 + kSB - 1) / kSB);
+    // The wave's columns in stages of 32: stage j is the k steps 8 (j % SPB) ..
+    // 8 (j % SPB) + 7 of its slice of block c0 + (j / SPB) kSB; steps(j) is how
+    // many of them lie inside the block (uniform over the wave)
+    const int nstage = SPB * ((c1 - c0 + kSB - 1) / kSB);
     const auto steps = [&](int j) {
         if (j >= nstage) return 0;
-        const int Bb = c0 + (j / 2) * kSB, u0 = 8 * (j % 2);
+        const int Bb = c0 + (j / SPB) * kSB, u0 = 8 * (j % SPB);
         return max(0, min(8, (min(kSB, x.k - Bb) - k0 - 4 * u0 + 3) / 4));
     };
     // Four-wave workgroups load every step of a stage (clamped columns past the
@@ -443,7 +446,7 @@ __device__ __forceinline__ void fwd2_sums(const Sn& x, const cf* __restrict__ Ps
     // of at most 64 columns) load only the steps they use.
     const auto load = [&](int j, int nu, f4 (&a)[8], cf (&b)[8]) {
         if constexpr (NW > 1) j = min(j, nstage - 1);
-        const int Bb = c0 + (j / 2) * kSB, kb = k0 + 32 * (j % 2);
+        const int Bb = c0 + (j / SPB) * kSB, kb = k0 + 32 * (j % SPB);
         const int last = min(kSB, x.k - Bb) - 1;
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -808,11 +811,11 @@ __global__ void __launch_bounds__(256) k_rect_reduce(const int* __restrict__ nod
 // x rows [16 bx, 16 bx + 16) of the block: x[c] = sum_{k >= c} X[k][c] z[k],
 // z = y D^-1 - t[B + k] (t column stride rk; read for every front when all is set,
 // else only for fronts with rows below their pivots, zero for the others or when t
-// is null); D^-1 from k_dinv; wave w takes k in [c0 + 64 w, c0 + 64 w + 64) (NW = 1:
+// is null); D^-1 from k_dinv; wave w takes k in [c0 + 64 w + 256 j, + 64) (NW = 1:
 // blocks of at most 64 columns)
 template <typename T, int NW = 4>
 __global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ nodes, int B, FrontPlan fp,
-                                                      const C<T>* __restrict__ dinv, const C<T>* __restrict__ XT256,
+                                                      const C<T>* __restrict__ dinv, const C<T>* __restrict__ XTsb,
                                                       const C<T>* __restrict__ Y, const C<T>* __restrict__ tl, int rk,
                                                       int all, C<T>* __restrict__ X, int n) {
     __shared__ C<T> part[NW][256];
@@ -826,17 +829,17 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ no
     const int k0 = c0 + 64 * (static_cast<int>(threadIdx.x) / 64);
     const bool has_t = tl != nullptr && (all || x.m > x.k);
     const C<T>* dv = dinv + x.first + B;
-    const C<T>* XTs = XT256 + fp.x256_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
+    const C<T>* XTs = XTsb + fp.xsb_off[s] + static_cast<int64_t>(B / kSB) * kSB * kSB;
     const C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first + B) * kQ;
     const C<T>* tg = has_t ? tl + ((static_cast<int64_t>(blockIdx.y) * G + g) * rk + B) * kQ : nullptr;
     V4<T> accr = V4<T>{0, 0, 0, 0}, acci = V4<T>{0, 0, 0, 0};
-    if (k0 < nb) {
+    for (int kc = k0; kc < nb; kc += 64 * NW) {
         // every load (clamped indices) issued before the first wait, then z, then the MFMAs
         const int lane = static_cast<int>(threadIdx.x % 64), p = lane % 16, kq = lane / 16;
         C<T> a[16], yv[16], dw[16], tv[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int kk = min(k0 + 4 * u + kq, nb - 1);
+            const int kk = min(kc + 4 * u + kq, nb - 1);
             a[u] = XTs[static_cast<int64_t>(kk) * nb + min(c0 + p, nb - 1)];
             yv[u] = Yg[kk * kQ + p];
             dw[u] = dv[kk];
@@ -846,7 +849,7 @@ __global__ void __launch_bounds__(64 * NW) k_bwd_diag(const int* __restrict__ no
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             const C<T> z = gsub(gmul(yv[u], dw[u]), tv[u]);
-            const C<T> b = k0 + 4 * u + kq < nb ? z : C<T>{0, 0};
+            const C<T> b = kc + 4 * u + kq < nb ? z : C<T>{0, 0};
             accr = Mfma<T>::mac(a[u].re, b.re, accr);
             accr = Mfma<T>::mac(-a[u].im, b.im, accr);
             acci = Mfma<T>::mac(a[u].re, b.im, acci);
@@ -894,7 +897,7 @@ __device__ __forceinline__ void cmma_k(V4<T>& accr, V4<T>& acci, int steps, FA a
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_fwd_small(const int* __restrict__ nodes, FrontPlan fp,
-                                                   const C<T>* __restrict__ P, const C<T>* __restrict__ X256,
+                                                   const C<T>* __restrict__ P, const C<T>* __restrict__ Xsb,
                                                    const C<T>* __restrict__ bt, int n, int64_t R, C<T>* W,
                                                    C<T>* __restrict__ Y) {
     extern __shared__ unsigned char smem_raw[];
@@ -924,7 +927,7 @@ __global__ void __launch_bounds__(256) k_fwd_small(const int* __restrict__ nodes
     __syncthreads();
     // y = X w[0:k] (X[r][j] is stored as zero for j > r)
     const int nb = x.k;
-    const C<T>* Xs = X256 + fp.x256_off[s];
+    const C<T>* Xs = Xsb + fp.xsb_off[s];
     C<T>* Yg = Y + (static_cast<int64_t>(g) * n + x.first) * kQ;
     for (int t = wave; 16 * t < nb; t += 4) {
         const int r0 = 16 * t;
@@ -1016,17 +1019,17 @@ void launch_inv_blocks(hipStream_t st, const int2* pairs, int npairs, const Fron
 
 template <typename T>
 void launch_fwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int max_tri_rows, int nrhs,
-                     const FrontPlan& fp, const void* P, const void* X256, void* W, void* Y, int n, int64_t R) {
+                     const FrontPlan& fp, const void* P, const void* Xsb, void* W, void* Y, int n, int64_t R) {
     if (count <= 0 || max_nb <= 0) return;
     const int G = rhs_groups(nrhs);
     // blocks of at most 64 columns (the levels of many small fronts): one-wave workgroups
     const bool one = max_nb <= 64;
     if (one)
         hipLaunchKernelGGL((k_fwd_diag<T, 1>), dim3((max_nb + 15) / 16, count, G), dim3(64), 0, st, nodes, B, fp,
-                           static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
+                           static_cast<const C<T>*>(Xsb), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
     else
         hipLaunchKernelGGL((k_fwd_diag<T, 4>), dim3((max_nb + 15) / 16, count, G), dim3(256), 0, st, nodes, B, fp,
-                           static_cast<const C<T>*>(X256), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
+                           static_cast<const C<T>*>(Xsb), static_cast<const C<T>*>(W), static_cast<C<T>*>(Y), n, R);
     if (max_tri_rows > 0) {
         if constexpr (std::is_same<T, float>::value) {
             if (one)
@@ -1120,18 +1123,18 @@ const void* launch_bwd_rect(hipStream_t st, const int* nodes, int count, int max
 
 template <typename T>
 void launch_bwd_step(hipStream_t st, const int* nodes, int count, int B, int max_nb, int nrhs, const FrontPlan& fp,
-                     const void* P, const void* dinv, const void* XT256, const void* Y, void* X, int n, void* t, int rk,
+                     const void* P, const void* dinv, const void* XTsb, const void* Y, void* X, int n, void* t, int rk,
                      bool all) {
     if (count <= 0 || max_nb <= 0) return;
     const int G = rhs_groups(nrhs);
     const int ctiles = (max_nb + 15) / 16;
     if (max_nb <= 64)
         hipLaunchKernelGGL((k_bwd_diag<T, 1>), dim3(ctiles, count, G), dim3(64), 0, st, nodes, B, fp,
-                           static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
+                           static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XTsb), static_cast<const C<T>*>(Y),
                            static_cast<const C<T>*>(t), rk, all ? 1 : 0, static_cast<C<T>*>(X), n);
     else
         hipLaunchKernelGGL((k_bwd_diag<T, 4>), dim3(ctiles, count, G), dim3(256), 0, st, nodes, B, fp,
-                           static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XT256), static_cast<const C<T>*>(Y),
+                           static_cast<const C<T>*>(dinv), static_cast<const C<T>*>(XTsb), static_cast<const C<T>*>(Y),
                            static_cast<const C<T>*>(t), rk, all ? 1 : 0, static_cast<C<T>*>(X), n);
     if (B == 0) return;
     // the columns before the block take its rows: t[0:B] += L[B:B+nb, 0:B]^T x_B
@@ -1157,12 +1160,12 @@ bool small_level(int max_k, int max_m) {
 
 template <typename T>
 void launch_fwd_small(hipStream_t st, const int* nodes, int count, int max_m, int nrhs, const FrontPlan& fp,
-                      const void* P, const void* X256, const void* b, int n, int64_t R, void* W, void* Y) {
+                      const void* P, const void* Xsb, const void* b, int n, int64_t R, void* W, void* Y) {
     if (count <= 0) return;
     const size_t lds = static_cast<size_t>(max_m + kSmallK) * kQ * sizeof(C<T>);
     allow_large_lds(k_fwd_small<T>, lds);
     hipLaunchKernelGGL(k_fwd_small<T>, dim3(count, rhs_groups(nrhs)), dim3(256), lds, st, nodes, fp,
-                       static_cast<const C<T>*>(P), static_cast<const C<T>*>(X256), static_cast<const C<T>*>(b), n, R,
+                       static_cast<const C<T>*>(P), static_cast<const C<T>*>(Xsb), static_cast<const C<T>*>(b), n, R,
                        static_cast<C<T>*>(W), static_cast<C<T>*>(Y));
 }
 

@@ -1,6 +1,6 @@
 """The solve's level structure on cubes whose levels exercise every path: levels
-of many small fronts (one-launch forward steps), block chains of 2 to 22 blocks of
-256 pivots, rows below the pivots in one launch per level with the forward sums
+of many small fronts (one-launch forward steps), block chains of 2 to 11 blocks of
+512 pivots, rows below the pivots in one launch per level with the forward sums
 split over column blocks, backward partial sums over several 1024-row chunks, and
 more than one group of 16 right-hand sides.
 
