@@ -358,7 +358,7 @@ int64_t factor_sequence(aphi_mesh* m, double pivot_floor, double& update_flops, 
         if (!L.large_count) continue;
         const int* lg = nodes + L.off + L.small_count;
         aphi_dev::launch_extend_add<T>(s, lg, m->ea_start.as<int>() + L.ea_off, L.large_count, L.ea_blocks,
-                                       (L.large_max_m + aphi_dev::kEaRows - 1) / aphi_dev::kEaRows, fp, m->P.p, m->CB.p);
+                                       L.large_max_m, fp, m->P.p, m->CB.p);
         int maxk = 0;
         for (int i = 0; i < L.large_count; ++i) maxk = std::max(maxk, L.k[static_cast<size_t>(L.small_count + i)]);
         // one update launch on stream `on`: the largest tile count over the level's

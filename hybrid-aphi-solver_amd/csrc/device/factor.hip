@@ -273,76 +273,141 @@ __global__ void __launch_bounds__(256) k_front_small(const int* __restrict__ nod
 // ------------------------------------------------------------------ large fronts
 
 // Extend-add for large fronts. Workgroup = (front, block of 64 front columns,
-// chunk of kEaRows front rows); it owns those entries exclusively, clears their
-// update-block part, then adds each child's contributions in child order --
-// deterministic without atomics. The child index ranges come from the
-// precomputed bounds; each wave takes two child columns at a time with all of
-// their rows (kEaRows / 64 per lane) loaded before any store.
+// chunk of ea_rows<T>() front rows); it owns those entries exclusively. It maps
+// the chunk's rows and columns to up to kEaChildren children at a time (LDS, -1
+// where the child has none; the child index ranges come from the precomputed
+// bounds), then takes the columns 16 / (ea_rows<T>() / 64) per wave at a time
+// with the entries in registers: wave w holds columns c0 + w + 4t (t < 16), lane l rows r0 + l + 64u.
+// In child order every entry adds its child entry if there is one. Each update-block
+// entry is then written once (zero where no child contributes) and each touched
+// pivot-column entry becomes P + (sum of its children's entries): one write per
+// entry, deterministic without atomics.
 template <typename T>
 __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ nodes, const int* __restrict__ blk_start,
                                                     int nnodes, FrontPlan fp, C<T>* P, C<T>* CB) {
-    constexpr int RL = kEaRows / 64;  // rows per lane per column
-    // locate the front this block belongs to
+    constexpr int R = ea_rows<T>();
+    constexpr int CT = 16;  // columns per wave
+    constexpr int NCH = kEaChildren;
+    __shared__ int invc[NCH][64];
+    __shared__ int invr[NCH][R];
+    __shared__ int cinfo[NCH][5];     // child columns [a, e) and rows [ia, ie) in the chunk, update rows
+    __shared__ int64_t coff[NCH][2];  // offsets of the child's update-row indices and update block
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / 64);
+    const int bx = static_cast<int>(blockIdx.x);
+    // the front of this block: 64-way search, blk_start[lo] <= bx < blk_start[hi]
     int lo = 0, hi = nnodes;
     while (hi - lo > 1) {
-        const int mid = (lo + hi) / 2;
-        if (blk_start[mid] <= static_cast<int>(blockIdx.x)) lo = mid;
-        else hi = mid;
+        const int step = (hi - lo + 63) / 64;
+        const int idx = lo + lane * step;
+        const unsigned long long le = __ballot(idx < hi && blk_start[idx] <= bx);
+        lo += (63 - __builtin_clzll(le)) * step;  // lane 0 always qualifies
+        hi = min(hi, lo + step);
     }
     const int s = nodes[lo];
     const FrontRef<T> fr = front<T>(fp, s, P, CB);
-    const int cblk = static_cast<int>(blockIdx.x) - blk_start[lo];
+    const int cblk = bx - blk_start[lo];
     const int c0 = cblk * 64;
     const int c1 = min(c0 + 64, fr.m);
-    const int r0 = max(c0, static_cast<int>(blockIdx.y) * kEaRows);
-    const int r1 = min(fr.m, (static_cast<int>(blockIdx.y) + 1) * kEaRows);
+    const int r0 = max(c0, static_cast<int>(blockIdx.y) * R);
+    const int r1 = min(fr.m, (static_cast<int>(blockIdx.y) + 1) * R);
     if (r0 >= r1) return;
     const int q0 = r0 / 64, q1 = (r1 + 63) / 64;  // row range in 64-blocks (r0, r1 multiples of 64 or m)
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int cfirst = fp.child_ptr[s], clast = fp.child_ptr[s + 1];
 
-    // clear owned update-block entries (lower part)
-    for (int j = max(c0, fr.k) + wave; j < c1; j += 4)
-        for (int i = max(j, r0) + lane; i < r1; i += 64) *fr.at(i, j) = C<T>{0, 0};
-
-    for (int c = fp.child_ptr[s]; c < fp.child_ptr[s + 1]; ++c) {
+    // maps of children [cc, cc + nc) for the whole chunk
+    auto build_maps = [&](int cc, int nc) {
+        __syncthreads();  // the previous maps are no longer read
+        for (int i = tid; i < NCH * 64; i += 256) (&invc[0][0])[i] = -1;
+        for (int i = tid; i < NCH * R; i += 256) (&invr[0][0])[i] = -1;
+        if (tid < nc) {
+            const int ch = fp.child[cc + tid];
+            const int* bnd = fp.ea_bound + fp.ea_bound_off[ch];
+            const int ck = fp.first[ch + 1] - fp.first[ch];
+            cinfo[tid][0] = bnd[cblk];
+            cinfo[tid][1] = bnd[cblk + 1];
+            cinfo[tid][2] = bnd[q0];
+            cinfo[tid][3] = bnd[q1];
+            cinfo[tid][4] = static_cast<int>(fp.row_ptr[ch + 1] - fp.row_ptr[ch]) - ck;
+            coff[tid][0] = fp.row_ptr[ch] + ck;
+            coff[tid][1] = fp.cb_off[ch];
+        }
         __syncthreads();
-        const int ch = fp.child[c];
-        const int cm = static_cast<int>(fp.row_ptr[ch + 1] - fp.row_ptr[ch]);
-        const int ck = fp.first[ch + 1] - fp.first[ch];
-        const int cu = cm - ck;
-        const int* rel = fp.rel + fp.row_ptr[ch] + ck;
-        const int* bnd = fp.ea_bound + fp.ea_bound_off[ch];
-        const int a = bnd[cblk], e = bnd[cblk + 1];   // child columns landing in [c0, c1)
-        const int ia = bnd[q0], ie = bnd[q1];         // child rows landing in [r0, r1)
-        const C<T>* cb = CB + fp.cb_off[ch];
-        for (int jb = a + 2 * wave; jb < e; jb += 8) {
-            int dr[2][RL];
-            C<T> v[2][RL];
-            int pj[2];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int jc = jb + h;
-                pj[h] = jc < e ? rel[jc] : 0;
-                const int base = max(jc, ia);
-#pragma unroll
-                for (int u = 0; u < RL; ++u) {
-                    const int ic = base + lane + 64 * u;
-                    const bool ok = jc < e && ic < ie;
-                    dr[h][u] = ok ? rel[ic] : -1;
-                    v[h][u] = ok ? cb[static_cast<int64_t>(jc) * cu + ic] : C<T>{0, 0};
-                }
+        for (int i = tid; i < nc * (64 + R); i += 256) {
+            const int c = i / (64 + R), r = i % (64 + R);
+            const int* rel = fp.rel + coff[c][0];
+            if (r < 64) {
+                const int jc = cinfo[c][0] + r;
+                if (jc < cinfo[c][1]) invc[c][rel[jc] - c0] = jc;
+            } else {
+                const int ic = cinfo[c][2] + (r - 64);
+                if (ic < cinfo[c][3]) invr[c][rel[ic] - r0] = ic;
             }
-            C<T> cur[2][RL];
+        }
+        __syncthreads();
+    };
+    const bool one_map = clast - cfirst <= NCH;
+    if (one_map && clast > cfirst) build_maps(cfirst, clast - cfirst);
+
+    // passes of CP columns per wave; a pass reads and writes whole R-row column
+    // segments (RL rows per lane), 16 entries per lane
+    constexpr int RL = R / 64, CP = 16 / RL;
+    static_assert(RL * CP == 16 && CT % CP == 0, "16 entries per lane and pass");
+    for (int q = 0; q < CT / CP; ++q) {
+        C<T> acc[CP][RL];
+        uint32_t touched = 0;  // bit g * RL + u: a child contributed to (column g, row u)
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+        for (int g = 0; g < CP; ++g)
 #pragma unroll
-                for (int u = 0; u < RL; ++u)
-                    if (dr[h][u] >= 0) cur[h][u] = *fr.at(dr[h][u], pj[h]);
+            for (int u = 0; u < RL; ++u) acc[g][u] = C<T>{0, 0};
+        for (int cc = cfirst; cc < clast; cc += NCH) {
+            const int nc = min(NCH, clast - cc);
+            if (!one_map) build_maps(cc, nc);
+            for (int c = 0; c < nc; ++c) {
+                if (cinfo[c][0] >= cinfo[c][1] || cinfo[c][2] >= cinfo[c][3]) continue;  // nothing in this chunk
+                const int cu = __builtin_amdgcn_readfirstlane(cinfo[c][4]);
+                const C<T>* cb = CB + coff[c][1];
+                int cr[RL];
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+                for (int u = 0; u < RL; ++u) cr[u] = invr[c][lane + 64 * u];
+                C<T> v[CP][RL];
+                bool ok[CP][RL];
 #pragma unroll
-                for (int u = 0; u < RL; ++u)
-                    if (dr[h][u] >= 0) *fr.at(dr[h][u], pj[h]) = gadd(cur[h][u], v[h][u]);
+                for (int g = 0; g < CP; ++g) {
+                    const int cj = __builtin_amdgcn_readfirstlane(invc[c][wave + 4 * (CP * q + g)]);
+                    const C<T>* col = cb + static_cast<int64_t>(cj) * cu;
+#pragma unroll
+                    for (int u = 0; u < RL; ++u) {
+                        // cr >= cj >= 0 puts the entry in the child's lower part (and the
+                        // parent's: the relative indices increase)
+                        ok[g][u] = cj >= 0 && cr[u] >= cj;
+                        v[g][u] = C<T>{0, 0};
+                        if (ok[g][u]) v[g][u] = col[cr[u]];
+                    }
+                }
+#pragma unroll
+                for (int g = 0; g < CP; ++g)
+#pragma unroll
+                    for (int u = 0; u < RL; ++u)
+                        if (ok[g][u]) {
+                            acc[g][u] = gadd(acc[g][u], v[g][u]);
+                            touched |= 1u << (g * RL + u);
+                        }
+            }
+        }
+        // update-block entries (j >= k) always, pivot-column entries where touched
+#pragma unroll
+        for (int g = 0; g < CP; ++g) {
+            const int j = c0 + wave + 4 * (CP * q + g);
+            if (j >= c1) break;
+#pragma unroll
+            for (int u = 0; u < RL; ++u) {
+                const int i = r0 + lane + 64 * u;
+                if (i >= r1 || i < j) continue;
+                C<T>* dst = fr.at(i, j);
+                if (j >= fr.k) *dst = acc[g][u];
+                else if (touched >> (g * RL + u) & 1u) *dst = gadd(*dst, acc[g][u]);
+            }
         }
     }
 }
@@ -949,7 +1014,8 @@ void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, 
 
 template <typename T>
 void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
-                       int row_chunks, const FrontPlan& fp, void* P, void* CB) {
+                       int max_m, const FrontPlan& fp, void* P, void* CB) {
+    const int row_chunks = (max_m + ea_rows<T>() - 1) / ea_rows<T>();
     if (count <= 0 || total_blocks <= 0 || row_chunks <= 0) return;
     hipLaunchKernelGGL(k_extend_add<T>, dim3(total_blocks, row_chunks), dim3(256), 0, st, nodes, blk_start, count, fp,
                        static_cast<C<T>*>(P), static_cast<C<T>*>(CB));

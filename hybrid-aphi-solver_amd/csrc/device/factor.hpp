@@ -9,7 +9,10 @@
 namespace aphi_dev {
 
 constexpr int kNB = 64;       // block-column width of the blocked front factorization and solves
-constexpr int kEaRows = 256;  // rows per extend-add workgroup (a multiple of 64)
+// rows per extend-add workgroup (64, 128, 256, 512 or 1024): a pass holds R / 64
+// rows of 16 / (R / 64) columns per lane
+template <typename T> constexpr int ea_rows() { return 256; }
+constexpr int kEaChildren = 8;  // children an extend-add workgroup maps at once
 constexpr int kOuter = 512;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
 constexpr int kWCols = 2 * kOuter;  // columns of a front's W window (complex float)
@@ -43,7 +46,7 @@ void launch_front_small(hipStream_t st, const int* nodes, int count, int max_m, 
                         void* xinv, double pivot_floor, FactorStatsDev* stats);
 template <typename T>
 void launch_extend_add(hipStream_t st, const int* nodes, const int* blk_start, int count, int total_blocks,
-                       int row_chunks, const FrontPlan& fp, void* P, void* CB);
+                       int max_m, const FrontPlan& fp, void* P, void* CB);
 // Rank-(c1 - c0) update with the L columns [c0, c1 = min(c0 + width, k)) of each
 // front, of the columns [max(c1, min(tlo, k)), thi < 0 ? m : min(thi, k)) and
 // every row below them.
