@@ -350,9 +350,10 @@ __global__ void __launch_bounds__(256) k_extend_add(const int* __restrict__ node
     if (one_map && clast > cfirst) build_maps(cfirst, clast - cfirst);
 
     // passes of CP columns per wave; a pass reads and writes whole R-row column
-    // segments (RL rows per lane), 16 entries per lane
-    constexpr int RL = R / 64, CP = 16 / RL;
-    static_assert(RL * CP == 16 && CT % CP == 0, "16 entries per lane and pass");
+    // segments (RL rows per lane), EA entries per lane
+    constexpr int EA = ea_entries<T>();
+    constexpr int RL = R / 64, CP = EA / RL;
+    static_assert(RL * CP == EA && CT % CP == 0 && EA <= 32, "EA entries per lane and pass");
     for (int q = 0; q < CT / CP; ++q) {
         C<T> acc[CP][RL];
         uint32_t touched = 0;  // bit g * RL + u: a child contributed to (column g, row u)

@@ -9,9 +9,10 @@
 namespace aphi_dev {
 
 constexpr int kNB = 64;       // block-column width of the blocked front factorization and solves
-// rows per extend-add workgroup (64, 128, 256, 512 or 1024): a pass holds R / 64
-// rows of 16 / (R / 64) columns per lane
+// rows per extend-add workgroup (64, 128, 256, 512 or 1024) and entries per lane
+// and pass: a pass holds R / 64 rows of ea_entries / (R / 64) columns per lane
 template <typename T> constexpr int ea_rows() { return 256; }
+template <typename T> constexpr int ea_entries() { return sizeof(T) == 4 ? 16 : 8; }
 constexpr int kEaChildren = 8;  // children an extend-add workgroup maps at once
 constexpr int kOuter = 512;   // columns per outer block: the trailing update is rank kOuter
 constexpr int kPanelTail = 64;  // zeroed entries after the factor storage (reads past a column's end)
