@@ -26,6 +26,31 @@ def grid_matrix(k, seed):
     scaled symmetrically by factors spread over four decades (the equilibration's
     work)."""
     n, rp, ci = grid_pattern(k)
+    return n, rp, ci, dominant_values(n, rp, ci, seed)
+
+
+def star_pattern(hub, arms, arm, reach, seed):
+    """Upper triangle of a clique of `hub` vertices with `arms` cliques of `arm`
+    vertices around it, each arm vertex joined to about half of `reach` hub
+    vertices drawn per arm: the hub's front has one child per arm."""
+    rng = np.random.default_rng(seed)
+    n = hub + arms * arm
+    adj = [set(range(i + 1, hub)) if i < hub else set() for i in range(n)]
+    for a in range(arms):
+        base = hub + a * arm
+        targets = rng.choice(hub, size=reach, replace=False)
+        for i in range(arm):
+            adj[base + i].update(range(base + i + 1, base + arm))
+            for t in targets[rng.random(reach) < 0.5]:
+                adj[min(base + i, int(t))].add(max(base + i, int(t)))
+    rp, ci = [0], []
+    for i in range(n):
+        ci += [i] + sorted(adj[i])
+        rp.append(len(ci))
+    return n, np.array(rp, dtype=np.int64), np.array(ci, dtype=np.int64)
+
+
+def dominant_values(n, rp, ci, seed):
     rng = np.random.default_rng(seed)
     rows = np.repeat(np.arange(n), np.diff(rp))
     off = rows != ci
@@ -38,7 +63,7 @@ def grid_matrix(k, seed):
     vals[diag] = 1.05 * rowsum[rows[diag]] + 0.1 + 1j * rng.uniform(0.0, 2.0, diag.sum())
     s = 10.0 ** rng.uniform(-2, 2, n)
     vals *= s[rows] * s[ci]
-    return n, rp, ci, vals
+    return vals
 
 
 @pytest.fixture(scope="module")
@@ -126,3 +151,33 @@ def test_grid_matrix_fp32_refined(gpu_context):
     finally:
         m.close()
 
+
+
+@pytest.mark.parametrize("shape", [(120, 12, 30, 60, 1), (160, 20, 24, 80, 2)])
+def test_hub_front_with_many_children(gpu_context, shape):
+    """A front with more children than the extend-add maps at once (12 children
+    of at most 96 rows; 20 children of 104 rows, themselves large fronts): FP64
+    to the oracle's solution (relative error <= 1e-10), FP32 + refinement to
+    backward error <= 1e-12."""
+    n, rp, ci = star_pattern(*shape)
+    vals = dominant_values(n, rp, ci, shape[-1])
+    q = A.Problem.from_csr(n, rp, ci)
+    parent, rows = q.ints("super_parent"), np.diff(q.ints("super_row_ptr"))
+    children = np.bincount(parent[parent >= 0], minlength=rows.size)
+    assert children[rows > 96].max() > 8
+    m = A.Mesh(gpu_context, q)
+    try:
+        rng = np.random.default_rng(shape[-1] + 7)
+        B = rng.normal(size=(3, n)) + 1j * rng.normal(size=(3, n))
+        m.upload_system(vals, B[0])
+        X, rep = m.solve_symmetric(dev(B), precision=A.APHI_FP64)
+        assert rep["ok"]
+        ox, _, st = O.csr_solve_symmetric_raw(n, rp, ci, vals, B, q.ints("perm"), iterations=10)
+        assert st["ok"]
+        Xg = X.cpu().numpy()
+        for r in range(B.shape[0]):
+            assert np.linalg.norm(Xg[r] - ox[r]) / np.linalg.norm(ox[r]) <= 1e-10
+        X, rep = m.solve_symmetric(dev(B), precision=A.APHI_FP32, max_refinements=40, tolerance=1e-12)
+        assert rep["ok"] and rep["precision_used"] == A.APHI_FP32 and rep["backward_error"] <= 1e-12
+    finally:
+        m.close()
