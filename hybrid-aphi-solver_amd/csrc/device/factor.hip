@@ -276,9 +276,10 @@ __global__ void __launch_bounds__(256) k_front_small(const int* __restrict__ nod
 // chunk of ea_rows<T>() front rows); it owns those entries exclusively. It maps
 // the chunk's rows and columns to up to kEaChildren children at a time (LDS, -1
 // where the child has none; the child index ranges come from the precomputed
-// bounds), then takes the columns 16 / (ea_rows<T>() / 64) per wave at a time
-// with the entries in registers: wave w holds columns c0 + w + 4t (t < 16), lane l rows r0 + l + 64u.
-// In child order every entry adds its child entry if there is one. Each update-block
+// bounds), then takes the columns ea_entries<T>() / (ea_rows<T>() / 64) per wave
+// at a time with the entries in registers: wave w holds columns c0 + w + 4t
+// (t < 16), lane l rows r0 + l + 64u. In child order every entry adds its child
+// entry if there is one. Each update-block
 // entry is then written once (zero where no child contributes) and each touched
 // pivot-column entry becomes P + (sum of its children's entries): one write per
 // entry, deterministic without atomics.
